@@ -1,0 +1,155 @@
+/*
+ * piernik_hip.h -- C-ABI of the MI355X (gfx950) split Riemann sweep.
+ *
+ * Drop-in for PIERNIK's per-block solver operator.  The reference binds
+ *   procedure(solve_cg_sub), pointer :: solve_cg          (src/scheme/sweeps.F90:167)
+ * to solve_cg_riemann (src/scheme/Riemann_Solver/solve_cg_riemann.F90:68-107)
+ * and calls it once per block, direction and RK stage (sweeps.F90:245).
+ * pk_solve_cg() below is that call; the other entry points carry the state
+ * the reference keeps in Fortran module globals (flind, global, hdc) and the
+ * device-resident stepping that replaces the host loops around it.
+ *
+ * Conventions
+ *   - Plain C types only; every entry point returns 0 on success and a
+ *     PK_E* code otherwise; pk_last_error() gives the message (the Fortran
+ *     adapter passes it to die(), as the reference does on any error).
+ *   - Host arrays use the reference layout (column-major, ghosts included):
+ *       u(nvar, nxg, nyg, nzg), b(3, nxg, nyg, nzg), psi(nxg, nyg, nzg)
+ *     with nxg = nx + 2*nb in every active direction (src/base/named_array.F90).
+ *   - The library owns device mirrors of registered blocks; they stay
+ *     resident between calls.  pk_upload / pk_download move data explicitly.
+ *   - Directions: 0 = x, 1 = y, 2 = z (the reference's xdim..zdim minus one).
+ *   - RK stages use the reference's enumerators: EULER = 10000,
+ *     RK2_1 = 10001, RK2_2 = 10002 (src/base/constants.F90:230-238).
+ *   - One host thread drives the library; all work is ordered on one HIP
+ *     stream.  Calls that return values to the host synchronise that stream.
+ */
+#ifndef PIERNIK_HIP_H
+#define PIERNIK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_OK 0
+#define PK_EINVAL 1      /* bad argument or configuration */
+#define PK_ENOBLOCK 2    /* unknown cg_id */
+#define PK_EHIP 3        /* HIP runtime error */
+#define PK_ESTATE 4      /* call out of order (e.g. before pk_init) */
+#define PK_ENOMEM 5
+
+#define PK_EULER 10000
+#define PK_RK2_1 10001
+#define PK_RK2_2 10002
+
+/* limiter names of src/scheme/Riemann_Solver/fluxlimiters.F90:90-103 */
+#define PK_LIM_VANLEER 0
+#define PK_LIM_MINMOD 1
+#define PK_LIM_MONCEN 2
+#define PK_LIM_SUPERBEE 3
+
+/* One fluid of flind (src/fluids/fluidtypes.F90:279-343).  Adiabatic only. */
+typedef struct pk_fluid {
+    double gamma;          /* fl%gam */
+    int32_t magnetized;    /* fl%is_magnetized: HLLD + GLM (riemann_hlld); else HLLC (riemann_hlld_u) */
+    int32_t reserved;
+} pk_fluid;
+
+/* The &NUMERICAL_SETUP knobs the sweep reads (src/base/global.F90:118-120,205-262). */
+typedef struct pk_config {
+    int32_t nfluids;           /* 1 (this build); the magnetized fluid, if any, is fluid 0 */
+    int32_t limiter;           /* 'limiter'   -> PK_LIM_* */
+    int32_t limiter_b;         /* 'limiter_b' -> PK_LIM_* */
+    int32_t integration_order; /* 1 (EULER) or 2 (RK2) */
+    pk_fluid fluid[2];
+    double smalld;             /* density floor (sources.F90:357) */
+    double smallei;            /* internal-energy floor (sources.F90:426) */
+    int32_t use_smalld;
+    int32_t use_smallei;
+    double cfl;                /* CFL number of time_step (timestep.F90:492) */
+    double cfl_glm;            /* GLM speed factor, ch = cfl_glm*min(dl)/dt (hdc.F90:122) */
+    double glm_alpha;          /* GLM damping (hdc.F90:325) */
+    int32_t has_dir[3];        /* dom%has_dir: active directions */
+    int32_t tile_x[2];         /* sweep tile shape (pencils, cells) along x; 0 = autotuned default */
+    int32_t tile_yz[2];        /* sweep tile shape (pencils, cells) along y and z; 0 = default */
+} pk_config;
+
+/* Floor bookkeeping accumulated since the last reset
+ * (dn_negative, ei_negative: sources.F90:350-351,425; magic mass :356-358). */
+typedef struct pk_stage_report {
+    int32_t dn_negative;
+    int32_t ei_negative;
+    double magic_mass;     /* mass added by the density floor, interior cells */
+} pk_stage_report;
+
+/* ---- life cycle ---------------------------------------------------------- */
+
+/* init_fluids/init_global/set_interpolations in one call (initpiernik.F90:190-195). */
+int pk_init(const pk_config *cfg);
+int pk_finalize(void);
+const char *pk_last_error(void);
+
+/* ---- block registry (grid containers) ------------------------------------ */
+
+/* Register one cg: n = interior cells per direction, nb = ghost layers,
+ * dl = cell sizes.  Host pointers may be NULL (device-only block); psi and
+ * psih are required when a fluid is magnetized.  Allocates device mirrors. */
+int pk_register_block(int64_t cg_id, const int32_t n[3], int32_t nb, const double dl[3],
+                      double *u, double *uh, double *b, double *bh, double *psi, double *psih);
+int pk_unregister_block(int64_t cg_id);
+
+/* Host <-> device copies of the registered host arrays (u, b, psi). */
+int pk_upload(int64_t cg_id);
+int pk_download(int64_t cg_id);
+/* Same from / to caller-given host arrays in the reference layout. */
+int pk_upload_from(int64_t cg_id, const double *u, const double *b, const double *psi);
+int pk_download_to(int64_t cg_id, double *u, double *b, double *psi);
+
+/* ---- the operator --------------------------------------------------------- */
+
+/* solve_cg_sub(cg, ddim, istep): one RK stage of the split Riemann update of
+ * one block along ddim, on the device mirror (sweeps.F90:140-151 interface;
+ * solve_cg_riemann.F90:68-307 semantics: at the first stage the state is
+ * kept as uh, every transverse-interior pencil is updated over its whole
+ * length except the two end cells, floors apply to all cells).  chspeed is
+ * the GLM speed of hdc.F90:90-140. */
+int pk_solve_cg(int64_t cg_id, int32_t ddim, int32_t istep, double dt, double chspeed);
+
+/* Both RK2 stages of one directional sweep in one pass (requires nb >= 4
+ * and ghost cells consistent with the neighbours along ddim, as after
+ * update_boundaries).  psi_fac multiplies psi on output (GLM damping fused
+ * into the last sweep of a substep; 1 otherwise). */
+int pk_sweep(int64_t cg_id, int32_t ddim, double dt, double chspeed, double psi_fac);
+
+/* Periodic ghost refresh of all variables of one block, all directions,
+ * corners included (update_boundaries on a single periodic block). */
+int pk_fill_ghosts(int64_t cg_id);
+
+/* psi *= fac over the whole block (glmdamping, hdc.F90:338). */
+int pk_glm_damp(int64_t cg_id, double fac);
+
+/* timestep_fluid for every fluid of one block (timestep.F90:431-495):
+ * dt_out = cfl * min over cells and directions of dl/c, c_out = max c. */
+int pk_timestep_fluid(int64_t cg_id, double *dt_out, double *c_out);
+
+/* fluid_update_full on one periodic block (fluidupdate.F90:99-136):
+ * update_chspeed, X->Y->Z sweeps, GLM damping, Z->Y->X sweeps, GLM damping. */
+int pk_fluid_update(int64_t cg_id, double dt);
+
+/* nsteps iterations of the main loop (piernik.F90:124-170) with
+ * cflcontrol='warn': time_step, then fluid_update.  *t_out = simulated time,
+ * *dt_out = the last dt. */
+int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out);
+
+/* Floor bookkeeping of one block since the last reset. */
+int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset);
+
+/* Wait for all queued work. */
+int pk_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,315 @@
+/*
+ * block.c -- block-level part of the oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * Restates, for one grid block ("cg") with periodic boundaries:
+ *   solve_cg_riemann / solve_cg_ub / solve_cg_u
+ *                       src/scheme/Riemann_Solver/solve_cg_riemann.F90:68-307
+ *   get_sweep + sweep permutations
+ *                       src/base/named_array.F90:327-348,
+ *                       src/fluids/fluidtypes.F90:318-320, fluidindex.F90:190-192
+ *   periodic ghost refresh (update_boundaries on a single periodic block)
+ *                       src/scheme/sweeps.F90:66-107, src/grid/cg_list_bnd.F90:251-328
+ *   update_chspeed, glmdamping
+ *                       src/scheme/Riemann_Solver/hdc.F90:90-140, 289-349
+ *   time_step, timestep_fluid, ion_cs, neu_cs
+ *                       src/base/timestep.F90:112-240, 431-497,
+ *                       src/fluids/ionized/initionized.F90:89-145,
+ *                       src/fluids/neutral/initneutral.F90:82-109
+ *   fluid_update_full, make_3sweeps, sweep
+ *                       src/scheme/fluidupdate.F90:99-232, src/scheme/sweeps.F90:114-279
+ * with cflcontrol='warn' (no step retries), tend never binding, and no
+ * source terms other than the vanishing internal ones.
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+#define EULER 10000
+#define RK2_1 10001
+#define RK2_2 10002
+
+static const double small_c = 1.17549435082228750797e-38; /* tiny(real(1.0,4)), constants.F90:104 */
+
+static int gh(const or_params *p, const or_block *cg, int d) { return p->has_dir[d] ? cg->nb : 0; }
+
+static void gdims(const or_params *p, const or_block *cg, int ng[3])
+{
+    for (int d = 0; d < 3; d++) ng[d] = cg->n[d] + 2 * gh(p, cg, d);
+}
+
+static int any_mag(const or_params *p)
+{
+    for (int f = 0; f < p->nfluids; f++)
+        if (p->fl[f].magnetized) return 1;
+    return 0;
+}
+
+/* index of sweep-order variable k of fluid f in xyz order (involutions) */
+static int swp_var(int ddim, int k)
+{
+    static const int perm[3][5] = {{0, 1, 2, 3, 4}, {0, 2, 1, 3, 4}, {0, 3, 2, 1, 4}};
+    return perm[ddim][k];
+}
+static int swp_mag(int ddim, int k)
+{
+    static const int perm[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}};
+    return perm[ddim][k];
+}
+
+static double rk_coef(int istep) { return istep == RK2_1 ? 0.5 : 1.0; } /* constants.F90:237-238 */
+static int first_stage(const or_params *p) { return p->integration_order == 1 ? EULER : RK2_1; }
+
+/* solve_cg_riemann for one block, one direction, one stage (in place). */
+void or_solve_cg(const or_params *p, or_block *cg, int ddim, int istep,
+                 double dt, double chspeed, or_report *rep)
+{
+    int ng[3];
+    gdims(p, cg, ng);
+    int nv = p->nvar, mhd = any_mag(p);
+    int n = ng[ddim];
+    int t1 = (ddim == 0) ? 1 : 0, t2 = (ddim == 2) ? 1 : 2;
+    size_t sd[3] = {1, (size_t)ng[0], (size_t)ng[0] * ng[1]};
+    double dtodx = (rk_coef(istep) * dt) / cg->dl[ddim];
+    double dvol = 1.0;
+    for (int d = 0; d < 3; d++)
+        if (p->has_dir[d]) dvol *= cg->dl[d];
+    int first = (istep == first_stage(p));
+    int nbp = gh(p, cg, ddim);
+
+    double *u0 = (double *)malloc(sizeof(double) * (size_t)n * nv);
+    double *u = (double *)malloc(sizeof(double) * (size_t)n * nv);
+    double *u1 = (double *)malloc(sizeof(double) * (size_t)n * nv);
+    double *b0 = (double *)malloc(sizeof(double) * (size_t)n * 4);
+    double *b = (double *)malloc(sizeof(double) * (size_t)n * 4);
+    double *b1 = (double *)malloc(sizeof(double) * (size_t)n * 4);
+
+    for (int c2 = gh(p, cg, t2); c2 < gh(p, cg, t2) + cg->n[t2]; c2++) {
+        for (int c1 = gh(p, cg, t1); c1 < gh(p, cg, t1) + cg->n[t1]; c1++) {
+            size_t base = (size_t)c1 * sd[t1] + (size_t)c2 * sd[t2];
+            for (int i = 0; i < n; i++) {
+                size_t cell = base + (size_t)i * sd[ddim];
+                if (first) {
+                    for (int v = 0; v < nv; v++) cg->uh[cell * nv + v] = cg->u[cell * nv + v];
+                    if (mhd) {
+                        for (int c = 0; c < 3; c++) cg->bh[cell * 3 + c] = cg->b[cell * 3 + c];
+                        cg->psih[cell] = cg->psi[cell];
+                    }
+                }
+                for (int f = 0; f < p->nfluids; f++) {
+                    int o = p->fl[f].beg;
+                    for (int k = 0; k < 5; k++) {
+                        u0[(size_t)(o + k) * n + i] = cg->uh[cell * nv + o + swp_var(ddim, k)];
+                        u[(size_t)(o + k) * n + i] = cg->u[cell * nv + o + swp_var(ddim, k)];
+                    }
+                }
+                if (mhd) {
+                    for (int k = 0; k < 3; k++) {
+                        b0[(size_t)k * n + i] = cg->bh[cell * 3 + swp_mag(ddim, k)];
+                        b[(size_t)k * n + i] = cg->b[cell * 3 + swp_mag(ddim, k)];
+                    }
+                    b0[(size_t)3 * n + i] = cg->psih[cell];
+                    b[(size_t)3 * n + i] = cg->psi[cell];
+                }
+            }
+            or_pencil_stage(p, n, dtodx, chspeed, u0, u, b0, b, u1, b1, nbp, dvol, rep);
+            for (int i = 0; i < n; i++) {
+                size_t cell = base + (size_t)i * sd[ddim];
+                for (int f = 0; f < p->nfluids; f++) {
+                    int o = p->fl[f].beg;
+                    for (int k = 0; k < 5; k++)
+                        cg->u[cell * nv + o + swp_var(ddim, k)] = u1[(size_t)(o + k) * n + i];
+                }
+                if (mhd) {
+                    for (int k = 0; k < 3; k++) cg->b[cell * 3 + swp_mag(ddim, k)] = b1[(size_t)k * n + i];
+                    cg->psi[cell] = b1[(size_t)3 * n + i];
+                }
+            }
+        }
+    }
+    free(u0); free(u); free(u1); free(b0); free(b); free(b1);
+}
+
+/* Copy every ghost cell from its periodic image (all_fluid_boundaries +
+ * all_mag_boundaries on one periodic block, corners included). */
+void or_fill_ghosts_periodic(const or_params *p, or_block *cg)
+{
+    int ng[3], g[3];
+    gdims(p, cg, ng);
+    for (int d = 0; d < 3; d++) g[d] = gh(p, cg, d);
+    int nv = p->nvar, mhd = any_mag(p);
+    for (int k = 0; k < ng[2]; k++)
+        for (int j = 0; j < ng[1]; j++)
+            for (int i = 0; i < ng[0]; i++) {
+                int idx[3] = {i, j, k}, src[3], ghost = 0;
+                for (int d = 0; d < 3; d++) {
+                    int r = idx[d] - g[d];
+                    if (r < 0 || r >= cg->n[d]) ghost = 1;
+                    r = ((r % cg->n[d]) + cg->n[d]) % cg->n[d];
+                    src[d] = r + g[d];
+                }
+                if (!ghost) continue;
+                size_t dc = (size_t)i + (size_t)ng[0] * ((size_t)j + (size_t)ng[1] * k);
+                size_t sc = (size_t)src[0] + (size_t)ng[0] * ((size_t)src[1] + (size_t)ng[1] * src[2]);
+                for (int v = 0; v < nv; v++) cg->u[dc * nv + v] = cg->u[sc * nv + v];
+                if (mhd) {
+                    for (int c = 0; c < 3; c++) cg->b[dc * 3 + c] = cg->b[sc * 3 + c];
+                    cg->psi[dc] = cg->psi[sc];
+                }
+            }
+}
+
+static double min_dl(const or_params *p, const or_block *cg)
+{
+    double m = DBL_MAX;
+    for (int d = 0; d < 3; d++)
+        if (p->has_dir[d] && cg->dl[d] < m) m = cg->dl[d];
+    return m;
+}
+
+/* hdc.F90:112-122 with ch_grid = .true. */
+double or_update_chspeed(const or_params *p, const or_block *cg, double dt)
+{
+    double ch = DBL_MAX;
+    double c = (p->cfl_glm * min_dl(p, cg)) / dt;
+    if (c < ch) ch = c;
+    return ch;
+}
+
+/* hdc.F90:313-331 */
+double or_glm_factor(const or_params *p, const or_block *cg, double chspeed, double dt)
+{
+    double fac = 0.0;
+    double f = (p->glm_alpha * chspeed) / (min_dl(p, cg) / dt);
+    if (f > fac) fac = f;
+    return exp(-fac);
+}
+
+/* hdc.F90:338: psi (ghosts included) *= fac */
+void or_glmdamping(const or_params *p, or_block *cg, double fac)
+{
+    int ng[3];
+    gdims(p, cg, ng);
+    size_t nc = (size_t)ng[0] * ng[1] * ng[2];
+    for (size_t c = 0; c < nc; c++) cg->psi[c] = cg->psi[c] * fac;
+}
+
+/* timestep_fluid (timestep.F90:431-495) for fluid ifl with ion_cs / neu_cs */
+void or_timestep_fluid(const or_params *p, const or_block *cg, int ifl, double *dt_out, double *c_out)
+{
+    int ng[3], g[3];
+    gdims(p, cg, ng);
+    for (int d = 0; d < 3; d++) g[d] = gh(p, cg, d);
+    const or_fluid *fl = &p->fl[ifl];
+    int nv = p->nvar, o = fl->beg;
+    double gam = fl->gam, gam_1 = fl->gam - 1.0;
+    double c_fl = small_c, dt_proc[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (int k = g[2]; k < g[2] + cg->n[2]; k++)
+        for (int j = g[1]; j < g[1] + cg->n[1]; j++)
+            for (int i = g[0]; i < g[0] + cg->n[0]; i++) {
+                size_t cell = (size_t)i + (size_t)ng[0] * ((size_t)j + (size_t)ng[1] * k);
+                const double *uc = cg->u + cell * nv + o;
+                double v[3];
+                if (uc[0] > 0.0) {
+                    for (int d = 0; d < 3; d++) v[d] = fabs(uc[1 + d] / uc[0]);
+                } else {
+                    v[0] = v[1] = v[2] = 0.0;
+                }
+                double ek = (0.5 * ((uc[1] * uc[1] + uc[2] * uc[2]) + uc[3] * uc[3])) / uc[0];
+                double cs;
+                if (fl->magnetized) {
+                    const double *bc = cg->b + cell * 3;
+                    double pmag = 0.5 * ((bc[0] * bc[0] + bc[1] * bc[1]) + bc[2] * bc[2]);
+                    double ps = (uc[4] - ek) * gam_1 + (2.0 - gam) * pmag;
+                    double pr = ps - pmag;
+                    cs = sqrt(fabs((2.0 * pmag + gam * pr) / uc[0]));
+                } else {
+                    double pr = (uc[4] - ek) * gam_1;
+                    cs = sqrt(fabs((gam * pr) / uc[0]));
+                }
+                double c[3];
+                for (int d = 0; d < 3; d++) {
+                    c[d] = v[d] + cs;
+                    if (small_c > c[d]) c[d] = small_c; /* max(v + cs, small) */
+                }
+                double cm = c[0];
+                if (c[1] > cm) cm = c[1];
+                if (c[2] > cm) cm = c[2];
+                if (cm > c_fl) c_fl = cm;
+                for (int d = 0; d < 3; d++) {
+                    if (p->has_dir[d] && c[d] > 0.0) {
+                        double t = cg->dl[d] / c[d];
+                        if (t < dt_proc[d]) dt_proc[d] = t;
+                    } else {
+                        dt_proc[d] = DBL_MAX;
+                    }
+                }
+            }
+    double mn = dt_proc[0];
+    if (dt_proc[1] < mn) mn = dt_proc[1];
+    if (dt_proc[2] < mn) mn = dt_proc[2];
+    *dt_out = p->cfl * mn;
+    *c_out = c_fl;
+}
+
+/* time_step (timestep.F90:112-225) with dt_initial=-1, dt_max_grow=2,
+ * dt_max=huge and tend never binding.  nstep counts completed steps. */
+double or_time_step(const or_params *p, const or_block *cg, int nstep, double dt_old, double *c_all)
+{
+    double dt = DBL_MAX, c = 0.0;
+    for (int f = 0; f < p->nfluids; f++) {
+        double dt_, c_;
+        or_timestep_fluid(p, cg, f, &dt_, &c_);
+        if (dt_ < dt) dt = dt_;
+        if (c_ > c) c = c_;
+    }
+    if (nstep < 1) {
+        double t = 1.0 * dt; /* dt_initial = -1: min(dt, -dt_initial*dt) */
+        if (t < dt) dt = t;
+    } else if (dt_old > 0.0) {
+        double t = dt_old * 2.0;
+        if (t < dt) dt = t;
+    }
+    if (c_all) *c_all = c;
+    return dt;
+}
+
+static void sweep(const or_params *p, or_block *cg, int ddim, double dt, double chspeed, or_report *rep)
+{
+    int s0 = first_stage(p), s1 = (p->integration_order == 1) ? EULER : RK2_2;
+    for (int istep = s0; istep <= s1; istep++) {
+        or_solve_cg(p, cg, ddim, istep, dt, chspeed, rep);
+        or_fill_ghosts_periodic(p, cg);
+    }
+}
+
+/* fluid_update_full (fluidupdate.F90:99-136) for one periodic block. */
+void or_fluid_update(const or_params *p, or_block *cg, double dt, or_report *rep)
+{
+    int mhd = any_mag(p);
+    double ch = mhd ? or_update_chspeed(p, cg, dt) : 0.0;
+    for (int half = 0; half < 2; half++) {
+        for (int s = 0; s < 3; s++) {
+            int d = half == 0 ? s : 2 - s;
+            if (p->has_dir[d]) sweep(p, cg, d, dt, ch, rep);
+        }
+        if (mhd) or_glmdamping(p, cg, or_glm_factor(p, cg, ch, dt));
+    }
+}
+
+/* nsteps iterations of the main loop (piernik.F90:124-170): time_step then
+ * fluid_update.  Returns the simulated time 2*sum(dt). */
+double or_run(const or_params *p, or_block *cg, int nsteps, or_report *rep)
+{
+    double t = 0.0, dt = -1.0;
+    or_fill_ghosts_periodic(p, cg);
+    for (int s = 0; s < nsteps; s++) {
+        double c;
+        dt = or_time_step(p, cg, s, dt, &c);
+        or_fluid_update(p, cg, dt, rep);
+        t = t + dt;
+        t = t + dt;
+    }
+    return t;
+}

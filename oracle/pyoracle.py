@@ -1,0 +1,250 @@
+"""ctypes binding of oracle/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: the checker for piernik_amd.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+LIM = {"vanleer": 0, "minmod": 1, "moncen": 2, "superbee": 3}
+EULER, RK2_1, RK2_2 = 10000, 10001, 10002
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+class Fluid(ctypes.Structure):
+    _fields_ = [("gam", ctypes.c_double), ("magnetized", ctypes.c_int32), ("beg", ctypes.c_int32)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nfluids", ctypes.c_int32), ("fl", Fluid * 2), ("nvar", ctypes.c_int32),
+        ("limiter", ctypes.c_int32), ("limiter_b", ctypes.c_int32), ("integration_order", ctypes.c_int32),
+        ("use_smalld", ctypes.c_int32), ("use_smallei", ctypes.c_int32),
+        ("smalld", ctypes.c_double), ("smallei", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("cfl_glm", ctypes.c_double), ("glm_alpha", ctypes.c_double),
+        ("has_dir", ctypes.c_int32 * 3),
+    ]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("dn_negative", ctypes.c_int32), ("ei_negative", ctypes.c_int32),
+                ("magic_mass", ctypes.c_double * 2)]
+
+
+class Block(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("nb", ctypes.c_int32), ("dl", ctypes.c_double * 3)] + [
+        (nm, ctypes.c_void_p) for nm in ("u", "uh", "b", "bh", "psi", "psih")]
+
+
+_lib = None
+P = ctypes.POINTER
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        dp = P(ctypes.c_double)
+        L.or_spacing.argtypes = [ctypes.c_double]
+        L.or_spacing.restype = ctypes.c_double
+        L.or_limiter.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+        L.or_linear.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        L.or_utoq.argtypes = [P(Params), ctypes.c_int, dp, dp, dp]
+        L.or_hllc.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double, dp]
+        L.or_hlld.argtypes = [ctypes.c_int, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.or_floors.argtypes = [P(Params), ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_double, P(Report)]
+        L.or_pencil_stage.argtypes = [P(Params), ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp, dp, dp,
+                                      dp, dp, ctypes.c_int, ctypes.c_double, P(Report)]
+        L.or_solve_cg.argtypes = [P(Params), P(Block), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, P(Report)]
+        L.or_fill_ghosts_periodic.argtypes = [P(Params), P(Block)]
+        L.or_update_chspeed.argtypes = [P(Params), P(Block), ctypes.c_double]
+        L.or_update_chspeed.restype = ctypes.c_double
+        L.or_glm_factor.argtypes = [P(Params), P(Block), ctypes.c_double, ctypes.c_double]
+        L.or_glm_factor.restype = ctypes.c_double
+        L.or_glmdamping.argtypes = [P(Params), P(Block), ctypes.c_double]
+        L.or_timestep_fluid.argtypes = [P(Params), P(Block), ctypes.c_int, dp, dp]
+        L.or_time_step.argtypes = [P(Params), P(Block), ctypes.c_int, ctypes.c_double, dp]
+        L.or_time_step.restype = ctypes.c_double
+        L.or_fluid_update.argtypes = [P(Params), P(Block), ctypes.c_double, P(Report)]
+        L.or_run.argtypes = [P(Params), P(Block), ctypes.c_int, P(Report)]
+        L.or_run.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(P(ctypes.c_double))
+
+
+def params(gamma=5.0 / 3.0, magnetized=False, limiter="vanleer", limiter_b=None, integration_order=2,
+           smalld=1e-10, smallei=1e-10, use_smalld=True, use_smallei=True, cfl=0.7, cfl_glm=None,
+           glm_alpha=0.1, has_dir=(True, True, True)):
+    p = Params()
+    p.nfluids = 1
+    p.fl[0].gam = gamma
+    p.fl[0].magnetized = int(magnetized)
+    p.fl[0].beg = 0
+    p.nvar = 5
+    p.limiter = LIM[limiter]
+    p.limiter_b = LIM[limiter if limiter_b is None else limiter_b]
+    p.integration_order = integration_order
+    p.use_smalld, p.use_smallei = int(use_smalld), int(use_smallei)
+    p.smalld, p.smallei = smalld, smallei
+    p.cfl = cfl
+    p.cfl_glm = cfl if cfl_glm is None else cfl_glm
+    p.glm_alpha = glm_alpha
+    for d in range(3):
+        p.has_dir[d] = int(has_dir[d])
+    return p
+
+
+def params_from_config(cfg):
+    """Oracle parameters equal to a piernik_amd.Config."""
+    return params(gamma=cfg.gamma, magnetized=cfg.magnetized, limiter=cfg.limiter, limiter_b=cfg.limiter_b,
+                  integration_order=cfg.integration_order, smalld=cfg.smalld, smallei=cfg.smallei,
+                  use_smalld=cfg.use_smalld, use_smallei=cfg.use_smallei, cfl=cfg.cfl, cfl_glm=cfg.cfl_glm,
+                  glm_alpha=cfg.glm_alpha, has_dir=cfg.has_dir)
+
+
+class OBlock:
+    """Host block in the reference layout, driven by the oracle."""
+
+    def __init__(self, n, nb, dl, magnetized, has_dir=(True, True, True)):
+        self.n, self.nb, self.dl = tuple(n), nb, tuple(dl)
+        g = tuple(nb if h else 0 for h in has_dir)
+        self.ng = tuple(n[d] + 2 * g[d] for d in range(3))
+        self.u = np.zeros((5,) + self.ng, order="F")
+        self.uh = np.zeros_like(self.u)
+        self.b = np.zeros((3,) + self.ng, order="F") if magnetized else None
+        self.bh = np.zeros_like(self.b) if magnetized else None
+        self.psi = np.zeros(self.ng, order="F") if magnetized else None
+        self.psih = np.zeros_like(self.psi) if magnetized else None
+        self.rep = Report()
+
+    @classmethod
+    def from_cg(cls, cg):
+        o = cls(cg.n, cg.nb, cg.dl, cg.magnetized, cg.has_dir)
+        o.u[...] = cg.u
+        if cg.magnetized:
+            o.b[...] = cg.b
+            o.psi[...] = cg.psi
+        return o
+
+    def c(self):
+        blk = Block()
+        for d in range(3):
+            blk.n[d] = self.n[d]
+            blk.dl[d] = self.dl[d]
+        blk.nb = self.nb
+        for nm in ("u", "uh", "b", "bh", "psi", "psih"):
+            a = getattr(self, nm)
+            setattr(blk, nm, None if a is None else a.ctypes.data_as(ctypes.c_void_p))
+        return blk
+
+
+def solve_cg(p, ob, ddim, istep, dt, chspeed=0.0):
+    blk = ob.c()
+    lib().or_solve_cg(ctypes.byref(p), ctypes.byref(blk), ddim, istep, dt, chspeed, ctypes.byref(ob.rep))
+
+
+def fill_ghosts(p, ob):
+    blk = ob.c()
+    lib().or_fill_ghosts_periodic(ctypes.byref(p), ctypes.byref(blk))
+
+
+def time_step(p, ob, nstep, dt_old):
+    blk = ob.c()
+    c = ctypes.c_double()
+    dt = lib().or_time_step(ctypes.byref(p), ctypes.byref(blk), nstep, dt_old, ctypes.byref(c))
+    return dt, c.value
+
+
+def timestep_fluid(p, ob):
+    blk = ob.c()
+    dt, c = ctypes.c_double(), ctypes.c_double()
+    lib().or_timestep_fluid(ctypes.byref(p), ctypes.byref(blk), 0, ctypes.byref(dt), ctypes.byref(c))
+    return dt.value, c.value
+
+
+def update_chspeed(p, ob, dt):
+    blk = ob.c()
+    return lib().or_update_chspeed(ctypes.byref(p), ctypes.byref(blk), dt)
+
+
+def glm_factor(p, ob, ch, dt):
+    blk = ob.c()
+    return lib().or_glm_factor(ctypes.byref(p), ctypes.byref(blk), ch, dt)
+
+
+def fluid_update(p, ob, dt):
+    blk = ob.c()
+    lib().or_fluid_update(ctypes.byref(p), ctypes.byref(blk), dt, ctypes.byref(ob.rep))
+
+
+def run(p, ob, nsteps):
+    blk = ob.c()
+    return lib().or_run(ctypes.byref(p), ctypes.byref(blk), nsteps, ctypes.byref(ob.rep))
+
+
+# ---- pencil-level helpers (column-major (n, nv) arrays) -------------------
+
+def limiter(kind, q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    dq = np.empty_like(q)
+    lib().or_limiter(LIM[kind], len(q), _dp(q), _dp(dq))
+    return dq
+
+
+def linear(kind, q):
+    """q: (n, nv) -> ql, qr: (n-1, nv)."""
+    q = np.asfortranarray(q, dtype=np.float64)
+    n, nv = q.shape
+    ql = np.zeros((n - 1, nv), order="F")
+    qr = np.zeros((n - 1, nv), order="F")
+    lib().or_linear(LIM[kind], n, nv, _dp(q), _dp(ql), _dp(qr))
+    return ql, qr
+
+
+def utoq(p, u, b=None):
+    u = np.asfortranarray(u, dtype=np.float64)
+    q = np.zeros_like(u, order="F")
+    bb = None if b is None else np.asfortranarray(b, dtype=np.float64)
+    lib().or_utoq(ctypes.byref(p), u.shape[0], _dp(u), None if bb is None else _dp(bb), _dp(q))
+    return q
+
+
+def hllc(ql, qr, gam):
+    ql = np.asfortranarray(ql, dtype=np.float64)
+    qr = np.asfortranarray(qr, dtype=np.float64)
+    f = np.zeros_like(ql, order="F")
+    lib().or_hllc(ql.shape[0], _dp(ql), _dp(qr), gam, _dp(f))
+    return f
+
+
+def hlld(ql, qr, bl, br, gam, ch):
+    """ql, qr: (m, 5) prims; bl, br: (m, 4) [Bn, Bt1, Bt2, psi] -> f (m, 5), bf (m, 4)."""
+    ql = np.asfortranarray(ql, dtype=np.float64)
+    qr = np.asfortranarray(qr, dtype=np.float64)
+    bl = np.asfortranarray(bl, dtype=np.float64)
+    br = np.asfortranarray(br, dtype=np.float64)
+    f = np.zeros_like(ql, order="F")
+    bf = np.zeros_like(bl, order="F")
+    lib().or_hlld(ql.shape[0], _dp(ql), _dp(qr), _dp(bl), _dp(br), gam, ch, _dp(f), _dp(bf))
+    return f, bf
+
+
+def spacing(x):
+    return lib().or_spacing(float(x))

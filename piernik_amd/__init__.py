@@ -1,0 +1,322 @@
+"""piernik_amd -- MI355X (gfx950) split Riemann sweep of PIERNIK.
+
+Python host mirror of the reference operator interface over the C-ABI of
+libpiernik_hip.so (include/piernik_hip.h).  Names follow the reference:
+
+  Config                 &NUMERICAL_SETUP + fluid gamma (src/base/global.F90:205-262)
+  GridContainer (cg)     one grid block, reference array layout (named_array.F90)
+  solve_cg(cg, ddim, istep, dt, chspeed)
+                         solve_cg_sub / solve_cg_riemann (sweeps.F90:140-151,
+                         solve_cg_riemann.F90:68-107)
+  sweep(cg, ddim, dt, chspeed)    both RK2 stages of one directional sweep
+  update_boundaries(cg)  periodic ghost refresh (sweeps.F90:66-107)
+  time_step(cg, ...)     timestep_fluid + caps (timestep.F90:112-240)
+  fluid_update(cg, dt)   fluid_update_full (fluidupdate.F90:99-136)
+  run(cg, nsteps)        the main loop (piernik.F90:124-170)
+
+Errors raise PiernikError with the library's message, where the reference
+calls die().  There is no CPU fallback: without the built library every call
+raises.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+__all__ = [
+    "PiernikError", "Config", "GridContainer", "load_library", "init", "finalize",
+    "solve_cg", "sweep", "update_boundaries", "glmdamping", "timestep_fluid", "time_step",
+    "update_chspeed", "glm_factor", "fluid_update", "run", "get_report", "synchronize",
+    "EULER", "RK2_1", "RK2_2", "LIMITERS",
+]
+
+EULER, RK2_1, RK2_2 = 10000, 10001, 10002          # src/base/constants.F90:230-232
+LIMITERS = {"vanleer": 0, "minmod": 1, "moncen": 2, "superbee": 3}  # fluxlimiters.F90:90-103
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpiernik_hip.so")
+
+
+class PiernikError(RuntimeError):
+    """An error reported by the library (the reference would call die())."""
+
+
+class _Fluid(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("magnetized", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("nfluids", ctypes.c_int32), ("limiter", ctypes.c_int32), ("limiter_b", ctypes.c_int32),
+        ("integration_order", ctypes.c_int32), ("fluid", _Fluid * 2),
+        ("smalld", ctypes.c_double), ("smallei", ctypes.c_double),
+        ("use_smalld", ctypes.c_int32), ("use_smallei", ctypes.c_int32),
+        ("cfl", ctypes.c_double), ("cfl_glm", ctypes.c_double), ("glm_alpha", ctypes.c_double),
+        ("has_dir", ctypes.c_int32 * 3), ("tile_x", ctypes.c_int32 * 2), ("tile_yz", ctypes.c_int32 * 2),
+    ]
+
+
+class _Report(ctypes.Structure):
+    _fields_ = [("dn_negative", ctypes.c_int32), ("ei_negative", ctypes.c_int32), ("magic_mass", ctypes.c_double)]
+
+
+_lib = None
+
+_PROTOS = {
+    "pk_init": [ctypes.POINTER(_Config)],
+    "pk_finalize": [],
+    "pk_register_block": [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                          ctypes.POINTER(ctypes.c_double)] + [ctypes.c_void_p] * 6,
+    "pk_unregister_block": [ctypes.c_int64],
+    "pk_upload": [ctypes.c_int64],
+    "pk_download": [ctypes.c_int64],
+    "pk_upload_from": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "pk_download_to": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "pk_solve_cg": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
+    "pk_sweep": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+    "pk_fill_ghosts": [ctypes.c_int64],
+    "pk_glm_damp": [ctypes.c_int64, ctypes.c_double],
+    "pk_timestep_fluid": [ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "pk_fluid_update": [ctypes.c_int64, ctypes.c_double],
+    "pk_run": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "pk_get_report": [ctypes.c_int64, ctypes.POINTER(_Report), ctypes.c_int32],
+    "pk_synchronize": [],
+}
+
+
+def load_library(path=None):
+    """Load libpiernik_hip.so; raise PiernikError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise PiernikError(f"{p} not found: build it with `python -m piernik_amd.build`")
+    lib = ctypes.CDLL(p)
+    for name, args in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.pk_last_error.argtypes = []
+    lib.pk_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _call(name, *args):
+    lib = load_library()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        raise PiernikError(f"{name}: {lib.pk_last_error().decode()} (code {rc})")
+
+
+class Config:
+    """Run configuration, named as the reference's namelists.
+
+    Defaults follow src/base/global.F90:205-262 except the floors, which the
+    reference derives from the initial state (initfluids.F90:212-320) and which
+    must be given explicitly here (``sanitize_smallx`` reproduces that rule).
+    """
+
+    def __init__(self, gamma=5.0 / 3.0, magnetized=False, limiter="vanleer", limiter_b=None,
+                 integration_order=2, cfl=0.7, cfl_glm=None, glm_alpha=0.1, smalld=1e-10, smallei=1e-10,
+                 use_smalld=True, use_smallei=True, has_dir=(True, True, True), tile_x=(0, 0), tile_yz=(0, 0)):
+        self.gamma = float(gamma)
+        self.magnetized = bool(magnetized)
+        self.limiter = limiter
+        self.limiter_b = limiter if limiter_b is None else limiter_b
+        self.integration_order = int(integration_order)
+        self.cfl = float(cfl)
+        self.cfl_glm = self.cfl if cfl_glm is None else float(cfl_glm)
+        self.glm_alpha = float(glm_alpha)
+        self.smalld = float(smalld)
+        self.smallei = float(smallei)
+        self.use_smalld = bool(use_smalld)
+        self.use_smallei = bool(use_smallei)
+        self.has_dir = tuple(bool(h) for h in has_dir)
+        self.tile_x = tuple(tile_x)
+        self.tile_yz = tuple(tile_yz)
+
+    @property
+    def nvar(self):
+        return 5
+
+    def to_c(self):
+        c = _Config()
+        c.nfluids = 1
+        if self.limiter not in LIMITERS or self.limiter_b not in LIMITERS:
+            raise PiernikError(f"unknown limiter {self.limiter!r}/{self.limiter_b!r}")
+        c.limiter = LIMITERS[self.limiter]
+        c.limiter_b = LIMITERS[self.limiter_b]
+        c.integration_order = self.integration_order
+        c.fluid[0].gamma = self.gamma
+        c.fluid[0].magnetized = int(self.magnetized)
+        c.smalld, c.smallei = self.smalld, self.smallei
+        c.use_smalld, c.use_smallei = int(self.use_smalld), int(self.use_smallei)
+        c.cfl, c.cfl_glm, c.glm_alpha = self.cfl, self.cfl_glm, self.glm_alpha
+        for d in range(3):
+            c.has_dir[d] = int(self.has_dir[d])
+        c.tile_x[0], c.tile_x[1] = self.tile_x
+        c.tile_yz[0], c.tile_yz[1] = self.tile_yz
+        return c
+
+
+def sanitize_smallx(u, b=None, gamma=5.0 / 3.0, nb=4, has_dir=(True, True, True)):
+    """The reference's default floors (initfluids.F90:240-312): smalld is the
+    minimum initial density itself; smallp is 1e-4 of the minimum pressure."""
+    sl = tuple(slice(nb, -nb) if h else slice(None) for h in has_dir)
+    dn = u[(0,) + sl]
+    e = u[(4,) + sl] - 0.5 * (u[(1,) + sl] ** 2 + u[(2,) + sl] ** 2 + u[(3,) + sl] ** 2) / dn
+    if b is not None:
+        e = e - 0.5 * (b[(0,) + sl] ** 2 + b[(1,) + sl] ** 2 + b[(2,) + sl] ** 2)
+    return float(dn.min()), float((e / (gamma - 1.0)).min() * 1e-4)
+
+
+class GridContainer:
+    """One grid block ("cg") with host arrays in the reference layout.
+
+    u(5, nxg, nyg, nzg), b(3, nxg, nyg, nzg), psi(nxg, nyg, nzg), Fortran
+    order, ghost cells included (nxg = nx + 2*nb in active directions).
+    """
+
+    _next_id = 1
+
+    def __init__(self, n, nb=4, dl=None, magnetized=False, has_dir=(True, True, True), cg_id=None):
+        self.n = tuple(int(x) for x in n)
+        self.nb = int(nb)
+        self.has_dir = tuple(bool(h) for h in has_dir)
+        self.g = tuple(self.nb if h else 0 for h in self.has_dir)
+        self.ng = tuple(self.n[d] + 2 * self.g[d] for d in range(3))
+        if dl is None:
+            dl = tuple(1.0 / self.n[d] for d in range(3))
+        self.dl = tuple(float(x) for x in dl)
+        self.magnetized = bool(magnetized)
+        self.u = np.zeros((5,) + self.ng, dtype=np.float64, order="F")
+        self.uh = np.zeros_like(self.u)
+        self.b = np.zeros((3,) + self.ng, dtype=np.float64, order="F") if magnetized else None
+        self.bh = np.zeros_like(self.b) if magnetized else None
+        self.psi = np.zeros(self.ng, dtype=np.float64, order="F") if magnetized else None
+        self.psih = np.zeros_like(self.psi) if magnetized else None
+        if cg_id is None:
+            cg_id = GridContainer._next_id
+            GridContainer._next_id += 1
+        self.cg_id = int(cg_id)
+        self.registered = False
+
+    def interior(self, a):
+        """View of the interior cells of a host array (leading axis kept)."""
+        sl = tuple(slice(g, g + n) for g, n in zip(self.g, self.n))
+        return a[(slice(None),) + sl] if a.ndim == 4 else a[sl]
+
+    # -- device mirror ------------------------------------------------------
+    def register(self):
+        if self.registered:
+            return
+        n = (ctypes.c_int32 * 3)(*self.n)
+        dl = (ctypes.c_double * 3)(*self.dl)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        _call("pk_register_block", self.cg_id, n, self.nb, dl, ptr(self.u), ptr(self.uh), ptr(self.b),
+              ptr(self.bh), ptr(self.psi), ptr(self.psih))
+        self.registered = True
+
+    def upload(self):
+        self.register()
+        _call("pk_upload", self.cg_id)
+
+    def download(self):
+        _call("pk_download", self.cg_id)
+
+    def unregister(self):
+        if self.registered:
+            _call("pk_unregister_block", self.cg_id)
+            self.registered = False
+
+
+_cfg = None
+
+
+def init(cfg):
+    """Configure the library (init_global / init_fluids / set_interpolations)."""
+    global _cfg
+    c = cfg.to_c()
+    _call("pk_init", ctypes.byref(c))
+    _cfg = cfg
+
+
+def finalize():
+    global _cfg
+    if _lib is not None:
+        _call("pk_finalize")
+    _cfg = None
+
+
+def solve_cg(cg, ddim, istep, dt, chspeed=0.0):
+    """One RK stage of one block along ddim (0,1,2) -- solve_cg_sub."""
+    _call("pk_solve_cg", cg.cg_id, int(ddim), int(istep), float(dt), float(chspeed))
+
+
+def sweep(cg, ddim, dt, chspeed=0.0, psi_fac=1.0):
+    """Both RK2 stages of one directional sweep in one fused pass."""
+    _call("pk_sweep", cg.cg_id, int(ddim), float(dt), float(chspeed), float(psi_fac))
+
+
+def update_boundaries(cg):
+    """Periodic refresh of every ghost cell of the block on the device."""
+    _call("pk_fill_ghosts", cg.cg_id)
+
+
+def glmdamping(cg, fac):
+    _call("pk_glm_damp", cg.cg_id, float(fac))
+
+
+def timestep_fluid(cg):
+    """(dt, c_max) of timestep_fluid: dt = cfl * min(dl/c)."""
+    dt, c = ctypes.c_double(), ctypes.c_double()
+    _call("pk_timestep_fluid", cg.cg_id, ctypes.byref(dt), ctypes.byref(c))
+    return dt.value, c.value
+
+
+def time_step(cg, nstep, dt_old):
+    """time_step with dt_initial=-1, dt_max_grow=2, dt_max huge (timestep.F90:200-225)."""
+    dt, _ = timestep_fluid(cg)
+    if nstep >= 1 and dt_old > 0.0:
+        dt = min(dt, dt_old * 2.0)
+    return dt
+
+
+def _min_dl(cg):
+    return min(cg.dl[d] for d in range(3) if cg.has_dir[d])
+
+
+def update_chspeed(cg, dt, cfg=None):
+    """hdc.F90:112-122 with ch_grid: ch = cfl_glm*min(dl)/dt."""
+    cfg = cfg or _cfg
+    return min(1.7976931348623157e308, (cfg.cfl_glm * _min_dl(cg)) / dt)
+
+
+def glm_factor(cg, chspeed, dt, cfg=None):
+    """hdc.F90:313-331: exp(-max(0, glm_alpha*ch/(min(dl)/dt)))."""
+    cfg = cfg or _cfg
+    return math.exp(-max(0.0, (cfg.glm_alpha * chspeed) / (_min_dl(cg) / dt)))
+
+
+def fluid_update(cg, dt):
+    _call("pk_fluid_update", cg.cg_id, float(dt))
+
+
+def run(cg, nsteps):
+    """nsteps main-loop iterations on the device; returns (t, last dt)."""
+    t, dt = ctypes.c_double(), ctypes.c_double()
+    _call("pk_run", cg.cg_id, int(nsteps), ctypes.byref(t), ctypes.byref(dt))
+    return t.value, dt.value
+
+
+def get_report(cg, reset=False):
+    r = _Report()
+    _call("pk_get_report", cg.cg_id, ctypes.byref(r), int(bool(reset)))
+    return {"dn_negative": bool(r.dn_negative), "ei_negative": bool(r.ei_negative), "magic_mass": r.magic_mass}
+
+
+def synchronize():
+    _call("pk_synchronize")

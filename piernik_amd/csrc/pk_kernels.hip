@@ -1,0 +1,456 @@
+// pk_kernels.hip -- gfx950 kernels of the split Riemann sweep.
+//
+// Device layout of one grid block: SoA planes of FP64, one plane per
+// variable in xyz order [rho, mx, my, mz, E (, Bx, By, Bz, psi)], each plane
+// the ghost-padded box (x fastest):  a[v*sv + x + sy*y + sz*z].
+//
+// The sweep kernel maps one lane to one cell of a tile of TP pencils x TS
+// cells along the sweep direction and exchanges neighbour values (primitive
+// states, face states, fluxes) through LDS.  Lanes along x are consecutive,
+// so every global load and store is coalesced in all three directions.  In
+// FUSED mode both RK2 stages (rk_coef = 1/2, 1; constants.F90:237-238) run in
+// one pass: the tile recomputes stage 1 on a 4-cell halo, so U is read once
+// and U1 written once; intermediates never leave the CU.  In STAGE mode one
+// stage runs over whole pencils exactly as one solve_cg_riemann call does
+// (solve_cg_riemann.F90:170-228), pencil ends included.
+#include <hip/hip_runtime.h>
+#include "pk_math.hpp"
+#include "pk_kernels.hpp"
+
+namespace pk {
+
+template <int DIR>
+__device__ __forceinline__ int plane_of(int k)
+{
+    // sweep-order variable k -> xyz plane (fluidtypes.F90:318-320, fluidindex.F90:190-192)
+    if (DIR == 0) return k;
+    if (DIR == 1) {
+        constexpr int p[9] = {0, 2, 1, 3, 4, 6, 5, 7, 8};
+        return p[k];
+    }
+    constexpr int p[9] = {0, 3, 2, 1, 4, 7, 6, 5, 8};
+    return p[k];
+}
+
+// One RK stage of the tile.  A holds primitives then fluxes, B face states.
+template <class P>
+__device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)[P::NV],
+                                           const double (&W0)[P::NV], double coef, bool upd,
+                                           bool lo_end, bool hi_end, double *__restrict__ A,
+                                           double *__restrict__ B, int t, int tm, int tp, int nt,
+                                           double (&Wn)[P::NV])
+{
+    constexpr int NV = P::NV;
+    double q[NV];
+    prim<P>(W, a.gam_1, q);
+#pragma unroll
+    for (int v = 0; v < NV; v++) A[v * nt + t] = q[v];
+    __syncthreads();
+
+    double fp[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const int lim = (v < 5) ? a.lim : a.limb;
+        const double dl = q[v] - A[v * nt + tm];
+        const double dr = A[v * nt + tp] - q[v];
+        double dq;
+        if (lo_end)
+            dq = (lim == LIM_VANLEER) ? 0.0 : limit(lim, dr, dr);
+        else if (hi_end)
+            dq = (lim == LIM_VANLEER) ? 0.0 : limit(lim, dl, dl);
+        else
+            dq = limit(lim, dl, dr);
+        const double h = 0.5 * dq;
+        fp[v] = q[v] + h;
+        B[v * nt + t] = q[v] - h;
+    }
+    __syncthreads();
+
+    double fr[NV], F[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) fr[v] = B[v * nt + tp];
+    if constexpr (P::MAG)
+        flux_hlld(fp, fr, a.gam, a.ch, F);
+    else
+        flux_hllc(fp, fr, a.gam, F);
+#pragma unroll
+    for (int v = 0; v < NV; v++) A[v * nt + t] = F[v];
+    __syncthreads();
+
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double fm = A[v * nt + tm];
+        const double un = upd ? W0[v] + coef * (fm - F[v]) : W[v];
+        Wn[v] = (v < 5) ? un + 0.0 : un;  // internal_sources adds +0 to the fluid part
+    }
+}
+
+template <int DIR, class P, bool FUSED>
+__global__ __launch_bounds__(P::MAG ? 512 : 1024) void k_sweep(const SweepArgs a)
+{
+    constexpr int NV = P::NV;
+    constexpr int HALO = FUSED ? 4 : 2;
+    extern __shared__ double lds[];
+    const int nt = blockDim.x;
+    const int t = threadIdx.x;
+    double *LA = lds;
+    double *LB = lds + NV * nt;
+
+    constexpr int ds = DIR;
+    constexpr int d1 = (DIR == 0) ? 1 : 0;
+    constexpr int d2 = (DIR == 2) ? 1 : 2;
+
+    int p, s, tm, tp;
+    if (DIR == 0) {
+        s = t % a.TS;
+        p = t / a.TS;
+        tm = t - 1;
+        tp = t + 1;
+    } else {
+        p = t % a.TP;
+        s = t / a.TP;
+        tm = t - a.TP;
+        tp = t + a.TP;
+    }
+    tm = tm < 0 ? 0 : tm;
+    tp = tp >= nt ? nt - 1 : tp;
+
+    const int bid = blockIdx.x;
+    const int chunk = bid % a.chunks;
+    const int rest = bid / a.chunks;
+    const int i1t = rest % a.tiles1;
+    const int i2 = rest / a.tiles1;
+
+    const int p1 = i1t * a.TP + p;
+    const bool pvalid = p1 < a.n[d1];
+    const int c1 = a.g[d1] + (pvalid ? p1 : a.n[d1] - 1);
+    const int c2 = a.g[d2] + i2;
+    const int ng = a.ng[ds];
+    const int os = (FUSED ? a.g[ds] : 0) + chunk * a.out;
+    const int oe = FUSED ? a.g[ds] + a.n[ds] : ng;
+    const int cs = os - HALO + s;
+    const bool svalid = cs >= 0 && cs < ng;
+    const int csc = cs < 0 ? 0 : (cs >= ng ? ng - 1 : cs);
+    const bool outl = pvalid && s >= HALO && s < HALO + a.out && cs < oe;
+
+    int xyz[3];
+    xyz[ds] = csc;
+    xyz[d1] = c1;
+    xyz[d2] = c2;
+    const long long idx = (long long)xyz[0] + (long long)a.sy * xyz[1] + (long long)a.sz * xyz[2];
+
+    double W[NV], W0[NV], Wn[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx];
+
+    unsigned flags = 0u;
+    double dmass = 0.0;
+    if constexpr (FUSED) {
+        double Ws[NV];
+        tile_stage<P>(a, W, W, a.c1, true, false, false, LA, LB, t, tm, tp, nt, Ws);
+        double dm = 0.0;
+        unsigned fl = 0u;
+        floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+        if (outl) {
+            flags |= fl;
+            dmass += dm;
+        }
+        tile_stage<P>(a, Ws, W, a.c2, true, false, false, LB, LA, t, tm, tp, nt, Wn);
+    } else {
+        const bool lo = cs == 0, hi = cs == ng - 1;
+        if (outl && a.base != a.src) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) W0[k] = a.base[(long long)plane_of<DIR>(k) * a.sv + idx];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NV; k++) W0[k] = W[k];
+        }
+        tile_stage<P>(a, W, W0, a.c1, !(lo || hi), lo, hi, LA, LB, t, tm, tp, nt, Wn);
+    }
+    {
+        double dm = 0.0;
+        unsigned fl = 0u;
+        floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+        const bool inner = cs >= a.g[ds] && cs < a.g[ds] + a.n[ds];
+        if (outl) {
+            flags |= fl;
+            if (inner) dmass += dm;
+        }
+    }
+    if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
+
+    if (outl && svalid) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + idx] = Wn[k];
+    }
+    // floor bookkeeping: rare, one atomic per lane that saw something
+    if (flags) atomicOr(a.flags, flags);
+    if (dmass != 0.0) atomicAdd(a.dmass, dmass);
+}
+
+// Periodic refresh of the ghost layers along direction DIR over the full
+// extent of the other two directions (run x, then y, then z for corners).
+template <int DIR>
+__global__ void k_fill_ghosts(const GhostArgs a)
+{
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int d1 = (DIR == 0) ? 1 : 0;
+    constexpr int d2 = (DIR == 2) ? 1 : 2;
+    const int g = a.g[DIR];
+    const long long per_plane = (long long)(2 * g) * a.ng[d1] * a.ng[d2];
+    if (tid >= per_plane * a.nplanes) return;
+    const int v = (int)(tid / per_plane);
+    long long r = tid % per_plane;
+    // fastest index: x when DIR != 0, else the ghost layer index
+    int xyz[3];
+    int gi;
+    if (DIR == 0) {
+        gi = (int)(r % (2 * g));
+        r /= (2 * g);
+        xyz[d1] = (int)(r % a.ng[d1]);
+        xyz[d2] = (int)(r / a.ng[d1]);
+    } else {
+        xyz[d1] = (int)(r % a.ng[d1]);
+        r /= a.ng[d1];
+        gi = (int)(r % (2 * g));
+        xyz[d2] = (int)(r / (2 * g));
+    }
+    const int n = a.n[DIR];
+    const int dstc = (gi < g) ? gi : n + gi;          // low ghosts 0..g-1, high ghosts g+n..
+    const int srcc = (gi < g) ? gi + n : gi;          // periodic images inside the interior
+    int s3[3] = {xyz[0], xyz[1], xyz[2]};
+    xyz[DIR] = dstc;
+    s3[DIR] = srcc;
+    const long long di = (long long)xyz[0] + (long long)a.sy * xyz[1] + (long long)a.sz * xyz[2];
+    const long long si = (long long)s3[0] + (long long)a.sy * s3[1] + (long long)a.sz * s3[2];
+    double *base = a.u + (long long)(a.plane0 + v) * a.sv;
+    base[di] = base[si];
+}
+
+__global__ void k_scale(double *__restrict__ x, long long n, double fac)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] * fac;
+}
+
+// timestep_fluid (timestep.F90:460-490) with ion_cs / neu_cs
+// (initionized.F90:137-141, initneutral.F90:102-105): per-workgroup partial
+// min of dl_d/c_d for d = x, y, z and max of c.
+template <bool MAG>
+__global__ __launch_bounds__(256) void k_dt_partial(const DtArgs a, double *__restrict__ part)
+{
+    __shared__ double red[4][256];
+    const int t = threadIdx.x;
+    const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+    double m0 = 1.7976931348623157e308, m1 = m0, m2 = m0, cmax = 1.17549435082228750797e-38;
+    for (long long c = (long long)blockIdx.x * blockDim.x + t; c < ncell; c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % a.n[0]);
+        const long long r = c / a.n[0];
+        const int j = (int)(r % a.n[1]);
+        const int k = (int)(r / a.n[1]);
+        const long long idx = (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
+        const double *u = a.u + (long long)a.plane0 * a.sv + idx;
+        const double d = u[0], mx = u[a.sv], my = u[2 * a.sv], mz = u[3 * a.sv], e = u[4 * a.sv];
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (d > 0.0) {
+            v0 = fabs(mx / d);
+            v1 = fabs(my / d);
+            v2 = fabs(mz / d);
+        }
+        const double ek = (0.5 * ((mx * mx + my * my) + mz * mz)) / d;
+        double cs;
+        if (MAG) {
+            const double *b = a.u + (long long)a.bplane * a.sv + idx;
+            const double bx = b[0], by = b[a.sv], bz = b[2 * a.sv];
+            const double pmag = 0.5 * ((bx * bx + by * by) + bz * bz);
+            const double ps = (e - ek) * a.gam_1 + (2.0 - a.gam) * pmag;
+            const double pr = ps - pmag;
+            cs = sqrt(fabs((2.0 * pmag + a.gam * pr) / d));
+        } else {
+            const double pr = (e - ek) * a.gam_1;
+            cs = sqrt(fabs((a.gam * pr) / d));
+        }
+        const double small = 1.17549435082228750797e-38;
+        const double c0 = fmx(v0 + cs, small), c1 = fmx(v1 + cs, small), c2 = fmx(v2 + cs, small);
+        cmax = fmx(cmax, fmx(fmx(c0, c1), c2));
+        if (a.has_dir[0]) m0 = fmn(m0, a.dl[0] / c0);
+        if (a.has_dir[1]) m1 = fmn(m1, a.dl[1] / c1);
+        if (a.has_dir[2]) m2 = fmn(m2, a.dl[2] / c2);
+    }
+    red[0][t] = m0;
+    red[1][t] = m1;
+    red[2][t] = m2;
+    red[3][t] = cmax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] = fmn(red[0][t], red[0][t + w]);
+            red[1][t] = fmn(red[1][t], red[1][t + w]);
+            red[2][t] = fmn(red[2][t], red[2][t + w]);
+            red[3][t] = fmx(red[3][t], red[3][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[4 * blockIdx.x + 0] = red[0][0];
+        part[4 * blockIdx.x + 1] = red[1][0];
+        part[4 * blockIdx.x + 2] = red[2][0];
+        part[4 * blockIdx.x + 3] = red[3][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dt_final(const double *__restrict__ part, int nparts,
+                                                  double *__restrict__ out)
+{
+    __shared__ double red[4][256];
+    const int t = threadIdx.x;
+    double m0 = 1.7976931348623157e308, m1 = m0, m2 = m0, cmax = 0.0;
+    for (int i = t; i < nparts; i += 256) {
+        m0 = fmn(m0, part[4 * i + 0]);
+        m1 = fmn(m1, part[4 * i + 1]);
+        m2 = fmn(m2, part[4 * i + 2]);
+        cmax = fmx(cmax, part[4 * i + 3]);
+    }
+    red[0][t] = m0;
+    red[1][t] = m1;
+    red[2][t] = m2;
+    red[3][t] = cmax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] = fmn(red[0][t], red[0][t + w]);
+            red[1][t] = fmn(red[1][t], red[1][t + w]);
+            red[2][t] = fmn(red[2][t], red[2][t + w]);
+            red[3][t] = fmx(red[3][t], red[3][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+        out[2] = red[2][0];
+        out[3] = red[3][0];
+    }
+}
+
+// Fortran AoS block (nc, nxg, nyg, nzg) <-> device planes [plane0 + c].
+__global__ void k_aos_to_soa(const double *__restrict__ aos, int nc, double *__restrict__ soa, long long sv,
+                             int plane0, int nx, int ny, int nz, int sy, int sz)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    if (i >= ncell) return;
+    const int x = (int)(i % nx);
+    const long long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const long long di = (long long)x + (long long)sy * y + (long long)sz * z;
+    for (int c = 0; c < nc; c++) soa[(long long)(plane0 + c) * sv + di] = aos[i * nc + c];
+}
+
+__global__ void k_soa_to_aos(const double *__restrict__ soa, long long sv, int plane0, int nc,
+                             double *__restrict__ aos, int nx, int ny, int nz, int sy, int sz)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    if (i >= ncell) return;
+    const int x = (int)(i % nx);
+    const long long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const long long si = (long long)x + (long long)sy * y + (long long)sz * z;
+    for (int c = 0; c < nc; c++) aos[i * nc + c] = soa[(long long)(plane0 + c) * sv + si];
+}
+
+// ------------------------------------------------------------ launchers ----
+
+template <int DIR, class P, bool FUSED>
+static hipError_t launch_sweep_t(const SweepArgs &a, hipStream_t st)
+{
+    const int nt = a.TP * a.TS;
+    const size_t lds = sizeof(double) * 2 * P::NV * nt;
+    static bool lds_raised = false;  // allow dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU)
+    if (!lds_raised) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_sweep<DIR, P, FUSED>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        lds_raised = true;
+    }
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const long long nblk = (long long)a.chunks * a.tiles1 * a.n[(DIR == 2) ? 1 : 2];
+    hipLaunchKernelGGL((k_sweep<DIR, P, FUSED>), dim3((unsigned)nblk), dim3(nt), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st)
+{
+#define PK_CASE(D, PP, F) \
+    if (dir == D && (mhd == PP::MAG) && fused == F) return launch_sweep_t<D, PP, F>(a, st);
+    PK_CASE(0, HD, true) PK_CASE(1, HD, true) PK_CASE(2, HD, true)
+    PK_CASE(0, HD, false) PK_CASE(1, HD, false) PK_CASE(2, HD, false)
+    PK_CASE(0, MHD, true) PK_CASE(1, MHD, true) PK_CASE(2, MHD, true)
+    PK_CASE(0, MHD, false) PK_CASE(1, MHD, false) PK_CASE(2, MHD, false)
+#undef PK_CASE
+    return hipErrorInvalidValue;
+}
+
+size_t sweep_lds_bytes(bool mhd, int TP, int TS) { return sizeof(double) * 2 * (mhd ? 9 : 5) * TP * TS; }
+
+hipError_t launch_fill_ghosts(int dir, const GhostArgs &a, hipStream_t st)
+{
+    const int d1 = (dir == 0) ? 1 : 0, d2 = (dir == 2) ? 1 : 2;
+    const long long total = (long long)(2 * a.g[dir]) * a.ng[d1] * a.ng[d2] * a.nplanes;
+    if (total == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    if (dir == 0) hipLaunchKernelGGL(k_fill_ghosts<0>, dim3(blocks), dim3(256), 0, st, a);
+    if (dir == 1) hipLaunchKernelGGL(k_fill_ghosts<1>, dim3(blocks), dim3(256), 0, st, a);
+    if (dir == 2) hipLaunchKernelGGL(k_fill_ghosts<2>, dim3(blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(double *x, long long n, double fac, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, fac);
+    return hipGetLastError();
+}
+
+int dt_partial_blocks(const DtArgs &a)
+{
+    const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+    long long b = (ncell + 255) / 256;
+    if (b > 2048) b = 2048;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+hipError_t launch_dt(const DtArgs &a, double *part, double *out, hipStream_t st)
+{
+    const int nb = dt_partial_blocks(a);
+    if (a.magnetized)
+        hipLaunchKernelGGL(k_dt_partial<true>, dim3(nb), dim3(256), 0, st, a, part);
+    else
+        hipLaunchKernelGGL(k_dt_partial<false>, dim3(nb), dim3(256), 0, st, a, part);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, part, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(const double *aos, int nc, double *soa, long long sv, int plane0, const int ng[3],
+                             int sy, int sz, hipStream_t st)
+{
+    const long long n = (long long)ng[0] * ng[1] * ng[2];
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, aos, nc, soa, sv, plane0,
+                       ng[0], ng[1], ng[2], sy, sz);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(const double *soa, long long sv, int plane0, int nc, double *aos, const int ng[3],
+                             int sy, int sz, hipStream_t st)
+{
+    const long long n = (long long)ng[0] * ng[1] * ng[2];
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, soa, sv, plane0, nc, aos,
+                       ng[0], ng[1], ng[2], sy, sz);
+    return hipGetLastError();
+}
+
+}  // namespace pk

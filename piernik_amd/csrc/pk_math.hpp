@@ -1,0 +1,393 @@
+// pk_math.hpp -- per-cell and per-interface FP64 arithmetic of the split
+// Riemann sweep, for gfx950.
+//
+// Every function keeps the reference's floating-point evaluation order
+// (left-to-right sums, explicit parentheses, x**2 as x*x) and is compiled with
+// -ffp-contract=off, so each operation rounds exactly as PIERNIK's FP64 path:
+//   prim()        src/scheme/Riemann_Solver/interpolations.F90:69-106 (utoq)
+//   limit()       src/scheme/Riemann_Solver/fluxlimiters.F90:109-216
+//   flux_hllc()   src/scheme/Riemann_Solver/hlld.F90:615-841 (riemann_hlld_u)
+//   flux_hlld()   src/scheme/Riemann_Solver/hlld.F90:161-609 (riemann_hlld, DIVB_HDC)
+//   floors()      src/fluids/sources.F90:299-433 (care_for_positives)
+//
+// Variables are in sweep order: the normal component comes first.
+//   HD  (5):  rho, m_n, m_t1, m_t2, E            prims: rho, v_n, v_t1, v_t2, p
+//   MHD (9):  HD + B_n, B_t1, B_t2, psi          prims: HD prims + B, psi
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Callable from device code and, for host-side checks of the same
+// arithmetic, from host code.
+#define PK_HD __host__ __device__ __forceinline__
+
+namespace pk {
+
+enum Limiter : int { LIM_VANLEER = 0, LIM_MINMOD = 1, LIM_MONCEN = 2, LIM_SUPERBEE = 3 };
+
+// Fortran min/max on non-NaN operands.
+PK_HD double fmn(double a, double b) { return (b < a) ? b : a; }
+PK_HD double fmx(double a, double b) { return (b > a) ? b : a; }
+
+// Fortran SPACING(x) for real(8): 2**max(exponent(x)-53, -1022); TINY for 0.
+// For a biased exponent field E >= 53 that is the power of two with exponent
+// field E-52, otherwise 2**-1022.
+PK_HD double spacing(double x)
+{
+    const uint64_t bits = __builtin_bit_cast(uint64_t, x);
+    const uint64_t e = (bits >> 52) & 0x7ffull;
+    return (e >= 53) ? __builtin_bit_cast(double, (e - 52) << 52) : 2.2250738585072014e-308;
+}
+
+// One slope of the configured limiter from the backward (dl) and forward (dr)
+// differences.  fluxlimiters.F90: vanleer :125-131, minmod :159,
+// moncen :186, superbee :212-213.
+PK_HD double limit(int lim, double dl, double dr)
+{
+    if (lim == LIM_VANLEER) {
+        const double dcen = dl * dr;
+        return (dcen > 0.0) ? (2.0 * dcen) / (dl + dr) : 0.0;
+    }
+    const double s = copysign(1.0, dl) + copysign(1.0, dr);
+    const double a = fabs(dl), b = fabs(dr);
+    if (lim == LIM_MINMOD) return (s * fmn(a, b)) * 0.5;
+    if (lim == LIM_MONCEN) return (s * fmn(fmn(2.0 * a, 2.0 * b), 0.5 * fabs(dl + dr))) * 0.5;
+    const double m = (a > b) ? fmn(a, fabs(2.0 * dr)) : fmn(fabs(2.0 * dl), b);
+    return (s * 0.5) * m;
+}
+
+struct HD {
+    static constexpr int NV = 5;
+    static constexpr bool MAG = false;
+};
+struct MHD {
+    static constexpr int NV = 9;
+    static constexpr bool MAG = true;
+};
+
+// Conserved -> primitive (utoq).  gam_1 = gamma - 1 as stored by the
+// reference (fluidtypes.F90:336).
+template <class P>
+PK_HD void prim(const double (&u)[P::NV], double gam_1, double (&q)[P::NV])
+{
+    const double d = u[0];
+    q[0] = d;
+    q[1] = u[1] / d;
+    q[2] = u[2] / d;
+    q[3] = u[3] / d;
+    const double ek = (0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3])) / d;
+    double p = gam_1 * (u[4] - ek);
+    if constexpr (P::MAG) {
+        const double b2 = (u[5] * u[5] + u[6] * u[6]) + u[7] * u[7];
+        p = p - (0.5 * gam_1) * b2;
+        q[5] = u[5];
+        q[6] = u[6];
+        q[7] = u[7];
+        q[8] = u[8];
+    }
+    q[4] = p;
+}
+
+// HLLC (riemann_hlld_u, adiabatic) for one interface.  l, r: primitive face
+// states; f: fluxes of rho, m_n, m_t1, m_t2, E.
+PK_HD void flux_hllc(const double *l, const double *r, double gam, double *f)
+{
+    const double prl = l[4], prr = r[4];
+    const double enl = (l[4] / (gam - 1.0)) + (0.5 * l[0]) * ((l[1] * l[1] + l[2] * l[2]) + l[3] * l[3]);
+    const double enr = (r[4] / (gam - 1.0)) + (0.5 * r[0]) * ((r[1] * r[1] + r[2] * r[2]) + r[3] * r[3]);
+    const double c = sqrt(fmx((gam * l[4]) / l[0], (gam * r[4]) / r[0]));
+    const double sl = fmn(l[1], r[1]) - c;
+    const double sr = fmx(l[1], r[1]) + c;
+
+    double fl[5], fr[5];
+    fl[0] = l[0] * l[1];
+    fl[1] = l[0] * (l[1] * l[1]) + prl;
+    fl[2] = (l[0] * l[2]) * l[1];
+    fl[3] = (l[0] * l[3]) * l[1];
+    fl[4] = (enl + prl) * l[1];
+    fr[0] = r[0] * r[1];
+    fr[1] = r[0] * (r[1] * r[1]) + prr;
+    fr[2] = (r[0] * r[2]) * r[1];
+    fr[3] = (r[0] * r[3]) * r[1];
+    fr[4] = (enr + prr) * r[1];
+
+    if (sl >= 0.0) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fl[v];
+        return;
+    }
+    if (sr <= 0.0) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fr[v];
+        return;
+    }
+    const double auxr = (sr - r[1]) * r[0], auxl = (sl - l[1]) * l[0];
+    const double sm = (fabs(auxr - auxl) < spacing(auxr + auxl))
+                          ? (sl + sr) / 2.0
+                          : ((auxr * r[1] - prr) - (auxl * l[1] - prl)) / (auxr - auxl);
+    const double slvxl = sl - l[1], srvxr = sr - r[1];
+    const double dn_l = l[0] * slvxl, dn_r = r[0] * srvxr;
+    const double prt_star = 0.5 * ((prl + dn_l * (sm - l[1])) + (prr + dn_r * (sm - r[1])));
+    if (sm > 0.0) {
+        const double slsm = sl - sm;
+        const double d = dn_l / slsm;
+        const double us[5] = {d, d * sm, d * l[2], d * l[3],
+                              (((slvxl * enl) - (prl * l[1])) + (prt_star * sm)) / slsm};
+        const double U[5] = {l[0], l[0] * l[1], l[0] * l[2], l[0] * l[3], enl};
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fl[v] + sl * (us[v] - U[v]);
+    } else if (sm < 0.0) {
+        const double srsm = sr - sm;
+        const double d = dn_r / srsm;
+        const double us[5] = {d, d * sm, d * r[2], d * r[3],
+                              (((srvxr * enr) - (prr * r[1])) + (prt_star * sm)) / srsm};
+        const double U[5] = {r[0], r[0] * r[1], r[0] * r[2], r[0] * r[3], enr};
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fr[v] + sr * (us[v] - U[v]);
+    } else {
+        // sm == 0: half*(fl + sl*(us_l - U_l) + fr + sr*(us_r - U_r)), hlld.F90:822
+        const double slsm = sl - sm, srsm = sr - sm;
+        const double dl_ = dn_l / slsm, dr_ = dn_r / srsm;
+        const double usl[5] = {dl_, dl_ * sm, dl_ * l[2], dl_ * l[3],
+                               (((slvxl * enl) - (prl * l[1])) + (prt_star * sm)) / slsm};
+        const double usr[5] = {dr_, dr_ * sm, dr_ * r[2], dr_ * r[3],
+                               (((srvxr * enr) - (prr * r[1])) + (prt_star * sm)) / srsm};
+        const double Ul[5] = {l[0], l[0] * l[1], l[0] * l[2], l[0] * l[3], enl};
+        const double Ur[5] = {r[0], r[0] * r[1], r[0] * r[2], r[0] * r[3], enr};
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            f[v] = 0.5 * (((fl[v] + sl * (usl[v] - Ul[v])) + fr[v]) + sr * (usr[v] - Ur[v]));
+    }
+}
+
+// HLLD with the GLM psi flux (riemann_hlld, DIVB_HDC) for one interface.
+// l, r: primitive face states [rho, v_n, v_t1, v_t2, p, B_n, B_t1, B_t2, psi].
+// f: fluxes of [rho, m_n, m_t1, m_t2, E, B_n, B_t1, B_t2, psi].
+PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
+                                          double *f)
+{
+    // Dedner et al. eq. 42 (hlld.F90:229-230)
+    f[8] = ((ch * ch) * 0.5) * ((r[5] + l[5]) - (r[8] - l[8]) / ch);
+    f[5] = 0.5 * ((r[8] + l[8]) - ch * (r[5] - l[5]));
+
+    const double bxl = l[5], byl = l[6], bzl = l[7];
+    const double bxr = r[5], byr = r[6], bzr = r[7];
+    const double b2l = (bxl * bxl + byl * byl) + bzl * bzl;
+    const double b2r = (bxr * bxr + byr * byr) + bzr * bzr;
+    const double prl = l[4] + 0.5 * b2l;
+    const double prr = r[4] + 0.5 * b2r;
+    const double enl = ((l[4] / (gam - 1.0)) + (0.5 * l[0]) * ((l[1] * l[1] + l[2] * l[2]) + l[3] * l[3])) + 0.5 * b2l;
+    const double enr = ((r[4] / (gam - 1.0)) + (0.5 * r[0]) * ((r[1] * r[1] + r[2] * r[2]) + r[3] * r[3])) + 0.5 * b2r;
+    const double gpl = gam * l[4], gpr = gam * r[4];
+    const double tl = gpl + b2l, tr = gpr + b2r;
+    const double cl = (tl + sqrt(tl * tl - (4.0 * gpl) * (bxl * bxl))) / l[0];
+    const double cr = (tr + sqrt(tr * tr - (4.0 * gpr) * (bxr * bxr))) / r[0];
+    const double c = sqrt(0.5 * fmx(cl, cr));
+    const double sl = fmn(l[1], r[1]) - c;
+    const double sr = fmx(l[1], r[1]) + c;
+
+    double fl[5], fr[5];
+    fl[0] = l[0] * l[1];
+    fl[1] = (l[0] * (l[1] * l[1]) + prl) - bxl * bxl;
+    fl[2] = (l[0] * l[2]) * l[1] - bxl * byl;
+    fl[3] = (l[0] * l[3]) * l[1] - bxl * bzl;
+    fl[4] = (enl + prl) * l[1] - bxl * ((l[1] * bxl + l[2] * byl) + l[3] * bzl);
+    const double fbyl = byl * l[1] - bxl * l[2];
+    const double fbzl = bzl * l[1] - bxl * l[3];
+    fr[0] = r[0] * r[1];
+    fr[1] = (r[0] * (r[1] * r[1]) + prr) - bxr * bxr;
+    fr[2] = (r[0] * r[2]) * r[1] - bxr * byr;
+    fr[3] = (r[0] * r[3]) * r[1] - bxr * bzr;
+    fr[4] = (enr + prr) * r[1] - bxr * ((r[1] * bxr + r[2] * byr) + r[3] * bzr);
+    const double fbyr = byr * r[1] - bxr * r[2];
+    const double fbzr = bzr * r[1] - bxr * r[3];
+
+    if (sl >= 0.0) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fl[v];
+        f[6] = fbyl;
+        f[7] = fbzl;
+        return;
+    }
+    if (sr <= 0.0) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fr[v];
+        f[6] = fbyr;
+        f[7] = fbzr;
+        return;
+    }
+
+    const double auxr = (sr - r[1]) * r[0], auxl = (sl - l[1]) * l[0];
+    const double sm = (fabs(auxr - auxl) < spacing(auxr + auxl))
+                          ? (sl + sr) / 2.0
+                          : ((auxr * r[1] - prr) - (auxl * l[1] - prl)) / (auxr - auxl);
+    const double slsm = sl - sm, srsm = sr - sm;
+    const double slvxl = sl - l[1], srvxr = sr - r[1];
+    const double srmsl = sr - sl;
+    const double dn_l = l[0] * slvxl, dn_r = r[0] * srvxr;
+    const double b_lr = bxl * bxr;
+    const double b_lrgam = b_lr / gam;
+    const double prt_star = 0.5 * ((prl + dn_l * (sm - l[1])) + (prr + dn_r * (sm - r[1])));
+
+    // Transverse B of the star states with the HLL fallback (hlld.F90:362-378)
+    double bsly, bslz, bsry, bsrz;
+    {
+        const double c1 = dn_l * slsm - b_lr;
+        const bool okl = (fabs(c1) > spacing(b_lr)) && b_lrgam <= l[4];
+        const double c2 = dn_r * srsm - b_lr;
+        const bool okr = (fabs(c2) > spacing(b_lr)) && b_lrgam <= r[4];
+        // the HLL average, used where the star-state formula degenerates
+        double hy = 0.0, hz = 0.0;
+        if (!(okl && okr)) {
+            hy = ((sr * byr - sl * byl) - (fbyr - fbyl)) / srmsl;
+            hz = ((sr * bzr - sl * bzl) - (fbzr - fbzl)) / srmsl;
+        }
+        if (okl) {
+            const double k = dn_l * slvxl - b_lr;
+            bsly = (byl * k) / c1;
+            bslz = (bzl * k) / c1;
+        } else {
+            bsly = hy;
+            bslz = hz;
+        }
+        if (okr) {
+            const double k = dn_r * srvxr - b_lr;
+            bsry = (byr * k) / c2;
+            bsrz = (bzr * k) / c2;
+        } else {
+            bsry = hy;
+            bsrz = hz;
+        }
+    }
+    const double vsly = l[2] + (bxl / dn_l) * (byl - bsly);
+    const double vslz = l[3] + (bxl / dn_l) * (bzl - bslz);
+    const double vsry = r[2] + (bxr / dn_r) * (byr - bsry);
+    const double vsrz = r[3] + (bxr / dn_r) * (bzr - bsrz);
+
+    const double vb_l = (l[1] * bxl + l[2] * byl) + l[3] * bzl;
+    const double vb_r = (r[1] * bxr + r[2] * byr) + r[3] * bzr;
+    const double vb_sl = (sm * bxl + vsly * bsly) + vslz * bslz;
+    const double vb_sr = (sm * bxr + vsry * bsry) + vsrz * bsrz;
+
+    const double dsl = dn_l / slsm, dsr = dn_r / srsm;
+    const double usl[5] = {dsl, dsl * sm, dsl * vsly, dsl * vslz,
+                           ((((slvxl * enl) - (prl * l[1])) + (prt_star * sm)) + bxl * (vb_l - vb_sl)) / slsm};
+    const double usr[5] = {dsr, dsr * sm, dsr * vsry, dsr * vsrz,
+                           ((((srvxr * enr) - (prr * r[1])) + (prt_star * sm)) + bxr * (vb_r - vb_sr)) / srsm};
+    const double Ul[5] = {l[0], l[0] * l[1], l[0] * l[2], l[0] * l[3], enl};
+    const double Ur[5] = {r[0], r[0] * r[1], r[0] * r[2], r[0] * r[3], enr};
+
+    // Single-star fluxes, Eq. 64: F_L + S_L (U*_L - U_L)
+    bool left_star, right_star;
+    if (fabs(bxl) > 0.0) {
+        const double dn_lsqt = sqrt(dsl), dn_rsqt = sqrt(dsr);
+        const double alfven_l = sm - fabs(bxl) / dn_lsqt;
+        const double alfven_r = sm + fabs(bxr) / dn_rsqt;
+        left_star = alfven_l > 0.0;
+        right_star = !left_star && (alfven_r < 0.0);
+        if (!left_star && !right_star) {
+            // Alfven (double-star) states, hlld.F90:450-544
+            const double b_sig = (bxl >= 0.0) ? 1.0 : -1.0;
+            const double add = dn_lsqt + dn_rsqt;
+            const double mul = dn_lsqt * dn_rsqt;
+            const double v2y = ((dn_lsqt * vsly + dn_rsqt * vsry) + b_sig * (bsry - bsly)) / add;
+            const double v2z = ((dn_lsqt * vslz + dn_rsqt * vsrz) + b_sig * (bsrz - bslz)) / add;
+            const double b2x = 0.5 * (bxl + bxr);
+            const double b2y = ((dn_lsqt * bsry + dn_rsqt * bsly) + (b_sig * mul) * (vsry - vsly)) / add;
+            const double b2z = ((dn_lsqt * bsrz + dn_rsqt * bslz) + (b_sig * mul) * (vsrz - vslz)) / add;
+            const double vb2 = (sm * b2x + v2y * b2y) + v2z * b2z;
+            double fal[7], far_[7];
+            if (sm >= 0.0) {
+                const double u2[5] = {dsl, dsl * sm, dsl * v2y, dsl * v2z,
+                                      usl[4] - (b_sig * dn_lsqt) * (vb_sl - vb2)};
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                    fal[v] = ((fl[v] + alfven_l * u2[v]) - (alfven_l - sl) * usl[v]) - sl * Ul[v];
+                fal[5] = ((fbyl + alfven_l * b2y) - (alfven_l - sl) * bsly) - sl * byl;
+                fal[6] = ((fbzl + alfven_l * b2z) - (alfven_l - sl) * bslz) - sl * bzl;
+            }
+            if (sm <= 0.0) {
+                const double u2[5] = {dsr, dsr * sm, dsr * v2y, dsr * v2z,
+                                      usr[4] + (b_sig * dn_rsqt) * (vb_sr - vb2)};
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                    far_[v] = ((fr[v] + alfven_r * u2[v]) - (alfven_r - sr) * usr[v]) - sr * Ur[v];
+                far_[5] = ((fbyr + alfven_r * b2y) - (alfven_r - sr) * bsry) - sr * byr;
+                far_[6] = ((fbzr + alfven_r * b2z) - (alfven_r - sr) * bsrz) - sr * bzr;
+            }
+            if (sm > 0.0) {
+#pragma unroll
+                for (int v = 0; v < 5; v++) f[v] = fal[v];
+                f[6] = fal[5];
+                f[7] = fal[6];
+            } else if (sm < 0.0) {
+#pragma unroll
+                for (int v = 0; v < 5; v++) f[v] = far_[v];
+                f[6] = far_[5];
+                f[7] = far_[6];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; v++) f[v] = 0.5 * (fal[v] + far_[v]);
+                f[6] = 0.5 * (fal[5] + far_[5]);
+                f[7] = 0.5 * (fal[6] + far_[6]);
+            }
+            return;
+        }
+    } else {
+        // B_n == 0: the contact speed picks the side (hlld.F90:552-597)
+        left_star = sm > 0.0;
+        right_star = sm < 0.0;
+        if (!left_star && !right_star) {
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                f[v] = 0.5 * (((fl[v] + sl * (usl[v] - Ul[v])) + fr[v]) + sr * (usr[v] - Ur[v]));
+            f[6] = 0.5 * (((fbyl + sl * (bsly - byl)) + fbyr) + sr * (bsry - byr));
+            f[7] = 0.5 * (((fbzl + sl * (bslz - bzl)) + fbzr) + sr * (bsrz - bzr));
+            return;
+        }
+    }
+    if (left_star) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fl[v] + sl * (usl[v] - Ul[v]);
+        f[6] = fbyl + sl * (bsly - byl);
+        f[7] = fbzl + sl * (bslz - bzl);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = fr[v] + sr * (usr[v] - Ur[v]);
+        f[6] = fbyr + sr * (bsry - byr);
+        f[7] = fbzr + sr * (bsrz - bzr);
+    }
+}
+
+// Floors after the update (care_for_positives): density floor, then
+// internal-energy floor with the unconditional energy re-sum.  The B used for
+// the magnetic energy is the updated one, in sweep order.  Bit 0 of *flags:
+// a negative density was seen; bit 1: a negative internal energy was seen.
+// *dmass receives the mass the density floor added.
+template <class P>
+PK_HD void floors(double (&u)[P::NV], double smalld, double smallei,
+                                       bool use_smalld, bool use_smallei, unsigned &flags,
+                                       double &dmass)
+{
+    if (u[0] < 0.0) flags |= 1u;
+    if (use_smalld) {
+        const double nd = fmx(u[0], smalld);
+        dmass = nd - u[0];
+        u[0] = nd;
+    }
+    const double kin = (0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3])) / u[0];
+    double ei, mag = 0.0;
+    if constexpr (P::MAG) {
+        mag = 0.5 * ((u[5] * u[5] + u[6] * u[6]) + u[7] * u[7]);
+        ei = (u[4] - kin) - mag;
+    } else {
+        ei = u[4] - kin;
+    }
+    if (ei < 0.0) flags |= 2u;
+    if (use_smallei) ei = fmx(ei, smallei);
+    double e = ei + kin;
+    if constexpr (P::MAG) e = e + mag;
+    u[4] = e;
+}
+
+}  // namespace pk

@@ -1,0 +1,144 @@
+"""Synthetic initial states for the benchmark and test problems.
+
+Each returns a GridContainer (reference array layout, periodic ghosts
+filled) plus the matching Config.  Sizes and parameters are those of the
+workloads the sweep is measured on:
+
+  sedov_hd        3D Sedov blast, adiabatic hydro, HLLC + van Leer, CFL 0.7
+                  (rho=1, v=0, p=1e-5, E=1 spread over cells within 3 cells of
+                  the centre; 64^3, 4 ghosts, periodic)
+  mhd_turbulence  3D ideal MHD, HLLD + GLM, minmod, CFL 0.5: rho=1, p=0.6,
+                  gamma=5/3, Gaussian random v and A with P(k) ~ k^-2 on
+                  1<=k<=8, rms v=1, B = curl A scaled to plasma beta 1
+                  (256^3, 4 ghosts, periodic)
+  smooth_random   smooth seeded random state (rho, p = 1 +- 0.2, |v| <= 0.5,
+                  optional B) for parity tests on small blocks
+"""
+import numpy as np
+
+from . import Config, GridContainer
+
+
+def fill_ghosts_periodic(cg):
+    """Host-side periodic ghost refresh of every array (all directions, corners)."""
+    for arr in (cg.u, cg.b, cg.psi):
+        if arr is None:
+            continue
+        off = 1 if arr.ndim == 4 else 0
+        for d in range(3):
+            g, n = cg.g[d], cg.n[d]
+            if g == 0:
+                continue
+            ax = d + off
+            lo = [slice(None)] * arr.ndim
+            src = [slice(None)] * arr.ndim
+            lo[ax] = slice(0, g)
+            src[ax] = slice(n, n + g)
+            arr[tuple(lo)] = arr[tuple(src)]
+            hi = [slice(None)] * arr.ndim
+            src2 = [slice(None)] * arr.ndim
+            hi[ax] = slice(g + n, 2 * g + n)
+            src2[ax] = slice(g, 2 * g)
+            arr[tuple(hi)] = arr[tuple(src2)]
+
+
+def _set_state(cg, rho, vx, vy, vz, p, gamma, bx=None, by=None, bz=None):
+    ui = cg.interior(cg.u)
+    ui[0] = rho
+    ui[1] = rho * vx
+    ui[2] = rho * vy
+    ui[3] = rho * vz
+    e = p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz)
+    if bx is not None:
+        bi = cg.interior(cg.b)
+        bi[0], bi[1], bi[2] = bx, by, bz
+        e = e + 0.5 * (bx * bx + by * by + bz * bz)
+        cg.interior(cg.psi)[...] = 0.0
+    ui[4] = e
+    fill_ghosts_periodic(cg)
+
+
+def sedov_hd(n=64, nb=4, gamma=5.0 / 3.0, radius_cells=3.0, e_blast=1.0, p0=1e-5):
+    cg = GridContainer((n, n, n), nb=nb, dl=(1.0 / n,) * 3, magnetized=False)
+    x = (np.arange(n) + 0.5) / n - 0.5
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    inside = r <= radius_cells / n
+    dvol = (1.0 / n) ** 3
+    rho = np.ones((n, n, n))
+    zero = np.zeros((n, n, n))
+    p = np.full((n, n, n), p0)
+    _set_state(cg, rho, zero, zero, zero, p, gamma)
+    cg.interior(cg.u)[4][inside] += e_blast / (inside.sum() * dvol)
+    fill_ghosts_periodic(cg)
+    cfg = Config(gamma=gamma, magnetized=False, limiter="vanleer", cfl=0.7, smalld=1e-10, smallei=1e-10)
+    return cg, cfg
+
+
+def gaussian_field(n, seed, kmin=1.0, kmax=8.0, slope=-2.0, ncomp=3):
+    """Seeded periodic Gaussian random field(s) with shell spectrum E(k) ~ k^slope
+    on kmin <= |k| <= kmax; returns (ncomp, n, n, n) with zero mean."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kz1 = np.fft.rfftfreq(n, d=1.0 / n)
+    KX, KY, KZ = np.meshgrid(k1, k1, kz1, indexing="ij")
+    k = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    band = (k >= kmin) & (k <= kmax)
+    amp = np.zeros_like(k)
+    amp[band] = np.sqrt(k[band] ** slope / (4.0 * np.pi * k[band] ** 2))
+    out = np.empty((ncomp, n, n, n))
+    for c in range(ncomp):
+        ph = rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape)
+        out[c] = np.fft.irfftn(amp * ph, s=(n, n, n), axes=(0, 1, 2))
+    return out, (KX, KY, KZ, band, amp)
+
+
+def mhd_turbulence(n=256, nb=4, seed=1, gamma=5.0 / 3.0, p0=0.6, rho0=1.0, vrms=1.0, beta=1.0):
+    cg = GridContainer((n, n, n), nb=nb, dl=(1.0 / n,) * 3, magnetized=True)
+    v, _ = gaussian_field(n, seed)
+    v *= vrms / np.sqrt(np.mean(np.sum(v * v, axis=0)))
+    # B = curl A, spectrally, so that div B = 0 for the continuous field
+    rng_seed = seed + 1000
+    a, (KX, KY, KZ, band, amp) = gaussian_field(n, rng_seed)
+    ah = [np.fft.rfftn(a[c]) for c in range(3)]
+    kk = (2.0 * np.pi) * np.array([KX, KY, KZ])  # box length 1
+    bh = [1j * (kk[1] * ah[2] - kk[2] * ah[1]),
+          1j * (kk[2] * ah[0] - kk[0] * ah[2]),
+          1j * (kk[0] * ah[1] - kk[1] * ah[0])]
+    b = np.array([np.fft.irfftn(bh[c], s=(n, n, n), axes=(0, 1, 2)) for c in range(3)])
+    b2 = np.mean(np.sum(b * b, axis=0))
+    b *= np.sqrt(2.0 * p0 / beta / b2)
+    rho = np.full((n, n, n), rho0)
+    p = np.full((n, n, n), p0)
+    _set_state(cg, rho, v[0], v[1], v[2], p, gamma, b[0], b[1], b[2])
+    cfg = Config(gamma=gamma, magnetized=True, limiter="minmod", limiter_b="minmod", cfl=0.5, cfl_glm=0.5,
+                 glm_alpha=0.1, smalld=1e-4, smallei=1e-6)
+    return cg, cfg
+
+
+def smooth_random(n, nb=4, seed=0, magnetized=False, gamma=5.0 / 3.0, has_dir=(True, True, True), amp_b=0.7):
+    """Smooth random state: rho, p in 1 +- 0.2, |v| <= 0.5 (and B of order amp_b)."""
+    n = tuple(n)
+    cg = GridContainer(n, nb=nb, dl=tuple(1.0 / max(1, n[d]) for d in range(3)), magnetized=magnetized,
+                       has_dir=has_dir)
+    rng = np.random.default_rng(seed)
+    coords = np.meshgrid(*[(np.arange(n[d]) + 0.5) / n[d] for d in range(3)], indexing="ij")
+
+    def field(lo, hi):
+        f = np.zeros(n)
+        for _ in range(3):
+            kv = rng.integers(1, 3, size=3) * np.array([1 if h else 0 for h in has_dir])
+            ph = rng.uniform(0, 2 * np.pi)
+            f += np.sin(2 * np.pi * (kv[0] * coords[0] + kv[1] * coords[1] + kv[2] * coords[2]) + ph)
+        f = (f - f.min()) / max(f.max() - f.min(), 1e-30)
+        return lo + (hi - lo) * f
+
+    rho = field(0.8, 1.2)
+    p = field(0.8, 1.2)
+    vel = [field(-0.5, 0.5) / np.sqrt(3.0) for _ in range(3)]
+    if magnetized:
+        bb = [field(-amp_b, amp_b) for _ in range(3)]
+        _set_state(cg, rho, *vel, p, gamma, *bb)
+    else:
+        _set_state(cg, rho, *vel, p, gamma)
+    return cg
