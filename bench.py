@@ -1,0 +1,195 @@
+"""Benchmark of the split Riemann sweep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mhd256|sedov64]
+
+A step is one PIERNIK step (time_step + fluid_update_full: two dt-substeps,
+X->Y->Z then Z->Y->X, both RK2 stages per sweep) over the whole block, with
+the state resident in HBM.  The metric is FP64 cell-updates/s, where one
+cell-update advances one cell by one dt (two per step).
+
+Default workload: 3D ideal MHD, HLLD + GLM, minmod, CFL 0.5, single periodic
+256^3 block with 4 ghost layers, rho=1, p=0.6, gamma=5/3, seeded Gaussian
+random v and B = curl A (beta = 1).
+
+Prints one JSON line with the measured throughput, the roofline of the
+dominant kernel (the fused sweep; HIP events on the library's stream) and a
+CPU baseline timed on this host's cores (the oracle, a C restatement of the
+reference, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "FP64 MHD cell-updates/s at 256^3"
+
+
+def _cpu_worker(args):
+    """One CPU worker: the oracle on its own block of the workload for ~budget s."""
+    workload, n, seed, budget = args
+    from oracle import pyoracle as po
+    from piernik_amd import setups
+    if workload == "mhd256":
+        cg, cfg = setups.mhd_turbulence(n=n, seed=seed)
+    else:
+        cg, cfg = setups.sedov_hd(n=n)
+    p = po.params_from_config(cfg)
+    ob = po.OBlock.from_cg(cg)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        po.run(p, ob, 1)
+        steps += 1
+        if time.perf_counter() - t0 >= budget:
+            break
+    el = time.perf_counter() - t0
+    return steps * n ** 3 * 2, el
+
+
+def cpu_baseline(workload, budget=10.0):
+    """The oracle on all host cores: one process per core, each advancing its
+    own 64^3 block of the same problem, for about `budget` seconds."""
+    ncpu = os.cpu_count() or 1
+    cores = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(workload, 64, 100 + i, budget) for i in range(cores)])
+    wall = time.perf_counter() - t0
+    updates = sum(r[0] for r in res)
+    slowest = max(r[1] for r in res)
+    return {
+        "value": updates / slowest,
+        "unit": "cell-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle (C restatement of the reference sweep, one thread per process) on {cores} "
+                   f"independent periodic 64^3 blocks of the same {'MHD HLLD+GLM' if workload == 'mhd256' else 'HD HLLC'} "
+                   f"problem, whole PIERNIK steps for ~{budget:.0f} s each; {updates} cell-updates in {slowest:.2f} s "
+                   f"(wall {wall:.2f} s)"),
+    }
+
+
+def load_traffic(name):
+    """HBM bytes per sweep launch from committed rocprofv3 --pmc runs, if any."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["mhd256", "sedov64"], default="mhd256")
+    ap.add_argument("--n", type=int, default=None, help="override the block size (cells per side)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--tile-x", type=int, nargs=2, default=(0, 0))
+    ap.add_argument("--tile-yz", type=int, nargs=2, default=(0, 0))
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    # CPU baseline first, before this process touches the GPU
+    cpu = None if args.no_cpu else cpu_baseline(args.workload, args.cpu_budget)
+    print(f"[bench] cpu baseline done: {cpu and round(cpu['value'])} cell-updates/s", file=sys.stderr, flush=True)
+
+    import piernik_amd as pk
+    from piernik_amd import setups
+
+    if args.workload == "mhd256":
+        n = args.n or 256
+        cg, cfg = setups.mhd_turbulence(n=n, seed=1)
+        nv = 9
+    else:
+        n = args.n or 64
+        cg, cfg = setups.sedov_hd(n=n)
+        nv = 5
+    cfg.tile_x = tuple(args.tile_x)
+    cfg.tile_yz = tuple(args.tile_yz)
+    print(f"[bench] initial state {n}^3 ready", file=sys.stderr, flush=True)
+
+    pk.init(cfg)
+    cg.upload()
+    pk.run(cg, args.warmup)
+    pk.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    pk.profile_begin()
+    pk.synchronize()
+    t0 = time.perf_counter()
+    t_sim, dt_last = pk.run(cg, args.steps)
+    pk.synchronize()
+    t1 = time.perf_counter()
+    prof = pk.profile_end()
+    el = t1 - t0
+    cells = n ** 3
+    value = cells * 2 * args.steps / el
+
+    launches = sum(prof["launches"])
+    ms = sum(prof["ms"])
+    algo = sum(prof["algo_bytes"])
+    achieved = (algo / launches) / (ms / launches * 1e-3) / 1e9 if launches else 0.0
+    per_dir = [(prof["algo_bytes"][d] / (prof["ms"][d] * 1e-3) / 1e9) if prof["ms"][d] > 0 else None
+               for d in range(3)]
+
+    rep = pk.get_report(cg)
+    line = {
+        "metric": METRIC if args.workload == "mhd256" else "FP64 HD cell-updates/s at 64^3",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3D ideal MHD, HLLD+GLM, minmod, RK2, CFL 0.5, single periodic {n}^3 block, nb=4, "
+                         "Gaussian random v and B=curl A, beta=1" if args.workload == "mhd256" else
+                         f"3D Sedov blast, HD, HLLC, van Leer, RK2, CFL 0.7, single periodic {n}^3 block, nb=4"),
+            "cells": cells,
+            "nvar": nv,
+            "step": "time_step + fluid_update_full (2 dt-substeps x 3 fused RK2 sweeps)",
+            "sim_time": t_sim,
+            "last_dt": dt_last,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": load_traffic(f"sweep_{args.workload}_traffic.json") if n in (256, 64) else None,
+            "kernel": f"k_sweep (fused RK2, {'HLLD+GLM' if nv == 9 else 'HLLC'}), x/y/z",
+            "algo_bytes_per_launch": algo / launches if launches else None,
+            "avg_launch_ms": ms / launches if launches else None,
+            "per_dir_gbps": per_dir,
+        },
+        "cpu_baseline": cpu,
+        "floors": rep,
+        "adjoint_forward_cost_ratio": None,
+    }
+    print(json.dumps(line))
+    pk.finalize()
+
+
+if __name__ == "__main__":
+    main()

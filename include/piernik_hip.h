@@ -149,6 +149,21 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset);
 /* Wait for all queued work. */
 int pk_synchronize(void);
 
+/* ---- measurement (no counterpart in the reference) ------------------------ */
+
+/* Per-direction totals of the sweep kernels launched between
+ * pk_profile_begin() and pk_profile_end(), timed with HIP events recorded on
+ * the library's stream around each launch.  algo_bytes counts what the
+ * sweep must move: every interior cell read once and written once. */
+typedef struct pk_profile_stats {
+    int64_t launches[3];
+    double ms[3];
+    double algo_bytes[3];
+} pk_profile_stats;
+
+int pk_profile_begin(void);
+int pk_profile_end(pk_profile_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,7 @@ __all__ = [
     "PiernikError", "Config", "GridContainer", "load_library", "init", "finalize",
     "solve_cg", "sweep", "update_boundaries", "glmdamping", "timestep_fluid", "time_step",
     "update_chspeed", "glm_factor", "fluid_update", "run", "get_report", "synchronize",
+    "profile_begin", "profile_end",
     "EULER", "RK2_1", "RK2_2", "LIMITERS",
 ]
 
@@ -57,6 +58,10 @@ class _Config(ctypes.Structure):
     ]
 
 
+class _ProfStats(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_int64 * 3), ("ms", ctypes.c_double * 3), ("algo_bytes", ctypes.c_double * 3)]
+
+
 class _Report(ctypes.Structure):
     _fields_ = [("dn_negative", ctypes.c_int32), ("ei_negative", ctypes.c_int32), ("magic_mass", ctypes.c_double)]
 
@@ -82,6 +87,8 @@ _PROTOS = {
     "pk_run": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_get_report": [ctypes.c_int64, ctypes.POINTER(_Report), ctypes.c_int32],
     "pk_synchronize": [],
+    "pk_profile_begin": [],
+    "pk_profile_end": [ctypes.POINTER(_ProfStats)],
 }
 
 
@@ -320,3 +327,15 @@ def get_report(cg, reset=False):
 
 def synchronize():
     _call("pk_synchronize")
+
+
+def profile_begin():
+    """Start timing every sweep launch with HIP events on the library's stream."""
+    _call("pk_profile_begin")
+
+
+def profile_end():
+    """Stop timing; per direction: launches, total ms, algorithmic bytes."""
+    st = _ProfStats()
+    _call("pk_profile_end", ctypes.byref(st))
+    return {"launches": list(st.launches), "ms": list(st.ms), "algo_bytes": list(st.algo_bytes)}

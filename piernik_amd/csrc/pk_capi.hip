@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <vector>
 #include <string>
 
 #include "../../include/piernik_hip.h"
@@ -30,6 +31,7 @@ struct Block {
     long long sv = 0;
     int sy = 0, sz = 0;
     double *buf[2] = {nullptr, nullptr};
+    int xo = 0;  // leading offset that puts every interior row start on a 128 B line
     int cur = 0;
     int uh = -1;  // buffer holding uh after a first stage, -1 when none
     double *h_u = nullptr, *h_uh = nullptr, *h_b = nullptr, *h_bh = nullptr, *h_psi = nullptr,
@@ -49,6 +51,16 @@ hipStream_t g_stream = nullptr;
 std::string g_err;
 double *g_pinned = nullptr;  // small pinned buffer for scalar read-backs
 
+// Event pairs recorded around sweep launches while profiling is on.
+struct ProfRec {
+    hipEvent_t start, stop;
+    int dir;
+    double bytes;
+};
+bool g_prof = false;
+std::vector<ProfRec> g_prof_recs;
+std::vector<hipEvent_t> g_event_pool;
+
 int fail(int code, const char *fmt, ...)
 {
     char buf[512];
@@ -65,6 +77,9 @@ int fail(int code, const char *fmt, ...)
         hipError_t e_ = (x);                                                                            \
         if (e_ != hipSuccess) return fail(PK_EHIP, "%s failed: %s", #x, hipGetErrorString(e_));       \
     } while (0)
+
+// Address of cell (0,0,0) of plane 0 of state buffer i.
+double *state(Block *b, int i) { return b->buf[i] + b->xo; }
 
 int get_block(int64_t id, Block **b)
 {
@@ -114,21 +129,16 @@ void fill_common(const Block *b, pk::SweepArgs &a)
 // bound (512 lanes MHD, 1024 HD); halo = 4 (fused) or 2 (one stage).
 void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, int &chunks, int &tiles1)
 {
-    const int maxt = g_mhd ? 512 : 1024;
+    const int maxt = 512;
     const int halo = fused ? 4 : 2;
     const int span = fused ? b->n[d] : b->ng[d];  // cells that receive output
     const int d1 = (d == 0) ? 1 : 0;
     const int *ov = (d == 0) ? g_cfg.tile_x : g_cfg.tile_yz;
     int tp = ov[0], ts = ov[1];
-    if (d == 0) {
-        if (ts <= 0) ts = span + 2 * halo;
-        if (ts > maxt) ts = maxt;
-        if (tp <= 0) tp = maxt / ts;
-    } else {
-        if (tp <= 0) tp = b->n[d1] < 16 ? b->n[d1] : 16;
-        if (ts <= 0) ts = maxt / tp;
-        if (ts > span + 2 * halo) ts = span + 2 * halo;
-    }
+    if (ts <= 0) ts = 72;  // 64 outputs per tile in fused mode (measured best at 256^3)
+    if (ts > span + 2 * halo) ts = span + 2 * halo;
+    if (ts > maxt) ts = maxt;
+    if (tp <= 0) tp = maxt / ts;
     if (tp < 1) tp = 1;
     if (tp > b->n[d1]) tp = b->n[d1];
     while (tp * ts > maxt && tp > 1) tp--;
@@ -141,11 +151,35 @@ void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, 
     tiles1 = (b->n[d1] + tp - 1) / tp;
 }
 
+int take_event(hipEvent_t *e)
+{
+    if (!g_event_pool.empty()) {
+        *e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return PK_OK;
+    }
+    HIPCHK(hipEventCreate(e));
+    return PK_OK;
+}
+
 int launch(int d, bool fused, const pk::SweepArgs &a0, Block *b)
 {
     pk::SweepArgs a = a0;
     choose_tile(b, d, fused, a.TP, a.TS, a.out, a.chunks, a.tiles1);
+    ProfRec rec{};
+    if (g_prof) {
+        int rc;
+        if ((rc = take_event(&rec.start)) || (rc = take_event(&rec.stop))) return rc;
+        rec.dir = d;
+        const double cells = (double)b->n[0] * b->n[1] * b->n[2];
+        rec.bytes = cells * b->nplanes * 8.0 * 2.0;
+        HIPCHK(hipEventRecord(rec.start, g_stream));
+    }
     HIPCHK(pk::launch_sweep(d, g_mhd, fused, a, g_stream));
+    if (g_prof) {
+        HIPCHK(hipEventRecord(rec.stop, g_stream));
+        g_prof_recs.push_back(rec);
+    }
     return PK_OK;
 }
 
@@ -194,8 +228,8 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
     a.c1 = (rk_coef(istep) * dt) / b->dl[ddim];
     a.c2 = a.c1;
     const bool first = istep == first_stage();
-    double *src = b->buf[b->cur];
-    double *other = b->buf[1 - b->cur];
+    double *src = state(b, b->cur);
+    double *other = state(b, 1 - b->cur);
     if (first) {
         a.src = src;
         a.base = src;
@@ -208,7 +242,7 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
     } else {
         if (b->uh < 0 || b->uh == b->cur)
             return fail(PK_ESTATE, "stage %d called without a preceding first stage", istep);
-        double *uh = b->buf[b->uh];
+        double *uh = state(b, b->uh);
         a.src = src;
         a.base = uh;
         a.dst = uh;
@@ -227,7 +261,7 @@ int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac)
         int rc = do_stage(b, ddim, PK_EULER, dt, ch);
         if (rc) return rc;
         if (psi_fac != 1.0 && g_mhd) {
-            HIPCHK(pk::launch_scale(b->buf[b->cur] + 8 * b->sv, b->sv, psi_fac, g_stream));
+            HIPCHK(pk::launch_scale(state(b, b->cur) + 8 * b->sv, b->sz * b->ng[2], psi_fac, g_stream));
         }
         return PK_OK;
     }
@@ -238,9 +272,9 @@ int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac)
     a.c1 = (0.5 * dt) / b->dl[ddim];
     a.c2 = (1.0 * dt) / b->dl[ddim];
     a.psi_fac = psi_fac;
-    a.src = b->buf[b->cur];
+    a.src = state(b, b->cur);
     a.base = a.src;
-    a.dst = b->buf[1 - b->cur];
+    a.dst = state(b, 1 - b->cur);
     int rc = launch(ddim, true, a, b);
     if (rc) return rc;
     b->cur = 1 - b->cur;
@@ -251,7 +285,7 @@ int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac)
 int do_fill(Block *b)
 {
     pk::GhostArgs a{};
-    a.u = b->buf[b->cur];
+    a.u = state(b, b->cur);
     a.sv = b->sv;
     a.sy = b->sy;
     a.sz = b->sz;
@@ -272,7 +306,7 @@ int do_timestep(Block *b, double *dt_out, double *c_out)
     double dt = DBL_MAX, c = 0.0;
     for (int f = 0; f < g_cfg.nfluids; f++) {
         pk::DtArgs a{};
-        a.u = b->buf[b->cur];
+        a.u = state(b, b->cur);
         a.sv = b->sv;
         a.sy = b->sy;
         a.sz = b->sz;
@@ -336,10 +370,11 @@ int do_fluid_update(Block *b, double dt)
 
 int alloc_block(Block *b)
 {
-    const int pitch = (b->ng[0] + 7) / 8 * 8;
+    b->xo = (16 - b->g[0] % 16) % 16;
+    const int pitch = (b->xo + b->ng[0] + 15) / 16 * 16;
     b->sy = pitch;
     b->sz = pitch * b->ng[1];
-    long long sv = (long long)b->sz * b->ng[2];
+    long long sv = (long long)b->xo + (long long)b->sz * b->ng[2];
     sv = (sv + 31) / 32 * 32;
     b->sv = sv;
     const size_t bytes = sizeof(double) * (size_t)sv * b->nplanes;
@@ -516,7 +551,7 @@ int pk_upload(int64_t cg_id)
     if (rc) return rc;
     if (!b->h_u) return fail(PK_EINVAL, "block %lld has no host arrays", (long long)cg_id);
     if (g_mhd && (!b->h_b || !b->h_psi)) return fail(PK_EINVAL, "magnetized block needs b and psi host arrays");
-    return upload_state(b, b->h_u, b->h_b, b->h_psi, b->buf[b->cur]);
+    return upload_state(b, b->h_u, b->h_b, b->h_psi, state(b, b->cur));
 }
 
 int pk_download(int64_t cg_id)
@@ -525,8 +560,8 @@ int pk_download(int64_t cg_id)
     int rc = get_block(cg_id, &b);
     if (rc) return rc;
     if (!b->h_u) return fail(PK_EINVAL, "block %lld has no host arrays", (long long)cg_id);
-    if ((rc = download_state(b, b->h_u, b->h_b, b->h_psi, b->buf[b->cur]))) return rc;
-    if (b->uh >= 0 && b->h_uh) return download_state(b, b->h_uh, b->h_bh, b->h_psih, b->buf[b->uh]);
+    if ((rc = download_state(b, b->h_u, b->h_b, b->h_psi, state(b, b->cur)))) return rc;
+    if (b->uh >= 0 && b->h_uh) return download_state(b, b->h_uh, b->h_bh, b->h_psih, state(b, b->uh));
     return PK_OK;
 }
 
@@ -536,7 +571,7 @@ int pk_upload_from(int64_t cg_id, const double *u, const double *bb, const doubl
     int rc = get_block(cg_id, &b);
     if (rc) return rc;
     if (!u || (g_mhd && (!bb || !psi))) return fail(PK_EINVAL, "missing host array");
-    return upload_state(b, u, bb, psi, b->buf[b->cur]);
+    return upload_state(b, u, bb, psi, state(b, b->cur));
 }
 
 int pk_download_to(int64_t cg_id, double *u, double *bb, double *psi)
@@ -544,7 +579,7 @@ int pk_download_to(int64_t cg_id, double *u, double *bb, double *psi)
     Block *b;
     int rc = get_block(cg_id, &b);
     if (rc) return rc;
-    return download_state(b, u, bb, psi, b->buf[b->cur]);
+    return download_state(b, u, bb, psi, state(b, b->cur));
 }
 
 int pk_solve_cg(int64_t cg_id, int32_t ddim, int32_t istep, double dt, double chspeed)
@@ -583,7 +618,7 @@ int pk_glm_damp(int64_t cg_id, double fac)
     int rc = get_block(cg_id, &b);
     if (rc) return rc;
     if (!g_mhd) return PK_OK;
-    HIPCHK(pk::launch_scale(b->buf[b->cur] + 8 * b->sv, b->sv, fac, g_stream));
+    HIPCHK(pk::launch_scale(state(b, b->cur) + 8 * b->sv, (long long)b->sz * b->ng[2], fac, g_stream));
     return PK_OK;
 }
 
@@ -655,6 +690,35 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset)
         HIPCHK(hipMemsetAsync(b->d_dmass, 0, sizeof(double), g_stream));
         HIPCHK(hipStreamSynchronize(g_stream));
     }
+    return PK_OK;
+}
+
+int pk_profile_begin(void)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    for (auto &r : g_prof_recs) {
+        g_event_pool.push_back(r.start);
+        g_event_pool.push_back(r.stop);
+    }
+    g_prof_recs.clear();
+    g_prof = true;
+    return PK_OK;
+}
+
+int pk_profile_end(pk_profile_stats *out)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    HIPCHK(hipStreamSynchronize(g_stream));
+    pk_profile_stats st{};
+    for (auto &r : g_prof_recs) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, r.start, r.stop));
+        st.launches[r.dir] += 1;
+        st.ms[r.dir] += ms;
+        st.algo_bytes[r.dir] += r.bytes;
+    }
+    g_prof = false;
+    if (out) *out = st;
     return PK_OK;
 }
 

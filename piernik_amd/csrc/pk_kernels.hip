@@ -33,7 +33,8 @@ __device__ __forceinline__ int plane_of(int k)
 }
 
 // One RK stage of the tile.  A holds primitives then fluxes, B face states.
-template <class P>
+// LIM >= 0 fixes both limiters at compile time; LIM < 0 reads them from a.
+template <class P, int LIM>
 __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)[P::NV],
                                            const double (&W0)[P::NV], double coef, bool upd,
                                            bool lo_end, bool hi_end, double *__restrict__ A,
@@ -50,7 +51,7 @@ __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)
     double fp[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-        const int lim = (v < 5) ? a.lim : a.limb;
+        const int lim = (LIM >= 0) ? LIM : ((v < 5) ? a.lim : a.limb);
         const double dl = q[v] - A[v * nt + tm];
         const double dr = A[v * nt + tp] - q[v];
         double dq;
@@ -85,8 +86,8 @@ __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)
     }
 }
 
-template <int DIR, class P, bool FUSED>
-__global__ __launch_bounds__(P::MAG ? 512 : 1024) void k_sweep(const SweepArgs a)
+template <int DIR, class P, bool FUSED, int LIM>
+__global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
 {
     constexpr int NV = P::NV;
     constexpr int HALO = FUSED ? 4 : 2;
@@ -115,7 +116,11 @@ __global__ __launch_bounds__(P::MAG ? 512 : 1024) void k_sweep(const SweepArgs a
     tm = tm < 0 ? 0 : tm;
     tp = tp >= nt ? nt - 1 : tp;
 
-    const int bid = blockIdx.x;
+    // XCD-aware order: workgroup i runs on XCD i % 8, so give each XCD a
+    // contiguous range of tiles; tiles that share halo cells then share an L2.
+    int bid = blockIdx.x;
+    const int nfull = (int)(gridDim.x / 8u) * 8;
+    if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
     const int chunk = bid % a.chunks;
     const int rest = bid / a.chunks;
     const int i1t = rest % a.tiles1;
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(P::MAG ? 512 : 1024) void k_sweep(const SweepArgs a
     double dmass = 0.0;
     if constexpr (FUSED) {
         double Ws[NV];
-        tile_stage<P>(a, W, W, a.c1, true, false, false, LA, LB, t, tm, tp, nt, Ws);
+        tile_stage<P, LIM>(a, W, W, a.c1, true, false, false, LA, LB, t, tm, tp, nt, Ws);
         double dm = 0.0;
         unsigned fl = 0u;
         floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
@@ -155,7 +160,7 @@ __global__ __launch_bounds__(P::MAG ? 512 : 1024) void k_sweep(const SweepArgs a
             flags |= fl;
             dmass += dm;
         }
-        tile_stage<P>(a, Ws, W, a.c2, true, false, false, LB, LA, t, tm, tp, nt, Wn);
+        tile_stage<P, LIM>(a, Ws, W, a.c2, true, false, false, LB, LA, t, tm, tp, nt, Wn);
     } else {
         const bool lo = cs == 0, hi = cs == ng - 1;
         if (outl && a.base != a.src) {
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(P::MAG ? 512 : 1024) void k_sweep(const SweepArgs a
 #pragma unroll
             for (int k = 0; k < NV; k++) W0[k] = W[k];
         }
-        tile_stage<P>(a, W, W0, a.c1, !(lo || hi), lo, hi, LA, LB, t, tm, tp, nt, Wn);
+        tile_stage<P, LIM>(a, W, W0, a.c1, !(lo || hi), lo, hi, LA, LB, t, tm, tp, nt, Wn);
     }
     {
         double dm = 0.0;
@@ -364,28 +369,43 @@ __global__ void k_soa_to_aos(const double *__restrict__ soa, long long sv, int p
 
 // ------------------------------------------------------------ launchers ----
 
-template <int DIR, class P, bool FUSED>
+template <int DIR, class P, bool FUSED, int LIM>
 static hipError_t launch_sweep_t(const SweepArgs &a, hipStream_t st)
 {
     const int nt = a.TP * a.TS;
     const size_t lds = sizeof(double) * 2 * P::NV * nt;
     static bool lds_raised = false;  // allow dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU)
     if (!lds_raised) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_sweep<DIR, P, FUSED>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_sweep<DIR, P, FUSED, LIM>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         lds_raised = true;
     }
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > 160 * 1024 || nt > 512) return hipErrorInvalidValue;
     const long long nblk = (long long)a.chunks * a.tiles1 * a.n[(DIR == 2) ? 1 : 2];
-    hipLaunchKernelGGL((k_sweep<DIR, P, FUSED>), dim3((unsigned)nblk), dim3(nt), lds, st, a);
+    hipLaunchKernelGGL((k_sweep<DIR, P, FUSED, LIM>), dim3((unsigned)nblk), dim3(nt), lds, st, a);
     return hipGetLastError();
+}
+
+template <int DIR, class P, bool FUSED>
+static hipError_t launch_lim(const SweepArgs &a, hipStream_t st)
+{
+    if (a.lim == a.limb) {
+        switch (a.lim) {
+        case LIM_VANLEER: return launch_sweep_t<DIR, P, FUSED, LIM_VANLEER>(a, st);
+        case LIM_MINMOD: return launch_sweep_t<DIR, P, FUSED, LIM_MINMOD>(a, st);
+        case LIM_MONCEN: return launch_sweep_t<DIR, P, FUSED, LIM_MONCEN>(a, st);
+        case LIM_SUPERBEE: return launch_sweep_t<DIR, P, FUSED, LIM_SUPERBEE>(a, st);
+        default: break;
+        }
+    }
+    return launch_sweep_t<DIR, P, FUSED, -1>(a, st);
 }
 
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st)
 {
 #define PK_CASE(D, PP, F) \
-    if (dir == D && (mhd == PP::MAG) && fused == F) return launch_sweep_t<D, PP, F>(a, st);
+    if (dir == D && (mhd == PP::MAG) && fused == F) return launch_lim<D, PP, F>(a, st);
     PK_CASE(0, HD, true) PK_CASE(1, HD, true) PK_CASE(2, HD, true)
     PK_CASE(0, HD, false) PK_CASE(1, HD, false) PK_CASE(2, HD, false)
     PK_CASE(0, MHD, true) PK_CASE(1, MHD, true) PK_CASE(2, MHD, true)
