@@ -1,0 +1,214 @@
+!
+! piernik_hip.F90 -- ISO_C_BINDING interface to libpiernik_hip.so
+!
+! The Fortran view of include/piernik_hip.h.  PIERNIK's host code uses this
+! module from an adapter with the solve_cg_sub signature
+! (src/scheme/sweeps.F90:140-151); see the adapter listed in the
+! repository's integration notes.  Every function returns 0 on success; on a
+! nonzero code the adapter passes pk_error_message() to die(), as the
+! reference does for any error.
+!
+module piernik_hip
+
+   use, intrinsic :: iso_c_binding, only: c_int32_t, c_int64_t, c_double, c_ptr, c_char, c_null_char, &
+        &                                  c_f_pointer, c_associated
+
+   implicit none
+
+   private
+   public :: pk_fluid, pk_config, pk_stage_report, pk_profile_stats
+   public :: pk_init, pk_finalize, pk_register_block, pk_unregister_block, pk_upload, pk_download, &
+        &    pk_upload_from, pk_download_to, pk_solve_cg, pk_sweep, pk_fill_ghosts, pk_glm_damp, &
+        &    pk_timestep_fluid, pk_fluid_update, pk_run, pk_get_report, pk_synchronize, &
+        &    pk_profile_begin, pk_profile_end, pk_error_message
+   public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
+
+   ! the reference's stage enumerators (src/base/constants.F90:230-232)
+   integer(c_int32_t), parameter :: PK_EULER = 10000, PK_RK2_1 = 10001, PK_RK2_2 = 10002
+   integer(c_int32_t), parameter :: PK_LIM_VANLEER = 0, PK_LIM_MINMOD = 1, PK_LIM_MONCEN = 2, PK_LIM_SUPERBEE = 3
+
+   type, bind(C) :: pk_fluid
+      real(c_double)     :: gamma
+      integer(c_int32_t) :: magnetized
+      integer(c_int32_t) :: reserved
+   end type pk_fluid
+
+   type, bind(C) :: pk_config
+      integer(c_int32_t) :: nfluids
+      integer(c_int32_t) :: limiter
+      integer(c_int32_t) :: limiter_b
+      integer(c_int32_t) :: integration_order
+      type(pk_fluid)     :: fluid(2)
+      real(c_double)     :: smalld
+      real(c_double)     :: smallei
+      integer(c_int32_t) :: use_smalld
+      integer(c_int32_t) :: use_smallei
+      real(c_double)     :: cfl
+      real(c_double)     :: cfl_glm
+      real(c_double)     :: glm_alpha
+      integer(c_int32_t) :: has_dir(3)
+      integer(c_int32_t) :: tile_x(2)
+      integer(c_int32_t) :: tile_yz(2)
+   end type pk_config
+
+   type, bind(C) :: pk_stage_report
+      integer(c_int32_t) :: dn_negative
+      integer(c_int32_t) :: ei_negative
+      real(c_double)     :: magic_mass
+   end type pk_stage_report
+
+   type, bind(C) :: pk_profile_stats
+      integer(c_int64_t) :: launches(3)
+      real(c_double)     :: ms(3)
+      real(c_double)     :: algo_bytes(3)
+   end type pk_profile_stats
+
+   interface
+
+      integer(c_int32_t) function pk_init(cfg) bind(C, name="pk_init")
+         import :: c_int32_t, pk_config
+         type(pk_config), intent(in) :: cfg
+      end function pk_init
+
+      integer(c_int32_t) function pk_finalize() bind(C, name="pk_finalize")
+         import :: c_int32_t
+      end function pk_finalize
+
+      type(c_ptr) function pk_last_error() bind(C, name="pk_last_error")
+         import :: c_ptr
+      end function pk_last_error
+
+      ! u(nvar, nxg, nyg, nzg), b(3, ...), psi(...): the reference's arrays, passed as c_loc()
+      integer(c_int32_t) function pk_register_block(cg_id, n, nb, dl, u, uh, b, bh, psi, psih) &
+           &                                        bind(C, name="pk_register_block")
+         import :: c_int32_t, c_int64_t, c_double, c_ptr
+         integer(c_int64_t), value      :: cg_id
+         integer(c_int32_t), intent(in) :: n(3)
+         integer(c_int32_t), value      :: nb
+         real(c_double),     intent(in) :: dl(3)
+         type(c_ptr),        value      :: u, uh, b, bh, psi, psih
+      end function pk_register_block
+
+      integer(c_int32_t) function pk_unregister_block(cg_id) bind(C, name="pk_unregister_block")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id
+      end function pk_unregister_block
+
+      integer(c_int32_t) function pk_upload(cg_id) bind(C, name="pk_upload")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id
+      end function pk_upload
+
+      integer(c_int32_t) function pk_download(cg_id) bind(C, name="pk_download")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id
+      end function pk_download
+
+      integer(c_int32_t) function pk_upload_from(cg_id, u, b, psi) bind(C, name="pk_upload_from")
+         import :: c_int32_t, c_int64_t, c_ptr
+         integer(c_int64_t), value :: cg_id
+         type(c_ptr),        value :: u, b, psi
+      end function pk_upload_from
+
+      integer(c_int32_t) function pk_download_to(cg_id, u, b, psi) bind(C, name="pk_download_to")
+         import :: c_int32_t, c_int64_t, c_ptr
+         integer(c_int64_t), value :: cg_id
+         type(c_ptr),        value :: u, b, psi
+      end function pk_download_to
+
+      ! solve_cg_sub(cg, ddim, istep): ddim here is 0-based (xdim - 1 .. zdim - 1)
+      integer(c_int32_t) function pk_solve_cg(cg_id, ddim, istep, dt, chspeed) bind(C, name="pk_solve_cg")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: ddim, istep
+         real(c_double),     value :: dt, chspeed
+      end function pk_solve_cg
+
+      integer(c_int32_t) function pk_sweep(cg_id, ddim, dt, chspeed, psi_fac) bind(C, name="pk_sweep")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: ddim
+         real(c_double),     value :: dt, chspeed, psi_fac
+      end function pk_sweep
+
+      integer(c_int32_t) function pk_fill_ghosts(cg_id) bind(C, name="pk_fill_ghosts")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id
+      end function pk_fill_ghosts
+
+      integer(c_int32_t) function pk_glm_damp(cg_id, fac) bind(C, name="pk_glm_damp")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         real(c_double),     value :: fac
+      end function pk_glm_damp
+
+      integer(c_int32_t) function pk_timestep_fluid(cg_id, dt_out, c_out) bind(C, name="pk_timestep_fluid")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value       :: cg_id
+         real(c_double),     intent(out) :: dt_out, c_out
+      end function pk_timestep_fluid
+
+      integer(c_int32_t) function pk_fluid_update(cg_id, dt) bind(C, name="pk_fluid_update")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         real(c_double),     value :: dt
+      end function pk_fluid_update
+
+      integer(c_int32_t) function pk_run(cg_id, nsteps, t_out, dt_out) bind(C, name="pk_run")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value       :: cg_id
+         integer(c_int32_t), value       :: nsteps
+         real(c_double),     intent(out) :: t_out, dt_out
+      end function pk_run
+
+      integer(c_int32_t) function pk_get_report(cg_id, rep, reset) bind(C, name="pk_get_report")
+         import :: c_int32_t, c_int64_t, pk_stage_report
+         integer(c_int64_t),    value       :: cg_id
+         type(pk_stage_report), intent(out) :: rep
+         integer(c_int32_t),    value       :: reset
+      end function pk_get_report
+
+      integer(c_int32_t) function pk_synchronize() bind(C, name="pk_synchronize")
+         import :: c_int32_t
+      end function pk_synchronize
+
+      integer(c_int32_t) function pk_profile_begin() bind(C, name="pk_profile_begin")
+         import :: c_int32_t
+      end function pk_profile_begin
+
+      integer(c_int32_t) function pk_profile_end(st) bind(C, name="pk_profile_end")
+         import :: c_int32_t, pk_profile_stats
+         type(pk_profile_stats), intent(out) :: st
+      end function pk_profile_end
+
+   end interface
+
+contains
+
+   !> pk_last_error() as a Fortran string, for die()
+   function pk_error_message() result(msg)
+
+      implicit none
+
+      character(len=:), allocatable :: msg
+      character(kind=c_char), dimension(:), pointer :: p
+      type(c_ptr) :: cp
+      integer :: n
+
+      cp = pk_last_error()
+      if (.not. c_associated(cp)) then
+         msg = ""
+         return
+      endif
+      call c_f_pointer(cp, p, [1024])
+      n = 0
+      do while (n < 1024)
+         if (p(n + 1) == c_null_char) exit
+         n = n + 1
+      enddo
+      allocate(character(len=n) :: msg)
+      msg = transfer(p(1:n), msg)
+
+   end function pk_error_message
+
+end module piernik_hip
