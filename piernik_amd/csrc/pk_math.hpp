@@ -25,9 +25,10 @@ namespace pk {
 
 enum Limiter : int { LIM_VANLEER = 0, LIM_MINMOD = 1, LIM_MONCEN = 2, LIM_SUPERBEE = 3 };
 
-// Fortran min/max on non-NaN operands.
-PK_HD double fmn(double a, double b) { return (b < a) ? b : a; }
-PK_HD double fmx(double a, double b) { return (b > a) ? b : a; }
+// Fortran min/max.  The operands here are never NaN and never a +0/-0 pair
+// whose sign could reach a result, so the hardware min/max is exact.
+PK_HD double fmn(double a, double b) { return fmin(a, b); }
+PK_HD double fmx(double a, double b) { return fmax(a, b); }
 
 // Fortran SPACING(x) for real(8): 2**max(exponent(x)-53, -1022); TINY for 0.
 // For a biased exponent field E >= 53 that is the power of two with exponent
@@ -128,22 +129,23 @@ PK_HD void flux_hllc(const double *l, const double *r, double gam, double *f)
     const double slvxl = sl - l[1], srvxr = sr - r[1];
     const double dn_l = l[0] * slvxl, dn_r = r[0] * srvxr;
     const double prt_star = 0.5 * ((prl + dn_l * (sm - l[1])) + (prr + dn_r * (sm - r[1])));
-    if (sm > 0.0) {
-        const double slsm = sl - sm;
-        const double d = dn_l / slsm;
-        const double us[5] = {d, d * sm, d * l[2], d * l[3],
-                              (((slvxl * enl) - (prl * l[1])) + (prt_star * sm)) / slsm};
-        const double U[5] = {l[0], l[0] * l[1], l[0] * l[2], l[0] * l[3], enl};
+    if (sm != 0.0) {
+        // one side only: pick its inputs, then evaluate the reference's
+        // formula once (hlld.F90:771-815)
+        const bool L = sm > 0.0;
+        const double s_ = L ? sl : sr;
+        const double svx = L ? slvxl : srvxr;
+        const double en = L ? enl : enr;
+        const double pr = L ? prl : prr;
+        const double dn = L ? dn_l : dn_r;
+        const double* q = L ? l : r;
+        const double* f0 = L ? fl : fr;
+        const double ssm = s_ - sm;
+        const double d = dn / ssm;
+        const double us[5] = {d, d * sm, d * q[2], d * q[3], (((svx * en) - (pr * q[1])) + (prt_star * sm)) / ssm};
+        const double U[5] = {q[0], q[0] * q[1], q[0] * q[2], q[0] * q[3], en};
 #pragma unroll
-        for (int v = 0; v < 5; v++) f[v] = fl[v] + sl * (us[v] - U[v]);
-    } else if (sm < 0.0) {
-        const double srsm = sr - sm;
-        const double d = dn_r / srsm;
-        const double us[5] = {d, d * sm, d * r[2], d * r[3],
-                              (((srvxr * enr) - (prr * r[1])) + (prt_star * sm)) / srsm};
-        const double U[5] = {r[0], r[0] * r[1], r[0] * r[2], r[0] * r[3], enr};
-#pragma unroll
-        for (int v = 0; v < 5; v++) f[v] = fr[v] + sr * (us[v] - U[v]);
+        for (int v = 0; v < 5; v++) f[v] = f0[v] + s_ * (us[v] - U[v]);
     } else {
         // sm == 0: half*(fl + sl*(us_l - U_l) + fr + sr*(us_r - U_r)), hlld.F90:822
         const double slsm = sl - sm, srsm = sr - sm;
@@ -277,14 +279,17 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
     const double Ul[5] = {l[0], l[0] * l[1], l[0] * l[2], l[0] * l[3], enl};
     const double Ur[5] = {r[0], r[0] * r[1], r[0] * r[2], r[0] * r[3], enr};
 
-    // Single-star fluxes, Eq. 64: F_L + S_L (U*_L - U_L)
-    bool left_star, right_star;
+    // Single-star fluxes, Eq. 64: F_L + S_L (U*_L - U_L).  Where a formula
+    // exists for both sides, the side's inputs are selected first and the
+    // formula is evaluated once, so a wave with lanes on both sides does the
+    // work once (a - x*y is evaluated as a + ((-1)*x)*y, which is exact).
+    bool left;
     if (fabs(bxl) > 0.0) {
         const double dn_lsqt = sqrt(dsl), dn_rsqt = sqrt(dsr);
         const double alfven_l = sm - fabs(bxl) / dn_lsqt;
         const double alfven_r = sm + fabs(bxr) / dn_rsqt;
-        left_star = alfven_l > 0.0;
-        right_star = !left_star && (alfven_r < 0.0);
+        const bool left_star = alfven_l > 0.0;
+        const bool right_star = !left_star && (alfven_r < 0.0);
         if (!left_star && !right_star) {
             // Alfven (double-star) states, hlld.F90:450-544
             const double b_sig = (bxl >= 0.0) ? 1.0 : -1.0;
@@ -296,48 +301,43 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
             const double b2y = ((dn_lsqt * bsry + dn_rsqt * bsly) + (b_sig * mul) * (vsry - vsly)) / add;
             const double b2z = ((dn_lsqt * bsrz + dn_rsqt * bslz) + (b_sig * mul) * (vsrz - vslz)) / add;
             const double vb2 = (sm * b2x + v2y * b2y) + v2z * b2z;
-            double fal[7], far_[7];
-            if (sm >= 0.0) {
-                const double u2[5] = {dsl, dsl * sm, dsl * v2y, dsl * v2z,
-                                      usl[4] - (b_sig * dn_lsqt) * (vb_sl - vb2)};
+            if (sm != 0.0) {
+                const bool L = sm > 0.0;
+                const double* us = L ? usl : usr;
+                const double* U = L ? Ul : Ur;
+                const double* f0 = L ? fl : fr;
+                const double al = L ? alfven_l : alfven_r;
+                const double s_ = L ? sl : sr;
+                const double sq = L ? dn_lsqt : dn_rsqt;
+                const double vbs = L ? vb_sl : vb_sr;
+                const double sg = L ? -1.0 : 1.0;
+                const double fby0 = L ? fbyl : fbyr, fbz0 = L ? fbzl : fbzr;
+                const double bsy = L ? bsly : bsry, bsz = L ? bslz : bsrz;
+                const double by0 = L ? byl : byr, bz0 = L ? bzl : bzr;
+                const double u2[5] = {us[0], us[0] * sm, us[0] * v2y, us[0] * v2z,
+                                      us[4] + (sg * (b_sig * sq)) * (vbs - vb2)};
 #pragma unroll
-                for (int v = 0; v < 5; v++)
-                    fal[v] = ((fl[v] + alfven_l * u2[v]) - (alfven_l - sl) * usl[v]) - sl * Ul[v];
-                fal[5] = ((fbyl + alfven_l * b2y) - (alfven_l - sl) * bsly) - sl * byl;
-                fal[6] = ((fbzl + alfven_l * b2z) - (alfven_l - sl) * bslz) - sl * bzl;
-            }
-            if (sm <= 0.0) {
-                const double u2[5] = {dsr, dsr * sm, dsr * v2y, dsr * v2z,
-                                      usr[4] + (b_sig * dn_rsqt) * (vb_sr - vb2)};
-#pragma unroll
-                for (int v = 0; v < 5; v++)
-                    far_[v] = ((fr[v] + alfven_r * u2[v]) - (alfven_r - sr) * usr[v]) - sr * Ur[v];
-                far_[5] = ((fbyr + alfven_r * b2y) - (alfven_r - sr) * bsry) - sr * byr;
-                far_[6] = ((fbzr + alfven_r * b2z) - (alfven_r - sr) * bsrz) - sr * bzr;
-            }
-            if (sm > 0.0) {
-#pragma unroll
-                for (int v = 0; v < 5; v++) f[v] = fal[v];
-                f[6] = fal[5];
-                f[7] = fal[6];
-            } else if (sm < 0.0) {
-#pragma unroll
-                for (int v = 0; v < 5; v++) f[v] = far_[v];
-                f[6] = far_[5];
-                f[7] = far_[6];
+                for (int v = 0; v < 5; v++) f[v] = ((f0[v] + al * u2[v]) - (al - s_) * us[v]) - s_ * U[v];
+                f[6] = ((fby0 + al * b2y) - (al - s_) * bsy) - s_ * by0;
+                f[7] = ((fbz0 + al * b2z) - (al - s_) * bsz) - s_ * bz0;
             } else {
+                const double u2l[5] = {dsl, dsl * sm, dsl * v2y, dsl * v2z, usl[4] - (b_sig * dn_lsqt) * (vb_sl - vb2)};
+                const double u2r[5] = {dsr, dsr * sm, dsr * v2y, dsr * v2z, usr[4] + (b_sig * dn_rsqt) * (vb_sr - vb2)};
 #pragma unroll
-                for (int v = 0; v < 5; v++) f[v] = 0.5 * (fal[v] + far_[v]);
-                f[6] = 0.5 * (fal[5] + far_[5]);
-                f[7] = 0.5 * (fal[6] + far_[6]);
+                for (int v = 0; v < 5; v++)
+                    f[v] = 0.5 * ((((fl[v] + alfven_l * u2l[v]) - (alfven_l - sl) * usl[v]) - sl * Ul[v]) +
+                                  (((fr[v] + alfven_r * u2r[v]) - (alfven_r - sr) * usr[v]) - sr * Ur[v]));
+                f[6] = 0.5 * ((((fbyl + alfven_l * b2y) - (alfven_l - sl) * bsly) - sl * byl) +
+                              (((fbyr + alfven_r * b2y) - (alfven_r - sr) * bsry) - sr * byr));
+                f[7] = 0.5 * ((((fbzl + alfven_l * b2z) - (alfven_l - sl) * bslz) - sl * bzl) +
+                              (((fbzr + alfven_r * b2z) - (alfven_r - sr) * bsrz) - sr * bzr));
             }
             return;
         }
+        left = left_star;
     } else {
         // B_n == 0: the contact speed picks the side (hlld.F90:552-597)
-        left_star = sm > 0.0;
-        right_star = sm < 0.0;
-        if (!left_star && !right_star) {
+        if (sm == 0.0) {
 #pragma unroll
             for (int v = 0; v < 5; v++)
                 f[v] = 0.5 * (((fl[v] + sl * (usl[v] - Ul[v])) + fr[v]) + sr * (usr[v] - Ur[v]));
@@ -345,17 +345,17 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
             f[7] = 0.5 * (((fbzl + sl * (bslz - bzl)) + fbzr) + sr * (bsrz - bzr));
             return;
         }
+        left = sm > 0.0;
     }
-    if (left_star) {
+    {
+        const double* us = left ? usl : usr;
+        const double* U = left ? Ul : Ur;
+        const double* f0 = left ? fl : fr;
+        const double s_ = left ? sl : sr;
 #pragma unroll
-        for (int v = 0; v < 5; v++) f[v] = fl[v] + sl * (usl[v] - Ul[v]);
-        f[6] = fbyl + sl * (bsly - byl);
-        f[7] = fbzl + sl * (bslz - bzl);
-    } else {
-#pragma unroll
-        for (int v = 0; v < 5; v++) f[v] = fr[v] + sr * (usr[v] - Ur[v]);
-        f[6] = fbyr + sr * (bsry - byr);
-        f[7] = fbzr + sr * (bsrz - bzr);
+        for (int v = 0; v < 5; v++) f[v] = f0[v] + s_ * (us[v] - U[v]);
+        f[6] = (left ? fbyl : fbyr) + s_ * ((left ? bsly : bsry) - (left ? byl : byr));
+        f[7] = (left ? fbzl : fbzr) + s_ * ((left ? bslz : bsrz) - (left ? bzl : bzr));
     }
 }
 
