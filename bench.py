@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--tile-x", type=int, nargs=2, default=(0, 0))
     ap.add_argument("--tile-yz", type=int, nargs=2, default=(0, 0))
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -125,8 +126,11 @@ def main():
     pk.init(cfg)
     cg.upload()
     pk.run(cg, args.warmup)
+    tiles = None
+    if not args.no_autotune and args.tile_x == (0, 0) and args.tile_yz == (0, 0):
+        tiles = pk.autotune(cg)
     pk.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    print(f"[bench] warmup done, tiles {tiles}", file=sys.stderr, flush=True)
 
     pk.profile_begin()
     pk.synchronize()
@@ -170,6 +174,7 @@ def main():
             "sim_time": t_sim,
             "last_dt": dt_last,
             "parallelism": "1 GPU",
+            "tiles_pencils_x_cells": {"xyz"[d]: list(t) for d, t in (tiles or {}).items()},
         },
         "roofline": {
             "bound": "hbm",

@@ -149,6 +149,15 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset);
 /* Wait for all queued work. */
 int pk_synchronize(void);
 
+/* ---- tuning (no counterpart in the reference) ------------------------------ */
+
+/* Time candidate tile shapes of the fused sweep on this block (the state is
+ * read, never advanced) and keep the fastest per direction for every block
+ * of the same shape and physics.  dt and chspeed should be representative.
+ * tiles_out (may be NULL) receives (pencils, cells) for x, y, z; 0 for an
+ * inactive direction.  A shape given in pk_config overrides the tuned one. */
+int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6]);
+
 /* ---- measurement (no counterpart in the reference) ------------------------ */
 
 /* Per-direction totals of the sweep kernels launched between

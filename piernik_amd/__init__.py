@@ -28,7 +28,7 @@ __all__ = [
     "PiernikError", "Config", "GridContainer", "load_library", "init", "finalize",
     "solve_cg", "sweep", "update_boundaries", "glmdamping", "timestep_fluid", "time_step",
     "update_chspeed", "glm_factor", "fluid_update", "run", "get_report", "synchronize",
-    "profile_begin", "profile_end",
+    "profile_begin", "profile_end", "autotune",
     "EULER", "RK2_1", "RK2_2", "LIMITERS",
 ]
 
@@ -87,6 +87,7 @@ _PROTOS = {
     "pk_run": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_get_report": [ctypes.c_int64, ctypes.POINTER(_Report), ctypes.c_int32],
     "pk_synchronize": [],
+    "pk_autotune": [ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int32)],
     "pk_profile_begin": [],
     "pk_profile_end": [ctypes.POINTER(_ProfStats)],
 }
@@ -327,6 +328,18 @@ def get_report(cg, reset=False):
 
 def synchronize():
     _call("pk_synchronize")
+
+
+def autotune(cg, dt=None, chspeed=None):
+    """Pick the fastest tile shape of the fused sweep per direction on this
+    block; returns {(dir): (pencils, cells)}."""
+    if dt is None:
+        dt = timestep_fluid(cg)[0]
+    if chspeed is None:
+        chspeed = update_chspeed(cg, dt)
+    out = (ctypes.c_int32 * 6)()
+    _call("pk_autotune", cg.cg_id, float(dt), float(chspeed), out)
+    return {d: (out[2 * d], out[2 * d + 1]) for d in range(3) if cg.has_dir[d]}
 
 
 def profile_begin():

@@ -51,6 +51,24 @@ hipStream_t g_stream = nullptr;
 std::string g_err;
 double *g_pinned = nullptr;  // small pinned buffer for scalar read-backs
 
+// Tile shapes chosen by pk_autotune, per block shape, direction and physics.
+struct TuneKey {
+    int n0, n1, n2, dir, mhd, lim, limb;
+    bool operator<(const TuneKey &o) const
+    {
+        const int a[7] = {n0, n1, n2, dir, mhd, lim, limb}, b[7] = {o.n0, o.n1, o.n2, o.dir, o.mhd, o.lim, o.limb};
+        for (int i = 0; i < 7; i++)
+            if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+};
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+
+TuneKey tune_key(const Block *b, int d)
+{
+    return TuneKey{b->n[0], b->n[1], b->n[2], d, g_mhd ? 1 : 0, g_cfg.limiter, g_cfg.limiter_b};
+}
+
 // Event pairs recorded around sweep launches while profiling is on.
 struct ProfRec {
     hipEvent_t start, stop;
@@ -127,15 +145,17 @@ void fill_common(const Block *b, pk::SweepArgs &a)
 
 // Tile shape along direction d: TP pencils x TS cells, within the launch
 // bound (512 lanes MHD, 1024 HD); halo = 4 (fused) or 2 (one stage).
-void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, int &chunks, int &tiles1)
+// Tile shape along direction d: TP pencils x TS cells (TP*TS <= 512 lanes),
+// halo 4 (fused) or 2 (one stage).  Precedence: the shape given in pk_config,
+// then the one pk_autotune measured for this block shape, then 3 x 72
+// (4-wave workgroups, two per CU; measured best at 256^3 MHD).
+void tile_geometry(const Block *b, int d, bool fused, int tp, int ts, int &TP, int &TS, int &out, int &chunks,
+                   int &tiles1)
 {
     const int maxt = 512;
     const int halo = fused ? 4 : 2;
     const int span = fused ? b->n[d] : b->ng[d];  // cells that receive output
     const int d1 = (d == 0) ? 1 : 0;
-    const int *ov = (d == 0) ? g_cfg.tile_x : g_cfg.tile_yz;
-    int tp = ov[0], ts = ov[1];
-    if (ts <= 0) ts = 72;  // 64 outputs per tile in fused mode (measured best at 256^3)
     if (ts > span + 2 * halo) ts = span + 2 * halo;
     if (ts > maxt) ts = maxt;
     if (tp <= 0) tp = maxt / ts;
@@ -149,6 +169,24 @@ void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, 
     out = ts - 2 * halo;
     chunks = (span + out - 1) / out;
     tiles1 = (b->n[d1] + tp - 1) / tp;
+}
+
+void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, int &chunks, int &tiles1)
+{
+    const int *ov = (d == 0) ? g_cfg.tile_x : g_cfg.tile_yz;
+    int tp = ov[0], ts = ov[1];
+    if (ts <= 0 && fused) {
+        auto it = g_tuned.find(tune_key(b, d));
+        if (it != g_tuned.end()) {
+            tp = it->second.first;
+            ts = it->second.second;
+        }
+    }
+    if (ts <= 0) {
+        ts = 72;
+        if (tp <= 0) tp = 3;
+    }
+    tile_geometry(b, d, fused, tp, ts, TP, TS, out, chunks, tiles1);
 }
 
 int take_event(hipEvent_t *e)
@@ -690,6 +728,78 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset)
         HIPCHK(hipMemsetAsync(b->d_dmass, 0, sizeof(double), g_stream));
         HIPCHK(hipStreamSynchronize(g_stream));
     }
+    return PK_OK;
+}
+
+int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
+{
+    Block *b;
+    int rc = get_block(cg_id, &b);
+    if (rc) return rc;
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "autotuning covers the fused RK2 sweep only");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int ts_list[] = {24, 40, 48, 56, 64, 72, 80, 96, 104, 136, 200, 264};
+    const int waves_list[] = {1, 2, 3, 4, 6, 8};
+    for (int d = 0; d < 3; d++) {
+        if (tiles_out) tiles_out[2 * d] = tiles_out[2 * d + 1] = 0;
+        if (!g_cfg.has_dir[d]) continue;
+        if (b->g[d] < 4) return fail(PK_EINVAL, "fused RK2 sweep needs nb >= 4");
+        pk::SweepArgs a{};
+        fill_common(b, a);
+        a.ch = chspeed;
+        a.c1 = (0.5 * dt) / b->dl[d];
+        a.c2 = (1.0 * dt) / b->dl[d];
+        a.src = state(b, b->cur);
+        a.base = a.src;
+        a.dst = state(b, 1 - b->cur);
+        std::vector<std::pair<int, int>> seen;
+        float best = 1e30f;
+        std::pair<int, int> best_t(0, 0);
+        for (int ts : ts_list) {
+            for (int w : waves_list) {
+                int TP, TS, out, chunks, tiles1;
+                int tp = (64 * w) / (ts < b->n[d] + 8 ? ts : b->n[d] + 8);
+                if (tp < 1) continue;
+                tile_geometry(b, d, true, tp, ts, TP, TS, out, chunks, tiles1);
+                if (TP * TS > 512) continue;
+                bool dup = false;
+                for (auto &p : seen) dup = dup || (p.first == TP && p.second == TS);
+                if (dup) continue;
+                seen.push_back({TP, TS});
+                a.TP = TP;
+                a.TS = TS;
+                a.out = out;
+                a.chunks = chunks;
+                a.tiles1 = tiles1;
+                HIPCHK(pk::launch_sweep(d, g_mhd, true, a, g_stream));  // warm-up
+                HIPCHK(hipEventRecord(e0, g_stream));
+                for (int rep = 0; rep < 3; rep++) HIPCHK(pk::launch_sweep(d, g_mhd, true, a, g_stream));
+                HIPCHK(hipEventRecord(e1, g_stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_t = {TP, TS};
+                }
+            }
+        }
+        g_tuned[tune_key(b, d)] = best_t;
+        if (tiles_out) {
+            tiles_out[2 * d] = best_t.first;
+            tiles_out[2 * d + 1] = best_t.second;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the candidates wrote the other buffer only; the state is unchanged.
+    // The floor bookkeeping they accumulated is discarded.
+    HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
+    HIPCHK(hipMemsetAsync(b->d_dmass, 0, sizeof(double), g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
 }
 

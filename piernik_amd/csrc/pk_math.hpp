@@ -162,11 +162,42 @@ PK_HD void flux_hllc(const double *l, const double *r, double gam, double *f)
     }
 }
 
+// One side of an HLLD interface: face state q = [rho, v_n, v_t1, v_t2, p,
+// B_n, B_t1, B_t2], |B|^2 and total pressure.  Gives the total energy
+// (hlld.F90:263-264), the physical flux (:296-308) and the conserved state.
+struct HlldSide {
+    double en;
+    double f[5], fby, fbz;
+    double U[5];
+};
+
+PK_HD void hlld_side(const double *q, double b2, double pr, double gam, HlldSide &s)
+{
+    s.en = ((q[4] / (gam - 1.0)) + (0.5 * q[0]) * ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3])) + 0.5 * b2;
+    s.f[0] = q[0] * q[1];
+    s.f[1] = (q[0] * (q[1] * q[1]) + pr) - q[5] * q[5];
+    s.f[2] = (q[0] * q[2]) * q[1] - q[5] * q[6];
+    s.f[3] = (q[0] * q[3]) * q[1] - q[5] * q[7];
+    s.f[4] = (s.en + pr) * q[1] - q[5] * ((q[1] * q[5] + q[2] * q[6]) + q[3] * q[7]);
+    s.fby = q[6] * q[1] - q[5] * q[2];
+    s.fbz = q[7] * q[1] - q[5] * q[3];
+    s.U[0] = q[0];
+    s.U[1] = q[0] * q[1];
+    s.U[2] = q[0] * q[2];
+    s.U[3] = q[0] * q[3];
+    s.U[4] = s.en;
+}
+
 // HLLD with the GLM psi flux (riemann_hlld, DIVB_HDC) for one interface.
 // l, r: primitive face states [rho, v_n, v_t1, v_t2, p, B_n, B_t1, B_t2, psi].
 // f: fluxes of [rho, m_n, m_t1, m_t2, E, B_n, B_t1, B_t2, psi].
-PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
-                                          double *f)
+//
+// Every branch of the reference that has a left and a right variant is
+// evaluated for the one side the interface needs: that side's inputs are
+// selected first and the reference's formula is applied once (a - x*y is
+// written a + ((-1)*x)*y, which rounds identically).  Only the two sm == 0
+// averages, which need both sides, evaluate both.
+PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, double *f)
 {
     // Dedner et al. eq. 42 (hlld.F90:229-230)
     f[8] = ((ch * ch) * 0.5) * ((r[5] + l[5]) - (r[8] - l[8]) / ch);
@@ -178,8 +209,6 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
     const double b2r = (bxr * bxr + byr * byr) + bzr * bzr;
     const double prl = l[4] + 0.5 * b2l;
     const double prr = r[4] + 0.5 * b2r;
-    const double enl = ((l[4] / (gam - 1.0)) + (0.5 * l[0]) * ((l[1] * l[1] + l[2] * l[2]) + l[3] * l[3])) + 0.5 * b2l;
-    const double enr = ((r[4] / (gam - 1.0)) + (0.5 * r[0]) * ((r[1] * r[1] + r[2] * r[2]) + r[3] * r[3])) + 0.5 * b2r;
     const double gpl = gam * l[4], gpr = gam * r[4];
     const double tl = gpl + b2l, tr = gpr + b2r;
     const double cl = (tl + sqrt(tl * tl - (4.0 * gpl) * (bxl * bxl))) / l[0];
@@ -188,34 +217,15 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
     const double sl = fmn(l[1], r[1]) - c;
     const double sr = fmx(l[1], r[1]) + c;
 
-    double fl[5], fr[5];
-    fl[0] = l[0] * l[1];
-    fl[1] = (l[0] * (l[1] * l[1]) + prl) - bxl * bxl;
-    fl[2] = (l[0] * l[2]) * l[1] - bxl * byl;
-    fl[3] = (l[0] * l[3]) * l[1] - bxl * bzl;
-    fl[4] = (enl + prl) * l[1] - bxl * ((l[1] * bxl + l[2] * byl) + l[3] * bzl);
-    const double fbyl = byl * l[1] - bxl * l[2];
-    const double fbzl = bzl * l[1] - bxl * l[3];
-    fr[0] = r[0] * r[1];
-    fr[1] = (r[0] * (r[1] * r[1]) + prr) - bxr * bxr;
-    fr[2] = (r[0] * r[2]) * r[1] - bxr * byr;
-    fr[3] = (r[0] * r[3]) * r[1] - bxr * bzr;
-    fr[4] = (enr + prr) * r[1] - bxr * ((r[1] * bxr + r[2] * byr) + r[3] * bzr);
-    const double fbyr = byr * r[1] - bxr * r[2];
-    const double fbzr = bzr * r[1] - bxr * r[3];
-
-    if (sl >= 0.0) {
+    if (sl >= 0.0 || sr <= 0.0) {
+        // supersonic: the upwind physical flux (hlld.F90:312-319)
+        const bool L = sl >= 0.0;
+        HlldSide s;
+        hlld_side(L ? l : r, L ? b2l : b2r, L ? prl : prr, gam, s);
 #pragma unroll
-        for (int v = 0; v < 5; v++) f[v] = fl[v];
-        f[6] = fbyl;
-        f[7] = fbzl;
-        return;
-    }
-    if (sr <= 0.0) {
-#pragma unroll
-        for (int v = 0; v < 5; v++) f[v] = fr[v];
-        f[6] = fbyr;
-        f[7] = fbzr;
+        for (int v = 0; v < 5; v++) f[v] = s.f[v];
+        f[6] = s.fby;
+        f[7] = s.fbz;
         return;
     }
 
@@ -230,6 +240,8 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
     const double b_lr = bxl * bxr;
     const double b_lrgam = b_lr / gam;
     const double prt_star = 0.5 * ((prl + dn_l * (sm - l[1])) + (prr + dn_r * (sm - r[1])));
+    const double fbyl = byl * l[1] - bxl * l[2], fbzl = bzl * l[1] - bxl * l[3];
+    const double fbyr = byr * r[1] - bxr * r[2], fbzr = bzr * r[1] - bxr * r[3];
 
     // Transverse B of the star states with the HLL fallback (hlld.F90:362-378)
     double bsly, bslz, bsry, bsrz;
@@ -238,7 +250,6 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
         const bool okl = (fabs(c1) > spacing(b_lr)) && b_lrgam <= l[4];
         const double c2 = dn_r * srsm - b_lr;
         const bool okr = (fabs(c2) > spacing(b_lr)) && b_lrgam <= r[4];
-        // the HLL average, used where the star-state formula degenerates
         double hy = 0.0, hz = 0.0;
         if (!(okl && okr)) {
             hy = ((sr * byr - sl * byl) - (fbyr - fbyl)) / srmsl;
@@ -265,97 +276,104 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch,
     const double vslz = l[3] + (bxl / dn_l) * (bzl - bslz);
     const double vsry = r[2] + (bxr / dn_r) * (byr - bsry);
     const double vsrz = r[3] + (bxr / dn_r) * (bzr - bsrz);
-
-    const double vb_l = (l[1] * bxl + l[2] * byl) + l[3] * bzl;
-    const double vb_r = (r[1] * bxr + r[2] * byr) + r[3] * bzr;
-    const double vb_sl = (sm * bxl + vsly * bsly) + vslz * bslz;
-    const double vb_sr = (sm * bxr + vsry * bsry) + vsrz * bsrz;
-
     const double dsl = dn_l / slsm, dsr = dn_r / srsm;
-    const double usl[5] = {dsl, dsl * sm, dsl * vsly, dsl * vslz,
-                           ((((slvxl * enl) - (prl * l[1])) + (prt_star * sm)) + bxl * (vb_l - vb_sl)) / slsm};
-    const double usr[5] = {dsr, dsr * sm, dsr * vsry, dsr * vsrz,
-                           ((((srvxr * enr) - (prr * r[1])) + (prt_star * sm)) + bxr * (vb_r - vb_sr)) / srsm};
-    const double Ul[5] = {l[0], l[0] * l[1], l[0] * l[2], l[0] * l[3], enl};
-    const double Ur[5] = {r[0], r[0] * r[1], r[0] * r[2], r[0] * r[3], enr};
 
-    // Single-star fluxes, Eq. 64: F_L + S_L (U*_L - U_L).  Where a formula
-    // exists for both sides, the side's inputs are selected first and the
-    // formula is evaluated once, so a wave with lanes on both sides does the
-    // work once (a - x*y is evaluated as a + ((-1)*x)*y, which is exact).
-    bool left;
-    if (fabs(bxl) > 0.0) {
-        const double dn_lsqt = sqrt(dsl), dn_rsqt = sqrt(dsr);
-        const double alfven_l = sm - fabs(bxl) / dn_lsqt;
-        const double alfven_r = sm + fabs(bxr) / dn_rsqt;
-        const bool left_star = alfven_l > 0.0;
-        const bool right_star = !left_star && (alfven_r < 0.0);
-        if (!left_star && !right_star) {
-            // Alfven (double-star) states, hlld.F90:450-544
-            const double b_sig = (bxl >= 0.0) ? 1.0 : -1.0;
-            const double add = dn_lsqt + dn_rsqt;
-            const double mul = dn_lsqt * dn_rsqt;
-            const double v2y = ((dn_lsqt * vsly + dn_rsqt * vsry) + b_sig * (bsry - bsly)) / add;
-            const double v2z = ((dn_lsqt * vslz + dn_rsqt * vsrz) + b_sig * (bsrz - bslz)) / add;
-            const double b2x = 0.5 * (bxl + bxr);
-            const double b2y = ((dn_lsqt * bsry + dn_rsqt * bsly) + (b_sig * mul) * (vsry - vsly)) / add;
-            const double b2z = ((dn_lsqt * bsrz + dn_rsqt * bslz) + (b_sig * mul) * (vsrz - vslz)) / add;
-            const double vb2 = (sm * b2x + v2y * b2y) + v2z * b2z;
-            if (sm != 0.0) {
-                const bool L = sm > 0.0;
-                const double* us = L ? usl : usr;
-                const double* U = L ? Ul : Ur;
-                const double* f0 = L ? fl : fr;
-                const double al = L ? alfven_l : alfven_r;
-                const double s_ = L ? sl : sr;
-                const double sq = L ? dn_lsqt : dn_rsqt;
-                const double vbs = L ? vb_sl : vb_sr;
-                const double sg = L ? -1.0 : 1.0;
-                const double fby0 = L ? fbyl : fbyr, fbz0 = L ? fbzl : fbzr;
-                const double bsy = L ? bsly : bsry, bsz = L ? bslz : bsrz;
-                const double by0 = L ? byl : byr, bz0 = L ? bzl : bzr;
-                const double u2[5] = {us[0], us[0] * sm, us[0] * v2y, us[0] * v2z,
-                                      us[4] + (sg * (b_sig * sq)) * (vbs - vb2)};
-#pragma unroll
-                for (int v = 0; v < 5; v++) f[v] = ((f0[v] + al * u2[v]) - (al - s_) * us[v]) - s_ * U[v];
-                f[6] = ((fby0 + al * b2y) - (al - s_) * bsy) - s_ * by0;
-                f[7] = ((fbz0 + al * b2z) - (al - s_) * bsz) - s_ * bz0;
-            } else {
-                const double u2l[5] = {dsl, dsl * sm, dsl * v2y, dsl * v2z, usl[4] - (b_sig * dn_lsqt) * (vb_sl - vb2)};
-                const double u2r[5] = {dsr, dsr * sm, dsr * v2y, dsr * v2z, usr[4] + (b_sig * dn_rsqt) * (vb_sr - vb2)};
-#pragma unroll
-                for (int v = 0; v < 5; v++)
-                    f[v] = 0.5 * ((((fl[v] + alfven_l * u2l[v]) - (alfven_l - sl) * usl[v]) - sl * Ul[v]) +
-                                  (((fr[v] + alfven_r * u2r[v]) - (alfven_r - sr) * usr[v]) - sr * Ur[v]));
-                f[6] = 0.5 * ((((fbyl + alfven_l * b2y) - (alfven_l - sl) * bsly) - sl * byl) +
-                              (((fbyr + alfven_r * b2y) - (alfven_r - sr) * bsry) - sr * byr));
-                f[7] = 0.5 * ((((fbzl + alfven_l * b2z) - (alfven_l - sl) * bslz) - sl * bzl) +
-                              (((fbzr + alfven_r * b2z) - (alfven_r - sr) * bsrz) - sr * bzr));
-            }
-            return;
-        }
-        left = left_star;
+    // which region holds x/t = 0 (hlld.F90:410-599)
+    const bool bxnz = fabs(bxl) > 0.0;
+    double dn_lsqt = 0.0, dn_rsqt = 0.0, alfven_l = 0.0, alfven_r = 0.0;
+    bool dbl, L;
+    if (bxnz) {
+        dn_lsqt = sqrt(dsl);
+        dn_rsqt = sqrt(dsr);
+        alfven_l = sm - fabs(bxl) / dn_lsqt;
+        alfven_r = sm + fabs(bxr) / dn_rsqt;
+        const bool ls = alfven_l > 0.0, rs = !ls && (alfven_r < 0.0);
+        dbl = !ls && !rs;
+        L = dbl ? (sm > 0.0) : ls;
     } else {
-        // B_n == 0: the contact speed picks the side (hlld.F90:552-597)
-        if (sm == 0.0) {
+        dbl = false;
+        L = sm > 0.0;
+    }
+
+    // double-star transverse v and B (hlld.F90:458-474), needed by both averages
+    const double b_sig = (bxl >= 0.0) ? 1.0 : -1.0;
+    double v2y = 0.0, v2z = 0.0, b2y = 0.0, b2z = 0.0, vb2 = 0.0;
+    if (dbl) {
+        const double add = dn_lsqt + dn_rsqt;
+        const double mul = dn_lsqt * dn_rsqt;
+        v2y = ((dn_lsqt * vsly + dn_rsqt * vsry) + b_sig * (bsry - bsly)) / add;
+        v2z = ((dn_lsqt * vslz + dn_rsqt * vsrz) + b_sig * (bsrz - bslz)) / add;
+        const double b2x = 0.5 * (bxl + bxr);
+        b2y = ((dn_lsqt * bsry + dn_rsqt * bsly) + (b_sig * mul) * (vsry - vsly)) / add;
+        b2z = ((dn_lsqt * bsrz + dn_rsqt * bslz) + (b_sig * mul) * (vsrz - vslz)) / add;
+        vb2 = (sm * b2x + v2y * b2y) + v2z * b2z;
+    }
+
+    if (sm == 0.0 && (dbl || !bxnz)) {
+        // both sides, averaged: hlld.F90:522-543 (Alfven) and :578-596 (B_n = 0)
+        HlldSide a, b;
+        hlld_side(l, b2l, prl, gam, a);
+        hlld_side(r, b2r, prr, gam, b);
+        const double vb_l = (l[1] * bxl + l[2] * byl) + l[3] * bzl;
+        const double vb_r = (r[1] * bxr + r[2] * byr) + r[3] * bzr;
+        const double vb_sl = (sm * bxl + vsly * bsly) + vslz * bslz;
+        const double vb_sr = (sm * bxr + vsry * bsry) + vsrz * bsrz;
+        const double usl[5] = {dsl, dsl * sm, dsl * vsly, dsl * vslz,
+                               ((((slvxl * a.en) - (prl * l[1])) + (prt_star * sm)) + bxl * (vb_l - vb_sl)) / slsm};
+        const double usr[5] = {dsr, dsr * sm, dsr * vsry, dsr * vsrz,
+                               ((((srvxr * b.en) - (prr * r[1])) + (prt_star * sm)) + bxr * (vb_r - vb_sr)) / srsm};
+        if (dbl) {
+            const double u2l[5] = {dsl, dsl * sm, dsl * v2y, dsl * v2z, usl[4] - (b_sig * dn_lsqt) * (vb_sl - vb2)};
+            const double u2r[5] = {dsr, dsr * sm, dsr * v2y, dsr * v2z, usr[4] + (b_sig * dn_rsqt) * (vb_sr - vb2)};
 #pragma unroll
             for (int v = 0; v < 5; v++)
-                f[v] = 0.5 * (((fl[v] + sl * (usl[v] - Ul[v])) + fr[v]) + sr * (usr[v] - Ur[v]));
+                f[v] = 0.5 * ((((a.f[v] + alfven_l * u2l[v]) - (alfven_l - sl) * usl[v]) - sl * a.U[v]) +
+                              (((b.f[v] + alfven_r * u2r[v]) - (alfven_r - sr) * usr[v]) - sr * b.U[v]));
+            f[6] = 0.5 * ((((fbyl + alfven_l * b2y) - (alfven_l - sl) * bsly) - sl * byl) +
+                          (((fbyr + alfven_r * b2y) - (alfven_r - sr) * bsry) - sr * byr));
+            f[7] = 0.5 * ((((fbzl + alfven_l * b2z) - (alfven_l - sl) * bslz) - sl * bzl) +
+                          (((fbzr + alfven_r * b2z) - (alfven_r - sr) * bsrz) - sr * bzr));
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                f[v] = 0.5 * (((a.f[v] + sl * (usl[v] - a.U[v])) + b.f[v]) + sr * (usr[v] - b.U[v]));
             f[6] = 0.5 * (((fbyl + sl * (bsly - byl)) + fbyr) + sr * (bsry - byr));
             f[7] = 0.5 * (((fbzl + sl * (bslz - bzl)) + fbzr) + sr * (bsrz - bzr));
-            return;
         }
-        left = sm > 0.0;
+        return;
     }
-    {
-        const double* us = left ? usl : usr;
-        const double* U = left ? Ul : Ur;
-        const double* f0 = left ? fl : fr;
-        const double s_ = left ? sl : sr;
+
+    // the one side the interface needs
+    const double *q = L ? l : r;
+    const double s_ = L ? sl : sr;
+    const double svx = L ? slvxl : srvxr;
+    const double ssm = L ? slsm : srsm;
+    const double ds = L ? dsl : dsr;
+    const double pr = L ? prl : prr;
+    const double vsy = L ? vsly : vsry, vsz = L ? vslz : vsrz;
+    const double bsy = L ? bsly : bsry, bsz = L ? bslz : bsrz;
+    HlldSide o;
+    hlld_side(q, L ? b2l : b2r, pr, gam, o);
+    const double vb = (q[1] * q[5] + q[2] * q[6]) + q[3] * q[7];
+    const double vbs = (sm * q[5] + vsy * bsy) + vsz * bsz;
+    const double us[5] = {ds, ds * sm, ds * vsy, ds * vsz,
+                          ((((svx * o.en) - (pr * q[1])) + (prt_star * sm)) + q[5] * (vb - vbs)) / ssm};
+    if (dbl) {
+        // Alfven intermediate flux, Eq. 65 (hlld.F90:496-520)
+        const double al = L ? alfven_l : alfven_r;
+        const double sq = L ? dn_lsqt : dn_rsqt;
+        const double sg = L ? -1.0 : 1.0;
+        const double u2[5] = {ds, ds * sm, ds * v2y, ds * v2z, us[4] + (sg * (b_sig * sq)) * (vbs - vb2)};
 #pragma unroll
-        for (int v = 0; v < 5; v++) f[v] = f0[v] + s_ * (us[v] - U[v]);
-        f[6] = (left ? fbyl : fbyr) + s_ * ((left ? bsly : bsry) - (left ? byl : byr));
-        f[7] = (left ? fbzl : fbzr) + s_ * ((left ? bslz : bsrz) - (left ? bzl : bzr));
+        for (int v = 0; v < 5; v++) f[v] = ((o.f[v] + al * u2[v]) - (al - s_) * us[v]) - s_ * o.U[v];
+        f[6] = ((o.fby + al * b2y) - (al - s_) * bsy) - s_ * q[6];
+        f[7] = ((o.fbz + al * b2z) - (al - s_) * bsz) - s_ * q[7];
+    } else {
+        // single-star flux, Eq. 64
+#pragma unroll
+        for (int v = 0; v < 5; v++) f[v] = o.f[v] + s_ * (us[v] - o.U[v]);
+        f[6] = o.fby + s_ * (bsy - q[6]);
+        f[7] = o.fbz + s_ * (bsz - q[7]);
     }
 }
 

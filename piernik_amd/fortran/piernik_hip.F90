@@ -20,7 +20,7 @@ module piernik_hip
    public :: pk_init, pk_finalize, pk_register_block, pk_unregister_block, pk_upload, pk_download, &
         &    pk_upload_from, pk_download_to, pk_solve_cg, pk_sweep, pk_fill_ghosts, pk_glm_damp, &
         &    pk_timestep_fluid, pk_fluid_update, pk_run, pk_get_report, pk_synchronize, &
-        &    pk_profile_begin, pk_profile_end, pk_error_message
+        &    pk_autotune, pk_profile_begin, pk_profile_end, pk_error_message
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
 
    ! the reference's stage enumerators (src/base/constants.F90:230-232)
@@ -171,6 +171,13 @@ module piernik_hip
       integer(c_int32_t) function pk_synchronize() bind(C, name="pk_synchronize")
          import :: c_int32_t
       end function pk_synchronize
+
+      integer(c_int32_t) function pk_autotune(cg_id, dt, chspeed, tiles_out) bind(C, name="pk_autotune")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value       :: cg_id
+         real(c_double),     value       :: dt, chspeed
+         integer(c_int32_t), intent(out) :: tiles_out(6)
+      end function pk_autotune
 
       integer(c_int32_t) function pk_profile_begin() bind(C, name="pk_profile_begin")
          import :: c_int32_t

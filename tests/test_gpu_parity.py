@@ -143,3 +143,20 @@ def test_2d_mhd_bit_exact(oracle):
     assert t_g == t_o
     assert_same(cg, ob, "2d mhd")
     pk.finalize()
+
+
+def test_autotuned_tiles_keep_bit_exactness(oracle):
+    """Tile shapes change the schedule, never the arithmetic."""
+    cg, cfg = setups.mhd_turbulence(n=24, seed=13)
+    setup_lib(cfg)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    cg.upload()
+    tiles = pk.autotune(cg)
+    assert set(tiles) == {0, 1, 2} and all(tp * ts <= 512 for tp, ts in tiles.values())
+    t_g, _ = pk.run(cg, 2)
+    t_o = oracle.run(p, ob, 2)
+    cg.download()
+    assert t_g == t_o
+    assert_same(cg, ob, "autotuned 24^3")
+    pk.finalize()

@@ -77,6 +77,16 @@ def pairs(rng, m, mhd):
         qr[5 * k:6 * k, 5] = ql[5 * k:6 * k, 5]       # continuous B_n
         ql[6 * k:7 * k, 6:8] = 0.0                    # vanishing transverse B
         qr[6 * k:7 * k, 6:8] = 0.0
+        # exact mirror images (v_n, B_n -> -v_n, -B_n): the contact speed is
+        # exactly 0, which selects the two-sided averages of hlld.F90:522,578
+        j = slice(7 * k, 7 * k + k // 2)
+        qr[j] = ql[j]
+        qr[j, 1] = -ql[j, 1]
+        qr[j, 5] = -ql[j, 5]
+        j2 = slice(7 * k + k // 2, 8 * k)
+        ql[j2, 5] = 0.0
+        qr[j2] = ql[j2]
+        qr[j2, 1] = -ql[j2, 1]
     return ql, qr
 
 
@@ -136,6 +146,9 @@ def test_hlld(hm, oracle, gam, ch):
     hm.hm_hlld(m, dp(ql), dp(qr), gam, ch, dp(f))
     fo, bfo = oracle.hlld(ql[:, :5], qr[:, :5], ql[:, 5:9], qr[:, 5:9], gam, ch)
     ref = np.concatenate([fo, bfo], axis=1)
+    # the mirror pairs do reach the contact-speed-zero averages
+    k = m // 8
+    assert np.all(f[7 * k:8 * k, 0] == 0.0)
     bad = np.argwhere(~(f == ref))
     assert bad.size == 0, (bad[:5], f[tuple(bad[0])], ref[tuple(bad[0])])
 
