@@ -74,6 +74,9 @@ typedef struct pk_config {
     int32_t has_dir[3];        /* dom%has_dir: active directions */
     int32_t tile_x[2];         /* sweep tile shape (pencils, cells) along x; 0 = autotuned default */
     int32_t tile_yz[2];        /* sweep tile shape (pencils, cells) along y and z; 0 = default */
+    int32_t fast_math;         /* 0: every operation rounds as the reference's FP64 code (bit-exact);
+                                  1: FMA contraction and fast division/sqrt in the sweep (~1e-15 per op) */
+    int32_t reserved;
 } pk_config;
 
 /* Floor bookkeeping accumulated since the last reset

@@ -55,6 +55,7 @@ class _Config(ctypes.Structure):
         ("use_smalld", ctypes.c_int32), ("use_smallei", ctypes.c_int32),
         ("cfl", ctypes.c_double), ("cfl_glm", ctypes.c_double), ("glm_alpha", ctypes.c_double),
         ("has_dir", ctypes.c_int32 * 3), ("tile_x", ctypes.c_int32 * 2), ("tile_yz", ctypes.c_int32 * 2),
+        ("fast_math", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -129,7 +130,8 @@ class Config:
 
     def __init__(self, gamma=5.0 / 3.0, magnetized=False, limiter="vanleer", limiter_b=None,
                  integration_order=2, cfl=0.7, cfl_glm=None, glm_alpha=0.1, smalld=1e-10, smallei=1e-10,
-                 use_smalld=True, use_smallei=True, has_dir=(True, True, True), tile_x=(0, 0), tile_yz=(0, 0)):
+                 use_smalld=True, use_smallei=True, has_dir=(True, True, True), tile_x=(0, 0), tile_yz=(0, 0),
+                 fast_math=False):
         self.gamma = float(gamma)
         self.magnetized = bool(magnetized)
         self.limiter = limiter
@@ -145,6 +147,7 @@ class Config:
         self.has_dir = tuple(bool(h) for h in has_dir)
         self.tile_x = tuple(tile_x)
         self.tile_yz = tuple(tile_yz)
+        self.fast_math = bool(fast_math)
 
     @property
     def nvar(self):
@@ -167,6 +170,7 @@ class Config:
             c.has_dir[d] = int(self.has_dir[d])
         c.tile_x[0], c.tile_x[1] = self.tile_x
         c.tile_yz[0], c.tile_yz[1] = self.tile_yz
+        c.fast_math = int(self.fast_math)
         return c
 
 

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpiernik_hip.so")
 
-SOURCES = ["pk_kernels.hip", "pk_capi.hip"]
+SOURCES = ["pk_kernels.hip", "pk_kernels_fast.hip", "pk_capi.hip"]
 
 # -ffp-contract=off: every a*b+c rounds twice, as the reference's FP64 code.
 # -packed-fp32-ops off: no packed FP32 instructions are generated.
@@ -57,7 +57,12 @@ def build(verbose=False):
     for src in SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
-        jobs.append([hipcc, *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        flags = list(HIPFLAGS)
+        if src.endswith("_fast.hip"):
+            # the relaxed build of the same kernels: FMA contraction, fast division and sqrt
+            flags = [f for f in flags if f not in ("-ffp-contract=off", "-fno-fast-math")]
+            flags += ["-ffp-contract=fast", "-ffast-math"]
+        jobs.append([hipcc, *flags, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
     tmp = LIB + ".tmp"

@@ -66,7 +66,8 @@ std::map<TuneKey, std::pair<int, int>> g_tuned;
 
 TuneKey tune_key(const Block *b, int d)
 {
-    return TuneKey{b->n[0], b->n[1], b->n[2], d, g_mhd ? 1 : 0, g_cfg.limiter, g_cfg.limiter_b};
+    return TuneKey{b->n[0], b->n[1], b->n[2], d, (g_mhd ? 1 : 0) + (g_cfg.fast_math ? 2 : 0), g_cfg.limiter,
+                   g_cfg.limiter_b};
 }
 
 // Event pairs recorded around sweep launches while profiling is on.
@@ -189,6 +190,12 @@ void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, 
     tile_geometry(b, d, fused, tp, ts, TP, TS, out, chunks, tiles1);
 }
 
+hipError_t sweep_kernel(int d, bool fused, const pk::SweepArgs &a)
+{
+    return g_cfg.fast_math ? pk_fast::launch_sweep(d, g_mhd, fused, a, g_stream)
+                           : pk::launch_sweep(d, g_mhd, fused, a, g_stream);
+}
+
 int take_event(hipEvent_t *e)
 {
     if (!g_event_pool.empty()) {
@@ -213,7 +220,7 @@ int launch(int d, bool fused, const pk::SweepArgs &a0, Block *b)
         rec.bytes = cells * b->nplanes * 8.0 * 2.0;
         HIPCHK(hipEventRecord(rec.start, g_stream));
     }
-    HIPCHK(pk::launch_sweep(d, g_mhd, fused, a, g_stream));
+    HIPCHK(sweep_kernel(d, fused, a));
     if (g_prof) {
         HIPCHK(hipEventRecord(rec.stop, g_stream));
         g_prof_recs.push_back(rec);
@@ -774,9 +781,9 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
                 a.out = out;
                 a.chunks = chunks;
                 a.tiles1 = tiles1;
-                HIPCHK(pk::launch_sweep(d, g_mhd, true, a, g_stream));  // warm-up
+                HIPCHK(sweep_kernel(d, true, a));  // warm-up
                 HIPCHK(hipEventRecord(e0, g_stream));
-                for (int rep = 0; rep < 3; rep++) HIPCHK(pk::launch_sweep(d, g_mhd, true, a, g_stream));
+                for (int rep = 0; rep < 3; rep++) HIPCHK(sweep_kernel(d, true, a));
                 HIPCHK(hipEventRecord(e1, g_stream));
                 HIPCHK(hipEventSynchronize(e1));
                 float ms = 0.0f;

@@ -17,7 +17,11 @@
 #include "pk_math.hpp"
 #include "pk_kernels.hpp"
 
-namespace pk {
+namespace PK_KNS {
+
+using pk::DtArgs;
+using pk::GhostArgs;
+using pk::SweepArgs;
 
 template <int DIR>
 __device__ __forceinline__ int plane_of(int k)
@@ -446,7 +450,7 @@ int dt_partial_blocks(const DtArgs &a)
 
 hipError_t launch_dt(const DtArgs &a, double *part, double *out, hipStream_t st)
 {
-    const int nb = dt_partial_blocks(a);
+    const int nb = PK_KNS::dt_partial_blocks(a);
     if (a.magnetized)
         hipLaunchKernelGGL(k_dt_partial<true>, dim3(nb), dim3(256), 0, st, a, part);
     else
@@ -473,4 +477,4 @@ hipError_t launch_soa_to_aos(const double *soa, long long sv, int plane0, int nc
     return hipGetLastError();
 }
 
-}  // namespace pk
+}  // namespace PK_KNS

@@ -63,3 +63,8 @@ hipError_t launch_soa_to_aos(const double *soa, long long sv, int plane0, int nc
                              int sy, int sz, hipStream_t st);
 
 }  // namespace pk
+
+namespace pk_fast {
+// the sweep of pk_kernels_fast.hip (FMA contraction, fast division)
+hipError_t launch_sweep(int dir, bool mhd, bool fused, const pk::SweepArgs &a, hipStream_t st);
+}  // namespace pk_fast

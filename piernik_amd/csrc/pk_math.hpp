@@ -21,7 +21,13 @@
 // arithmetic, from host code.
 #define PK_HD __host__ __device__ __forceinline__
 
-namespace pk {
+// The same source is compiled twice: exactly (namespace pk) and with FMA
+// contraction and fast division (namespace pk_fast, pk_kernels_fast.hip).
+#ifndef PK_KNS
+#define PK_KNS pk
+#endif
+
+namespace PK_KNS {
 
 enum Limiter : int { LIM_VANLEER = 0, LIM_MINMOD = 1, LIM_MONCEN = 2, LIM_SUPERBEE = 3 };
 
@@ -408,4 +414,4 @@ PK_HD void floors(double (&u)[P::NV], double smalld, double smallei,
     u[4] = e;
 }
 
-}  // namespace pk
+}  // namespace PK_KNS

@@ -49,6 +49,8 @@ module piernik_hip
       integer(c_int32_t) :: has_dir(3)
       integer(c_int32_t) :: tile_x(2)
       integer(c_int32_t) :: tile_yz(2)
+      integer(c_int32_t) :: fast_math
+      integer(c_int32_t) :: reserved
    end type pk_config
 
    type, bind(C) :: pk_stage_report
