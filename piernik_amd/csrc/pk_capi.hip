@@ -41,7 +41,11 @@ struct Block {
     double *d_part = nullptr;
     double *d_dtout = nullptr;
     double *d_stage = nullptr;  // AoS staging for host transfers
+    unsigned long long *d_dtslots = nullptr;  // timestep reduction written by the last sweep of a step
+    bool dt_pending = false;                  // d_dtout holds timestep_fluid of the current state
 };
+
+const int kDtSlots = 1024;
 
 pk_config g_cfg;
 bool g_init = false;
@@ -142,6 +146,13 @@ void fill_common(const Block *b, pk::SweepArgs &a)
     a.dmass = b->d_dmass;
     a.psi_fac = 1.0;
     a.ch = 0.0;
+    a.images = 0;
+    a.dtslots = nullptr;
+    a.nslots = kDtSlots;
+    for (int d = 0; d < 3; d++) {
+        a.has_dir[d] = g_cfg.has_dir[d];
+        a.dl[d] = b->dl[d];
+    }
 }
 
 // Tile shape along direction d: TP pencils x TS cells, within the launch
@@ -267,6 +278,7 @@ int copy_transverse_ghosts(Block *b, int d, const double *src, double *dst)
 
 int do_stage(Block *b, int ddim, int istep, double dt, double ch)
 {
+    b->dt_pending = false;
     pk::SweepArgs a{};
     fill_common(b, a);
     a.ch = ch;
@@ -300,7 +312,10 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
     return PK_OK;
 }
 
-int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac)
+// One directional sweep, both RK2 stages.  images: write the periodic ghost
+// images too (the block is periodic on itself); want_dt: reduce the next
+// step's timestep_fluid from the cells written.
+int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool images = false, bool want_dt = false)
 {
     if (g_cfg.integration_order == 1) {
         int rc = do_stage(b, ddim, PK_EULER, dt, ch);
@@ -320,10 +335,20 @@ int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac)
     a.src = state(b, b->cur);
     a.base = a.src;
     a.dst = state(b, 1 - b->cur);
+    a.images = images ? 1 : 0;
+    if (want_dt) {
+        HIPCHK(pk::launch_dtslots_init(b->d_dtslots, kDtSlots, g_stream));
+        a.dtslots = b->d_dtslots;
+    }
     int rc = launch(ddim, true, a, b);
     if (rc) return rc;
     b->cur = 1 - b->cur;
     b->uh = -1;
+    b->dt_pending = false;
+    if (want_dt) {
+        HIPCHK(pk::launch_dtslots_final(b->d_dtslots, kDtSlots, b->d_dtout, g_stream));
+        b->dt_pending = true;
+    }
     return PK_OK;
 }
 
@@ -349,6 +374,17 @@ int do_fill(Block *b)
 int do_timestep(Block *b, double *dt_out, double *c_out)
 {
     double dt = DBL_MAX, c = 0.0;
+    if (b->dt_pending && g_cfg.nfluids == 1) {
+        // reduced by the last sweep from the cells it wrote (same formula, same state)
+        HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtout, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        double m = g_pinned[0];
+        if (g_pinned[1] < m) m = g_pinned[1];
+        if (g_pinned[2] < m) m = g_pinned[2];
+        *dt_out = g_cfg.cfl * m;
+        *c_out = g_pinned[3];
+        return PK_OK;
+    }
     for (int f = 0; f < g_cfg.nfluids; f++) {
         pk::DtArgs a{};
         a.u = state(b, b->cur);
@@ -405,9 +441,16 @@ int do_fluid_update(Block *b, double dt)
         for (int s = 0; s < 3; s++) {
             const int d = half == 0 ? s : 2 - s;
             if (!g_cfg.has_dir[d]) continue;
-            int rc = do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0);
+            int rc;
+            if (g_cfg.integration_order == 2) {
+                // the sweep refreshes the periodic ghosts itself; the last one of
+                // the step also reduces the next step's timestep
+                rc = do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0, true, half == 1 && d == last);
+            } else {
+                rc = do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0);
+                if (!rc) rc = do_fill(b);
+            }
             if (rc) return rc;
-            if ((rc = do_fill(b))) return rc;
         }
     }
     return PK_OK;
@@ -433,6 +476,7 @@ int alloc_block(Block *b)
     HIPCHK(hipMemsetAsync(b->d_dmass, 0, sizeof(double), g_stream));
     HIPCHK(hipMalloc((void **)&b->d_part, sizeof(double) * 4 * 2048));
     HIPCHK(hipMalloc((void **)&b->d_dtout, sizeof(double) * 4));
+    HIPCHK(hipMalloc((void **)&b->d_dtslots, sizeof(unsigned long long) * 4 * kDtSlots));
     const size_t ncell = (size_t)b->ng[0] * b->ng[1] * b->ng[2];
     HIPCHK(hipMalloc((void **)&b->d_stage, sizeof(double) * ncell * 5 * g_cfg.nfluids));
     return PK_OK;
@@ -447,6 +491,7 @@ void free_block(Block *b)
     if (b->d_part) (void)hipFree(b->d_part);
     if (b->d_dtout) (void)hipFree(b->d_dtout);
     if (b->d_stage) (void)hipFree(b->d_stage);
+    if (b->d_dtslots) (void)hipFree(b->d_dtslots);
     delete b;
 }
 
@@ -471,6 +516,7 @@ int download_array(Block *b, double *h, int nc, int plane0, const double *src)
 int upload_state(Block *b, const double *u, const double *bb, const double *psi, double *dst)
 {
     int rc;
+    b->dt_pending = false;
     if (u && (rc = upload_array(b, u, 5 * g_cfg.nfluids, 0, dst))) return rc;
     if (g_mhd) {
         if (bb && (rc = upload_array(b, bb, 3, 5, dst))) return rc;

@@ -90,6 +90,71 @@ __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)
     }
 }
 
+// timestep_fluid (timestep.F90:460-490, ion_cs / neu_cs) of the cell a lane
+// has just written, reduced over the wave and folded into one slot with
+// 64-bit atomics (the values are positive, so their bit patterns order as
+// the numbers do; min and max do not depend on the order of reduction).
+template <int DIR, class P>
+__device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double (&Wn)[P::NV], bool valid)
+{
+    const double small = 1.17549435082228750797e-38;
+    const double huge = 1.7976931348623157e308;
+    double mn[3] = {huge, huge, huge}, cmax = 0.0;
+    if (valid) {
+        const double d = Wn[0];
+        const double mx = Wn[plane_of<DIR>(1)], my = Wn[plane_of<DIR>(2)], mz = Wn[plane_of<DIR>(3)];
+        const double e = Wn[4];
+        const double m[3] = {mx, my, mz};
+        double v[3] = {0.0, 0.0, 0.0};
+        if (d > 0.0) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) v[q] = fabs(m[q] / d);
+        }
+        const double ek = (0.5 * ((mx * mx + my * my) + mz * mz)) / d;
+        double cs;
+        if constexpr (P::MAG) {
+            const double bx = Wn[plane_of<DIR>(5)], by = Wn[plane_of<DIR>(6)], bz = Wn[plane_of<DIR>(7)];
+            const double pmag = 0.5 * ((bx * bx + by * by) + bz * bz);
+            const double ps = (e - ek) * a.gam_1 + (2.0 - a.gam) * pmag;
+            const double pr = ps - pmag;
+            cs = sqrt(fabs((2.0 * pmag + a.gam * pr) / d));
+        } else {
+            const double pr = (e - ek) * a.gam_1;
+            cs = sqrt(fabs((a.gam * pr) / d));
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double c = fmx(v[q] + cs, small);
+            cmax = fmx(cmax, c);
+            if (a.has_dir[q]) mn[q] = a.dl[q] / c;
+        }
+    }
+    // xor butterfly over the lanes that exist (the last wave of a workgroup
+    // may be partial); partners beyond it are skipped, lane 0 ends with all.
+    const int lane = threadIdx.x & 63;
+    const int nact = min(64, (int)blockDim.x - (int)(threadIdx.x & ~63u));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const bool take = (lane ^ o) < nact;
+        double t[4];
+#pragma unroll
+        for (int q = 0; q < 3; q++) t[q] = __shfl_xor(mn[q], o, 64);
+        t[3] = __shfl_xor(cmax, o, 64);
+        if (take) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) mn[q] = fmn(mn[q], t[q]);
+            cmax = fmx(cmax, t[3]);
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const int slot = (int)((blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u) % (unsigned)a.nslots);
+        unsigned long long *s = a.dtslots + 4 * slot;
+#pragma unroll
+        for (int q = 0; q < 3; q++) atomicMin(s + q, (unsigned long long)__double_as_longlong(mn[q]));
+        atomicMax(s + 3, (unsigned long long)__double_as_longlong(cmax));
+    }
+}
+
 template <int DIR, class P, bool FUSED, int LIM>
 __global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
 {
@@ -186,11 +251,37 @@ __global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
             if (inner) dmass += dm;
         }
     }
+    if constexpr (FUSED) {
+        if (a.dtslots) dt_contribution<DIR, P>(a, Wn, outl && svalid);
+    }
     if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
 
     if (outl && svalid) {
 #pragma unroll
         for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + idx] = Wn[k];
+        if (FUSED && a.images) {
+            // periodic images: each ghost cell has exactly one interior pre-image
+            int off[3][3], cnt[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int ic = xyz[d] - a.g[d];
+                cnt[d] = 1;
+                off[d][0] = 0;
+                if (a.g[d] > 0) {
+                    if (ic < a.g[d]) off[d][cnt[d]++] = a.n[d];
+                    if (ic >= a.n[d] - a.g[d]) off[d][cnt[d]++] = -a.n[d];
+                }
+            }
+            for (int i0 = 0; i0 < cnt[0]; i0++)
+                for (int i1 = 0; i1 < cnt[1]; i1++)
+                    for (int i2 = 0; i2 < cnt[2]; i2++) {
+                        if ((i0 | i1 | i2) == 0) continue;
+                        const long long j =
+                            idx + off[0][i0] + (long long)a.sy * off[1][i1] + (long long)a.sz * off[2][i2];
+#pragma unroll
+                        for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + j] = Wn[k];
+                    }
+        }
     }
     // floor bookkeeping: rare, one atomic per lane that saw something
     if (flags) atomicOr(a.flags, flags);
@@ -369,6 +460,63 @@ __global__ void k_soa_to_aos(const double *__restrict__ soa, long long sv, int p
     const int z = (int)(r / ny);
     const long long si = (long long)x + (long long)sy * y + (long long)sz * z;
     for (int c = 0; c < nc; c++) aos[i * nc + c] = soa[(long long)(plane0 + c) * sv + si];
+}
+
+__global__ void k_dtslots_init(unsigned long long *slots, int nslots)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    const unsigned long long huge = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
+    slots[4 * i + 0] = huge;
+    slots[4 * i + 1] = huge;
+    slots[4 * i + 2] = huge;
+    slots[4 * i + 3] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_dtslots_final(const unsigned long long *__restrict__ slots, int nslots,
+                                                       double *__restrict__ out)
+{
+    __shared__ double red[4][256];
+    const int t = threadIdx.x;
+    double m0 = 1.7976931348623157e308, m1 = m0, m2 = m0, cmax = 0.0;
+    for (int i = t; i < nslots; i += 256) {
+        m0 = fmn(m0, __longlong_as_double((long long)slots[4 * i + 0]));
+        m1 = fmn(m1, __longlong_as_double((long long)slots[4 * i + 1]));
+        m2 = fmn(m2, __longlong_as_double((long long)slots[4 * i + 2]));
+        cmax = fmx(cmax, __longlong_as_double((long long)slots[4 * i + 3]));
+    }
+    red[0][t] = m0;
+    red[1][t] = m1;
+    red[2][t] = m2;
+    red[3][t] = cmax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] = fmn(red[0][t], red[0][t + w]);
+            red[1][t] = fmn(red[1][t], red[1][t + w]);
+            red[2][t] = fmn(red[2][t], red[2][t + w]);
+            red[3][t] = fmx(red[3][t], red[3][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+        out[2] = red[2][0];
+        out[3] = red[3][0];
+    }
+}
+
+hipError_t launch_dtslots_init(unsigned long long *slots, int nslots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dtslots_init, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, slots, nslots);
+    return hipGetLastError();
+}
+
+hipError_t launch_dtslots_final(const unsigned long long *slots, int nslots, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dtslots_final, dim3(1), dim3(256), 0, st, slots, nslots, out);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------ launchers ----

@@ -28,6 +28,13 @@ struct SweepArgs {
     int use_smalld, use_smallei;
     unsigned *flags;     // device: bit 0 negative density, bit 1 negative internal energy
     double *dmass;       // device: floor-added mass / cell volume
+    // fused mode only:
+    int images;          // also write every output cell's periodic ghost images (one periodic block)
+    unsigned long long *dtslots;  // if set: per slot, min of dl_d/c_d (d = x, y, z) and max c of the
+                                  // cells written, as FP64 bit patterns (timestep_fluid of the next step)
+    int nslots;
+    int has_dir[3];
+    double dl[3];
 };
 
 struct GhostArgs {
@@ -56,6 +63,8 @@ size_t sweep_lds_bytes(bool mhd, int TP, int TS);
 hipError_t launch_fill_ghosts(int dir, const GhostArgs &a, hipStream_t st);
 hipError_t launch_scale(double *x, long long n, double fac, hipStream_t st);
 int dt_partial_blocks(const DtArgs &a);
+hipError_t launch_dtslots_init(unsigned long long *slots, int nslots, hipStream_t st);
+hipError_t launch_dtslots_final(const unsigned long long *slots, int nslots, double *out, hipStream_t st);
 hipError_t launch_dt(const DtArgs &a, double *part, double *out, hipStream_t st);
 hipError_t launch_aos_to_soa(const double *aos, int nc, double *soa, long long sv, int plane0, const int ng[3],
                              int sy, int sz, hipStream_t st);
