@@ -1,6 +1,6 @@
 """Benchmark of the split Riemann sweep on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mhd256|sedov64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mhd256|sedov64|batch4096]
 
 A step is one PIERNIK step (time_step + fluid_update_full: two dt-substeps,
 X->Y->Z then Z->Y->X, both RK2 stages per sweep) over the whole block, with
@@ -14,6 +14,12 @@ bit-exact build.
 Default workload: 3D ideal MHD, HLLD + GLM, minmod, CFL 0.5, single periodic
 256^3 block with 4 ghost layers, rho=1, p=0.6, gamma=5/3, seeded Gaussian
 random v and B = curl A (beta = 1).
+
+--workload batch4096 times the batched block list instead: 4096 periodic
+blocks with edges drawn from 16, 24, 32, 48, 64 (seeded), an ionized MHD
+fluid (HLLD + GLM) and a neutral fluid (HLLC) per block, smooth seeded
+fields; one persistent launch per sweep and fluid replayed from a HIP graph,
+against the same launches issued directly and against one launch per block.
 
 Prints one JSON line with the measured throughput, the roofline of the
 dominant kernel (the fused sweep; HIP events on the library's stream) and a
@@ -41,6 +47,9 @@ def _cpu_worker(args):
     from piernik_amd import setups
     if workload == "mhd256":
         cg, cfg = setups.mhd_turbulence(n=n, seed=seed)
+    elif workload == "batch4096":
+        cfg = setups.batch_config()
+        cg = setups.batch_blocks([n], cfg, seed=seed)[0]
     else:
         cg, cfg = setups.sedov_hd(n=n)
     p = po.params_from_config(cfg)
@@ -58,13 +67,15 @@ def _cpu_worker(args):
 
 def cpu_baseline(workload, budget=10.0):
     """The oracle on all host cores: one process per core, each advancing its
-    own 64^3 block of the same problem, for about `budget` seconds."""
+    own block of the same problem (64^3; 32^3 two-fluid blocks for the
+    batched list) for about `budget` seconds."""
     ncpu = os.cpu_count() or 1
     cores = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    nblk = 32 if workload == "batch4096" else 64
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
-        res = pool.map(_cpu_worker, [(workload, 64, 100 + i, budget) for i in range(cores)])
+        res = pool.map(_cpu_worker, [(workload, nblk, 100 + i, budget) for i in range(cores)])
     wall = time.perf_counter() - t0
     updates = sum(r[0] for r in res)
     slowest = max(r[1] for r in res)
@@ -74,7 +85,8 @@ def cpu_baseline(workload, budget=10.0):
         "cores": cores,
         "kind": "port",
         "sample": (f"oracle (C restatement of the reference sweep, one thread per process) on {cores} "
-                   f"independent periodic 64^3 blocks of the same {'MHD HLLD+GLM' if workload == 'mhd256' else 'HD HLLC'} "
+                   f"independent periodic {nblk}^3 blocks of the same "
+                   f"{ {'mhd256': 'MHD HLLD+GLM', 'batch4096': 'two-fluid (MHD HLLD+GLM + HD HLLC)'}.get(workload, 'HD HLLC')} "
                    f"problem, whole PIERNIK steps for ~{budget:.0f} s each; {updates} cell-updates in {slowest:.2f} s "
                    f"(wall {wall:.2f} s)"),
     }
@@ -94,7 +106,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["mhd256", "sedov64"], default="mhd256")
+    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096"], default="mhd256")
+    ap.add_argument("--nblocks", type=int, default=4096, help="blocks of the batched list")
     ap.add_argument("--n", type=int, default=None, help="override the block size (cells per side)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
@@ -113,6 +126,9 @@ def main():
     # CPU baseline first, before this process touches the GPU
     cpu = None if args.no_cpu else cpu_baseline(args.workload, args.cpu_budget)
     print(f"[bench] cpu baseline done: {cpu and round(cpu['value'])} cell-updates/s", file=sys.stderr, flush=True)
+    if args.workload == "batch4096":
+        print(json.dumps(run_batch(args, cpu)))
+        return
 
     import piernik_amd as pk
     from piernik_amd import setups
@@ -204,6 +220,98 @@ def main():
     }
     print(json.dumps(line))
     pk.finalize()
+
+
+def run_batch(args, cpu):
+    """The batched list: graph replay vs direct persistent launches vs one
+    launch per block, same state, same arithmetic (bit-identical results)."""
+    import piernik_amd as pk
+    from piernik_amd import setups
+
+    sizes = setups.batch_sizes(args.nblocks, seed=0)
+    cfg = setups.batch_config()
+    cfg.fast_math = not args.exact
+    pk.init(cfg)
+    t0 = time.perf_counter()
+    setups.batch_blocks_device(sizes, cfg, seed=0)
+    pk.synchronize()
+    t_setup = time.perf_counter() - t0
+    cells = sum(n ** 3 for n in sizes)
+    print(f"[bench] {len(sizes)} blocks, {cells} cells generated on the GPU in {t_setup:.1f} s", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    if not args.no_autotune:
+        pk.autotune_all()
+    t_tune = time.perf_counter() - t0
+    print(f"[bench] autotune {t_tune:.1f} s", file=sys.stderr, flush=True)
+    modes = [("graph", pk.BATCH_GRAPH), ("persistent", pk.BATCH_PERSISTENT), ("per_block", pk.BATCH_PER_BLOCK)]
+    for name, mode in modes:
+        pk.run_all(max(1, args.warmup), mode)
+    ms = {}
+    t_sim = dt_last = None
+    for name, mode in modes:
+        pk.synchronize()
+        t0 = time.perf_counter()
+        t_sim, dt_last = pk.run_all(args.steps, mode)
+        pk.synchronize()
+        ms[name] = (time.perf_counter() - t0) / args.steps * 1e3
+        print(f"[bench] {name}: {ms[name]:.2f} ms/step", file=sys.stderr, flush=True)
+    pk.profile_begin()
+    pk.run_all(1, pk.BATCH_PERSISTENT)
+    prof = pk.profile_end()
+    launches = sum(prof["launches"])
+    kms = sum(prof["ms"])
+    algo = sum(prof["algo_bytes"])
+    achieved = (algo / (kms * 1e-3)) / 1e9 if kms > 0 else 0.0
+    value = cells * 2 / (ms["graph"] * 1e-3)
+    rep_sizes = {str(n): sizes.count(n) for n in sorted(set(sizes))}
+    return {
+        "metric": "FP64 two-fluid cell-updates/s, batched list of 4096 uneven blocks",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms["graph"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{len(sizes)} periodic blocks, edges {rep_sizes} (seed 0), nb=4, ionized MHD (HLLD+GLM) "
+                         "+ neutral HD (HLLC) per block, van Leer, RK2, CFL 0.7, rho, p = 1 +- 0.2, |v| <= 0.5"),
+            "cells": cells,
+            "nvar": 14,
+            "step": "time_step + fluid_update_full over the list (6 sweeps x 2 fluids)",
+            "sim_time": t_sim,
+            "last_dt": dt_last,
+            "parallelism": "1 GPU",
+            "arithmetic": ("FP64 with FMA contraction and fast division/sqrt" if not args.exact else
+                           "FP64, every operation rounded as the reference's code (bit-exact)"),
+            "setup_s": t_setup,
+            "autotune_s": t_tune,
+        },
+        "batching": {
+            "ms_per_step": ms,
+            "speedup_graph_vs_per_block": ms["per_block"] / ms["graph"],
+            "speedup_persistent_vs_per_block": ms["per_block"] / ms["persistent"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "k_sweep_list (persistent, fused RK2; MHD and HD launches), x/y/z",
+            "algo_bytes_per_launch": algo / launches if launches else None,
+            "avg_launch_ms": kms / launches if launches else None,
+            "per_dir_gbps": [(prof["algo_bytes"][d] / (prof["ms"][d] * 1e-3) / 1e9) if prof["ms"][d] > 0 else None
+                             for d in range(3)],
+        },
+        "cpu_baseline": cpu,
+    }
 
 
 if __name__ == "__main__":

@@ -59,7 +59,9 @@ typedef struct pk_fluid {
 
 /* The &NUMERICAL_SETUP knobs the sweep reads (src/base/global.F90:118-120,205-262). */
 typedef struct pk_config {
-    int32_t nfluids;           /* 1 (this build); the magnetized fluid, if any, is fluid 0 */
+    int32_t nfluids;           /* 1 or 2 (e.g. ionized + neutral); at most one magnetized
+                                  (solve_cg_riemann.F90:85-87).  Fluid f occupies components
+                                  5f..5f+4 of u, as flind%all_fluids(f)%fl%beg */
     int32_t limiter;           /* 'limiter'   -> PK_LIM_* */
     int32_t limiter_b;         /* 'limiter_b' -> PK_LIM_* */
     int32_t integration_order; /* 1 (EULER) or 2 (RK2) */
@@ -84,8 +86,14 @@ typedef struct pk_config {
 typedef struct pk_stage_report {
     int32_t dn_negative;
     int32_t ei_negative;
-    double magic_mass;     /* mass added by the density floor, interior cells */
+    double magic_mass[2];  /* per fluid: mass added by the density floor, interior cells
+                              (local_magic_mass, mass_defect.F90) */
 } pk_stage_report;
+
+/* Execution of the block-list entry points (pk_*_all). */
+#define PK_BATCH_GRAPH 0       /* one persistent launch per sweep and fluid, the step captured in a HIP graph */
+#define PK_BATCH_PERSISTENT 1  /* the same launches issued directly on the stream */
+#define PK_BATCH_PER_BLOCK 2   /* one launch per block, sweep and fluid, same tile shapes */
 
 /* ---- life cycle ---------------------------------------------------------- */
 
@@ -109,6 +117,10 @@ int pk_download(int64_t cg_id);
 /* Same from / to caller-given host arrays in the reference layout. */
 int pk_upload_from(int64_t cg_id, const double *u, const double *b, const double *psi);
 int pk_download_to(int64_t cg_id, double *u, double *b, double *psi);
+/* Same from / to caller-given DEVICE arrays in the reference layout (e.g.
+ * tensors of shape (nzg, nyg, nxg, nvar) on this GPU); no host staging. */
+int pk_upload_device(int64_t cg_id, const double *u, const double *b, const double *psi);
+int pk_download_device(int64_t cg_id, double *u, double *b, double *psi);
 
 /* ---- the operator --------------------------------------------------------- */
 
@@ -152,6 +164,22 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset);
 /* Wait for all queued work. */
 int pk_synchronize(void);
 
+/* ---- the whole block list --------------------------------------------------- */
+/* The registered blocks (ascending cg_id) advanced together, each periodic on
+ * itself, with one dt for all: the block loops of sweep()
+ * (sweeps.F90:230-260), time_step (timestep.F90:112-240: min over blocks),
+ * update_chspeed (hdc.F90:112-140: min over blocks) and glmdamping
+ * (hdc.F90:313-331: one factor from the max over blocks).  RK2 only.
+ * mode is a PK_BATCH_* value; every mode gives bit-identical results. */
+
+/* timestep_fluid over the list: dt = cfl * min over blocks, fluids, cells and
+ * directions of dl/c; c_out = max c. */
+int pk_timestep_all(double *dt_out, double *c_out);
+/* fluid_update_full over the list with the given dt. */
+int pk_fluid_update_all(int32_t mode, double dt);
+/* nsteps main-loop iterations over the list (as pk_run for one block). */
+int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out);
+
 /* ---- tuning (no counterpart in the reference) ------------------------------ */
 
 /* Time candidate tile shapes of the fused sweep on this block (the state is
@@ -160,6 +188,16 @@ int pk_synchronize(void);
  * tiles_out (may be NULL) receives (pencils, cells) for x, y, z; 0 for an
  * inactive direction.  A shape given in pk_config overrides the tuned one. */
 int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6]);
+
+/* Tile shapes of the persistent list sweep: per direction and fluid kind the
+ * workgroup size (128, 256 or 512 lanes) and, per block shape, the tile
+ * length, timed on the registered list (the state is read, never advanced;
+ * the floor bookkeeping of every block is reset).  chspeed <= 0: the
+ * list's own update_chspeed for this dt. */
+int pk_autotune_all(double dt, double chspeed);
+/* The list sweep's workgroup lanes and tile (pencils, cells) for one block,
+ * direction and fluid: out = {lanes, pencils, cells}. */
+int pk_list_tiles(int32_t ddim, int32_t ifluid, int64_t cg_id, int32_t out[3]);
 
 /* ---- measurement (no counterpart in the reference) ------------------------ */
 

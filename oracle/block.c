@@ -313,3 +313,59 @@ double or_run(const or_params *p, or_block *cg, int nsteps, or_report *rep)
     }
     return t;
 }
+
+/* ---- the block list ---------------------------------------------------- */
+
+/* fluid_update_full over a list of blocks, each periodic on itself: one
+ * update_chspeed over the list (hdc.F90:112-140, min over blocks), the sweep
+ * loops of make_3sweeps/sweep over every block (sweeps.F90:217-272), and
+ * glmdamping with one factor from the max over blocks (hdc.F90:313-331). */
+void or_fluid_update_list(const or_params *p, or_block *const *cgs, int nblk, double dt, or_report *reps)
+{
+    int mhd = any_mag(p);
+    double ch = DBL_MAX, fac = 1.0;
+    if (mhd) {
+        for (int i = 0; i < nblk; i++) {
+            double c = or_update_chspeed(p, cgs[i], dt);
+            if (c < ch) ch = c;
+        }
+        double f = 0.0;
+        for (int i = 0; i < nblk; i++) {
+            double r = (p->glm_alpha * ch) / (min_dl(p, cgs[i]) / dt);
+            if (r > f) f = r;
+        }
+        fac = exp(-f);
+    } else {
+        ch = 0.0;
+    }
+    for (int half = 0; half < 2; half++) {
+        for (int s = 0; s < 3; s++) {
+            int d = half == 0 ? s : 2 - s;
+            if (!p->has_dir[d]) continue;
+            for (int i = 0; i < nblk; i++) sweep(p, cgs[i], d, dt, ch, &reps[i]);
+        }
+        if (mhd)
+            for (int i = 0; i < nblk; i++) or_glmdamping(p, cgs[i], fac);
+    }
+}
+
+/* nsteps main-loop iterations over the list: time_step takes the min over
+ * blocks (timestep.F90:112-225), then fluid_update over the list. */
+double or_run_list(const or_params *p, or_block *const *cgs, int nblk, int nsteps, or_report *reps)
+{
+    double t = 0.0, dt = -1.0;
+    for (int i = 0; i < nblk; i++) or_fill_ghosts_periodic(p, cgs[i]);
+    for (int s = 0; s < nsteps; s++) {
+        double dtn = DBL_MAX;
+        for (int i = 0; i < nblk; i++) {
+            double c;
+            double d = or_time_step(p, cgs[i], s, dt, &c);
+            if (d < dtn) dtn = d;
+        }
+        dt = dtn;
+        or_fluid_update_list(p, cgs, nblk, dt, reps);
+        t = t + dt;
+        t = t + dt;
+    }
+    return t;
+}

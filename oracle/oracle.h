@@ -93,6 +93,8 @@ double or_time_step(const or_params *p, const or_block *cg, int nstep, double dt
                     double *c_all);
 void or_fluid_update(const or_params *p, or_block *cg, double dt, or_report *rep);
 double or_run(const or_params *p, or_block *cg, int nsteps, or_report *rep);
+void or_fluid_update_list(const or_params *p, or_block *const *cgs, int nblk, double dt, or_report *reps);
+double or_run_list(const or_params *p, or_block *const *cgs, int nblk, int nsteps, or_report *reps);
 
 #ifdef __cplusplus
 }

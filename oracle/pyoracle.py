@@ -81,6 +81,9 @@ def lib():
         L.or_fluid_update.argtypes = [P(Params), P(Block), ctypes.c_double, P(Report)]
         L.or_run.argtypes = [P(Params), P(Block), ctypes.c_int, P(Report)]
         L.or_run.restype = ctypes.c_double
+        L.or_fluid_update_list.argtypes = [P(Params), P(P(Block)), ctypes.c_int, ctypes.c_double, P(Report)]
+        L.or_run_list.argtypes = [P(Params), P(P(Block)), ctypes.c_int, ctypes.c_int, P(Report)]
+        L.or_run_list.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -91,13 +94,17 @@ def _dp(a):
 
 def params(gamma=5.0 / 3.0, magnetized=False, limiter="vanleer", limiter_b=None, integration_order=2,
            smalld=1e-10, smallei=1e-10, use_smalld=True, use_smallei=True, cfl=0.7, cfl_glm=None,
-           glm_alpha=0.1, has_dir=(True, True, True)):
+           glm_alpha=0.1, has_dir=(True, True, True), fluids=None):
+    """fluids: (gamma, magnetized) per fluid; default the one fluid (gamma, magnetized)."""
+    if fluids is None:
+        fluids = [(gamma, magnetized)]
     p = Params()
-    p.nfluids = 1
-    p.fl[0].gam = gamma
-    p.fl[0].magnetized = int(magnetized)
-    p.fl[0].beg = 0
-    p.nvar = 5
+    p.nfluids = len(fluids)
+    for f, (g, m) in enumerate(fluids):
+        p.fl[f].gam = g
+        p.fl[f].magnetized = int(m)
+        p.fl[f].beg = 5 * f
+    p.nvar = 5 * len(fluids)
     p.limiter = LIM[limiter]
     p.limiter_b = LIM[limiter if limiter_b is None else limiter_b]
     p.integration_order = integration_order
@@ -116,17 +123,17 @@ def params_from_config(cfg):
     return params(gamma=cfg.gamma, magnetized=cfg.magnetized, limiter=cfg.limiter, limiter_b=cfg.limiter_b,
                   integration_order=cfg.integration_order, smalld=cfg.smalld, smallei=cfg.smallei,
                   use_smalld=cfg.use_smalld, use_smallei=cfg.use_smallei, cfl=cfg.cfl, cfl_glm=cfg.cfl_glm,
-                  glm_alpha=cfg.glm_alpha, has_dir=cfg.has_dir)
+                  glm_alpha=cfg.glm_alpha, has_dir=cfg.has_dir, fluids=cfg.fluids)
 
 
 class OBlock:
     """Host block in the reference layout, driven by the oracle."""
 
-    def __init__(self, n, nb, dl, magnetized, has_dir=(True, True, True)):
+    def __init__(self, n, nb, dl, magnetized, has_dir=(True, True, True), nvar=5):
         self.n, self.nb, self.dl = tuple(n), nb, tuple(dl)
         g = tuple(nb if h else 0 for h in has_dir)
         self.ng = tuple(n[d] + 2 * g[d] for d in range(3))
-        self.u = np.zeros((5,) + self.ng, order="F")
+        self.u = np.zeros((nvar,) + self.ng, order="F")
         self.uh = np.zeros_like(self.u)
         self.b = np.zeros((3,) + self.ng, order="F") if magnetized else None
         self.bh = np.zeros_like(self.b) if magnetized else None
@@ -136,7 +143,7 @@ class OBlock:
 
     @classmethod
     def from_cg(cls, cg):
-        o = cls(cg.n, cg.nb, cg.dl, cg.magnetized, cg.has_dir)
+        o = cls(cg.n, cg.nb, cg.dl, cg.magnetized, cg.has_dir, nvar=cg.u.shape[0])
         o.u[...] = cg.u
         if cg.magnetized:
             o.b[...] = cg.b
@@ -197,6 +204,36 @@ def fluid_update(p, ob, dt):
 def run(p, ob, nsteps):
     blk = ob.c()
     return lib().or_run(ctypes.byref(p), ctypes.byref(blk), nsteps, ctypes.byref(ob.rep))
+
+
+def _block_array(obs):
+    blks = [ob.c() for ob in obs]
+    arr = (P(Block) * len(blks))(*[ctypes.pointer(b) for b in blks])
+    reps = (Report * len(obs))()
+    return blks, arr, reps
+
+
+def _store_reports(obs, reps):
+    for ob, r in zip(obs, reps):
+        ob.rep.dn_negative |= r.dn_negative
+        ob.rep.ei_negative |= r.ei_negative
+        for f in range(2):
+            ob.rep.magic_mass[f] += r.magic_mass[f]
+
+
+def fluid_update_list(p, obs, dt):
+    """fluid_update_full over a list of blocks (each periodic on itself)."""
+    blks, arr, reps = _block_array(obs)
+    lib().or_fluid_update_list(ctypes.byref(p), arr, len(obs), dt, reps)
+    _store_reports(obs, reps)
+
+
+def run_list(p, obs, nsteps):
+    """nsteps main-loop iterations over a list of blocks with one dt."""
+    blks, arr, reps = _block_array(obs)
+    t = lib().or_run_list(ctypes.byref(p), arr, len(obs), nsteps, reps)
+    _store_reports(obs, reps)
+    return t
 
 
 # ---- pencil-level helpers (column-major (n, nv) arrays) -------------------

@@ -13,6 +13,8 @@ libpiernik_hip.so (include/piernik_hip.h).  Names follow the reference:
   time_step(cg, ...)     timestep_fluid + caps (timestep.F90:112-240)
   fluid_update(cg, dt)   fluid_update_full (fluidupdate.F90:99-136)
   run(cg, nsteps)        the main loop (piernik.F90:124-170)
+  run_all(nsteps, mode)  the main loop over every registered block (the block
+                         loops of sweeps.F90:230-260), batched on the device
 
 Errors raise PiernikError with the library's message, where the reference
 calls die().  There is no CPU fallback: without the built library every call
@@ -28,12 +30,14 @@ __all__ = [
     "PiernikError", "Config", "GridContainer", "load_library", "init", "finalize",
     "solve_cg", "sweep", "update_boundaries", "glmdamping", "timestep_fluid", "time_step",
     "update_chspeed", "glm_factor", "fluid_update", "run", "get_report", "synchronize",
-    "profile_begin", "profile_end", "autotune",
-    "EULER", "RK2_1", "RK2_2", "LIMITERS",
+    "profile_begin", "profile_end", "autotune", "timestep_all", "fluid_update_all", "run_all",
+    "autotune_all", "list_tiles",
+    "EULER", "RK2_1", "RK2_2", "LIMITERS", "BATCH_GRAPH", "BATCH_PERSISTENT", "BATCH_PER_BLOCK",
 ]
 
 EULER, RK2_1, RK2_2 = 10000, 10001, 10002          # src/base/constants.F90:230-232
 LIMITERS = {"vanleer": 0, "minmod": 1, "moncen": 2, "superbee": 3}  # fluxlimiters.F90:90-103
+BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_BLOCK = 0, 1, 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpiernik_hip.so")
@@ -64,7 +68,8 @@ class _ProfStats(ctypes.Structure):
 
 
 class _Report(ctypes.Structure):
-    _fields_ = [("dn_negative", ctypes.c_int32), ("ei_negative", ctypes.c_int32), ("magic_mass", ctypes.c_double)]
+    _fields_ = [("dn_negative", ctypes.c_int32), ("ei_negative", ctypes.c_int32),
+                ("magic_mass", ctypes.c_double * 2)]
 
 
 _lib = None
@@ -79,6 +84,8 @@ _PROTOS = {
     "pk_download": [ctypes.c_int64],
     "pk_upload_from": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "pk_download_to": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "pk_upload_device": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "pk_download_device": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "pk_solve_cg": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
     "pk_sweep": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
     "pk_fill_ghosts": [ctypes.c_int64],
@@ -89,9 +96,40 @@ _PROTOS = {
     "pk_get_report": [ctypes.c_int64, ctypes.POINTER(_Report), ctypes.c_int32],
     "pk_synchronize": [],
     "pk_autotune": [ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int32)],
+    "pk_timestep_all": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "pk_fluid_update_all": [ctypes.c_int32, ctypes.c_double],
+    "pk_run_all": [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "pk_autotune_all": [ctypes.c_double, ctypes.c_double],
+    "pk_list_tiles": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)],
     "pk_profile_begin": [],
     "pk_profile_end": [ctypes.POINTER(_ProfStats)],
 }
+
+
+def _bind_torch_runtime():
+    """Bind to PyTorch's HIP runtime when PyTorch is installed.
+
+    PyTorch ships its own libamdhip64 (same soname as ROCm's).  If this
+    library loaded ROCm's first, a later `import torch` would bring a second
+    HIP runtime into the process and PyTorch would find no GPU.  Loading
+    PyTorch's copy first (without importing torch) makes both use one
+    runtime, so device tensors can be handed to pk_upload_device.  Set
+    PIERNIK_HIP_RUNTIME=system to keep ROCm's runtime (no PyTorch in the
+    process)."""
+    if os.environ.get("PIERNIK_HIP_RUNTIME", "") == "system":
+        return None
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    rt = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if not os.path.exists(rt):
+        return None
+    ctypes.CDLL(rt, mode=ctypes.RTLD_GLOBAL)
+    return rt
 
 
 def load_library(path=None):
@@ -102,6 +140,7 @@ def load_library(path=None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise PiernikError(f"{p} not found: build it with `python -m piernik_amd.build`")
+    _bind_torch_runtime()
     lib = ctypes.CDLL(p)
     for name, args in _PROTOS.items():
         fn = getattr(lib, name)
@@ -131,9 +170,14 @@ class Config:
     def __init__(self, gamma=5.0 / 3.0, magnetized=False, limiter="vanleer", limiter_b=None,
                  integration_order=2, cfl=0.7, cfl_glm=None, glm_alpha=0.1, smalld=1e-10, smallei=1e-10,
                  use_smalld=True, use_smallei=True, has_dir=(True, True, True), tile_x=(0, 0), tile_yz=(0, 0),
-                 fast_math=False):
-        self.gamma = float(gamma)
-        self.magnetized = bool(magnetized)
+                 fast_math=False, fluids=None):
+        # fluids: (gamma, magnetized) per fluid in flind order (at most two,
+        # at most one magnetized); by default the one fluid (gamma, magnetized)
+        if fluids is None:
+            fluids = [(gamma, magnetized)]
+        self.fluids = [(float(g), bool(m)) for g, m in fluids]
+        self.gamma = self.fluids[0][0]
+        self.magnetized = any(m for _, m in self.fluids)
         self.limiter = limiter
         self.limiter_b = limiter if limiter_b is None else limiter_b
         self.integration_order = int(integration_order)
@@ -150,19 +194,26 @@ class Config:
         self.fast_math = bool(fast_math)
 
     @property
+    def nfluids(self):
+        return len(self.fluids)
+
+    @property
     def nvar(self):
-        return 5
+        return 5 * self.nfluids
 
     def to_c(self):
         c = _Config()
-        c.nfluids = 1
+        if not 1 <= self.nfluids <= 2:
+            raise PiernikError(f"{self.nfluids} fluids: one or two are supported")
+        c.nfluids = self.nfluids
         if self.limiter not in LIMITERS or self.limiter_b not in LIMITERS:
             raise PiernikError(f"unknown limiter {self.limiter!r}/{self.limiter_b!r}")
         c.limiter = LIMITERS[self.limiter]
         c.limiter_b = LIMITERS[self.limiter_b]
         c.integration_order = self.integration_order
-        c.fluid[0].gamma = self.gamma
-        c.fluid[0].magnetized = int(self.magnetized)
+        for f, (g, m) in enumerate(self.fluids):
+            c.fluid[f].gamma = g
+            c.fluid[f].magnetized = int(m)
         c.smalld, c.smallei = self.smalld, self.smallei
         c.use_smalld, c.use_smallei = int(self.use_smalld), int(self.use_smallei)
         c.cfl, c.cfl_glm, c.glm_alpha = self.cfl, self.cfl_glm, self.glm_alpha
@@ -188,13 +239,14 @@ def sanitize_smallx(u, b=None, gamma=5.0 / 3.0, nb=4, has_dir=(True, True, True)
 class GridContainer:
     """One grid block ("cg") with host arrays in the reference layout.
 
-    u(5, nxg, nyg, nzg), b(3, nxg, nyg, nzg), psi(nxg, nyg, nzg), Fortran
-    order, ghost cells included (nxg = nx + 2*nb in active directions).
+    u(nvar, nxg, nyg, nzg), b(3, nxg, nyg, nzg), psi(nxg, nyg, nzg), Fortran
+    order, ghost cells included (nxg = nx + 2*nb in active directions);
+    nvar = 5 per fluid, fluid f at components 5f..5f+4.
     """
 
     _next_id = 1
 
-    def __init__(self, n, nb=4, dl=None, magnetized=False, has_dir=(True, True, True), cg_id=None):
+    def __init__(self, n, nb=4, dl=None, magnetized=False, has_dir=(True, True, True), cg_id=None, nfluids=1):
         self.n = tuple(int(x) for x in n)
         self.nb = int(nb)
         self.has_dir = tuple(bool(h) for h in has_dir)
@@ -204,7 +256,9 @@ class GridContainer:
             dl = tuple(1.0 / self.n[d] for d in range(3))
         self.dl = tuple(float(x) for x in dl)
         self.magnetized = bool(magnetized)
-        self.u = np.zeros((5,) + self.ng, dtype=np.float64, order="F")
+        self.nfluids = int(nfluids)
+        self.nvar = 5 * self.nfluids
+        self.u = np.zeros((self.nvar,) + self.ng, dtype=np.float64, order="F")
         self.uh = np.zeros_like(self.u)
         self.b = np.zeros((3,) + self.ng, dtype=np.float64, order="F") if magnetized else None
         self.bh = np.zeros_like(self.b) if magnetized else None
@@ -239,10 +293,28 @@ class GridContainer:
     def download(self):
         _call("pk_download", self.cg_id)
 
+    def upload_device(self, u, b=None, psi=None):
+        """Copy from device tensors in the reference layout (torch tensors of
+        shape (nzg, nyg, nxg, nvar), (nzg, nyg, nxg, 3), (nzg, nyg, nxg),
+        float64, contiguous, on this GPU) without host staging."""
+        self.register()
+        ptr = lambda t: None if t is None else ctypes.c_void_p(_device_ptr(t))
+        _call("pk_upload_device", self.cg_id, ptr(u), ptr(b), ptr(psi))
+
+    def download_device(self, u, b=None, psi=None):
+        ptr = lambda t: None if t is None else ctypes.c_void_p(_device_ptr(t))
+        _call("pk_download_device", self.cg_id, ptr(u), ptr(b), ptr(psi))
+
     def unregister(self):
         if self.registered:
             _call("pk_unregister_block", self.cg_id)
             self.registered = False
+
+
+def _device_ptr(t):
+    if not (t.is_cuda and t.is_contiguous() and str(t.dtype) == "torch.float64"):
+        raise PiernikError("device arrays must be contiguous float64 tensors on the GPU")
+    return t.data_ptr()
 
 
 _cfg = None
@@ -327,7 +399,8 @@ def run(cg, nsteps):
 def get_report(cg, reset=False):
     r = _Report()
     _call("pk_get_report", cg.cg_id, ctypes.byref(r), int(bool(reset)))
-    return {"dn_negative": bool(r.dn_negative), "ei_negative": bool(r.ei_negative), "magic_mass": r.magic_mass}
+    return {"dn_negative": bool(r.dn_negative), "ei_negative": bool(r.ei_negative),
+            "magic_mass": list(r.magic_mass)}
 
 
 def synchronize():
@@ -344,6 +417,43 @@ def autotune(cg, dt=None, chspeed=None):
     out = (ctypes.c_int32 * 6)()
     _call("pk_autotune", cg.cg_id, float(dt), float(chspeed), out)
     return {d: (out[2 * d], out[2 * d + 1]) for d in range(3) if cg.has_dir[d]}
+
+
+def timestep_all():
+    """(dt, c_max) of timestep_fluid over every registered block and fluid."""
+    dt, c = ctypes.c_double(), ctypes.c_double()
+    _call("pk_timestep_all", ctypes.byref(dt), ctypes.byref(c))
+    return dt.value, c.value
+
+
+def fluid_update_all(dt, mode=BATCH_GRAPH):
+    """fluid_update_full over every registered block (each periodic on itself)."""
+    _call("pk_fluid_update_all", int(mode), float(dt))
+
+
+def run_all(nsteps, mode=BATCH_GRAPH):
+    """nsteps main-loop iterations over every registered block with one dt;
+    returns (t, last dt).  mode: BATCH_GRAPH (persistent launches replayed
+    from a HIP graph), BATCH_PERSISTENT, or BATCH_PER_BLOCK (one launch per
+    block); all give identical results."""
+    t, dt = ctypes.c_double(), ctypes.c_double()
+    _call("pk_run_all", int(mode), int(nsteps), ctypes.byref(t), ctypes.byref(dt))
+    return t.value, dt.value
+
+
+def autotune_all(dt=None, chspeed=0.0):
+    """Tune the list sweep's workgroup size and per-shape tile lengths
+    (chspeed 0: the list's own update_chspeed for this dt)."""
+    if dt is None:
+        dt = timestep_all()[0]
+    _call("pk_autotune_all", float(dt), float(chspeed))
+
+
+def list_tiles(ddim, ifluid, cg):
+    """(workgroup lanes, pencils, cells) of the list sweep for one block."""
+    out = (ctypes.c_int32 * 3)()
+    _call("pk_list_tiles", int(ddim), int(ifluid), cg.cg_id, out)
+    return tuple(out)
 
 
 def profile_begin():

@@ -1,4 +1,4 @@
-// pk_capi.cpp -- host side of the C-ABI declared in include/piernik_hip.h.
+// pk_capi.hip -- host side of the C-ABI declared in include/piernik_hip.h.
 //
 // Keeps the per-run configuration (the reference's flind / &NUMERICAL_SETUP
 // globals), a registry of grid blocks with device mirrors, and drives the
@@ -6,17 +6,21 @@
 // (pk_fluid_update, pk_run) restates fluid_update_full / make_3sweeps /
 // sweep / time_step (src/scheme/fluidupdate.F90:99-232,
 // src/scheme/sweeps.F90:114-279, src/base/timestep.F90:112-240) for one
-// periodic block with cflcontrol='warn'.
+// periodic block with cflcontrol='warn'; the list driver (pk_*_all) does the
+// same for the whole registered block list, every block periodic on itself,
+// with one persistent launch per sweep and fluid, captured in a HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
-#include <vector>
 #include <string>
+#include <vector>
 
 #include "../../include/piernik_hip.h"
 #include "pk_kernels.hpp"
@@ -27,20 +31,17 @@ struct Block {
     int64_t id = 0;
     int n[3] = {1, 1, 1}, nb = 0, g[3] = {0, 0, 0}, ng[3] = {1, 1, 1};
     double dl[3] = {1, 1, 1};
-    int nplanes = 5;
     long long sv = 0;
     int sy = 0, sz = 0;
     double *buf[2] = {nullptr, nullptr};
-    int xo = 0;  // leading offset that puts every interior row start on a 128 B line
+    int xo = 0;  // leading offset that aligns every interior row start
     int cur = 0;
     int uh = -1;  // buffer holding uh after a first stage, -1 when none
     double *h_u = nullptr, *h_uh = nullptr, *h_b = nullptr, *h_bh = nullptr, *h_psi = nullptr,
            *h_psih = nullptr;
     unsigned *d_flags = nullptr;
-    double *d_dmass = nullptr;
-    double *d_part = nullptr;
+    double *d_dmass = nullptr;  // per fluid
     double *d_dtout = nullptr;
-    double *d_stage = nullptr;  // AoS staging for host transfers
     unsigned long long *d_dtslots = nullptr;  // timestep reduction written by the last sweep of a step
     bool dt_pending = false;                  // d_dtout holds timestep_fluid of the current state
 };
@@ -49,18 +50,26 @@ const int kDtSlots = 1024;
 
 pk_config g_cfg;
 bool g_init = false;
-bool g_mhd = false;
+bool g_mhd = false;       // some fluid is magnetized: B and psi planes exist
+int g_nfl = 1;
+int g_fplane[2] = {0, 5};  // first plane of each fluid's group
+bool g_fmag[2] = {false, false};
+int g_magf = -1;           // the magnetized fluid, -1 if none
+int g_nplanes = 5;
 std::map<int64_t, Block *> g_blocks;
 hipStream_t g_stream = nullptr;
 std::string g_err;
 double *g_pinned = nullptr;  // small pinned buffer for scalar read-backs
+double *g_stage = nullptr;   // AoS staging for host transfers, grown on demand
+size_t g_stage_bytes = 0;
+double *g_part = nullptr;    // partial results of the standalone timestep reduction
 
 // Tile shapes chosen by pk_autotune, per block shape, direction and physics.
 struct TuneKey {
-    int n0, n1, n2, dir, mhd, lim, limb;
+    int n0, n1, n2, dir, kind, lim, limb;
     bool operator<(const TuneKey &o) const
     {
-        const int a[7] = {n0, n1, n2, dir, mhd, lim, limb}, b[7] = {o.n0, o.n1, o.n2, o.dir, o.mhd, o.lim, o.limb};
+        const int a[7] = {n0, n1, n2, dir, kind, lim, limb}, b[7] = {o.n0, o.n1, o.n2, o.dir, o.kind, o.lim, o.limb};
         for (int i = 0; i < 7; i++)
             if (a[i] != b[i]) return a[i] < b[i];
         return false;
@@ -68,9 +77,9 @@ struct TuneKey {
 };
 std::map<TuneKey, std::pair<int, int>> g_tuned;
 
-TuneKey tune_key(const Block *b, int d)
+TuneKey tune_key(const Block *b, int d, bool mag, int nt = 0)
 {
-    return TuneKey{b->n[0], b->n[1], b->n[2], d, (g_mhd ? 1 : 0) + (g_cfg.fast_math ? 2 : 0), g_cfg.limiter,
+    return TuneKey{b->n[0], b->n[1], b->n[2], d, (mag ? 1 : 0) + (g_cfg.fast_math ? 2 : 0) + 4 * nt, g_cfg.limiter,
                    g_cfg.limiter_b};
 }
 
@@ -101,8 +110,19 @@ int fail(int code, const char *fmt, ...)
         if (e_ != hipSuccess) return fail(PK_EHIP, "%s failed: %s", #x, hipGetErrorString(e_));       \
     } while (0)
 
-// Address of cell (0,0,0) of plane 0 of state buffer i.
+#define RCCHK(x)                \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
+// Address of cell (0,0,0) of plane 0 of state buffer i, and of fluid f's group.
 double *state(Block *b, int i) { return b->buf[i] + b->xo; }
+double *fluid_state(Block *b, int i, int f) { return state(b, i) + (long long)g_fplane[f] * b->sv; }
+int nv_of(int f) { return g_fmag[f] ? 9 : 5; }
+
+void list_invalidate();
+void list_touch();
 
 int get_block(int64_t id, Block **b)
 {
@@ -124,7 +144,7 @@ double min_dl(const Block *b)
 int first_stage() { return g_cfg.integration_order == 1 ? PK_EULER : PK_RK2_1; }
 double rk_coef(int istep) { return istep == PK_RK2_1 ? 0.5 : 1.0; }  // constants.F90:237-238
 
-void fill_common(const Block *b, pk::SweepArgs &a)
+void fill_common(const Block *b, int f, pk::SweepArgs &a)
 {
     a.sv = b->sv;
     a.sy = b->sy;
@@ -136,14 +156,14 @@ void fill_common(const Block *b, pk::SweepArgs &a)
     }
     a.lim = g_cfg.limiter;
     a.limb = g_cfg.limiter_b;
-    a.gam = g_cfg.fluid[0].gamma;
-    a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
+    a.gam = g_cfg.fluid[f].gamma;
+    a.gam_1 = g_cfg.fluid[f].gamma - 1.0;
     a.smalld = g_cfg.smalld;
     a.smallei = g_cfg.smallei;
     a.use_smalld = g_cfg.use_smalld;
     a.use_smallei = g_cfg.use_smallei;
     a.flags = b->d_flags;
-    a.dmass = b->d_dmass;
+    a.dmass = b->d_dmass + f;
     a.psi_fac = 1.0;
     a.ch = 0.0;
     a.images = 0;
@@ -155,24 +175,22 @@ void fill_common(const Block *b, pk::SweepArgs &a)
     }
 }
 
-// Tile shape along direction d: TP pencils x TS cells, within the launch
-// bound (512 lanes MHD, 1024 HD); halo = 4 (fused) or 2 (one stage).
 // Tile shape along direction d: TP pencils x TS cells (TP*TS <= 512 lanes),
-// halo 4 (fused) or 2 (one stage).  Precedence: the shape given in pk_config,
-// then the one pk_autotune measured for this block shape, then 3 x 72
-// (4-wave workgroups, two per CU; measured best at 256^3 MHD).
+// halo 4 (fused) or 2 (one stage); pencils of the transverse interior are
+// flattened, so the last tile of a block is the only partial one.
 void tile_geometry(const Block *b, int d, bool fused, int tp, int ts, int &TP, int &TS, int &out, int &chunks,
-                   int &tiles1)
+                   int &ptiles)
 {
     const int maxt = 512;
     const int halo = fused ? 4 : 2;
     const int span = fused ? b->n[d] : b->ng[d];  // cells that receive output
-    const int d1 = (d == 0) ? 1 : 0;
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    const int npen = b->n[d1] * b->n[d2];
     if (ts > span + 2 * halo) ts = span + 2 * halo;
     if (ts > maxt) ts = maxt;
     if (tp <= 0) tp = maxt / ts;
     if (tp < 1) tp = 1;
-    if (tp > b->n[d1]) tp = b->n[d1];
+    if (tp > npen) tp = npen;
     while (tp * ts > maxt && tp > 1) tp--;
     if (tp * ts > maxt) ts = maxt / tp;
     if (ts < 2 * halo + 1) ts = 2 * halo + 1;
@@ -180,15 +198,17 @@ void tile_geometry(const Block *b, int d, bool fused, int tp, int ts, int &TP, i
     TS = ts;
     out = ts - 2 * halo;
     chunks = (span + out - 1) / out;
-    tiles1 = (b->n[d1] + tp - 1) / tp;
+    ptiles = (npen + tp - 1) / tp;
 }
 
-void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, int &chunks, int &tiles1)
+// Precedence: the shape given in pk_config, then the one pk_autotune
+// measured for this block shape, then 3 x 72 (4-wave workgroups, two per CU).
+void choose_tile(const Block *b, int d, bool fused, bool mag, int &TP, int &TS, int &out, int &chunks, int &ptiles)
 {
     const int *ov = (d == 0) ? g_cfg.tile_x : g_cfg.tile_yz;
     int tp = ov[0], ts = ov[1];
     if (ts <= 0 && fused) {
-        auto it = g_tuned.find(tune_key(b, d));
+        auto it = g_tuned.find(tune_key(b, d, mag));
         if (it != g_tuned.end()) {
             tp = it->second.first;
             ts = it->second.second;
@@ -198,13 +218,25 @@ void choose_tile(const Block *b, int d, bool fused, int &TP, int &TS, int &out, 
         ts = 72;
         if (tp <= 0) tp = 3;
     }
-    tile_geometry(b, d, fused, tp, ts, TP, TS, out, chunks, tiles1);
+    tile_geometry(b, d, fused, tp, ts, TP, TS, out, chunks, ptiles);
 }
 
-hipError_t sweep_kernel(int d, bool fused, const pk::SweepArgs &a)
+hipError_t sweep_kernel(int d, bool mag, bool fused, const pk::SweepArgs &a)
 {
-    return g_cfg.fast_math ? pk_fast::launch_sweep(d, g_mhd, fused, a, g_stream)
-                           : pk::launch_sweep(d, g_mhd, fused, a, g_stream);
+    return g_cfg.fast_math ? pk_fast::launch_sweep(d, mag, fused, a, g_stream)
+                           : pk::launch_sweep(d, mag, fused, a, g_stream);
+}
+
+hipError_t list_kernel(int d, bool mag, const pk::ListArgs &L, int nt, int grid)
+{
+    return g_cfg.fast_math ? pk_fast::launch_sweep_list(d, mag, g_cfg.limiter, g_cfg.limiter_b, L, nt, grid, g_stream)
+                           : pk::launch_sweep_list(d, mag, g_cfg.limiter, g_cfg.limiter_b, L, nt, grid, g_stream);
+}
+
+int list_resident(int d, bool mag, int nt)
+{
+    return g_cfg.fast_math ? pk_fast::sweep_list_resident(d, mag, g_cfg.limiter, g_cfg.limiter_b, nt)
+                           : pk::sweep_list_resident(d, mag, g_cfg.limiter, g_cfg.limiter_b, nt);
 }
 
 int take_event(hipEvent_t *e)
@@ -218,20 +250,28 @@ int take_event(hipEvent_t *e)
     return PK_OK;
 }
 
-int launch(int d, bool fused, const pk::SweepArgs &a0, Block *b)
+// Launch one sweep kernel (tile shape chosen here), timed while profiling.
+int launch_timed(int d, double bytes, const std::function<hipError_t()> &go);
+
+int launch(int d, bool fused, const pk::SweepArgs &a0, Block *b, int f)
 {
     pk::SweepArgs a = a0;
-    choose_tile(b, d, fused, a.TP, a.TS, a.out, a.chunks, a.tiles1);
+    choose_tile(b, d, fused, g_fmag[f], a.TP, a.TS, a.out, a.chunks, a.ptiles);
+    const double cells = (double)b->n[0] * b->n[1] * b->n[2];
+    return launch_timed(d, cells * nv_of(f) * 8.0 * 2.0, [&]() { return sweep_kernel(d, g_fmag[f], fused, a); });
+}
+
+int launch_timed(int d, double bytes, const std::function<hipError_t()> &go)
+{
     ProfRec rec{};
     if (g_prof) {
-        int rc;
-        if ((rc = take_event(&rec.start)) || (rc = take_event(&rec.stop))) return rc;
+        RCCHK(take_event(&rec.start));
+        RCCHK(take_event(&rec.stop));
         rec.dir = d;
-        const double cells = (double)b->n[0] * b->n[1] * b->n[2];
-        rec.bytes = cells * b->nplanes * 8.0 * 2.0;
+        rec.bytes = bytes;
         HIPCHK(hipEventRecord(rec.start, g_stream));
     }
-    HIPCHK(sweep_kernel(d, fused, a));
+    HIPCHK(go());
     if (g_prof) {
         HIPCHK(hipEventRecord(rec.stop, g_stream));
         g_prof_recs.push_back(rec);
@@ -270,7 +310,7 @@ int copy_transverse_ghosts(Block *b, int d, const double *src, double *dst)
 {
     const long long ncell = (long long)b->ng[0] * b->ng[1] * b->ng[2];
     hipLaunchKernelGGL(k_copy_transverse_ghosts, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, g_stream, src,
-                       dst, b->sv, b->nplanes, b->sy, b->sz, d, b->ng[0], b->ng[1], b->ng[2], b->g[0], b->g[1],
+                       dst, b->sv, g_nplanes, b->sy, b->sz, d, b->ng[0], b->ng[1], b->ng[2], b->g[0], b->g[1],
                        b->g[2], b->n[0], b->n[1], b->n[2]);
     HIPCHK(hipGetLastError());
     return PK_OK;
@@ -279,69 +319,61 @@ int copy_transverse_ghosts(Block *b, int d, const double *src, double *dst)
 int do_stage(Block *b, int ddim, int istep, double dt, double ch)
 {
     b->dt_pending = false;
-    pk::SweepArgs a{};
-    fill_common(b, a);
-    a.ch = ch;
-    a.c1 = (rk_coef(istep) * dt) / b->dl[ddim];
-    a.c2 = a.c1;
     const bool first = istep == first_stage();
-    double *src = state(b, b->cur);
-    double *other = state(b, 1 - b->cur);
+    if (!first && (b->uh < 0 || b->uh == b->cur))
+        return fail(PK_ESTATE, "stage %d called without a preceding first stage", istep);
+    const int src_i = b->cur, dst_i = first ? 1 - b->cur : b->uh;
+    for (int f = 0; f < g_nfl; f++) {
+        pk::SweepArgs a{};
+        fill_common(b, f, a);
+        a.ch = ch;
+        a.c1 = (rk_coef(istep) * dt) / b->dl[ddim];
+        a.c2 = a.c1;
+        a.src = fluid_state(b, src_i, f);
+        a.base = first ? a.src : fluid_state(b, dst_i, f);
+        a.dst = fluid_state(b, dst_i, f);
+        RCCHK(launch(ddim, false, a, b, f));
+    }
+    RCCHK(copy_transverse_ghosts(b, ddim, state(b, src_i), state(b, dst_i)));
     if (first) {
-        a.src = src;
-        a.base = src;
-        a.dst = other;
-        int rc = launch(ddim, false, a, b);
-        if (rc) return rc;
-        if ((rc = copy_transverse_ghosts(b, ddim, src, other))) return rc;
         b->uh = b->cur;
         b->cur = 1 - b->cur;
     } else {
-        if (b->uh < 0 || b->uh == b->cur)
-            return fail(PK_ESTATE, "stage %d called without a preceding first stage", istep);
-        double *uh = state(b, b->uh);
-        a.src = src;
-        a.base = uh;
-        a.dst = uh;
-        int rc = launch(ddim, false, a, b);
-        if (rc) return rc;
-        if ((rc = copy_transverse_ghosts(b, ddim, src, uh))) return rc;
         b->cur = b->uh;
         b->uh = -1;
     }
     return PK_OK;
 }
 
-// One directional sweep, both RK2 stages.  images: write the periodic ghost
-// images too (the block is periodic on itself); want_dt: reduce the next
-// step's timestep_fluid from the cells written.
+// One directional sweep, both RK2 stages, every fluid.  images: write the
+// periodic ghost images too (the block is periodic on itself); want_dt:
+// reduce the next step's timestep_fluid from the cells written.
 int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool images = false, bool want_dt = false)
 {
     if (g_cfg.integration_order == 1) {
-        int rc = do_stage(b, ddim, PK_EULER, dt, ch);
-        if (rc) return rc;
+        RCCHK(do_stage(b, ddim, PK_EULER, dt, ch));
         if (psi_fac != 1.0 && g_mhd) {
-            HIPCHK(pk::launch_scale(state(b, b->cur) + 8 * b->sv, b->sz * b->ng[2], psi_fac, g_stream));
+            HIPCHK(pk::launch_scale(fluid_state(b, b->cur, g_magf) + 8 * b->sv, (long long)b->sz * b->ng[2], psi_fac,
+                                    g_stream));
         }
         return PK_OK;
     }
     if (b->g[ddim] < 4) return fail(PK_EINVAL, "fused RK2 sweep needs nb >= 4 (nb = %d)", b->g[ddim]);
-    pk::SweepArgs a{};
-    fill_common(b, a);
-    a.ch = ch;
-    a.c1 = (0.5 * dt) / b->dl[ddim];
-    a.c2 = (1.0 * dt) / b->dl[ddim];
-    a.psi_fac = psi_fac;
-    a.src = state(b, b->cur);
-    a.base = a.src;
-    a.dst = state(b, 1 - b->cur);
-    a.images = images ? 1 : 0;
-    if (want_dt) {
-        HIPCHK(pk::launch_dtslots_init(b->d_dtslots, kDtSlots, g_stream));
-        a.dtslots = b->d_dtslots;
+    if (want_dt) HIPCHK(pk::launch_dtslots_init(b->d_dtslots, kDtSlots, g_stream));
+    for (int f = 0; f < g_nfl; f++) {
+        pk::SweepArgs a{};
+        fill_common(b, f, a);
+        a.ch = ch;
+        a.c1 = (0.5 * dt) / b->dl[ddim];
+        a.c2 = (1.0 * dt) / b->dl[ddim];
+        a.psi_fac = psi_fac;
+        a.src = fluid_state(b, b->cur, f);
+        a.base = a.src;
+        a.dst = fluid_state(b, 1 - b->cur, f);
+        a.images = images ? 1 : 0;
+        a.dtslots = want_dt ? b->d_dtslots : nullptr;
+        RCCHK(launch(ddim, true, a, b, f));
     }
-    int rc = launch(ddim, true, a, b);
-    if (rc) return rc;
     b->cur = 1 - b->cur;
     b->uh = -1;
     b->dt_pending = false;
@@ -365,27 +397,18 @@ int do_fill(Block *b)
         a.n[d] = b->n[d];
     }
     a.plane0 = 0;
-    a.nplanes = b->nplanes;
+    a.nplanes = g_nplanes;
     for (int d = 0; d < 3; d++)
         if (b->g[d] > 0) HIPCHK(pk::launch_fill_ghosts(d, a, g_stream));
     return PK_OK;
 }
 
-int do_timestep(Block *b, double *dt_out, double *c_out)
+// timestep_fluid of every fluid of one block, min over fluids; with
+// cfl > 0 the min commutes with the multiplication by cfl, so one combined
+// reduction over all fluids' cells gives the same dt.
+int enqueue_dt(Block *b, double *out)
 {
-    double dt = DBL_MAX, c = 0.0;
-    if (b->dt_pending && g_cfg.nfluids == 1) {
-        // reduced by the last sweep from the cells it wrote (same formula, same state)
-        HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtout, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-        HIPCHK(hipStreamSynchronize(g_stream));
-        double m = g_pinned[0];
-        if (g_pinned[1] < m) m = g_pinned[1];
-        if (g_pinned[2] < m) m = g_pinned[2];
-        *dt_out = g_cfg.cfl * m;
-        *c_out = g_pinned[3];
-        return PK_OK;
-    }
-    for (int f = 0; f < g_cfg.nfluids; f++) {
+    for (int f = 0; f < g_nfl; f++) {
         pk::DtArgs a{};
         a.u = state(b, b->cur);
         a.sv = b->sv;
@@ -398,40 +421,69 @@ int do_timestep(Block *b, double *dt_out, double *c_out)
             a.has_dir[d] = g_cfg.has_dir[d];
             a.dl[d] = b->dl[d];
         }
-        a.plane0 = 5 * f;
-        a.bplane = 5;
-        a.magnetized = g_cfg.fluid[f].magnetized;
+        a.plane0 = g_fplane[f];
+        a.bplane = g_magf >= 0 ? g_fplane[g_magf] + 5 : 0;
+        a.magnetized = g_fmag[f];
         a.gam = g_cfg.fluid[f].gamma;
         a.gam_1 = g_cfg.fluid[f].gamma - 1.0;
-        HIPCHK(pk::launch_dt(a, b->d_part, b->d_dtout, g_stream));
-        HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtout, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-        HIPCHK(hipStreamSynchronize(g_stream));
-        double m = g_pinned[0];
-        if (g_pinned[1] < m) m = g_pinned[1];
-        if (g_pinned[2] < m) m = g_pinned[2];
+        HIPCHK(pk::launch_dt(a, g_part, out + 4 * f, g_stream));
+    }
+    return PK_OK;
+}
+
+void combine_dt(const double *v, int nsets, double *dt_out, double *c_out)
+{
+    double dt = DBL_MAX, c = 0.0;
+    for (int f = 0; f < nsets; f++) {
+        double m = v[4 * f + 0];
+        if (v[4 * f + 1] < m) m = v[4 * f + 1];
+        if (v[4 * f + 2] < m) m = v[4 * f + 2];
         const double dtf = g_cfg.cfl * m;  // timestep.F90:492
         if (dtf < dt) dt = dtf;
-        if (g_pinned[3] > c) c = g_pinned[3];
+        if (v[4 * f + 3] > c) c = v[4 * f + 3];
     }
     *dt_out = dt;
     *c_out = c;
+}
+
+int do_timestep(Block *b, double *dt_out, double *c_out)
+{
+    if (b->dt_pending) {
+        // reduced by the last sweep from the cells it wrote (same formula, same state)
+        HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtout, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        combine_dt(g_pinned, 1, dt_out, c_out);
+        return PK_OK;
+    }
+    RCCHK(enqueue_dt(b, b->d_dtout));  // d_dtout holds 4 doubles per fluid
+    HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtout, 4 * g_nfl * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    combine_dt(g_pinned, g_nfl, dt_out, c_out);
     return PK_OK;
+}
+
+// update_chspeed with ch_grid (hdc.F90:112-122) and the glmdamping factor
+// (hdc.F90:313-331) over blocks with the given smallest cell sizes.
+void glm_scalars(const std::vector<double> &mdls, double dt, double *ch_out, double *fac_out)
+{
+    double ch = DBL_MAX;
+    for (double mdl : mdls) {
+        const double c = (g_cfg.cfl_glm * mdl) / dt;
+        if (c < ch) ch = c;
+    }
+    double f = 0.0;
+    for (double mdl : mdls) {
+        const double ff = (g_cfg.glm_alpha * ch) / (mdl / dt);
+        if (ff > f) f = ff;
+    }
+    *ch_out = ch;
+    *fac_out = std::exp(-f);
 }
 
 int do_fluid_update(Block *b, double dt)
 {
     double ch = 0.0, fac = 1.0;
-    if (g_mhd) {
-        // update_chspeed with ch_grid (hdc.F90:112-122) and the glmdamping factor (hdc.F90:313-331)
-        const double mdl = min_dl(b);
-        ch = DBL_MAX;
-        const double c = (g_cfg.cfl_glm * mdl) / dt;
-        if (c < ch) ch = c;
-        double f = 0.0;
-        const double ff = (g_cfg.glm_alpha * ch) / (mdl / dt);
-        if (ff > f) f = ff;
-        fac = std::exp(-f);
-    }
+    if (g_mhd) glm_scalars({min_dl(b)}, dt, &ch, &fac);
     for (int half = 0; half < 2; half++) {
         int last = -1;
         for (int s = 0; s < 3; s++) {
@@ -441,16 +493,14 @@ int do_fluid_update(Block *b, double dt)
         for (int s = 0; s < 3; s++) {
             const int d = half == 0 ? s : 2 - s;
             if (!g_cfg.has_dir[d]) continue;
-            int rc;
             if (g_cfg.integration_order == 2) {
                 // the sweep refreshes the periodic ghosts itself; the last one of
                 // the step also reduces the next step's timestep
-                rc = do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0, true, half == 1 && d == last);
+                RCCHK(do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0, true, half == 1 && d == last));
             } else {
-                rc = do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0);
-                if (!rc) rc = do_fill(b);
+                RCCHK(do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0));
+                RCCHK(do_fill(b));
             }
-            if (rc) return rc;
         }
     }
     return PK_OK;
@@ -458,27 +508,26 @@ int do_fluid_update(Block *b, double dt)
 
 int alloc_block(Block *b)
 {
-    b->xo = (16 - b->g[0] % 16) % 16;
-    const int pitch = (b->xo + b->ng[0] + 15) / 16 * 16;
+    // interior rows start on a 128 B line for wide blocks, 64 B for narrow ones
+    const int al = b->ng[0] >= 128 ? 16 : 8;
+    b->xo = (al - b->g[0] % al) % al;
+    const int pitch = (b->xo + b->ng[0] + al - 1) / al * al;
     b->sy = pitch;
     b->sz = pitch * b->ng[1];
     long long sv = (long long)b->xo + (long long)b->sz * b->ng[2];
     sv = (sv + 31) / 32 * 32;
     b->sv = sv;
-    const size_t bytes = sizeof(double) * (size_t)sv * b->nplanes;
+    const size_t bytes = sizeof(double) * (size_t)sv * g_nplanes;
     for (int i = 0; i < 2; i++) {
         HIPCHK(hipMalloc((void **)&b->buf[i], bytes));
         HIPCHK(hipMemsetAsync(b->buf[i], 0, bytes, g_stream));
     }
     HIPCHK(hipMalloc((void **)&b->d_flags, sizeof(unsigned)));
-    HIPCHK(hipMalloc((void **)&b->d_dmass, sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->d_dmass, 2 * sizeof(double)));
     HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
-    HIPCHK(hipMemsetAsync(b->d_dmass, 0, sizeof(double), g_stream));
-    HIPCHK(hipMalloc((void **)&b->d_part, sizeof(double) * 4 * 2048));
-    HIPCHK(hipMalloc((void **)&b->d_dtout, sizeof(double) * 4));
+    HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
+    HIPCHK(hipMalloc((void **)&b->d_dtout, sizeof(double) * 8));
     HIPCHK(hipMalloc((void **)&b->d_dtslots, sizeof(unsigned long long) * 4 * kDtSlots));
-    const size_t ncell = (size_t)b->ng[0] * b->ng[1] * b->ng[2];
-    HIPCHK(hipMalloc((void **)&b->d_stage, sizeof(double) * ncell * 5 * g_cfg.nfluids));
     return PK_OK;
 }
 
@@ -488,51 +537,454 @@ void free_block(Block *b)
         if (b->buf[i]) (void)hipFree(b->buf[i]);
     if (b->d_flags) (void)hipFree(b->d_flags);
     if (b->d_dmass) (void)hipFree(b->d_dmass);
-    if (b->d_part) (void)hipFree(b->d_part);
     if (b->d_dtout) (void)hipFree(b->d_dtout);
-    if (b->d_stage) (void)hipFree(b->d_stage);
     if (b->d_dtslots) (void)hipFree(b->d_dtslots);
     delete b;
 }
 
-int upload_array(Block *b, const double *h, int nc, int plane0, double *dst)
+int ensure_stage(size_t bytes)
+{
+    if (bytes <= g_stage_bytes) return PK_OK;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (g_stage) (void)hipFree(g_stage);
+    g_stage = nullptr;
+    g_stage_bytes = 0;
+    HIPCHK(hipMalloc((void **)&g_stage, bytes));
+    g_stage_bytes = bytes;
+    return PK_OK;
+}
+
+// Plane maps of the reference arrays: u(nvar) with fluid f at components
+// 5f..5f+4 (flind%all_fluids(f)%fl%beg), b(3) and psi of the magnetized fluid.
+pk::PlaneMap map_u()
+{
+    pk::PlaneMap m{};
+    m.nc = 5 * g_nfl;
+    for (int f = 0; f < g_nfl; f++)
+        for (int k = 0; k < 5; k++) m.planes[5 * f + k] = g_fplane[f] + k;
+    return m;
+}
+pk::PlaneMap map_b()
+{
+    pk::PlaneMap m{};
+    m.nc = 3;
+    for (int k = 0; k < 3; k++) m.planes[k] = g_fplane[g_magf] + 5 + k;
+    return m;
+}
+pk::PlaneMap map_psi()
+{
+    pk::PlaneMap m{};
+    m.nc = 1;
+    m.planes[0] = g_fplane[g_magf] + 8;
+    return m;
+}
+
+// from_device: the source array is in device memory (no host staging)
+int upload_array(Block *b, const double *h, const pk::PlaneMap &m, double *dst, bool from_device)
 {
     const size_t ncell = (size_t)b->ng[0] * b->ng[1] * b->ng[2];
-    HIPCHK(hipMemcpyAsync(b->d_stage, h, sizeof(double) * ncell * nc, hipMemcpyHostToDevice, g_stream));
-    HIPCHK(pk::launch_aos_to_soa(b->d_stage, nc, dst, b->sv, plane0, b->ng, b->sy, b->sz, g_stream));
+    const double *src = h;
+    if (!from_device) {
+        RCCHK(ensure_stage(sizeof(double) * ncell * m.nc));
+        HIPCHK(hipMemcpyAsync(g_stage, h, sizeof(double) * ncell * m.nc, hipMemcpyHostToDevice, g_stream));
+        src = g_stage;
+    }
+    HIPCHK(pk::launch_aos_to_soa(src, m, dst, b->sv, b->ng, b->sy, b->sz, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
 }
 
-int download_array(Block *b, double *h, int nc, int plane0, const double *src)
+int download_array(Block *b, double *h, const pk::PlaneMap &m, const double *src, bool to_device)
 {
     const size_t ncell = (size_t)b->ng[0] * b->ng[1] * b->ng[2];
-    HIPCHK(pk::launch_soa_to_aos(src, b->sv, plane0, nc, b->d_stage, b->ng, b->sy, b->sz, g_stream));
-    HIPCHK(hipMemcpyAsync(h, b->d_stage, sizeof(double) * ncell * nc, hipMemcpyDeviceToHost, g_stream));
+    if (to_device) {
+        HIPCHK(pk::launch_soa_to_aos(src, b->sv, m, h, b->ng, b->sy, b->sz, g_stream));
+    } else {
+        RCCHK(ensure_stage(sizeof(double) * ncell * m.nc));
+        HIPCHK(pk::launch_soa_to_aos(src, b->sv, m, g_stage, b->ng, b->sy, b->sz, g_stream));
+        HIPCHK(hipMemcpyAsync(h, g_stage, sizeof(double) * ncell * m.nc, hipMemcpyDeviceToHost, g_stream));
+    }
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
 }
 
-int upload_state(Block *b, const double *u, const double *bb, const double *psi, double *dst)
+int upload_state(Block *b, const double *u, const double *bb, const double *psi, double *dst, bool dev = false)
 {
-    int rc;
     b->dt_pending = false;
-    if (u && (rc = upload_array(b, u, 5 * g_cfg.nfluids, 0, dst))) return rc;
+    if (u) RCCHK(upload_array(b, u, map_u(), dst, dev));
     if (g_mhd) {
-        if (bb && (rc = upload_array(b, bb, 3, 5, dst))) return rc;
-        if (psi && (rc = upload_array(b, psi, 1, 8, dst))) return rc;
+        if (bb) RCCHK(upload_array(b, bb, map_b(), dst, dev));
+        if (psi) RCCHK(upload_array(b, psi, map_psi(), dst, dev));
     }
     return PK_OK;
 }
 
-int download_state(Block *b, double *u, double *bb, double *psi, const double *src)
+int download_state(Block *b, double *u, double *bb, double *psi, const double *src, bool dev = false)
 {
-    int rc;
-    if (u && (rc = download_array(b, u, 5 * g_cfg.nfluids, 0, src))) return rc;
+    if (u) RCCHK(download_array(b, u, map_u(), src, dev));
     if (g_mhd) {
-        if (bb && (rc = download_array(b, bb, 3, 5, src))) return rc;
-        if (psi && (rc = download_array(b, psi, 1, 8, src))) return rc;
+        if (bb) RCCHK(download_array(b, bb, map_b(), src, dev));
+        if (psi) RCCHK(download_array(b, psi, map_psi(), src, dev));
     }
+    return PK_OK;
+}
+
+// ---------------------------------------------------------------- block list
+//
+// The whole registered list (ascending cg_id), swept by one persistent
+// launch per direction and fluid.  Per block and (direction, fluid) the
+// host builds the kernel arguments once; the per-step scalars live in
+// device memory (StepScalars), so the step's launches are captured once in a
+// HIP graph and replayed.
+
+struct LaunchTab {
+    int nt = 256;                       // workgroup lanes of the persistent launch
+    std::vector<pk::SweepArgs> h_args;  // per block, src = buffer of the block's cur at build time
+    std::vector<unsigned> h_first;      // per block first item, nblocks + 1 entries
+    pk::SweepArgs *d_args = nullptr;
+    unsigned *d_first = nullptr;
+    unsigned *d_items = nullptr;
+    size_t cap_blocks = 0, cap_items = 0;
+    unsigned nitems = 0;
+    int grid = 0;
+};
+
+const int kQueues = 16;
+
+struct ListState {
+    bool built = false;
+    std::vector<Block *> blocks;
+    std::vector<int> cur0;     // each block's cur when the tables were built
+    std::vector<double> mdl;   // min(dl) per block
+    LaunchTab tab[3][2];
+    int nt_pref[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // workgroup lanes chosen by pk_autotune_all (0: default)
+    unsigned *d_queue = nullptr;
+    pk::StepScalars *d_step = nullptr;
+    pk::StepScalars *h_step = nullptr;  // pinned
+    unsigned long long *d_dtslots = nullptr;
+    double *d_dtout = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool fresh = false;  // ghosts current and d_dtout holds the timestep of the current state
+};
+ListState g_list;
+
+void destroy_graph()
+{
+    if (g_list.exec) (void)hipGraphExecDestroy(g_list.exec);
+    if (g_list.graph) (void)hipGraphDestroy(g_list.graph);
+    g_list.exec = nullptr;
+    g_list.graph = nullptr;
+}
+
+void list_invalidate()
+{
+    g_list.built = false;
+    g_list.fresh = false;
+    destroy_graph();
+}
+
+void list_touch() { g_list.fresh = false; }
+
+void free_tab(LaunchTab &t)
+{
+    if (t.d_args) (void)hipFree(t.d_args);
+    if (t.d_first) (void)hipFree(t.d_first);
+    if (t.d_items) (void)hipFree(t.d_items);
+    t = LaunchTab();
+}
+
+void list_free()
+{
+    (void)hipStreamSynchronize(g_stream);
+    destroy_graph();
+    for (auto &row : g_list.tab)
+        for (auto &t : row) free_tab(t);
+    if (g_list.d_queue) (void)hipFree(g_list.d_queue);
+    if (g_list.d_step) (void)hipFree(g_list.d_step);
+    if (g_list.h_step) (void)hipHostFree(g_list.h_step);
+    if (g_list.d_dtslots) (void)hipFree(g_list.d_dtslots);
+    if (g_list.d_dtout) (void)hipFree(g_list.d_dtout);
+    g_list = ListState();
+}
+
+// Default list tile along d for nt lanes: the split of the pencil into
+// chunks that needs the fewest nt-lane tiles; ties go to the longer tile.
+int list_default_ts(const Block *b, int d, int nt)
+{
+    const int span = b->n[d];
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    const long long npen = (long long)b->n[d1] * b->n[d2];
+    long long best = -1;
+    int best_ts = 0;
+    for (int k = 1; k <= span; k++) {
+        const int out = (span + k - 1) / k;
+        const int ts = out + 8;
+        if (ts > nt) continue;
+        const int tp = nt / ts;
+        const long long tiles = (long long)((span + out - 1) / out) * ((npen + tp - 1) / tp);
+        if (best < 0 || tiles < best) {
+            best = tiles;
+            best_ts = ts;
+        }
+        if (out <= 1) break;
+    }
+    return best_ts > 0 ? best_ts : 9;
+}
+
+int list_ts(const Block *b, int d, int f, int nt)
+{
+    auto it = g_tuned.find(tune_key(b, d, g_fmag[f], nt));
+    if (it != g_tuned.end()) return it->second.second;
+    return list_default_ts(b, d, nt);
+}
+
+// Kernel arguments and item table of one persistent launch over `blks`.
+int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int nt,
+              const std::vector<int> *ts_override = nullptr)
+{
+    t.nt = nt;
+    t.h_args.resize(blks.size());
+    t.h_first.resize(blks.size() + 1);
+    unsigned items = 0;
+    for (size_t i = 0; i < blks.size(); i++) {
+        Block *b = blks[i];
+        pk::SweepArgs &a = t.h_args[i];
+        a = pk::SweepArgs{};
+        fill_common(b, f, a);
+        a.src = fluid_state(b, b->cur, f);
+        a.base = a.src;
+        a.dst = fluid_state(b, 1 - b->cur, f);
+        a.images = 1;
+        const int ts = ts_override ? (*ts_override)[i] : list_ts(b, d, f, nt);
+        tile_geometry(b, d, true, nt / ts, ts, a.TP, a.TS, a.out, a.chunks, a.ptiles);
+        if (a.TP * a.TS > nt) return fail(PK_EINVAL, "tile %d x %d exceeds %d lanes", a.TP, a.TS, nt);
+        t.h_first[i] = items;
+        const unsigned long long n = (unsigned long long)a.chunks * (unsigned long long)a.ptiles;
+        if (items + n > 0xffffffffull) return fail(PK_EINVAL, "block list has too many tiles");
+        items += (unsigned)n;
+    }
+    t.h_first[blks.size()] = items;
+    t.nitems = items;
+    if (t.cap_blocks < blks.size() + 1) {
+        if (t.d_args) (void)hipFree(t.d_args);
+        if (t.d_first) (void)hipFree(t.d_first);
+        t.d_args = nullptr;
+        t.d_first = nullptr;
+        HIPCHK(hipMalloc((void **)&t.d_args, sizeof(pk::SweepArgs) * (blks.size() + 1)));
+        HIPCHK(hipMalloc((void **)&t.d_first, sizeof(unsigned) * (blks.size() + 1)));
+        t.cap_blocks = blks.size() + 1;
+    }
+    if (t.cap_items < items) {
+        if (t.d_items) (void)hipFree(t.d_items);
+        t.d_items = nullptr;
+        HIPCHK(hipMalloc((void **)&t.d_items, sizeof(unsigned) * items));
+        t.cap_items = items;
+    }
+    HIPCHK(hipMemcpyAsync(t.d_args, t.h_args.data(), sizeof(pk::SweepArgs) * blks.size(), hipMemcpyHostToDevice,
+                          g_stream));
+    HIPCHK(hipMemcpyAsync(t.d_first, t.h_first.data(), sizeof(unsigned) * (blks.size() + 1), hipMemcpyHostToDevice,
+                          g_stream));
+    HIPCHK(pk::launch_fill_items(t.d_first, (int)blks.size(), t.d_items, g_stream));
+    t.grid = list_resident(d, g_fmag[f], nt);
+    if (t.grid <= 0) return fail(PK_EHIP, "no resident workgroups for the list sweep (nt = %d)", nt);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return PK_OK;
+}
+
+int list_build()
+{
+    if (g_list.built) {
+        bool same = g_list.blocks.size() == g_blocks.size();
+        for (size_t i = 0; same && i < g_list.blocks.size(); i++) same = g_list.blocks[i]->cur == g_list.cur0[i];
+        if (same) return PK_OK;
+        destroy_graph();
+    }
+    if (g_blocks.empty()) return fail(PK_ESTATE, "no registered blocks");
+    if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "the batched list sweep is the fused RK2 sweep");
+    g_list.blocks.clear();
+    g_list.cur0.clear();
+    g_list.mdl.clear();
+    for (auto &kv : g_blocks) {
+        Block *b = kv.second;
+        for (int d = 0; d < 3; d++)
+            if (g_cfg.has_dir[d] && b->g[d] < 4)
+                return fail(PK_EINVAL, "block %lld: the fused RK2 sweep needs nb >= 4", (long long)b->id);
+        g_list.blocks.push_back(b);
+        g_list.cur0.push_back(b->cur);
+        g_list.mdl.push_back(min_dl(b));
+    }
+    for (int d = 0; d < 3; d++) {
+        if (!g_cfg.has_dir[d]) continue;
+        for (int f = 0; f < g_nfl; f++) {
+            const int nt = g_list.nt_pref[d][g_fmag[f]] > 0 ? g_list.nt_pref[d][g_fmag[f]] : 256;
+            RCCHK(build_tab(g_list.tab[d][f], g_list.blocks, d, f, nt));
+        }
+    }
+    if (!g_list.d_queue) HIPCHK(hipMalloc((void **)&g_list.d_queue, sizeof(unsigned) * kQueues));
+    if (!g_list.d_step) HIPCHK(hipMalloc((void **)&g_list.d_step, sizeof(pk::StepScalars)));
+    if (!g_list.h_step) HIPCHK(hipHostMalloc((void **)&g_list.h_step, sizeof(pk::StepScalars), hipHostMallocDefault));
+    if (!g_list.d_dtslots) HIPCHK(hipMalloc((void **)&g_list.d_dtslots, sizeof(unsigned long long) * 4 * kDtSlots));
+    if (!g_list.d_dtout) HIPCHK(hipMalloc((void **)&g_list.d_dtout, sizeof(double) * 4));
+    g_list.built = true;
+    return PK_OK;
+}
+
+// The sweeps of one step in order: (direction, apply psi damping, reduce dt).
+struct SweepStep {
+    int d;
+    bool fac, dt;
+};
+std::vector<SweepStep> step_sweeps()
+{
+    std::vector<SweepStep> v;
+    for (int half = 0; half < 2; half++) {
+        int last = -1;
+        for (int s = 0; s < 3; s++) {
+            const int d = half == 0 ? s : 2 - s;
+            if (g_cfg.has_dir[d]) last = d;
+        }
+        for (int s = 0; s < 3; s++) {
+            const int d = half == 0 ? s : 2 - s;
+            if (g_cfg.has_dir[d]) v.push_back({d, d == last, half == 1 && d == last});
+        }
+    }
+    return v;
+}
+
+// Enqueue the launches of one step on g_stream.  mode PK_BATCH_PER_BLOCK
+// launches the single-block sweep once per block with the same tile shapes;
+// the persistent modes launch k_sweep_list once per sweep and fluid.
+int enqueue_step(int mode, const pk::StepScalars &sc)
+{
+    const std::vector<SweepStep> sw = step_sweeps();
+    const bool persistent = mode != PK_BATCH_PER_BLOCK;
+    if (sw.size() * (size_t)g_nfl > (size_t)kQueues) return fail(PK_EINVAL, "too many launches per step");
+    if (persistent) HIPCHK(pk::launch_zero_u32(g_list.d_queue, kQueues, g_stream));
+    HIPCHK(pk::launch_dtslots_init(g_list.d_dtslots, kDtSlots, g_stream));
+    int q = 0;
+    for (size_t k = 0; k < sw.size(); k++) {
+        const int d = sw[k].d;
+        const int parity = (int)(k % 2);
+        for (int f = 0; f < g_nfl; f++) {
+            LaunchTab &t = g_list.tab[d][f];
+            double bytes = 0.0;
+            for (Block *b : g_list.blocks) bytes += (double)b->n[0] * b->n[1] * b->n[2] * nv_of(f) * 16.0;
+            if (persistent) {
+                pk::ListArgs L{};
+                L.args = t.d_args;
+                L.first = t.d_first;
+                L.items = t.d_items;
+                L.nitems = t.nitems;
+                L.queue = g_list.d_queue + q++;
+                L.parity = parity;
+                L.apply_fac = sw[k].fac && g_fmag[f];
+                L.step = g_list.d_step;
+                L.dtslots = sw[k].dt ? g_list.d_dtslots : nullptr;
+                L.nslots = kDtSlots;
+                if (mode == PK_BATCH_GRAPH) {
+                    HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
+                } else {
+                    RCCHK(launch_timed(d, bytes, [&]() { return list_kernel(d, g_fmag[f], L, t.nt, t.grid); }));
+                }
+            } else {
+                for (size_t i = 0; i < g_list.blocks.size(); i++) {
+                    pk::SweepArgs a = t.h_args[i];
+                    if (parity) {
+                        const double *s0 = a.src;
+                        a.src = a.dst;
+                        a.dst = const_cast<double *>(s0);
+                    }
+                    a.base = a.src;
+                    a.c1 = (0.5 * sc.dt) / a.dl[d];
+                    a.c2 = (1.0 * sc.dt) / a.dl[d];
+                    a.ch = sc.ch;
+                    a.psi_fac = (sw[k].fac && g_fmag[f]) ? sc.psi_fac : 1.0;
+                    a.dtslots = sw[k].dt ? g_list.d_dtslots : nullptr;
+                    HIPCHK(sweep_kernel(d, g_fmag[f], true, a));
+                }
+            }
+        }
+    }
+    HIPCHK(pk::launch_dtslots_final(g_list.d_dtslots, kDtSlots, g_list.d_dtout, g_stream));
+    return PK_OK;
+}
+
+int ensure_graph()
+{
+    if (g_list.exec) return PK_OK;
+    // kernel attributes were set when the tables were built (outside capture)
+    pk::StepScalars sc{};
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(PK_BATCH_GRAPH, sc);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(g_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIPCHK(e);
+    g_list.graph = g;
+    HIPCHK(hipGraphInstantiate(&g_list.exec, g_list.graph, nullptr, nullptr, 0));
+    return PK_OK;
+}
+
+// Ghost refresh of every block and the list timestep of the current state.
+int list_prepare(double *dt_out, double *c_out)
+{
+    double dt = DBL_MAX, c = 0.0;
+    for (Block *b : g_list.blocks) {
+        RCCHK(do_fill(b));
+        double dtb, cb;
+        RCCHK(do_timestep(b, &dtb, &cb));
+        if (dtb < dt) dt = dtb;
+        if (cb > c) c = cb;
+    }
+    *dt_out = dt;
+    *c_out = c;
+    return PK_OK;
+}
+
+int list_dt(double *dt_out, double *c_out)
+{
+    if (g_list.fresh) {
+        HIPCHK(hipMemcpyAsync(g_pinned, g_list.d_dtout, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        combine_dt(g_pinned, 1, dt_out, c_out);
+        return PK_OK;
+    }
+    return list_prepare(dt_out, c_out);
+}
+
+int list_step(int mode, double dt)
+{
+    pk::StepScalars sc{};
+    sc.dt = dt;
+    sc.ch = 0.0;
+    sc.psi_fac = 1.0;
+    if (g_mhd) glm_scalars(g_list.mdl, dt, &sc.ch, &sc.psi_fac);
+    // the pinned slot may still feed the previous step's copy
+    HIPCHK(hipStreamSynchronize(g_stream));
+    *g_list.h_step = sc;
+    HIPCHK(hipMemcpyAsync(g_list.d_step, g_list.h_step, sizeof sc, hipMemcpyHostToDevice, g_stream));
+    if (mode == PK_BATCH_GRAPH) {
+        RCCHK(ensure_graph());
+        HIPCHK(hipGraphLaunch(g_list.exec, g_stream));
+    } else {
+        RCCHK(enqueue_step(mode, sc));
+    }
+    for (Block *b : g_list.blocks) {
+        b->uh = -1;
+        b->dt_pending = false;
+    }
+    g_list.fresh = true;
+    return PK_OK;
+}
+
+int check_mode(int mode)
+{
+    if (mode != PK_BATCH_GRAPH && mode != PK_BATCH_PERSISTENT && mode != PK_BATCH_PER_BLOCK)
+        return fail(PK_EINVAL, "unknown batch mode %d", mode);
     return PK_OK;
 }
 
@@ -545,26 +997,49 @@ const char *pk_last_error(void) { return g_err.c_str(); }
 int pk_init(const pk_config *cfg)
 {
     if (!cfg) return fail(PK_EINVAL, "null config");
-    if (cfg->nfluids != 1) return fail(PK_EINVAL, "nfluids = %d: this build runs one fluid per block", cfg->nfluids);
+    if (cfg->nfluids < 1 || cfg->nfluids > 2) return fail(PK_EINVAL, "nfluids = %d (1 or 2)", cfg->nfluids);
     if (cfg->limiter < 0 || cfg->limiter > 3 || cfg->limiter_b < 0 || cfg->limiter_b > 3)
         return fail(PK_EINVAL, "unknown limiter");
     if (cfg->integration_order != 1 && cfg->integration_order != 2)
         return fail(PK_EINVAL, "integration_order must be 1 or 2");
-    if (!(cfg->fluid[0].gamma > 1.0)) return fail(PK_EINVAL, "adiabatic fluid needs gamma > 1");
+    int nmag = 0;
+    for (int f = 0; f < cfg->nfluids; f++) {
+        if (!(cfg->fluid[f].gamma > 1.0)) return fail(PK_EINVAL, "adiabatic fluid %d needs gamma > 1", f);
+        if (cfg->fluid[f].magnetized) nmag++;
+    }
+    // solve_cg_riemann.F90:85-87: at most one magnetized fluid
+    if (nmag > 1) return fail(PK_EINVAL, "at most one fluid may be magnetized");
     if (!(cfg->has_dir[0] || cfg->has_dir[1] || cfg->has_dir[2])) return fail(PK_EINVAL, "no active direction");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(PK_EHIP, "no HIP device");
+    if (g_init) {
+        for (auto &kv : g_blocks) free_block(kv.second);
+        g_blocks.clear();
+        list_free();
+    }
     if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    if (!g_pinned) HIPCHK(hipHostMalloc((void **)&g_pinned, 8 * sizeof(double), hipHostMallocDefault));
+    if (!g_pinned) HIPCHK(hipHostMalloc((void **)&g_pinned, 16 * sizeof(double), hipHostMallocDefault));
+    if (!g_part) HIPCHK(hipMalloc((void **)&g_part, sizeof(double) * 4 * 2048));
     g_cfg = *cfg;
-    g_mhd = cfg->fluid[0].magnetized != 0;
+    g_nfl = cfg->nfluids;
+    g_magf = -1;
+    int p = 0;
+    for (int f = 0; f < g_nfl; f++) {
+        g_fmag[f] = cfg->fluid[f].magnetized != 0;
+        g_fplane[f] = p;
+        p += g_fmag[f] ? 9 : 5;
+        if (g_fmag[f]) g_magf = f;
+    }
+    g_nplanes = p;
+    g_mhd = g_magf >= 0;
     g_init = true;
     return PK_OK;
 }
 
 int pk_finalize(void)
 {
+    list_free();
     for (auto &kv : g_blocks) free_block(kv.second);
     g_blocks.clear();
     if (g_stream) {
@@ -576,6 +1051,11 @@ int pk_finalize(void)
         (void)hipHostFree(g_pinned);
         g_pinned = nullptr;
     }
+    if (g_stage) (void)hipFree(g_stage);
+    g_stage = nullptr;
+    g_stage_bytes = 0;
+    if (g_part) (void)hipFree(g_part);
+    g_part = nullptr;
     g_init = false;
     return PK_OK;
 }
@@ -603,12 +1083,15 @@ int pk_register_block(int64_t cg_id, const int32_t n[3], int32_t nb, const doubl
             delete blk;
             return fail(PK_EINVAL, "n[%d] = %d smaller than nb = %d", d, n[d], nb);
         }
+        if (!(dl[d] > 0.0)) {
+            delete blk;
+            return fail(PK_EINVAL, "dl[%d] must be positive", d);
+        }
         blk->n[d] = n[d];
         blk->g[d] = g_cfg.has_dir[d] ? nb : 0;
         blk->ng[d] = n[d] + 2 * blk->g[d];
         blk->dl[d] = dl[d];
     }
-    blk->nplanes = g_mhd ? 9 : 5;
     blk->h_u = u;
     blk->h_uh = uh;
     blk->h_b = b;
@@ -621,15 +1104,16 @@ int pk_register_block(int64_t cg_id, const int32_t n[3], int32_t nb, const doubl
         return rc;
     }
     g_blocks[cg_id] = blk;
+    list_invalidate();
     return PK_OK;
 }
 
 int pk_unregister_block(int64_t cg_id)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     (void)hipStreamSynchronize(g_stream);
+    list_invalidate();
     g_blocks.erase(cg_id);
     free_block(b);
     return PK_OK;
@@ -638,20 +1122,19 @@ int pk_unregister_block(int64_t cg_id)
 int pk_upload(int64_t cg_id)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (!b->h_u) return fail(PK_EINVAL, "block %lld has no host arrays", (long long)cg_id);
     if (g_mhd && (!b->h_b || !b->h_psi)) return fail(PK_EINVAL, "magnetized block needs b and psi host arrays");
+    list_touch();
     return upload_state(b, b->h_u, b->h_b, b->h_psi, state(b, b->cur));
 }
 
 int pk_download(int64_t cg_id)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (!b->h_u) return fail(PK_EINVAL, "block %lld has no host arrays", (long long)cg_id);
-    if ((rc = download_state(b, b->h_u, b->h_b, b->h_psi, state(b, b->cur)))) return rc;
+    RCCHK(download_state(b, b->h_u, b->h_b, b->h_psi, state(b, b->cur)));
     if (b->uh >= 0 && b->h_uh) return download_state(b, b->h_uh, b->h_bh, b->h_psih, state(b, b->uh));
     return PK_OK;
 }
@@ -659,65 +1142,79 @@ int pk_download(int64_t cg_id)
 int pk_upload_from(int64_t cg_id, const double *u, const double *bb, const double *psi)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (!u || (g_mhd && (!bb || !psi))) return fail(PK_EINVAL, "missing host array");
+    list_touch();
     return upload_state(b, u, bb, psi, state(b, b->cur));
 }
 
 int pk_download_to(int64_t cg_id, double *u, double *bb, double *psi)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     return download_state(b, u, bb, psi, state(b, b->cur));
+}
+
+int pk_upload_device(int64_t cg_id, const double *u, const double *bb, const double *psi)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    if (!u || (g_mhd && (!bb || !psi))) return fail(PK_EINVAL, "missing device array");
+    list_touch();
+    return upload_state(b, u, bb, psi, state(b, b->cur), true);
+}
+
+int pk_download_device(int64_t cg_id, double *u, double *bb, double *psi)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    return download_state(b, u, bb, psi, state(b, b->cur), true);
 }
 
 int pk_solve_cg(int64_t cg_id, int32_t ddim, int32_t istep, double dt, double chspeed)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
     const bool ok = (g_cfg.integration_order == 1) ? istep == PK_EULER : (istep == PK_RK2_1 || istep == PK_RK2_2);
     if (!ok) return fail(PK_EINVAL, "istep %d does not match integration_order %d", istep, g_cfg.integration_order);
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    list_touch();
     return do_stage(b, ddim, istep, dt, chspeed);
 }
 
 int pk_sweep(int64_t cg_id, int32_t ddim, double dt, double chspeed, double psi_fac)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    list_touch();
     return do_sweep(b, ddim, dt, chspeed, psi_fac);
 }
 
 int pk_fill_ghosts(int64_t cg_id)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     return do_fill(b);
 }
 
 int pk_glm_damp(int64_t cg_id, double fac)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (!g_mhd) return PK_OK;
-    HIPCHK(pk::launch_scale(state(b, b->cur) + 8 * b->sv, (long long)b->sz * b->ng[2], fac, g_stream));
+    list_touch();
+    b->dt_pending = false;
+    HIPCHK(pk::launch_scale(fluid_state(b, b->cur, g_magf) + 8 * b->sv, (long long)b->sz * b->ng[2], fac, g_stream));
     return PK_OK;
 }
 
 int pk_timestep_fluid(int64_t cg_id, double *dt_out, double *c_out)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (!dt_out || !c_out) return fail(PK_EINVAL, "null output");
     return do_timestep(b, dt_out, c_out);
 }
@@ -725,29 +1222,29 @@ int pk_timestep_fluid(int64_t cg_id, double *dt_out, double *c_out)
 int pk_fluid_update(int64_t cg_id, double dt)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    list_touch();
     return do_fluid_update(b, dt);
 }
 
 int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (nsteps < 0) return fail(PK_EINVAL, "nsteps < 0");
+    list_touch();
     double t = 0.0, dt = -1.0, dt_old = -1.0;
-    if ((rc = do_fill(b))) return rc;
+    RCCHK(do_fill(b));
     for (int s = 0; s < nsteps; s++) {
         double c;
-        if ((rc = do_timestep(b, &dt, &c))) return rc;
+        RCCHK(do_timestep(b, &dt, &c));
         // time_step caps (timestep.F90:200-208) with dt_initial = -1, dt_max_grow = 2
         if (s >= 1 && dt_old > 0.0) {
             const double cap = dt_old * 2.0;
             if (cap < dt) dt = cap;
         }
-        if ((rc = do_fluid_update(b, dt))) return rc;
+        RCCHK(do_fluid_update(b, dt));
         t = t + dt;
         t = t + dt;
         dt_old = dt;
@@ -761,12 +1258,11 @@ int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out)
 int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset)
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     unsigned fl = 0;
-    double dm = 0.0;
+    double dm[2] = {0.0, 0.0};
     HIPCHK(hipMemcpyAsync(&fl, b->d_flags, sizeof fl, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(&dm, b->d_dmass, sizeof dm, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(dm, b->d_dmass, sizeof dm, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     if (rep) {
         double dvol = 1.0;
@@ -774,11 +1270,11 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset)
             if (g_cfg.has_dir[d]) dvol *= b->dl[d];
         rep->dn_negative = (fl & 1u) ? 1 : 0;
         rep->ei_negative = (fl & 2u) ? 1 : 0;
-        rep->magic_mass = dm * dvol;
+        for (int f = 0; f < 2; f++) rep->magic_mass[f] = f < g_nfl ? dm[f] * dvol : 0.0;
     }
     if (reset) {
         HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
-        HIPCHK(hipMemsetAsync(b->d_dmass, 0, sizeof(double), g_stream));
+        HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
         HIPCHK(hipStreamSynchronize(g_stream));
     }
     return PK_OK;
@@ -787,8 +1283,7 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset)
 int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
 {
     Block *b;
-    int rc = get_block(cg_id, &b);
-    if (rc) return rc;
+    RCCHK(get_block(cg_id, &b));
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
     if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "autotuning covers the fused RK2 sweep only");
     hipEvent_t e0, e1;
@@ -800,50 +1295,52 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
         if (tiles_out) tiles_out[2 * d] = tiles_out[2 * d + 1] = 0;
         if (!g_cfg.has_dir[d]) continue;
         if (b->g[d] < 4) return fail(PK_EINVAL, "fused RK2 sweep needs nb >= 4");
-        pk::SweepArgs a{};
-        fill_common(b, a);
-        a.ch = chspeed;
-        a.c1 = (0.5 * dt) / b->dl[d];
-        a.c2 = (1.0 * dt) / b->dl[d];
-        a.src = state(b, b->cur);
-        a.base = a.src;
-        a.dst = state(b, 1 - b->cur);
-        std::vector<std::pair<int, int>> seen;
-        float best = 1e30f;
-        std::pair<int, int> best_t(0, 0);
-        for (int ts : ts_list) {
-            for (int w : waves_list) {
-                int TP, TS, out, chunks, tiles1;
-                int tp = (64 * w) / (ts < b->n[d] + 8 ? ts : b->n[d] + 8);
-                if (tp < 1) continue;
-                tile_geometry(b, d, true, tp, ts, TP, TS, out, chunks, tiles1);
-                if (TP * TS > 512) continue;
-                bool dup = false;
-                for (auto &p : seen) dup = dup || (p.first == TP && p.second == TS);
-                if (dup) continue;
-                seen.push_back({TP, TS});
-                a.TP = TP;
-                a.TS = TS;
-                a.out = out;
-                a.chunks = chunks;
-                a.tiles1 = tiles1;
-                HIPCHK(sweep_kernel(d, true, a));  // warm-up
-                HIPCHK(hipEventRecord(e0, g_stream));
-                for (int rep = 0; rep < 3; rep++) HIPCHK(sweep_kernel(d, true, a));
-                HIPCHK(hipEventRecord(e1, g_stream));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    best_t = {TP, TS};
+        for (int f = 0; f < g_nfl; f++) {
+            pk::SweepArgs a{};
+            fill_common(b, f, a);
+            a.ch = chspeed;
+            a.c1 = (0.5 * dt) / b->dl[d];
+            a.c2 = (1.0 * dt) / b->dl[d];
+            a.src = fluid_state(b, b->cur, f);
+            a.base = a.src;
+            a.dst = fluid_state(b, 1 - b->cur, f);
+            std::vector<std::pair<int, int>> seen;
+            float best = 1e30f;
+            std::pair<int, int> best_t(0, 0);
+            for (int ts : ts_list) {
+                for (int w : waves_list) {
+                    int TP, TS, out, chunks, ptiles;
+                    int tp = (64 * w) / (ts < b->n[d] + 8 ? ts : b->n[d] + 8);
+                    if (tp < 1) continue;
+                    tile_geometry(b, d, true, tp, ts, TP, TS, out, chunks, ptiles);
+                    if (TP * TS > 512) continue;
+                    bool dup = false;
+                    for (auto &p : seen) dup = dup || (p.first == TP && p.second == TS);
+                    if (dup) continue;
+                    seen.push_back({TP, TS});
+                    a.TP = TP;
+                    a.TS = TS;
+                    a.out = out;
+                    a.chunks = chunks;
+                    a.ptiles = ptiles;
+                    HIPCHK(sweep_kernel(d, g_fmag[f], true, a));  // warm-up
+                    HIPCHK(hipEventRecord(e0, g_stream));
+                    for (int rep = 0; rep < 3; rep++) HIPCHK(sweep_kernel(d, g_fmag[f], true, a));
+                    HIPCHK(hipEventRecord(e1, g_stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        best_t = {TP, TS};
+                    }
                 }
             }
-        }
-        g_tuned[tune_key(b, d)] = best_t;
-        if (tiles_out) {
-            tiles_out[2 * d] = best_t.first;
-            tiles_out[2 * d + 1] = best_t.second;
+            g_tuned[tune_key(b, d, g_fmag[f])] = best_t;
+            if (tiles_out && f == 0) {
+                tiles_out[2 * d] = best_t.first;
+                tiles_out[2 * d + 1] = best_t.second;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -851,9 +1348,185 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
     // the candidates wrote the other buffer only; the state is unchanged.
     // The floor bookkeeping they accumulated is discarded.
     HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
-    HIPCHK(hipMemsetAsync(b->d_dmass, 0, sizeof(double), g_stream));
+    HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
+}
+
+// ---- the block list ----------------------------------------------------------
+
+int pk_timestep_all(double *dt_out, double *c_out)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (!dt_out || !c_out) return fail(PK_EINVAL, "null output");
+    RCCHK(list_build());
+    return list_dt(dt_out, c_out);
+}
+
+int pk_fluid_update_all(int32_t mode, double dt)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    RCCHK(check_mode(mode));
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    RCCHK(list_build());
+    if (!g_list.fresh) {
+        for (Block *b : g_list.blocks) RCCHK(do_fill(b));
+    }
+    return list_step(mode, dt);
+}
+
+int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    RCCHK(check_mode(mode));
+    if (nsteps < 0) return fail(PK_EINVAL, "nsteps < 0");
+    RCCHK(list_build());
+    double t = 0.0, dt = -1.0, dt_old = -1.0;
+    for (int s = 0; s < nsteps; s++) {
+        double c;
+        RCCHK(list_dt(&dt, &c));
+        if (s >= 1 && dt_old > 0.0) {
+            const double cap = dt_old * 2.0;
+            if (cap < dt) dt = cap;
+        }
+        RCCHK(list_step(mode, dt));
+        t = t + dt;
+        t = t + dt;
+        dt_old = dt;
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (t_out) *t_out = t;
+    if (dt_out) *dt_out = dt;
+    return PK_OK;
+}
+
+int pk_autotune_all(double dt, double chspeed)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    RCCHK(list_build());
+    pk::StepScalars sc{};
+    sc.dt = dt;
+    sc.ch = chspeed;
+    sc.psi_fac = 1.0;
+    if (!(chspeed > 0.0) && g_mhd) {
+        double fac;
+        glm_scalars(g_list.mdl, dt, &sc.ch, &fac);
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    *g_list.h_step = sc;
+    HIPCHK(hipMemcpyAsync(g_list.d_step, g_list.h_step, sizeof sc, hipMemcpyHostToDevice, g_stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    // block shapes present in the list, with up to ~8M cells of each for timing
+    std::map<std::vector<int>, std::vector<Block *>> classes;
+    for (Block *b : g_list.blocks) {
+        auto &v = classes[{b->n[0], b->n[1], b->n[2]}];
+        long long cells = 0;
+        for (Block *o : v) cells += (long long)o->n[0] * o->n[1] * o->n[2];
+        if (cells < 8ll * 1024 * 1024) v.push_back(b);
+    }
+    LaunchTab tmp;
+    int rc = PK_OK;
+    const int nt_list[] = {128, 256, 512};
+    for (int d = 0; d < 3 && !rc; d++) {
+        if (!g_cfg.has_dir[d]) continue;
+        for (int f = 0; f < g_nfl && !rc; f++) {
+            double best_total = 1e300;
+            int best_nt = 256;
+            std::map<std::vector<int>, int> best_ts_at;
+            for (int nt : nt_list) {
+                if (list_resident(d, g_fmag[f], nt) <= 0) continue;
+                double total = 0.0;
+                std::map<std::vector<int>, int> ts_at;
+                for (auto &cl : classes) {
+                    Block *b0 = cl.second[0];
+                    std::vector<int> cand;
+                    for (int k = 1; k <= 8; k++) {
+                        const int out = (b0->n[d] + k - 1) / k;
+                        const int ts = out + 8;
+                        if (ts <= nt && std::find(cand.begin(), cand.end(), ts) == cand.end()) cand.push_back(ts);
+                    }
+                    float best = 1e30f;
+                    int best_ts = cand.empty() ? 9 : cand[0];
+                    for (int ts : cand) {
+                        std::vector<int> tsv(cl.second.size(), ts);
+                        if ((rc = build_tab(tmp, cl.second, d, f, nt, &tsv))) break;
+                        pk::ListArgs L{};
+                        L.args = tmp.d_args;
+                        L.first = tmp.d_first;
+                        L.items = tmp.d_items;
+                        L.nitems = tmp.nitems;
+                        L.queue = g_list.d_queue;
+                        L.parity = 0;
+                        L.apply_fac = 0;
+                        L.step = g_list.d_step;
+                        L.dtslots = nullptr;
+                        L.nslots = kDtSlots;
+                        float ms = 0.0f;
+                        for (int rep = 0; rep < 2; rep++) {  // warm-up, then timed
+                            HIPCHK(pk::launch_zero_u32(g_list.d_queue, 1, g_stream));
+                            HIPCHK(hipEventRecord(e0, g_stream));
+                            HIPCHK(list_kernel(d, g_fmag[f], L, nt, tmp.grid));
+                            HIPCHK(hipEventRecord(e1, g_stream));
+                            HIPCHK(hipEventSynchronize(e1));
+                            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                        }
+                        if (ms < best) {
+                            best = ms;
+                            best_ts = ts;
+                        }
+                    }
+                    if (rc) break;
+                    ts_at[cl.first] = best_ts;
+                    total += best;
+                }
+                if (rc) break;
+                if (total < best_total) {
+                    best_total = total;
+                    best_nt = nt;
+                    best_ts_at = ts_at;
+                }
+            }
+            g_list.nt_pref[d][g_fmag[f]] = best_nt;
+            for (auto &cl : classes)
+                g_tuned[tune_key(cl.second[0], d, g_fmag[f], best_nt)] = {best_nt / best_ts_at[cl.first],
+                                                                          best_ts_at[cl.first]};
+        }
+    }
+    free_tab(tmp);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    // the candidates wrote the other buffers only; their floor bookkeeping is discarded
+    for (Block *b : g_list.blocks) {
+        HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
+        HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    const bool fresh = g_list.fresh;
+    list_invalidate();
+    RCCHK(list_build());
+    g_list.fresh = fresh;
+    return PK_OK;
+}
+
+int pk_list_tiles(int32_t ddim, int32_t ifluid, int64_t cg_id, int32_t out[3])
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
+    if (ifluid < 0 || ifluid >= g_nfl) return fail(PK_EINVAL, "bad fluid %d", ifluid);
+    RCCHK(list_build());
+    for (size_t i = 0; i < g_list.blocks.size(); i++) {
+        if (g_list.blocks[i]->id != cg_id) continue;
+        const LaunchTab &t = g_list.tab[ddim][ifluid];
+        out[0] = t.nt;
+        out[1] = t.h_args[i].TP;
+        out[2] = t.h_args[i].TS;
+        return PK_OK;
+    }
+    return fail(PK_ENOBLOCK, "unknown cg_id %lld", (long long)cg_id);
 }
 
 int pk_profile_begin(void)

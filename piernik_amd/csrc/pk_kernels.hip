@@ -21,6 +21,8 @@ namespace PK_KNS {
 
 using pk::DtArgs;
 using pk::GhostArgs;
+using pk::ListArgs;
+using pk::StepScalars;
 using pk::SweepArgs;
 
 template <int DIR>
@@ -95,7 +97,8 @@ __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)
 // 64-bit atomics (the values are positive, so their bit patterns order as
 // the numbers do; min and max do not depend on the order of reduction).
 template <int DIR, class P>
-__device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double (&Wn)[P::NV], bool valid)
+__device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double (&Wn)[P::NV], bool valid,
+                                                unsigned slot_id)
 {
     const double small = 1.17549435082228750797e-38;
     const double huge = 1.7976931348623157e308;
@@ -147,7 +150,7 @@ __device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double
         }
     }
     if ((threadIdx.x & 63) == 0) {
-        const int slot = (int)((blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u) % (unsigned)a.nslots);
+        const int slot = (int)((slot_id * (blockDim.x / 64u) + threadIdx.x / 64u) % (unsigned)a.nslots);
         unsigned long long *s = a.dtslots + 4 * slot;
 #pragma unroll
         for (int q = 0; q < 3; q++) atomicMin(s + q, (unsigned long long)__double_as_longlong(mn[q]));
@@ -155,16 +158,18 @@ __device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double
     }
 }
 
+// One tile of a sweep: chunk `bid % chunks` along the sweep direction times
+// TP consecutive pencils of the transverse interior, flattened with d1
+// fastest (tile `bid / chunks`).  Lanes t >= TP*TS of a larger workgroup
+// take part in the barriers only.  slot_id picks the timestep slot.
 template <int DIR, class P, bool FUSED, int LIM>
-__global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
+__device__ __forceinline__ void sweep_tile(const SweepArgs &a, int bid, unsigned slot_id, double *__restrict__ LA,
+                                           double *__restrict__ LB)
 {
     constexpr int NV = P::NV;
     constexpr int HALO = FUSED ? 4 : 2;
-    extern __shared__ double lds[];
     const int nt = blockDim.x;
     const int t = threadIdx.x;
-    double *LA = lds;
-    double *LB = lds + NV * nt;
 
     constexpr int ds = DIR;
     constexpr int d1 = (DIR == 0) ? 1 : 0;
@@ -185,20 +190,15 @@ __global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
     tm = tm < 0 ? 0 : tm;
     tp = tp >= nt ? nt - 1 : tp;
 
-    // XCD-aware order: workgroup i runs on XCD i % 8, so give each XCD a
-    // contiguous range of tiles; tiles that share halo cells then share an L2.
-    int bid = blockIdx.x;
-    const int nfull = (int)(gridDim.x / 8u) * 8;
-    if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
     const int chunk = bid % a.chunks;
-    const int rest = bid / a.chunks;
-    const int i1t = rest % a.tiles1;
-    const int i2 = rest / a.tiles1;
-
-    const int p1 = i1t * a.TP + p;
-    const bool pvalid = p1 < a.n[d1];
-    const int c1 = a.g[d1] + (pvalid ? p1 : a.n[d1] - 1);
-    const int c2 = a.g[d2] + i2;
+    const int ptile = bid / a.chunks;
+    const int n1 = a.n[d1];
+    const int npen = n1 * a.n[d2];
+    const int pf = ptile * a.TP + p;
+    const bool pvalid = t < a.TP * a.TS && pf < npen;
+    const int pfc = pf < npen ? pf : npen - 1;
+    const int c1 = a.g[d1] + pfc % n1;
+    const int c2 = a.g[d2] + pfc / n1;
     const int ng = a.ng[ds];
     const int os = (FUSED ? a.g[ds] : 0) + chunk * a.out;
     const int oe = FUSED ? a.g[ds] + a.n[ds] : ng;
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
         }
     }
     if constexpr (FUSED) {
-        if (a.dtslots) dt_contribution<DIR, P>(a, Wn, outl && svalid);
+        if (a.dtslots) dt_contribution<DIR, P>(a, Wn, outl && svalid, slot_id);
     }
     if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
 
@@ -286,6 +286,78 @@ __global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
     // floor bookkeeping: rare, one atomic per lane that saw something
     if (flags) atomicOr(a.flags, flags);
     if (dmass != 0.0) atomicAdd(a.dmass, dmass);
+}
+
+template <int DIR, class P, bool FUSED, int LIM>
+__global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
+{
+    extern __shared__ double lds[];
+    // XCD-aware order: workgroup i runs on XCD i % 8, so give each XCD a
+    // contiguous range of tiles; tiles that share halo cells then share an L2.
+    int bid = blockIdx.x;
+    const int nfull = (int)(gridDim.x / 8u) * 8;
+    if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
+    sweep_tile<DIR, P, FUSED, LIM>(a, bid, blockIdx.x, lds, lds + P::NV * blockDim.x);
+}
+
+// Persistent fused sweep over a block list.  Each workgroup takes the next
+// tile from the queue (one atomic per tile), loads that block's arguments
+// and runs the tile; it leaves when the queue is empty, so every workgroup
+// ends after at most nitems + gridDim.x fetches in total.
+template <int DIR, class P, int LIM>
+__global__ __launch_bounds__(512) void k_sweep_list(const ListArgs L)
+{
+    extern __shared__ double lds[];
+    __shared__ unsigned s_item;
+    const StepScalars st = *L.step;
+    for (;;) {
+        __syncthreads();  // the previous tile is done with LDS and s_item
+        if (threadIdx.x == 0) s_item = atomicAdd(L.queue, 1u);
+        __syncthreads();
+        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)s_item);
+        if (item >= L.nitems) break;
+        // the tables are read-only while the kernel runs: read them through the
+        // constant address space so the uniform loads go to scalar registers
+        typedef const __attribute__((address_space(4))) unsigned cu32;
+        const int blk = __builtin_amdgcn_readfirstlane((int)((cu32 *)L.items)[item]);
+        SweepArgs a;
+        {
+            static_assert(sizeof(SweepArgs) % 4 == 0, "SweepArgs is copied in dwords");
+            cu32 *w = (cu32 *)(L.args + blk);
+            unsigned *aw = reinterpret_cast<unsigned *>(&a);
+#pragma unroll
+            for (int i = 0; i < (int)(sizeof(SweepArgs) / 4); i++) aw[i] = w[i];
+        }
+        const unsigned first = ((cu32 *)L.first)[blk];
+        if (L.parity) {
+            const double *s0 = a.src;
+            a.src = a.dst;
+            a.dst = const_cast<double *>(s0);
+        }
+        a.base = a.src;
+        a.c1 = (0.5 * st.dt) / a.dl[DIR];  // rk_coef * dt / dl, constants.F90:237-238
+        a.c2 = (1.0 * st.dt) / a.dl[DIR];
+        a.ch = st.ch;
+        a.psi_fac = L.apply_fac ? st.psi_fac : 1.0;
+        a.dtslots = L.dtslots;
+        a.nslots = L.nslots;
+        a.images = 1;
+        sweep_tile<DIR, P, true, LIM>(a, (int)(item - first), item, lds, lds + P::NV * blockDim.x);
+    }
+}
+
+__global__ void k_fill_items(const unsigned *__restrict__ first, int nblocks, unsigned *__restrict__ items)
+{
+    // one workgroup per block writes its range of the item table
+    const int b = blockIdx.x;
+    if (b >= nblocks) return;
+    for (unsigned i = first[b] + threadIdx.x; i < first[b + 1]; i += blockDim.x) items[i] = (unsigned)b;
+}
+
+__global__ void k_zero_u32(unsigned *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // Periodic refresh of the ghost layers along direction DIR over the full
@@ -433,9 +505,9 @@ __global__ __launch_bounds__(256) void k_dt_final(const double *__restrict__ par
     }
 }
 
-// Fortran AoS block (nc, nxg, nyg, nzg) <-> device planes [plane0 + c].
-__global__ void k_aos_to_soa(const double *__restrict__ aos, int nc, double *__restrict__ soa, long long sv,
-                             int plane0, int nx, int ny, int nz, int sy, int sz)
+// Fortran AoS block (nc, nxg, nyg, nzg) <-> device planes m.planes[c].
+__global__ void k_aos_to_soa(const double *__restrict__ aos, const pk::PlaneMap m, double *__restrict__ soa,
+                             long long sv, int nx, int ny, int nz, int sy, int sz)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long ncell = (long long)nx * ny * nz;
@@ -445,10 +517,10 @@ __global__ void k_aos_to_soa(const double *__restrict__ aos, int nc, double *__r
     const int y = (int)(r % ny);
     const int z = (int)(r / ny);
     const long long di = (long long)x + (long long)sy * y + (long long)sz * z;
-    for (int c = 0; c < nc; c++) soa[(long long)(plane0 + c) * sv + di] = aos[i * nc + c];
+    for (int c = 0; c < m.nc; c++) soa[(long long)m.planes[c] * sv + di] = aos[i * m.nc + c];
 }
 
-__global__ void k_soa_to_aos(const double *__restrict__ soa, long long sv, int plane0, int nc,
+__global__ void k_soa_to_aos(const double *__restrict__ soa, long long sv, const pk::PlaneMap m,
                              double *__restrict__ aos, int nx, int ny, int nz, int sy, int sz)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -459,7 +531,7 @@ __global__ void k_soa_to_aos(const double *__restrict__ soa, long long sv, int p
     const int y = (int)(r % ny);
     const int z = (int)(r / ny);
     const long long si = (long long)x + (long long)sy * y + (long long)sz * z;
-    for (int c = 0; c < nc; c++) aos[i * nc + c] = soa[(long long)(plane0 + c) * sv + si];
+    for (int c = 0; c < m.nc; c++) aos[i * m.nc + c] = soa[(long long)m.planes[c] * sv + si];
 }
 
 __global__ void k_dtslots_init(unsigned long long *slots, int nslots)
@@ -534,7 +606,7 @@ static hipError_t launch_sweep_t(const SweepArgs &a, hipStream_t st)
         lds_raised = true;
     }
     if (lds > 160 * 1024 || nt > 512) return hipErrorInvalidValue;
-    const long long nblk = (long long)a.chunks * a.tiles1 * a.n[(DIR == 2) ? 1 : 2];
+    const long long nblk = (long long)a.chunks * a.ptiles;
     hipLaunchKernelGGL((k_sweep<DIR, P, FUSED, LIM>), dim3((unsigned)nblk), dim3(nt), lds, st, a);
     return hipGetLastError();
 }
@@ -564,6 +636,83 @@ hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipSt
     PK_CASE(0, MHD, false) PK_CASE(1, MHD, false) PK_CASE(2, MHD, false)
 #undef PK_CASE
     return hipErrorInvalidValue;
+}
+
+// Persistent list sweep: launch (resident == nullptr) or report how many
+// workgroups of nt lanes the device holds at once.
+template <int DIR, class P, int LIM>
+static hipError_t list_t(const ListArgs *L, int nt, int grid, hipStream_t st, int *resident)
+{
+    const void *fn = (const void *)k_sweep_list<DIR, P, LIM>;
+    const size_t lds = sizeof(double) * 2 * P::NV * nt;
+    const size_t lds_max = 160 * 1024 - 256;  // static LDS (the queue slot) comes out of the same 160 KiB
+    static bool lds_raised = false;
+    if (!lds_raised) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);
+        if (e != hipSuccess) return e;
+        lds_raised = true;
+    }
+    if (nt < 64 || nt > 512 || nt % 64 != 0 || lds > lds_max) return hipErrorInvalidValue;
+    if (resident) {
+        int per = 0, dev = 0, ncu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        *resident = per * ncu;
+        return e;
+    }
+    if (grid <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_sweep_list<DIR, P, LIM>), dim3((unsigned)grid), dim3(nt), lds, st, *L);
+    return hipGetLastError();
+}
+
+template <int DIR, class P>
+static hipError_t list_lim(int lim, int limb, const ListArgs *L, int nt, int grid, hipStream_t st, int *resident)
+{
+    if (lim == limb) {
+        switch (lim) {
+        case LIM_VANLEER: return list_t<DIR, P, LIM_VANLEER>(L, nt, grid, st, resident);
+        case LIM_MINMOD: return list_t<DIR, P, LIM_MINMOD>(L, nt, grid, st, resident);
+        case LIM_MONCEN: return list_t<DIR, P, LIM_MONCEN>(L, nt, grid, st, resident);
+        case LIM_SUPERBEE: return list_t<DIR, P, LIM_SUPERBEE>(L, nt, grid, st, resident);
+        default: break;
+        }
+    }
+    return list_t<DIR, P, -1>(L, nt, grid, st, resident);
+}
+
+static hipError_t list_dispatch(int dir, bool mhd, int lim, int limb, const ListArgs *L, int nt, int grid,
+                                hipStream_t st, int *resident)
+{
+    if (mhd) {
+        if (dir == 0) return list_lim<0, MHD>(lim, limb, L, nt, grid, st, resident);
+        if (dir == 1) return list_lim<1, MHD>(lim, limb, L, nt, grid, st, resident);
+        if (dir == 2) return list_lim<2, MHD>(lim, limb, L, nt, grid, st, resident);
+    } else {
+        if (dir == 0) return list_lim<0, HD>(lim, limb, L, nt, grid, st, resident);
+        if (dir == 1) return list_lim<1, HD>(lim, limb, L, nt, grid, st, resident);
+        if (dir == 2) return list_lim<2, HD>(lim, limb, L, nt, grid, st, resident);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep_list(int dir, bool mhd, int lim, int limb, const ListArgs &L, int nt, int grid,
+                             hipStream_t st)
+{
+    if (grid <= 0) {
+        int r = 0;
+        hipError_t e = list_dispatch(dir, mhd, lim, limb, &L, nt, 0, st, &r);
+        if (e != hipSuccess) return e;
+        grid = r > 0 ? r : 1;
+    }
+    return list_dispatch(dir, mhd, lim, limb, &L, nt, grid, st, nullptr);
+}
+
+int sweep_list_resident(int dir, bool mhd, int lim, int limb, int nt)
+{
+    int r = 0;
+    if (list_dispatch(dir, mhd, lim, limb, nullptr, nt, 0, nullptr, &r) != hipSuccess) return 0;
+    return r;
 }
 
 size_t sweep_lds_bytes(bool mhd, int TP, int TS) { return sizeof(double) * 2 * (mhd ? 9 : 5) * TP * TS; }
@@ -607,21 +756,35 @@ hipError_t launch_dt(const DtArgs &a, double *part, double *out, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_aos_to_soa(const double *aos, int nc, double *soa, long long sv, int plane0, const int ng[3],
+hipError_t launch_aos_to_soa(const double *aos, const pk::PlaneMap &m, double *soa, long long sv, const int ng[3],
                              int sy, int sz, hipStream_t st)
 {
     const long long n = (long long)ng[0] * ng[1] * ng[2];
-    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, aos, nc, soa, sv, plane0,
-                       ng[0], ng[1], ng[2], sy, sz);
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, aos, m, soa, sv, ng[0],
+                       ng[1], ng[2], sy, sz);
     return hipGetLastError();
 }
 
-hipError_t launch_soa_to_aos(const double *soa, long long sv, int plane0, int nc, double *aos, const int ng[3],
+hipError_t launch_soa_to_aos(const double *soa, long long sv, const pk::PlaneMap &m, double *aos, const int ng[3],
                              int sy, int sz, hipStream_t st)
 {
     const long long n = (long long)ng[0] * ng[1] * ng[2];
-    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, soa, sv, plane0, nc, aos,
-                       ng[0], ng[1], ng[2], sy, sz);
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, soa, sv, m, aos, ng[0],
+                       ng[1], ng[2], sy, sz);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_items(const unsigned *first, int nblocks, unsigned *items, hipStream_t st)
+{
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_items, dim3((unsigned)nblocks), dim3(256), 0, st, first, nblocks, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

@@ -16,7 +16,7 @@ struct SweepArgs {
     int n[3];            // interior extents
     int TP, TS;          // tile: TP pencils x TS cells along the sweep
     int chunks;          // tiles along the sweep direction
-    int tiles1;          // tiles along the fastest transverse direction
+    int ptiles;          // tiles over the transverse interior, pencils flattened (d1 fastest)
     int out;             // output cells per tile along the sweep (TS-8 fused, TS-4 per stage)
     int lim, limb;       // limiter for fluid primitives / for B and psi
     double gam, gam_1;
@@ -45,6 +45,12 @@ struct GhostArgs {
     int plane0, nplanes;
 };
 
+// Fortran AoS component c of nc <-> device plane planes[c].
+struct PlaneMap {
+    int nc;
+    int planes[16];
+};
+
 struct DtArgs {
     const double *u;
     long long sv;
@@ -58,7 +64,39 @@ struct DtArgs {
     double gam, gam_1;
 };
 
+// Per-step scalars of the batched sweep, in device memory so that a captured
+// graph replays every step unchanged.
+struct StepScalars {
+    double dt;       // the step's dt; the sweep forms (rk_coef*dt)/dl per block
+    double ch;       // GLM cleaning speed
+    double psi_fac;  // GLM damping factor applied by the last sweep of each substep
+    double pad;
+};
+
+// One persistent launch over a block list: workgroups take tiles from a
+// device queue until it is empty.  Tile `item` belongs to block items[item]
+// and is that block's tile item - first[block].
+struct ListArgs {
+    const SweepArgs *args;   // per block: geometry, pointers (src = buffer 0), tile shape
+    const unsigned *first;   // per block: first item (nblocks + 1 entries)
+    const unsigned *items;   // per item: block index
+    unsigned nitems;
+    unsigned *queue;         // item counter, zero before the launch
+    int parity;              // 1: read buffer 1, write buffer 0
+    int apply_fac;           // multiply psi by step->psi_fac
+    const StepScalars *step;
+    unsigned long long *dtslots;  // if set: fold timestep_fluid of the cells written into these slots
+    int nslots;
+};
+
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st);
+// Persistent batched fused sweep with workgroups of nt lanes; grid = 0 picks
+// the number of workgroups the device holds at once.
+hipError_t launch_sweep_list(int dir, bool mhd, int lim, int limb, const ListArgs &L, int nt, int grid,
+                             hipStream_t st);
+int sweep_list_resident(int dir, bool mhd, int lim, int limb, int nt);
+hipError_t launch_fill_items(const unsigned *first, int nblocks, unsigned *items, hipStream_t st);
+hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st);
 size_t sweep_lds_bytes(bool mhd, int TP, int TS);
 hipError_t launch_fill_ghosts(int dir, const GhostArgs &a, hipStream_t st);
 hipError_t launch_scale(double *x, long long n, double fac, hipStream_t st);
@@ -66,14 +104,17 @@ int dt_partial_blocks(const DtArgs &a);
 hipError_t launch_dtslots_init(unsigned long long *slots, int nslots, hipStream_t st);
 hipError_t launch_dtslots_final(const unsigned long long *slots, int nslots, double *out, hipStream_t st);
 hipError_t launch_dt(const DtArgs &a, double *part, double *out, hipStream_t st);
-hipError_t launch_aos_to_soa(const double *aos, int nc, double *soa, long long sv, int plane0, const int ng[3],
+hipError_t launch_aos_to_soa(const double *aos, const PlaneMap &m, double *soa, long long sv, const int ng[3],
                              int sy, int sz, hipStream_t st);
-hipError_t launch_soa_to_aos(const double *soa, long long sv, int plane0, int nc, double *aos, const int ng[3],
+hipError_t launch_soa_to_aos(const double *soa, long long sv, const PlaneMap &m, double *aos, const int ng[3],
                              int sy, int sz, hipStream_t st);
 
 }  // namespace pk
 
 namespace pk_fast {
-// the sweep of pk_kernels_fast.hip (FMA contraction, fast division)
+// the sweeps of pk_kernels_fast.hip (FMA contraction, fast division)
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const pk::SweepArgs &a, hipStream_t st);
+hipError_t launch_sweep_list(int dir, bool mhd, int lim, int limb, const pk::ListArgs &L, int nt, int grid,
+                             hipStream_t st);
+int sweep_list_resident(int dir, bool mhd, int lim, int limb, int nt);
 }  // namespace pk_fast

@@ -20,12 +20,16 @@ module piernik_hip
    public :: pk_init, pk_finalize, pk_register_block, pk_unregister_block, pk_upload, pk_download, &
         &    pk_upload_from, pk_download_to, pk_solve_cg, pk_sweep, pk_fill_ghosts, pk_glm_damp, &
         &    pk_timestep_fluid, pk_fluid_update, pk_run, pk_get_report, pk_synchronize, &
-        &    pk_autotune, pk_profile_begin, pk_profile_end, pk_error_message
+        &    pk_autotune, pk_profile_begin, pk_profile_end, pk_error_message, &
+        &    pk_upload_device, pk_download_device, pk_timestep_all, pk_fluid_update_all, pk_run_all, &
+        &    pk_autotune_all, pk_list_tiles
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
+   public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
 
    ! the reference's stage enumerators (src/base/constants.F90:230-232)
    integer(c_int32_t), parameter :: PK_EULER = 10000, PK_RK2_1 = 10001, PK_RK2_2 = 10002
    integer(c_int32_t), parameter :: PK_LIM_VANLEER = 0, PK_LIM_MINMOD = 1, PK_LIM_MONCEN = 2, PK_LIM_SUPERBEE = 3
+   integer(c_int32_t), parameter :: PK_BATCH_GRAPH = 0, PK_BATCH_PERSISTENT = 1, PK_BATCH_PER_BLOCK = 2
 
    type, bind(C) :: pk_fluid
       real(c_double)     :: gamma
@@ -56,7 +60,7 @@ module piernik_hip
    type, bind(C) :: pk_stage_report
       integer(c_int32_t) :: dn_negative
       integer(c_int32_t) :: ei_negative
-      real(c_double)     :: magic_mass
+      real(c_double)     :: magic_mass(2)   ! per fluid
    end type pk_stage_report
 
    type, bind(C) :: pk_profile_stats
@@ -180,6 +184,49 @@ module piernik_hip
          real(c_double),     value       :: dt, chspeed
          integer(c_int32_t), intent(out) :: tiles_out(6)
       end function pk_autotune
+
+      ! device arrays (e.g. from a GPU allocation), reference layout
+      integer(c_int32_t) function pk_upload_device(cg_id, u, b, psi) bind(C, name="pk_upload_device")
+         import :: c_int32_t, c_int64_t, c_ptr
+         integer(c_int64_t), value :: cg_id
+         type(c_ptr),        value :: u, b, psi
+      end function pk_upload_device
+
+      integer(c_int32_t) function pk_download_device(cg_id, u, b, psi) bind(C, name="pk_download_device")
+         import :: c_int32_t, c_int64_t, c_ptr
+         integer(c_int64_t), value :: cg_id
+         type(c_ptr),        value :: u, b, psi
+      end function pk_download_device
+
+      ! the whole registered block list (the block loops of sweeps.F90:230-260)
+      integer(c_int32_t) function pk_timestep_all(dt_out, c_out) bind(C, name="pk_timestep_all")
+         import :: c_int32_t, c_double
+         real(c_double), intent(out) :: dt_out, c_out
+      end function pk_timestep_all
+
+      integer(c_int32_t) function pk_fluid_update_all(mode, dt) bind(C, name="pk_fluid_update_all")
+         import :: c_int32_t, c_double
+         integer(c_int32_t), value :: mode
+         real(c_double),     value :: dt
+      end function pk_fluid_update_all
+
+      integer(c_int32_t) function pk_run_all(mode, nsteps, t_out, dt_out) bind(C, name="pk_run_all")
+         import :: c_int32_t, c_double
+         integer(c_int32_t), value       :: mode, nsteps
+         real(c_double),     intent(out) :: t_out, dt_out
+      end function pk_run_all
+
+      integer(c_int32_t) function pk_autotune_all(dt, chspeed) bind(C, name="pk_autotune_all")
+         import :: c_int32_t, c_double
+         real(c_double), value :: dt, chspeed
+      end function pk_autotune_all
+
+      integer(c_int32_t) function pk_list_tiles(ddim, ifluid, cg_id, tiles) bind(C, name="pk_list_tiles")
+         import :: c_int32_t, c_int64_t
+         integer(c_int32_t), value       :: ddim, ifluid
+         integer(c_int64_t), value       :: cg_id
+         integer(c_int32_t), intent(out) :: tiles(3)
+      end function pk_list_tiles
 
       integer(c_int32_t) function pk_profile_begin() bind(C, name="pk_profile_begin")
          import :: c_int32_t

@@ -13,6 +13,11 @@ workloads the sweep is measured on:
                   (256^3, 4 ghosts, periodic)
   smooth_random   smooth seeded random state (rho, p = 1 +- 0.2, |v| <= 0.5,
                   optional B) for parity tests on small blocks
+  batch_blocks    the batched block list: uneven blocks (edges drawn from
+                  16, 24, 32, 48, 64), an ionized MHD fluid and a neutral
+                  fluid per block, smooth seeded fields rho, p = 1 +- 0.2,
+                  |v| <= 0.5 (host arrays, or generated on the GPU by
+                  batch_blocks_device)
 """
 import numpy as np
 
@@ -142,3 +147,164 @@ def smooth_random(n, nb=4, seed=0, magnetized=False, gamma=5.0 / 3.0, has_dir=(T
     else:
         _set_state(cg, rho, *vel, p, gamma)
     return cg
+
+
+# ---- batched block list: uneven blocks, two fluids ---------------------------
+
+BATCH_SIZES = (16, 24, 32, 48, 64)
+
+
+def batch_config(fluids=((5.0 / 3.0, True), (5.0 / 3.0, False)), limiter="vanleer", cfl=0.7, has_dir=(True, True, True)):
+    """Ionized fluid (HLLD + GLM) and neutral fluid (HLLC), PIERNIK's default
+    limiter (van Leer for both) and CFL 0.7, floors below any value the
+    smooth state reaches."""
+    return Config(fluids=list(fluids), limiter=limiter, limiter_b=limiter, cfl=cfl, cfl_glm=cfl, glm_alpha=0.1,
+                  smalld=1e-4, smallei=1e-6, has_dir=has_dir)
+
+
+def batch_sizes(nblocks, seed=0, sizes=BATCH_SIZES):
+    """Interior edge length of each block, drawn uniformly from `sizes`."""
+    rng = np.random.default_rng([seed, 7777])
+    return [int(s) for s in rng.choice(np.asarray(sizes), size=nblocks)]
+
+
+def _batch_params(idx, seed, nfluids, mag, has_dir):
+    """Mode tables of every field of block idx: per field 3 modes of integer
+    wave vectors in [-2, 2]^3 (inactive directions 0), phases and weights."""
+    rng = np.random.default_rng([seed, int(idx)])
+    act = np.array([1 if h else 0 for h in has_dir])
+    fields = []
+    for f in range(nfluids):
+        modes = []
+        for _ in range(5):  # rho, p, vx, vy, vz
+            kv = rng.integers(-2, 3, size=(3, 3)) * act
+            kv[np.all(kv == 0, axis=1)] = act  # no constant mode
+            modes.append((kv, rng.uniform(0.0, 2.0 * np.pi, 3), rng.uniform(0.5, 1.0, 3)))
+        fields.append(modes)
+    bmodes = None
+    if mag:
+        # B_x(y, z), B_y(z, x), B_z(x, y): each component independent of its
+        # own coordinate, so div B = 0 for the continuous field
+        bmodes = []
+        for c in range(3):
+            kv = rng.integers(1, 3, size=3) * act
+            kv[c] = 0
+            bmodes.append((kv, rng.uniform(0.0, 2.0 * np.pi)))
+    return fields, bmodes
+
+
+def _batch_eval(xp, coords, fields, bmodes, gammas, mags, b0):
+    """Conserved state from the mode tables; coords = (X, Y, Z) broadcastable
+    arrays of cell centres in [0, 1).  Returns list of (5,) fluid arrays and B."""
+    two_pi = 2.0 * np.pi
+
+    def series(kv, ph, w):
+        s = 0.0
+        for m in range(3):
+            s = s + w[m] * xp.sin(two_pi * (kv[m][0] * coords[0] + kv[m][1] * coords[1] + kv[m][2] * coords[2])
+                                  + ph[m])
+        return s / float(np.sum(w))
+
+    bb = None
+    if bmodes is not None:
+        bb = [b0 * xp.sin(two_pi * (kv[0] * coords[0] + kv[1] * coords[1] + kv[2] * coords[2]) + ph)
+              for kv, ph in bmodes]
+    out = []
+    for f, modes in enumerate(fields):
+        rho = 1.0 + 0.2 * series(*modes[0])
+        p = 1.0 + 0.2 * series(*modes[1])
+        v = [(0.5 / np.sqrt(3.0)) * series(*modes[2 + d]) for d in range(3)]
+        e = p / (gammas[f] - 1.0) + 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+        if mags[f]:
+            e = e + 0.5 * (bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2])
+        out.append([rho, rho * v[0], rho * v[1], rho * v[2], e])
+    return out, bb
+
+
+def _shape_of(n, has_dir):
+    """Block shape from an edge length (cube over the active directions) or a triple."""
+    if isinstance(n, (tuple, list)):
+        return tuple(int(x) for x in n)
+    return tuple(int(n) if h else 1 for h in has_dir)
+
+
+def _wrapped_centres(n, g):
+    """Cell centres of a ghost-padded axis, ghosts at their periodic images."""
+    i = np.arange(n + 2 * g) - g
+    return ((i % n) + 0.5) / n
+
+
+def batch_blocks(sizes, cfg, seed=0, nb=4, b0=0.3, first_id=1):
+    """Host blocks (GridContainer, reference layout) of the batched list:
+    rho = 1 +- 0.2, p = 1 +- 0.2, |v| <= 0.5 per fluid, B of amplitude b0 for
+    the magnetized fluid, psi = 0; cubes of unit size (dl = 1/n)."""
+    gammas = [g for g, _ in cfg.fluids]
+    mags = [m for _, m in cfg.fluids]
+    out = []
+    for i, n in enumerate(sizes):
+        shape = _shape_of(n, cfg.has_dir)
+        cg = GridContainer(shape, nb=nb, dl=tuple(1.0 / s for s in shape), magnetized=cfg.magnetized,
+                           has_dir=cfg.has_dir, cg_id=first_id + i, nfluids=cfg.nfluids)
+        ax = [_wrapped_centres(shape[d], cg.g[d]) for d in range(3)]
+        X, Y, Z = np.meshgrid(*ax, indexing="ij")
+        fields, bmodes = _batch_params(i, seed, cfg.nfluids, cfg.magnetized, cfg.has_dir)
+        fl, bb = _batch_eval(np, (X, Y, Z), fields, bmodes, gammas, mags, b0)
+        for f in range(cfg.nfluids):
+            for k in range(5):
+                cg.u[5 * f + k] = fl[f][k]
+        if cfg.magnetized:
+            for c in range(3):
+                cg.b[c] = bb[c]
+            cg.psi[...] = 0.0
+        out.append(cg)
+    return out
+
+
+def batch_blocks_device(sizes, cfg, seed=0, nb=4, b0=0.3, first_id=1, device="cuda"):
+    """The same state generated on the GPU with torch and copied into the
+    library's device mirrors (no host arrays); returns the registered
+    GridContainers (their host arrays are not filled)."""
+    import torch
+
+    gammas = [g for g, _ in cfg.fluids]
+    mags = [m for _, m in cfg.fluids]
+    out = []
+    for i, n in enumerate(sizes):
+        shape = _shape_of(n, cfg.has_dir)
+        cg = _DeviceBlock(shape, nb, cfg, first_id + i)
+        ax = [torch.as_tensor(_wrapped_centres(shape[d], cg.g[d]), dtype=torch.float64, device=device)
+              for d in range(3)]
+        X = ax[0].view(1, 1, -1)
+        Y = ax[1].view(1, -1, 1)
+        Z = ax[2].view(-1, 1, 1)
+        fields, bmodes = _batch_params(i, seed, cfg.nfluids, cfg.magnetized, cfg.has_dir)
+        fl, bb = _batch_eval(torch, (X, Y, Z), fields, bmodes, gammas, mags, b0)
+        full = (cg.ng[2], cg.ng[1], cg.ng[0])
+        u = torch.stack([torch.broadcast_to(torch.as_tensor(c, dtype=torch.float64, device=device), full)
+                         for f in range(cfg.nfluids) for c in fl[f]], dim=-1).contiguous()
+        b = psi = None
+        if cfg.magnetized:
+            b = torch.stack([torch.broadcast_to(c, full) for c in bb], dim=-1).contiguous()
+            psi = torch.zeros(full, dtype=torch.float64, device=device)
+        cg.upload_device(u, b, psi)
+        del u, b, psi
+        out.append(cg)
+    return out
+
+
+class _DeviceBlock(GridContainer):
+    """A GridContainer registered without host arrays (device-resident)."""
+
+    def __init__(self, shape, nb, cfg, cg_id):
+        self.n = tuple(shape)
+        self.nb = nb
+        self.has_dir = tuple(cfg.has_dir)
+        self.g = tuple(nb if h else 0 for h in self.has_dir)
+        self.ng = tuple(self.n[d] + 2 * self.g[d] for d in range(3))
+        self.dl = tuple(1.0 / s for s in self.n)
+        self.magnetized = cfg.magnetized
+        self.nfluids = cfg.nfluids
+        self.nvar = 5 * self.nfluids
+        self.u = self.uh = self.b = self.bh = self.psi = self.psih = None
+        self.cg_id = int(cg_id)
+        self.registered = False

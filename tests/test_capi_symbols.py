@@ -28,7 +28,8 @@ def test_header_declares_entry_points():
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built (python -m piernik_amd.build)")
 def test_library_exports_every_declared_symbol():
-    lib = ctypes.CDLL(LIB)
+    import piernik_amd as pk
+    lib = pk.load_library()
     missing = [n for n in declared_symbols() if not hasattr(lib, n)]
     assert not missing, missing
 

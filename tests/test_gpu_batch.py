@@ -1,0 +1,167 @@
+"""The block-list path (pk_*_all) against the oracle's list driver.
+
+Bit-exact for every array element: the persistent list kernel, the graph
+replay and the per-block launches run the same arithmetic as the
+single-block sweep, which is bit-exact against the oracle.  Blocks are
+uneven (non-cubic shapes, different cell sizes) and carry an ionized MHD
+fluid and a neutral HLLC fluid with different gammas.
+"""
+import numpy as np
+import pytest
+
+import piernik_amd as pk
+from piernik_amd import setups
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"graph": pk.BATCH_GRAPH, "persistent": pk.BATCH_PERSISTENT, "per_block": pk.BATCH_PER_BLOCK}
+SHAPES = [(16, 16, 16), (8, 12, 10), (24, 8, 8), (12, 20, 16), (6, 6, 6)]
+ION = (5.0 / 3.0, True)
+NEU = (1.4, False)
+
+
+def start(cfg, cgs):
+    pk.load_library()
+    pk.finalize()
+    pk.init(cfg)
+    for cg in cgs:
+        cg.upload()
+
+
+def assert_same(cg, ob, where=""):
+    for name in ("u", "b", "psi"):
+        a, o = getattr(cg, name), getattr(ob, name)
+        if a is None:
+            continue
+        bad = np.argwhere(~(a == o))
+        assert bad.size == 0, (f"{where} block {cg.cg_id} {cg.n} {name}: {len(bad)} mismatches, first at "
+                               f"{tuple(bad[0])}: gpu={a[tuple(bad[0])]!r} oracle={o[tuple(bad[0])]!r}")
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_two_fluid_list_bit_exact(oracle, mode):
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    cgs = setups.batch_blocks(SHAPES, cfg, seed=9)
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    print(f"two-fluid list, mode {mode}, shapes {SHAPES}", flush=True)
+    start(cfg, cgs)
+    t_g, _ = pk.run_all(3, MODES[mode])
+    t_o = oracle.run_list(p, obs, 3)
+    assert t_g == t_o
+    for cg, ob in zip(cgs, obs):
+        cg.download()
+        assert_same(cg, ob, mode)
+        rep = pk.get_report(cg)
+        assert rep["magic_mass"] == list(ob.rep.magic_mass)
+    pk.finalize()
+
+
+@pytest.mark.parametrize("fluid,limiter", [(ION, "minmod"), (NEU, "vanleer"), (ION, "superbee")])
+def test_one_fluid_list_bit_exact(oracle, fluid, limiter):
+    cfg = setups.batch_config(fluids=(fluid,), limiter=limiter)
+    cgs = setups.batch_blocks(SHAPES[:4], cfg, seed=3)
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    print(f"one-fluid list {fluid} {limiter}", flush=True)
+    start(cfg, cgs)
+    t_g, _ = pk.run_all(2)
+    t_o = oracle.run_list(p, obs, 2)
+    assert t_g == t_o
+    for cg, ob in zip(cgs, obs):
+        cg.download()
+        assert_same(cg, ob, f"{fluid} {limiter}")
+    pk.finalize()
+
+
+def test_2d_two_fluid_list_bit_exact(oracle):
+    cfg = setups.batch_config(fluids=(ION, NEU), has_dir=(True, True, False))
+    cgs = setups.batch_blocks([(16, 24, 1), (12, 8, 1), (32, 16, 1)], cfg, seed=4)
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    print("2d two-fluid list", flush=True)
+    start(cfg, cgs)
+    t_g, _ = pk.run_all(3)
+    t_o = oracle.run_list(p, obs, 3)
+    assert t_g == t_o
+    for cg, ob in zip(cgs, obs):
+        cg.download()
+        assert_same(cg, ob, "2d")
+    pk.finalize()
+
+
+def test_list_calls_continue_and_timestep(oracle):
+    """run_all twice (the second call starts from the dt and ghosts the first
+    left on the device), then fluid_update_all with a given dt, and
+    timestep_all, each against the oracle."""
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    cgs = setups.batch_blocks(SHAPES[:3], cfg, seed=12)
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    start(cfg, cgs)
+    dt0, _ = pk.timestep_all()
+    for ob in obs:
+        oracle.fill_ghosts(p, ob)
+    assert dt0 == min(oracle.time_step(p, ob, 0, -1.0)[0] for ob in obs)
+    pk.run_all(2)
+    oracle.run_list(p, obs, 2)
+    t_g, _ = pk.run_all(1)
+    t_o = oracle.run_list(p, obs, 1)
+    assert t_g == t_o
+    dt = 0.5 * pk.timestep_all()[0]
+    pk.fluid_update_all(dt, pk.BATCH_PERSISTENT)
+    oracle.fluid_update_list(p, obs, dt)
+    for cg, ob in zip(cgs, obs):
+        cg.download()
+        assert_same(cg, ob, "continued")
+    pk.finalize()
+
+
+def test_autotuned_list_bit_exact(oracle):
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    cgs = setups.batch_blocks(SHAPES, cfg, seed=21)
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    start(cfg, cgs)
+    pk.autotune_all()
+    seen = set()
+    for cg in cgs:
+        for d in range(3):
+            for f in range(2):
+                lanes, tp, ts = pk.list_tiles(d, f, cg)
+                assert lanes in (128, 256, 512) and 1 <= tp and tp * ts <= lanes
+                seen.add(lanes)
+    t_g, _ = pk.run_all(2)
+    t_o = oracle.run_list(p, obs, 2)
+    assert t_g == t_o
+    for cg, ob in zip(cgs, obs):
+        cg.download()
+        assert_same(cg, ob, "autotuned")
+    pk.finalize()
+
+
+def test_device_generated_state_matches_host():
+    """batch_blocks_device fills the mirrors from GPU tensors through
+    pk_upload_device; the state equals the host generator's to rounding of
+    sin (torch vs numpy), and pk_download_device returns it unchanged."""
+    import torch
+
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    sizes = [16, 24]
+    host = setups.batch_blocks(sizes, cfg, seed=5, first_id=100)
+    pk.load_library()
+    pk.finalize()
+    pk.init(cfg)
+    dev = setups.batch_blocks_device(sizes, cfg, seed=5, first_id=1)
+    for h, d in zip(host, dev):
+        ng = d.ng
+        u = torch.empty((ng[2], ng[1], ng[0], 10), dtype=torch.float64, device="cuda")
+        b = torch.empty((ng[2], ng[1], ng[0], 3), dtype=torch.float64, device="cuda")
+        psi = torch.empty((ng[2], ng[1], ng[0]), dtype=torch.float64, device="cuda")
+        d.download_device(u, b, psi)
+        uh = u.cpu().numpy().transpose(3, 2, 1, 0)
+        bh = b.cpu().numpy().transpose(3, 2, 1, 0)
+        assert np.max(np.abs(uh - h.u)) <= 1e-13
+        assert np.max(np.abs(bh - h.b)) <= 1e-13
+        assert float(psi.abs().max()) == 0.0
+    pk.finalize()
