@@ -101,6 +101,16 @@ _PROTOS = {
     "pk_run_all": [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_autotune_all": [ctypes.c_double, ctypes.c_double],
     "pk_list_tiles": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)],
+    "pk_adjoint_setup": [ctypes.c_int64, ctypes.c_int32],
+    "pk_state_save": [ctypes.c_int64, ctypes.c_int32],
+    "pk_state_load": [ctypes.c_int64, ctypes.c_int32],
+    "pk_adjoint_set": [ctypes.c_int64, ctypes.c_void_p],
+    "pk_adjoint_get": [ctypes.c_int64, ctypes.c_void_p],
+    "pk_fluid_update_record": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double],
+    "pk_sweep_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                         ctypes.POINTER(ctypes.c_double)],
+    "pk_fluid_update_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)],
+    "pk_timestep_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double],
     "pk_profile_begin": [],
     "pk_profile_end": [ctypes.POINTER(_ProfStats)],
 }
@@ -312,8 +322,13 @@ class GridContainer:
 
 
 def _device_ptr(t):
+    """Device address of a torch tensor handed to the library.  The library
+    works on its own stream: torch's queued work on the tensor is finished
+    first (the library's calls return with their own work finished)."""
     if not (t.is_cuda and t.is_contiguous() and str(t.dtype) == "torch.float64"):
         raise PiernikError("device arrays must be contiguous float64 tensors on the GPU")
+    import torch
+    torch.cuda.synchronize(t.device)
     return t.data_ptr()
 
 

@@ -1,0 +1,114 @@
+"""Differentiable rollout of the hydro sweep for PyTorch.
+
+A rollout is `nsteps` iterations of PIERNIK's main loop on one periodic
+hydro block (time_step then fluid_update_full; timestep.F90:112-240,
+fluidupdate.F90:99-136) on the GPU.  `rollout(u0, plan)` is a
+torch.autograd.Function: the forward pass keeps the state every
+`checkpoint_every` steps, the backward pass recomputes each segment from its
+checkpoint (keeping the state before every sweep) and runs the library's
+hand-written adjoint sweeps backwards through it.
+
+dt is part of the differentiated map: each step's dt = cfl * min(dl/c)
+depends on the state at the start of the step (through the minimising
+cell), except where the growth cap dt <= 2 dt_old binds, where it depends on
+the previous dt.
+
+Tensors use the reference layout u(5, nxg, nyg, nzg) in Fortran order, i.e.
+torch shape (nzg, nyg, nxg, 5), float64, on the GPU; ghost cells included
+(their input values are replaced by periodic images, so their gradient is 0).
+"""
+import ctypes
+
+import torch
+
+from . import PiernikError, _call, _device_ptr, timestep_fluid, fluid_update, update_boundaries
+from .setups import _DeviceBlock
+
+
+class RolloutPlan:
+    """One registered device block and the slots a checkpointed rollout uses."""
+
+    def __init__(self, cfg, n, nsteps, checkpoint_every=8, nb=4, cg_id=900001):
+        if cfg.magnetized or cfg.nfluids != 1:
+            raise PiernikError("the adjoint covers one hydro fluid")
+        self.cfg = cfg
+        self.nsteps = int(nsteps)
+        self.k = max(1, int(checkpoint_every))
+        self.nsw = sum(1 for h in cfg.has_dir if h) * 2
+        self.nck = (self.nsteps + self.k - 1) // self.k
+        shape = tuple(int(n) if h else 1 for h in cfg.has_dir) if not isinstance(n, (tuple, list)) else tuple(n)
+        self.cg = _DeviceBlock(shape, nb, cfg, cg_id)
+        self.cg.register()
+        _call("pk_adjoint_setup", self.cg.cg_id, self.nck + self.nsw * self.k)
+        self.dts = []
+        self.capped = []
+
+    @property
+    def shape(self):
+        ng = self.cg.ng
+        return (ng[2], ng[1], ng[0], 5)
+
+    def run(self, u0, record=True):
+        """Forward rollout from u0 (device tensor); returns the final state."""
+        cg = self.cg
+        cg.upload_device(u0.contiguous())
+        update_boundaries(cg)
+        self.dts, self.capped = [], []
+        dt_old = -1.0
+        for s in range(self.nsteps):
+            if record and s % self.k == 0:
+                _call("pk_state_save", cg.cg_id, s // self.k)
+            dt, _ = timestep_fluid(cg)
+            cap = False
+            if s >= 1 and dt_old > 0.0 and dt_old * 2.0 < dt:  # timestep.F90:200-208
+                dt, cap = dt_old * 2.0, True
+            fluid_update(cg, dt)
+            self.dts.append(dt)
+            self.capped.append(cap)
+            dt_old = dt
+        out = torch.empty(self.shape, dtype=torch.float64, device=u0.device)
+        cg.download_device(out)
+        return out
+
+    def backward(self, g):
+        """dL/du0 for dL/du_final = g (device tensor)."""
+        cg = self.cg
+        _call("pk_adjoint_set", cg.cg_id, ctypes.c_void_p(_device_ptr(g.contiguous())))
+        carry = 0.0
+        for seg in reversed(range(self.nck)):
+            s0, s1 = seg * self.k, min(seg * self.k + self.k, self.nsteps)
+            _call("pk_state_load", cg.cg_id, seg)
+            for s in range(s0, s1):
+                _call("pk_fluid_update_record", cg.cg_id, self.nck + (s - s0) * self.nsw, self.dts[s])
+            for s in reversed(range(s0, s1)):
+                slot = self.nck + (s - s0) * self.nsw
+                dtb = ctypes.c_double(0.0)
+                _call("pk_fluid_update_adjoint", cg.cg_id, slot, self.dts[s], ctypes.byref(dtb))
+                total = dtb.value + carry
+                if self.capped[s]:
+                    carry = 2.0 * total  # dt_s = 2 dt_{s-1}
+                else:
+                    _call("pk_timestep_adjoint", cg.cg_id, slot, total)
+                    carry = 0.0
+        out = torch.empty(self.shape, dtype=torch.float64, device=g.device)
+        _call("pk_adjoint_get", cg.cg_id, ctypes.c_void_p(_device_ptr(out)))
+        return out
+
+    def close(self):
+        self.cg.unregister()
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, plan):
+        ctx.plan = plan
+        return plan.run(u0.detach(), record=True)
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.plan.backward(g.detach()), None
+
+
+def rollout(u0, plan):
+    """Final state of `plan.nsteps` steps from u0, differentiable w.r.t. u0."""
+    return _Rollout.apply(u0, plan)

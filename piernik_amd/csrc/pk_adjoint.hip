@@ -1,0 +1,490 @@
+// pk_adjoint.hip -- reverse mode of the fused RK2 hydro sweep (gfx950).
+//
+// For one directional sweep of a periodic block with hydro fluids
+//   U* = floors(U + c1 R(U)),  U' = floors(U + c2 R(U*)),
+//   R(W)_i = F(W)_{i-1/2} - F(W)_{i+1/2},  c1 = (dt/2)/dl,  c2 = dt/dl,
+// (solve_cg_riemann.F90:366 inside the RK2 loop of sweeps.F90:217-272), the
+// kernel takes lam' = dL/dU' and returns lam = dL/dU, plus dL/d(dt), with
+// F = HLLC of the limited piecewise-linear face states (hlld.F90:615-841,
+// interpolations.F90:112-204).
+//
+// One lane is one cell of a tile of TP pencils x TS cells along the sweep.
+// The tile first recomputes both stages from U (forward values are the
+// forward sweep's bit for bit: same functions, same order, no contraction),
+// then runs the chain rule backwards; neighbour terms go through LDS.  The
+// output cells are lanes 8 .. TS-9: eight cells of halo on each side.
+// Reads wrap periodically along the sweep, so ghost cells are not used.
+#include <hip/hip_runtime.h>
+#include "pk_adjoint.hpp"
+#include "pk_adjoint_kernels.hpp"
+
+namespace pk {
+
+template <int DIR>
+__device__ __forceinline__ int plane_of_hd(int k)
+{
+    if (DIR == 0) return k;
+    if (DIR == 1) {
+        constexpr int p[5] = {0, 2, 1, 3, 4};
+        return p[k];
+    }
+    constexpr int p[5] = {0, 3, 2, 1, 4};
+    return p[k];
+}
+
+// limited face states of cell k from q_{k-1}, q_k, q_{k+1}
+template <int LIM>
+__device__ __forceinline__ void faces(int lim, const double (&qm)[5], const double (&q)[5], const double (&qp)[5],
+                                      double (&fp)[5], double (&fm)[5])
+{
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        const double dq = limit((LIM >= 0) ? LIM : lim, q[v] - qm[v], qp[v] - q[v]);
+        const double h = 0.5 * dq;
+        fp[v] = q[v] + h;
+        fm[v] = q[v] - h;
+    }
+}
+
+__device__ __forceinline__ void lds_get(const double *A, int nt, int t, double (&x)[5])
+{
+#pragma unroll
+    for (int v = 0; v < 5; v++) x[v] = A[v * nt + t];
+}
+__device__ __forceinline__ void lds_put(double *A, int nt, int t, const double (&x)[5])
+{
+#pragma unroll
+    for (int v = 0; v < 5; v++) A[v * nt + t] = x[v];
+}
+
+// q_bar of cell k from its face-state adjoints (fpb: left state of k+1/2,
+// fmb: right state of k-1/2) through the limiter: the own-cell part and the
+// dl/dr adjoints that the neighbours take.
+template <int LIM>
+__device__ __forceinline__ void faces_vjp(int lim, const double (&qm)[5], const double (&q)[5],
+                                          const double (&qp)[5], const double (&fpb)[5], const double (&fmb)[5],
+                                          double (&qb)[5], double (&dlb)[5], double (&drb)[5])
+{
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        const double dqb = 0.5 * fpb[v] - 0.5 * fmb[v];
+        limit_vjp((LIM >= 0) ? LIM : lim, q[v] - qm[v], qp[v] - q[v], dqb, dlb[v], drb[v]);
+        qb[v] = (fpb[v] + fmb[v]) + (dlb[v] - drb[v]);
+    }
+}
+
+template <int DIR, int LIM>
+__global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
+{
+    extern __shared__ double lds[];
+    const int nt = blockDim.x;
+    const int t = threadIdx.x;
+    double *Q1 = lds, *Q2 = lds + 5 * nt, *S1 = lds + 10 * nt, *S2 = lds + 15 * nt, *S3 = lds + 20 * nt;
+
+    constexpr int d1 = (DIR == 0) ? 1 : 0;
+    constexpr int d2 = (DIR == 2) ? 1 : 2;
+    const int step = (DIR == 0) ? 1 : a.TP;
+    int p, s;
+    if (DIR == 0) {
+        s = t % a.TS;
+        p = t / a.TS;
+    } else {
+        p = t % a.TP;
+        s = t / a.TP;
+    }
+    const int tm = max(t - step, 0), tp = min(t + step, nt - 1), tpp = min(t + 2 * step, nt - 1);
+
+    const int bid = blockIdx.x;
+    const int chunk = bid % a.chunks;
+    const int ptile = bid / a.chunks;
+    const int n1 = a.n[d1], nd = a.n[DIR];
+    const int npen = n1 * a.n[d2];
+    const int pf = ptile * a.TP + p;
+    const bool pvalid = t < a.TP * a.TS && pf < npen;
+    const int pfc = pf < npen ? pf : npen - 1;
+    const int c1i = a.g[d1] + pfc % n1;
+    const int c2i = a.g[d2] + pfc / n1;
+    const int cs = chunk * a.out - 8 + s;  // interior coordinate along the sweep, before wrapping
+    const int ci = ((cs % nd) + nd) % nd;
+    const bool outl = pvalid && s >= 8 && s < 8 + a.out && cs < nd;
+
+    int xyz[3];
+    xyz[DIR] = a.g[DIR] + ci;
+    xyz[d1] = c1i;
+    xyz[d2] = c2i;
+    const long long idx = (long long)xyz[0] + (long long)a.sy * xyz[1] + (long long)a.sz * xyz[2];
+
+    double U[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) U[k] = a.u[(long long)plane_of_hd<DIR>(k) * a.sv + idx];
+
+    // ---- forward: stage 1 ----
+    double q[5];
+    prim<HD>(U, a.gam_1, q);
+    lds_put(Q1, nt, t, q);
+    __syncthreads();
+    double qm[5], qp[5], fp[5], fm[5], F[5], R1[5], Us_pre[5], Us[5];
+    lds_get(Q1, nt, tm, qm);
+    lds_get(Q1, nt, tp, qp);
+    faces<LIM>(a.lim, qm, q, qp, fp, fm);
+    lds_put(S1, nt, t, fm);
+    __syncthreads();
+    {
+        double r[5];
+        lds_get(S1, nt, tp, r);
+        flux_hllc(fp, r, a.gam, F);
+        lds_put(S2, nt, t, F);
+    }
+    __syncthreads();
+    {
+        double Fm[5];
+        lds_get(S2, nt, tm, Fm);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            R1[v] = Fm[v] - F[v];
+            Us_pre[v] = (U[v] + a.c1 * R1[v]) + 0.0;
+            Us[v] = Us_pre[v];
+        }
+        unsigned fl = 0u;
+        double dm = 0.0;
+        floors<HD>(Us, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+    }
+    // ---- forward: stage 2 ----
+    double qs[5];
+    prim<HD>(Us, a.gam_1, qs);
+    lds_put(Q2, nt, t, qs);
+    __syncthreads();
+    lds_get(Q2, nt, tm, qm);
+    lds_get(Q2, nt, tp, qp);
+    faces<LIM>(a.lim, qm, qs, qp, fp, fm);
+    lds_put(S1, nt, t, fm);
+    __syncthreads();
+    {
+        double r[5];
+        lds_get(S1, nt, tp, r);
+        flux_hllc(fp, r, a.gam, F);
+        lds_put(S2, nt, t, F);
+    }
+    __syncthreads();
+    double lp[5];  // lam' through the floors of stage 2
+    double dtb = 0.0;
+    {
+        double Fm[5], R2[5], Up[5];
+        lds_get(S2, nt, tm, Fm);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            R2[v] = Fm[v] - F[v];
+            Up[v] = (U[v] + a.c2 * R2[v]) + 0.0;
+        }
+        // ---- reverse: stage 2 ----
+#pragma unroll
+        for (int k = 0; k < 5; k++) lp[k] = a.lam_out[(long long)plane_of_hd<DIR>(k) * a.sv + idx];
+        floors_vjp_hd(Up, a.smalld, a.smallei, a.use_smalld, a.use_smallei, lp);
+        if (outl) {
+            double acc = 0.0;
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc += lp[v] * R2[v];
+            dtb += acc / a.dl;  // c2 = dt/dl
+        }
+    }
+    lds_put(S1, nt, t, lp);
+    __syncthreads();
+    double fpb[5];
+    {
+        // interface k+1/2 of stage 2: l = left state of cell k, r = right state of cell k+1
+        double lpn[5], fb[5], q2[5], q3[5], l[5], r[5], junk[5], rb[5];
+        lds_get(S1, nt, tp, lpn);
+#pragma unroll
+        for (int v = 0; v < 5; v++) fb[v] = a.c2 * (lpn[v] - lp[v]);
+        lds_get(Q2, nt, tm, qm);
+        lds_get(Q2, nt, tp, q2);
+        lds_get(Q2, nt, tpp, q3);
+        faces<LIM>(a.lim, qm, qs, q2, l, junk);
+        faces<LIM>(a.lim, qs, q2, q3, junk, r);
+#pragma unroll
+        for (int v = 0; v < 5; v++) fpb[v] = rb[v] = 0.0;
+        hllc_vjp(l, r, a.gam, fb, fpb, rb);
+        lds_put(S2, nt, t, rb);  // belongs to cell k+1
+    }
+    __syncthreads();
+    double lsp[5];  // lam* through the floors of stage 1
+    {
+        double fmb[5], qb[5], dlb[5], drb[5];
+        lds_get(S2, nt, tm, fmb);
+        lds_get(Q2, nt, tm, qm);
+        lds_get(Q2, nt, tp, qp);
+        faces_vjp<LIM>(a.lim, qm, qs, qp, fpb, fmb, qb, dlb, drb);
+        lds_put(S1, nt, t, dlb);
+        lds_put(S3, nt, t, drb);
+        __syncthreads();
+        double dln[5], drp[5];
+        lds_get(S1, nt, tp, dln);
+        lds_get(S3, nt, tm, drp);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            qb[v] += drp[v] - dln[v];
+            lsp[v] = 0.0;
+        }
+        prim_vjp_hd(Us, a.gam_1, qb, lsp);
+        floors_vjp_hd(Us_pre, a.smalld, a.smallei, a.use_smalld, a.use_smallei, lsp);
+        if (outl) {
+            double acc = 0.0;
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc += lsp[v] * R1[v];
+            dtb += (0.5 * acc) / a.dl;  // c1 = (dt/2)/dl
+        }
+        lds_put(S2, nt, t, lsp);
+    }
+    __syncthreads();
+    // ---- reverse: stage 1 ----
+    {
+        double lsn[5], fb[5], q2[5], q3[5], l[5], r[5], junk[5], rb[5];
+        lds_get(S2, nt, tp, lsn);
+#pragma unroll
+        for (int v = 0; v < 5; v++) fb[v] = a.c1 * (lsn[v] - lsp[v]);
+        lds_get(Q1, nt, tm, qm);
+        lds_get(Q1, nt, tp, q2);
+        lds_get(Q1, nt, tpp, q3);
+        faces<LIM>(a.lim, qm, q, q2, l, junk);
+        faces<LIM>(a.lim, q, q2, q3, junk, r);
+#pragma unroll
+        for (int v = 0; v < 5; v++) fpb[v] = rb[v] = 0.0;
+        hllc_vjp(l, r, a.gam, fb, fpb, rb);
+        lds_put(S1, nt, t, rb);
+    }
+    __syncthreads();
+    double lam[5];
+    {
+        double fmb[5], qb[5], dlb[5], drb[5];
+        lds_get(S1, nt, tm, fmb);
+        lds_get(Q1, nt, tm, qm);
+        lds_get(Q1, nt, tp, qp);
+        faces_vjp<LIM>(a.lim, qm, q, qp, fpb, fmb, qb, dlb, drb);
+        lds_put(S2, nt, t, dlb);
+        lds_put(S3, nt, t, drb);
+        __syncthreads();
+        double dln[5], drp[5];
+        lds_get(S2, nt, tp, dln);
+        lds_get(S3, nt, tm, drp);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            qb[v] += drp[v] - dln[v];
+            lam[v] = 0.0;
+        }
+        prim_vjp_hd(U, a.gam_1, qb, lam);
+#pragma unroll
+        for (int v = 0; v < 5; v++) lam[v] = (lam[v] + lsp[v]) + lp[v];
+    }
+    if (outl) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) a.lam_in[(long long)plane_of_hd<DIR>(k) * a.sv + idx] = lam[k];
+    }
+    // dL/d(dt): per-wave sums in a fixed order, one slot per wave of the grid
+    const int lane = t & 63;
+    const int nact = min(64, nt - (t & ~63));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double other = __shfl_xor(dtb, o, 64);
+        if ((lane ^ o) < nact && lane < (lane ^ o)) dtb += other;
+        // lanes with the higher index keep their value; lane 0 ends with the sum
+    }
+    if (lane == 0) a.dtb_part[(long long)bid * ((nt + 63) / 64) + t / 64] = dtb;
+}
+
+// Sum of the per-wave partials in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ part, long long n,
+                                                   double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (long long i = t; i < n; i += 256) acc += part[i];
+    red[t] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] = red[t] + red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[0] = red[0];
+}
+
+// Adjoint of the periodic ghost refresh along DIR (k_fill_ghosts): every
+// ghost cell's adjoint is added to its interior image and cleared.  Run for
+// z, then y, then x (the refresh runs x, y, z).
+template <int DIR>
+__global__ void k_fold_ghosts(double *__restrict__ lam, long long sv, int sy, int sz, int ng0, int ng1, int ng2,
+                              int g0, int g1, int g2, int n0, int n1, int n2, int nplanes)
+{
+    const int ng[3] = {ng0, ng1, ng2}, g[3] = {g0, g1, g2}, n[3] = {n0, n1, n2};
+    constexpr int e1 = (DIR == 0) ? 1 : 0;
+    constexpr int e2 = (DIR == 2) ? 1 : 2;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // one thread per (plane, transverse cell, interior image along DIR); it
+    // owns the ghosts that copy that image, so no two threads touch one cell
+    const long long per = (long long)n[DIR] * ng[e1] * ng[e2];
+    if (tid >= per * nplanes) return;
+    const int v = (int)(tid / per);
+    long long r = tid % per;
+    const int i = (int)(r % n[DIR]);
+    r /= n[DIR];
+    int xyz[3];
+    xyz[e1] = (int)(r % ng[e1]);
+    xyz[e2] = (int)(r / ng[e1]);
+    double *base = lam + (long long)v * sv;
+    xyz[DIR] = g[DIR] + i;
+    const long long dst = (long long)xyz[0] + (long long)sy * xyz[1] + (long long)sz * xyz[2];
+    double acc = base[dst];
+    // ghost layers whose image is i: low ghost j (0 <= j < g) copies g + n - g + j ... i.e. image n + j - g + g
+    for (int j = 0; j < g[DIR]; j++) {
+        // low ghost at j copies interior index n - g + j; high ghost at g + n + j copies interior index j
+        if (i == n[DIR] - g[DIR] + j) {
+            xyz[DIR] = j;
+            const long long gi = (long long)xyz[0] + (long long)sy * xyz[1] + (long long)sz * xyz[2];
+            acc += base[gi];
+            base[gi] = 0.0;
+        }
+        if (i == j) {
+            xyz[DIR] = g[DIR] + n[DIR] + j;
+            const long long gi = (long long)xyz[0] + (long long)sy * xyz[1] + (long long)sz * xyz[2];
+            acc += base[gi];
+            base[gi] = 0.0;
+        }
+    }
+    base[dst] = acc;
+}
+
+__global__ void k_zero_best(unsigned long long *best)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *best = ~0ull;
+}
+
+// timestep adjoint: the cell and direction where dl_d/c_d is smallest (the
+// smallest linear index among exact ties), then dL/dU there.
+__global__ __launch_bounds__(256) void k_dt_argmin(const DtAdjArgs a, unsigned long long *best)
+{
+    const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % a.n[0]);
+        const long long r = c / a.n[0];
+        const int j = (int)(r % a.n[1]);
+        const int k = (int)(r / a.n[1]);
+        const long long idx =
+            (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
+        const double *u = a.u + idx;
+        const double d = u[0], m[3] = {u[a.sv], u[2 * a.sv], u[3 * a.sv]}, e = u[4 * a.sv];
+        const double ek = (0.5 * ((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2])) / d;
+        const double pr = (e - ek) * a.gam_1;
+        const double cs = sqrt(fabs((a.gam * pr) / d));
+        for (int q = 0; q < 3; q++) {
+            if (!a.has_dir[q]) continue;
+            const double v = (d > 0.0) ? fabs(m[q] / d) : 0.0;
+            const double cc = fmx(v + cs, 1.17549435082228750797e-38);
+            if (a.dl[q] / cc == a.mn) atomicMin(best, (unsigned long long)(c * 3 + q));
+        }
+    }
+}
+
+__global__ void k_dt_apply(const DtAdjArgs a, const unsigned long long *best, double *lam)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const unsigned long long b = *best;
+    if (b == ~0ull) return;
+    const long long c = (long long)(b / 3);
+    const int q = (int)(b % 3);
+    const int i = (int)(c % a.n[0]);
+    const long long r = c / a.n[0];
+    const int j = (int)(r % a.n[1]);
+    const int k = (int)(r / a.n[1]);
+    const long long idx = (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
+    double u[5], ub[5] = {0, 0, 0, 0, 0};
+    for (int v = 0; v < 5; v++) u[v] = a.u[(long long)v * a.sv + idx];
+    // dt = cfl * dl_q / c_q
+    dt_vjp_hd(u, q, a.dl[q], a.gam, a.gam_1, a.cfl * a.dtb, ub);
+    for (int v = 0; v < 5; v++) lam[(long long)v * a.sv + idx] += ub[v];
+}
+
+// ---- launchers ------------------------------------------------------------------
+
+template <int DIR, int LIM>
+static hipError_t adj_t(const AdjArgs &a, int nt, long long nblk, hipStream_t st)
+{
+    const size_t lds = sizeof(double) * 25 * nt;
+    static bool raised = false;
+    if (!raised) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_sweep_adjoint<DIR, LIM>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        raised = true;
+    }
+    hipLaunchKernelGGL((k_sweep_adjoint<DIR, LIM>), dim3((unsigned)nblk), dim3(nt), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int DIR>
+static hipError_t adj_lim(const AdjArgs &a, int nt, long long nblk, hipStream_t st)
+{
+    switch (a.lim) {
+    case LIM_VANLEER: return adj_t<DIR, LIM_VANLEER>(a, nt, nblk, st);
+    case LIM_MINMOD: return adj_t<DIR, LIM_MINMOD>(a, nt, nblk, st);
+    case LIM_MONCEN: return adj_t<DIR, LIM_MONCEN>(a, nt, nblk, st);
+    case LIM_SUPERBEE: return adj_t<DIR, LIM_SUPERBEE>(a, nt, nblk, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+long long sweep_adjoint_parts(const AdjArgs &a)
+{
+    const int nt = a.TP * a.TS;
+    return (long long)a.chunks * a.ptiles * ((nt + 63) / 64);
+}
+
+hipError_t launch_sweep_adjoint(int dir, const AdjArgs &a, double *dtb_out, hipStream_t st)
+{
+    const int nt = a.TP * a.TS;
+    if (nt < 1 || nt > 256 || a.out < 1 || a.out != a.TS - 16) return hipErrorInvalidValue;
+    const long long nblk = (long long)a.chunks * a.ptiles;
+    hipError_t e = hipErrorInvalidValue;
+    if (dir == 0) e = adj_lim<0>(a, nt, nblk, st);
+    if (dir == 1) e = adj_lim<1>(a, nt, nblk, st);
+    if (dir == 2) e = adj_lim<2>(a, nt, nblk, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, a.dtb_part, sweep_adjoint_parts(a), dtb_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_ghosts(double *lam, long long sv, int sy, int sz, const int ng[3], const int g[3],
+                              const int n[3], int nplanes, hipStream_t st)
+{
+    for (int d = 2; d >= 0; d--) {
+        if (g[d] == 0) continue;
+        const int e1 = (d == 0) ? 1 : 0, e2 = (d == 2) ? 1 : 2;
+        const long long total = (long long)n[d] * ng[e1] * ng[e2] * nplanes;
+        const unsigned blocks = (unsigned)((total + 255) / 256);
+        if (d == 0)
+            hipLaunchKernelGGL(k_fold_ghosts<0>, dim3(blocks), dim3(256), 0, st, lam, sv, sy, sz, ng[0], ng[1], ng[2],
+                               g[0], g[1], g[2], n[0], n[1], n[2], nplanes);
+        if (d == 1)
+            hipLaunchKernelGGL(k_fold_ghosts<1>, dim3(blocks), dim3(256), 0, st, lam, sv, sy, sz, ng[0], ng[1], ng[2],
+                               g[0], g[1], g[2], n[0], n[1], n[2], nplanes);
+        if (d == 2)
+            hipLaunchKernelGGL(k_fold_ghosts<2>, dim3(blocks), dim3(256), 0, st, lam, sv, sy, sz, ng[0], ng[1], ng[2],
+                               g[0], g[1], g[2], n[0], n[1], n[2], nplanes);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_dt_adjoint(const DtAdjArgs &a, unsigned long long *best, double *lam, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_best, dim3(1), dim3(1), 0, st, best);
+    const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+    long long blocks = (ncell + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_dt_argmin, dim3((unsigned)blocks), dim3(256), 0, st, a, best);
+    hipLaunchKernelGGL(k_dt_apply, dim3(1), dim3(64), 0, st, a, best, lam);
+    return hipGetLastError();
+}
+
+}  // namespace pk

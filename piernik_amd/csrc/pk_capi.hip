@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/piernik_hip.h"
+#include "pk_adjoint_kernels.hpp"
 #include "pk_kernels.hpp"
 
 namespace {
@@ -44,6 +45,14 @@ struct Block {
     double *d_dtout = nullptr;
     unsigned long long *d_dtslots = nullptr;  // timestep reduction written by the last sweep of a step
     bool dt_pending = false;                  // d_dtout holds timestep_fluid of the current state
+    // reverse mode (pk_adjoint_setup): saved states and the adjoint state
+    std::vector<double *> slots;
+    double *lam[2] = {nullptr, nullptr};
+    int lcur = 0;
+    double *d_dtb_part = nullptr;
+    long long dtb_parts = 0;
+    double *d_dtb = nullptr;
+    unsigned long long *d_best = nullptr;
 };
 
 const int kDtSlots = 1024;
@@ -531,8 +540,27 @@ int alloc_block(Block *b)
     return PK_OK;
 }
 
+void free_adjoint(Block *b)
+{
+    for (double *p : b->slots)
+        if (p) (void)hipFree(p);
+    b->slots.clear();
+    for (int i = 0; i < 2; i++) {
+        if (b->lam[i]) (void)hipFree(b->lam[i]);
+        b->lam[i] = nullptr;
+    }
+    if (b->d_dtb_part) (void)hipFree(b->d_dtb_part);
+    if (b->d_dtb) (void)hipFree(b->d_dtb);
+    if (b->d_best) (void)hipFree(b->d_best);
+    b->d_dtb_part = nullptr;
+    b->d_dtb = nullptr;
+    b->d_best = nullptr;
+    b->dtb_parts = 0;
+}
+
 void free_block(Block *b)
 {
+    free_adjoint(b);
     for (int i = 0; i < 2; i++)
         if (b->buf[i]) (void)hipFree(b->buf[i]);
     if (b->d_flags) (void)hipFree(b->d_flags);
@@ -627,6 +655,103 @@ int download_state(Block *b, double *u, double *bb, double *psi, const double *s
         if (psi) RCCHK(download_array(b, psi, map_psi(), src, dev));
     }
     return PK_OK;
+}
+
+// ------------------------------------------------------------- reverse mode
+//
+// Adjoint of the fused RK2 sweep for hydro fluids on periodic blocks.  The
+// state before every sweep of a recorded step is kept in a slot; the
+// adjoint state lam (same layout as the state) runs backwards through the
+// sweeps, each linearised at its saved input.
+
+int adj_check(Block *b)
+{
+    if (b->lam[0] == nullptr) return fail(PK_ESTATE, "pk_adjoint_setup has not been called for this block");
+    return PK_OK;
+}
+
+int slot_ptr(Block *b, int slot, double **p)
+{
+    if (slot < 0 || slot >= (int)b->slots.size())
+        return fail(PK_EINVAL, "slot %d outside 0..%d", slot, (int)b->slots.size() - 1);
+    *p = b->slots[slot];
+    return PK_OK;
+}
+
+size_t state_bytes(const Block *b) { return sizeof(double) * (size_t)b->sv * g_nplanes; }
+
+// tile of the adjoint sweep along d: TS - 16 output cells, TP pencils, <= 256 lanes
+void adj_tile(const Block *b, int d, int &TP, int &TS, int &out, int &chunks, int &ptiles)
+{
+    const int nt = 256;
+    const int span = b->n[d];
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    const long long npen = (long long)b->n[d1] * b->n[d2];
+    long long best = -1;
+    for (int k = 1; k <= span; k++) {
+        const int o = (span + k - 1) / k;
+        const int ts = o + 16;
+        if (ts > nt) continue;
+        const int tp = nt / ts;
+        const long long tiles = (long long)((span + o - 1) / o) * ((npen + tp - 1) / tp);
+        if (best < 0 || tiles < best) {
+            best = tiles;
+            TS = ts;
+            TP = tp;
+            out = o;
+        }
+        if (o <= 1) break;
+    }
+    if (TP > npen) TP = (int)npen;
+    chunks = (span + out - 1) / out;
+    ptiles = (int)((npen + TP - 1) / TP);
+}
+
+int do_sweep_adjoint(Block *b, const double *u, int d, double dt, double *dtb)
+{
+    pk::AdjArgs a{};
+    a.u = u + b->xo;
+    a.lam_out = b->lam[b->lcur] + b->xo;
+    a.lam_in = b->lam[1 - b->lcur] + b->xo;
+    a.dtb_part = b->d_dtb_part;
+    a.sv = b->sv;
+    a.sy = b->sy;
+    a.sz = b->sz;
+    for (int k = 0; k < 3; k++) {
+        a.ng[k] = b->ng[k];
+        a.g[k] = b->g[k];
+        a.n[k] = b->n[k];
+    }
+    adj_tile(b, d, a.TP, a.TS, a.out, a.chunks, a.ptiles);
+    if (pk::sweep_adjoint_parts(a) > b->dtb_parts) return fail(PK_EINVAL, "adjoint partial-sum buffer too small");
+    a.lim = g_cfg.limiter;
+    a.gam = g_cfg.fluid[0].gamma;
+    a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
+    a.c1 = (0.5 * dt) / b->dl[d];
+    a.c2 = (1.0 * dt) / b->dl[d];
+    a.dl = b->dl[d];
+    a.smalld = g_cfg.smalld;
+    a.smallei = g_cfg.smallei;
+    a.use_smalld = g_cfg.use_smalld;
+    a.use_smallei = g_cfg.use_smallei;
+    HIPCHK(pk::launch_sweep_adjoint(d, a, b->d_dtb, g_stream));
+    b->lcur = 1 - b->lcur;
+    HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtb, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    *dtb += g_pinned[0];
+    return PK_OK;
+}
+
+// The directions of the sweeps of one step in order (XYZ then ZYX).
+std::vector<int> step_dirs()
+{
+    std::vector<int> v;
+    for (int half = 0; half < 2; half++)
+        for (int s = 0; s < 3; s++) {
+            const int d = half == 0 ? s : 2 - s;
+            if (g_cfg.has_dir[d]) v.push_back(d);
+        }
+    return v;
 }
 
 // ---------------------------------------------------------------- block list
@@ -1527,6 +1652,191 @@ int pk_list_tiles(int32_t ddim, int32_t ifluid, int64_t cg_id, int32_t out[3])
         return PK_OK;
     }
     return fail(PK_ENOBLOCK, "unknown cg_id %lld", (long long)cg_id);
+}
+
+// ---- reverse mode -------------------------------------------------------------
+
+int pk_adjoint_setup(int64_t cg_id, int32_t nslots)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    if (g_mhd || g_nfl != 1) return fail(PK_EINVAL, "the adjoint covers one hydro (HLLC) fluid");
+    if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "the adjoint covers the RK2 sweep");
+    if (nslots < 1) return fail(PK_EINVAL, "nslots = %d", nslots);
+    for (int d = 0; d < 3; d++)
+        if (g_cfg.has_dir[d] && b->g[d] < 4) return fail(PK_EINVAL, "the fused RK2 sweep needs nb >= 4");
+    HIPCHK(hipStreamSynchronize(g_stream));
+    free_adjoint(b);
+    const size_t bytes = state_bytes(b);
+    for (int i = 0; i < 2; i++) {
+        HIPCHK(hipMalloc((void **)&b->lam[i], bytes));
+        HIPCHK(hipMemsetAsync(b->lam[i], 0, bytes, g_stream));
+    }
+    b->slots.assign(nslots, nullptr);
+    for (int i = 0; i < nslots; i++) HIPCHK(hipMalloc((void **)&b->slots[i], bytes));
+    long long parts = 1;
+    for (int d = 0; d < 3; d++) {
+        if (!g_cfg.has_dir[d]) continue;
+        int TP = 1, TS = 17, out = 1, chunks = 1, ptiles = 1;
+        adj_tile(b, d, TP, TS, out, chunks, ptiles);
+        const long long p = (long long)chunks * ptiles * ((TP * TS + 63) / 64);
+        if (p > parts) parts = p;
+    }
+    HIPCHK(hipMalloc((void **)&b->d_dtb_part, sizeof(double) * parts));
+    b->dtb_parts = parts;
+    HIPCHK(hipMalloc((void **)&b->d_dtb, sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->d_best, sizeof(unsigned long long)));
+    b->lcur = 0;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return PK_OK;
+}
+
+int pk_state_save(int64_t cg_id, int32_t slot)
+{
+    Block *b;
+    double *p;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    RCCHK(slot_ptr(b, slot, &p));
+    HIPCHK(hipMemcpyAsync(p, b->buf[b->cur], state_bytes(b), hipMemcpyDeviceToDevice, g_stream));
+    return PK_OK;
+}
+
+int pk_state_load(int64_t cg_id, int32_t slot)
+{
+    Block *b;
+    double *p;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    RCCHK(slot_ptr(b, slot, &p));
+    list_touch();
+    HIPCHK(hipMemcpyAsync(b->buf[b->cur], p, state_bytes(b), hipMemcpyDeviceToDevice, g_stream));
+    b->dt_pending = false;
+    b->uh = -1;
+    return PK_OK;
+}
+
+int pk_adjoint_set(int64_t cg_id, const double *lam)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    if (!lam) return fail(PK_EINVAL, "null adjoint array");
+    double *dst = b->lam[b->lcur];
+    HIPCHK(hipMemsetAsync(dst, 0, state_bytes(b), g_stream));
+    RCCHK(upload_array(b, lam, map_u(), dst + b->xo, true));
+    // the ghost cells are periodic copies of interior cells
+    HIPCHK(pk::launch_fold_ghosts(dst + b->xo, b->sv, b->sy, b->sz, b->ng, b->g, b->n, g_nplanes, g_stream));
+    return PK_OK;
+}
+
+int pk_adjoint_get(int64_t cg_id, double *lam)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    if (!lam) return fail(PK_EINVAL, "null adjoint array");
+    return download_array(b, lam, map_u(), b->lam[b->lcur] + b->xo, true);
+}
+
+int pk_sweep_adjoint(int64_t cg_id, int32_t slot, int32_t ddim, double dt, double *dt_bar)
+{
+    Block *b;
+    double *p;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    RCCHK(slot_ptr(b, slot, &p));
+    if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    double dtb = 0.0;
+    RCCHK(do_sweep_adjoint(b, p, ddim, dt, &dtb));
+    if (dt_bar) *dt_bar += dtb;
+    return PK_OK;
+}
+
+int pk_fluid_update_record(int64_t cg_id, int32_t slot0, double dt)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    const std::vector<int> dirs = step_dirs();
+    if (slot0 < 0 || slot0 + (int)dirs.size() > (int)b->slots.size())
+        return fail(PK_EINVAL, "slots %d..%d do not exist", slot0, slot0 + (int)dirs.size() - 1);
+    list_touch();
+    for (size_t i = 0; i < dirs.size(); i++) {
+        HIPCHK(hipMemcpyAsync(b->slots[slot0 + i], b->buf[b->cur], state_bytes(b), hipMemcpyDeviceToDevice,
+                              g_stream));
+        const bool last = i + 1 == dirs.size();
+        RCCHK(do_sweep(b, dirs[i], dt, 0.0, 1.0, true, last));
+    }
+    return PK_OK;
+}
+
+int pk_fluid_update_adjoint(int64_t cg_id, int32_t slot0, double dt, double *dt_bar)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    const std::vector<int> dirs = step_dirs();
+    if (slot0 < 0 || slot0 + (int)dirs.size() > (int)b->slots.size())
+        return fail(PK_EINVAL, "slots %d..%d do not exist", slot0, slot0 + (int)dirs.size() - 1);
+    double dtb = 0.0;
+    for (int i = (int)dirs.size() - 1; i >= 0; i--) RCCHK(do_sweep_adjoint(b, b->slots[slot0 + i], dirs[i], dt, &dtb));
+    if (dt_bar) *dt_bar += dtb;
+    return PK_OK;
+}
+
+int pk_timestep_adjoint(int64_t cg_id, int32_t slot, double dt_bar)
+{
+    Block *b;
+    double *p;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(adj_check(b));
+    RCCHK(slot_ptr(b, slot, &p));
+    pk::DtArgs a{};
+    a.u = p + b->xo;
+    a.sv = b->sv;
+    a.sy = b->sy;
+    a.sz = b->sz;
+    for (int d = 0; d < 3; d++) {
+        a.ng[d] = b->ng[d];
+        a.g[d] = b->g[d];
+        a.n[d] = b->n[d];
+        a.has_dir[d] = g_cfg.has_dir[d];
+        a.dl[d] = b->dl[d];
+    }
+    a.plane0 = 0;
+    a.bplane = 0;
+    a.magnetized = 0;
+    a.gam = g_cfg.fluid[0].gamma;
+    a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
+    HIPCHK(pk::launch_dt(a, g_part, b->d_dtout, g_stream));
+    HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtout, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    double mn = g_pinned[0];
+    if (g_pinned[1] < mn) mn = g_pinned[1];
+    if (g_pinned[2] < mn) mn = g_pinned[2];
+    b->dt_pending = false;
+    pk::DtAdjArgs q{};
+    q.u = a.u;
+    q.sv = b->sv;
+    q.sy = b->sy;
+    q.sz = b->sz;
+    for (int d = 0; d < 3; d++) {
+        q.n[d] = b->n[d];
+        q.g[d] = b->g[d];
+        q.has_dir[d] = g_cfg.has_dir[d];
+        q.dl[d] = b->dl[d];
+    }
+    q.gam = a.gam;
+    q.gam_1 = a.gam_1;
+    q.mn = mn;
+    q.cfl = g_cfg.cfl;
+    q.dtb = dt_bar;
+    HIPCHK(pk::launch_dt_adjoint(q, b->d_best, b->lam[b->lcur] + b->xo, g_stream));
+    return PK_OK;
 }
 
 int pk_profile_begin(void)

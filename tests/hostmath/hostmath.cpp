@@ -1,6 +1,6 @@
 // Host build of the sweep's per-interface arithmetic (piernik_amd/csrc/pk_math.hpp)
 // for bitwise comparison with the oracle on the CPU.  Test helper only.
-#include "../../piernik_amd/csrc/pk_math.hpp"
+#include "../../piernik_amd/csrc/pk_adjoint.hpp"
 
 extern "C" {
 
@@ -72,5 +72,57 @@ void hm_floors(int m, int mhd, double smalld, double smallei, int use_smalld, in
             for (int v = 0; v < 5; v++) u[v * m + i] = a[v];
         }
     }
+}
+
+// ---- reverse mode (pk_adjoint.hpp) ----
+void hm_prim_vjp_hd(double gam_1, const double *u, const double *qb, double *ub)
+{
+    double a[5], b[5], o[5] = {0, 0, 0, 0, 0};
+    for (int v = 0; v < 5; v++) {
+        a[v] = u[v];
+        b[v] = qb[v];
+    }
+    pk::prim_vjp_hd(a, gam_1, b, o);
+    for (int v = 0; v < 5; v++) ub[v] = o[v];
+}
+
+void hm_limit_vjp(int lim, double dl, double dr, double dqb, double *out)
+{
+    pk::limit_vjp(lim, dl, dr, dqb, out[0], out[1]);
+}
+
+void hm_hllc_vjp(const double *l, const double *r, double gam, const double *fb, double *lb, double *rb)
+{
+    for (int v = 0; v < 5; v++) lb[v] = rb[v] = 0.0;
+    pk::hllc_vjp(l, r, gam, fb, lb, rb);
+}
+
+void hm_floors_hd(double smalld, double smallei, int use_smalld, int use_smallei, double *u)
+{
+    double a[5];
+    for (int v = 0; v < 5; v++) a[v] = u[v];
+    unsigned fl = 0;
+    double dm = 0.0;
+    pk::floors<pk::HD>(a, smalld, smallei, use_smalld, use_smallei, fl, dm);
+    for (int v = 0; v < 5; v++) u[v] = a[v];
+}
+
+void hm_floors_vjp_hd(double smalld, double smallei, int use_smalld, int use_smallei, const double *u, double *ub)
+{
+    double a[5], b[5];
+    for (int v = 0; v < 5; v++) {
+        a[v] = u[v];
+        b[v] = ub[v];
+    }
+    pk::floors_vjp_hd(a, smalld, smallei, use_smalld, use_smallei, b);
+    for (int v = 0; v < 5; v++) ub[v] = b[v];
+}
+
+void hm_dt_vjp_hd(const double *u, int qdir, double dl, double gam, double tb, double *ub)
+{
+    double a[5], b[5] = {0, 0, 0, 0, 0};
+    for (int v = 0; v < 5; v++) a[v] = u[v];
+    pk::dt_vjp_hd(a, qdir, dl, gam, gam - 1.0, tb, b);
+    for (int v = 0; v < 5; v++) ub[v] = b[v];
 }
 }

@@ -5,39 +5,18 @@ A CPU-side filter for transcription mistakes in the device code; the GPU
 parity tests check the device build of the same functions."""
 import ctypes
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-HM = os.path.join(HERE, "hostmath")
-SRC = [os.path.join(HM, "hostmath.cpp"), os.path.join(HERE, "..", "piernik_amd", "csrc", "pk_math.hpp")]
-LIB = os.path.join(HM, "libhostmath.so")
 P = ctypes.POINTER(ctypes.c_double)
-
-
-def _build():
-    if os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(s) for s in SRC):
-        return
-    subprocess.run(["hipcc", "-x", "hip", "--offload-host-only", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
-                    "-std=c++17", "-o", LIB, SRC[0]], check=True, capture_output=True)
 
 
 @pytest.fixture(scope="module")
 def hm():
-    _build()
-    L = ctypes.CDLL(LIB)
-    L.hm_spacing.argtypes = [ctypes.c_double]
-    L.hm_spacing.restype = ctypes.c_double
-    L.hm_limit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
-    L.hm_limit.restype = ctypes.c_double
-    L.hm_prim.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P]
-    L.hm_hllc.argtypes = [ctypes.c_int, P, P, ctypes.c_double, P]
-    L.hm_hlld.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double, P]
-    L.hm_floors.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                            ctypes.c_int, P, ctypes.POINTER(ctypes.c_uint)]
-    return L
+    from hostmath import loader
+    return loader.load()
 
 
 def dp(a):
