@@ -106,7 +106,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096"], default="mhd256")
+    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128"], default="mhd256")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--nblocks", type=int, default=4096, help="blocks of the batched list")
     ap.add_argument("--n", type=int, default=None, help="override the block size (cells per side)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -128,6 +129,14 @@ def main():
     print(f"[bench] cpu baseline done: {cpu and round(cpu['value'])} cell-updates/s", file=sys.stderr, flush=True)
     if args.workload == "batch4096":
         print(json.dumps(run_batch(args, cpu)))
+        return
+    if args.workload == "adjoint128":
+        adj = adjoint_cost()
+        print(json.dumps({
+            "metric": "adjoint/forward cost ratio, 128^3 hydro rollout", "value": adj["ratio"], "unit": "x",
+            "n_gpus": 1, "steps": adj["steps"], "warmup": 1, "ms_per_step": adj["forward_ms"] / adj["steps"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": adj["workload"]}, "adjoint": adj}))
         return
 
     import piernik_amd as pk
@@ -174,6 +183,11 @@ def main():
                for d in range(3)]
 
     rep = pk.get_report(cg)
+    pk.finalize()
+    adj = None
+    if not args.no_adjoint:
+        print("[bench] adjoint cost ratio (128^3 hydro rollout)", file=sys.stderr, flush=True)
+        adj = adjoint_cost()
     line = {
         "metric": METRIC if args.workload == "mhd256" else "FP64 HD cell-updates/s at 64^3",
         "value": value,
@@ -216,10 +230,62 @@ def main():
         },
         "cpu_baseline": cpu,
         "floors": rep,
-        "adjoint_forward_cost_ratio": None,
+        "adjoint_forward_cost_ratio": adj and adj["ratio"],
+        "adjoint": adj,
     }
     print(json.dumps(line))
+
+
+def adjoint_cost(n=128, steps=32, every=8):
+    """Cost of the gradient of a hydro rollout: t(forward with checkpoints +
+    backward with segment recompute) / t(forward), on the 128^3 HD problem
+    (rho = 1 + 0.1 G, p = 1, v = 0, HLLC, van Leer, CFL 0.5, 32 steps,
+    checkpoint every 8, loss 0.5*||rho - rho_target||^2)."""
+    import torch
+    import piernik_amd as pk
+    from piernik_amd import adjoint, setups
+
+    u, target, cfg = setups.adjoint_hydro(n, seed=0)
+    pk.init(cfg)
+    plan = adjoint.RolloutPlan(cfg, n, steps, checkpoint_every=every)
+    u0 = setups.to_device_layout(u)
+    tgt = torch.as_tensor(target).cuda()
+
+    def grad_of(uf):
+        g = torch.zeros_like(uf)
+        g[4:-4, 4:-4, 4:-4, 0] = (uf[4:-4, 4:-4, 4:-4, 0] - tgt.permute(2, 1, 0))
+        return g
+
+    uf = plan.run(u0)  # warm-up
+    plan.backward(grad_of(uf))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.run(u0, record=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    uf = plan.run(u0, record=True)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    g = grad_of(uf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.backward(g)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    plan.close()
     pk.finalize()
+    return {
+        "ratio": (t_fwd + t_bwd) / t_plain,
+        "forward_ms": t_plain * 1e3,
+        "forward_checkpointed_ms": t_fwd * 1e3,
+        "backward_ms": t_bwd * 1e3,
+        "steps": steps,
+        "checkpoint_every": every,
+        "workload": (f"3D hydro, HLLC, van Leer, RK2, CFL 0.5, periodic {n}^3, rho = 1 + 0.1 G (|k| <= 6), "
+                     f"p = 1, v = 0, {steps} steps, checkpoint every {every}, loss 0.5*||rho - rho_target||^2"),
+        "definition": "(forward with checkpoints + backward with segment recompute) / plain forward",
+    }
 
 
 def run_batch(args, cpu):

@@ -308,3 +308,45 @@ class _DeviceBlock(GridContainer):
         self.u = self.uh = self.b = self.bh = self.psi = self.psih = None
         self.cg_id = int(cg_id)
         self.registered = False
+
+
+# ---- differentiable hydro rollout ------------------------------------------------
+
+
+def unit_gaussian(n, seed, kmax=6.0):
+    """Seeded periodic Gaussian random field on n^3 with 0 < |k| <= kmax,
+    normalised to zero mean and unit variance."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kz = np.fft.rfftfreq(n, d=1.0 / n)
+    KX, KY, KZ = np.meshgrid(k1, k1, kz, indexing="ij")
+    k = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    band = (k > 0) & (k <= kmax)
+    ph = rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape)
+    f = np.fft.irfftn(ph * band, s=(n, n, n), axes=(0, 1, 2))
+    return f / np.sqrt(np.mean(f * f))
+
+
+def adjoint_hydro(n=128, seed=0, gamma=5.0 / 3.0, amp=0.1):
+    """Initial state and target of the adjoint rollout: rho = 1 + amp*G
+    (G unit-variance Gaussian, |k| <= 6), p = 1, v = 0; target density from
+    seed + 1.  Returns (u (5, n, n, n) interior, rho_target (n, n, n), cfg)."""
+    rho = 1.0 + amp * unit_gaussian(n, seed)
+    u = np.zeros((5, n, n, n))
+    u[0] = rho
+    u[4] = 1.0 / (gamma - 1.0)
+    target = 1.0 + amp * unit_gaussian(n, seed + 1)
+    cfg = Config(gamma=gamma, magnetized=False, limiter="vanleer", cfl=0.5, smalld=1e-10, smallei=1e-10)
+    return u, target, cfg
+
+
+def to_device_layout(u, nb=4):
+    """(5, nx, ny, nz) interior array (numpy or torch) -> torch tensor
+    (nzg, nyg, nxg, 5) on the GPU with periodic ghost cells, the reference
+    layout u(5, nxg, nyg, nzg)."""
+    import torch
+
+    t = torch.as_tensor(np.ascontiguousarray(u) if isinstance(u, np.ndarray) else u, dtype=torch.float64)
+    for ax in (1, 2, 3):
+        t = torch.cat([t.narrow(ax, t.shape[ax] - nb, nb), t, t.narrow(ax, 0, nb)], dim=ax)
+    return t.permute(3, 2, 1, 0).contiguous().cuda()

@@ -37,4 +37,11 @@ def load():
     L.hm_floors_hd.argtypes = [d, d, i, i, P]
     L.hm_floors_vjp_hd.argtypes = [d, d, i, i, P, P]
     L.hm_dt_vjp_hd.argtypes = [P, i, d, d, d, P]
+    L.hm_hlld_vjp.argtypes = [P, P, d, d, P, P, P, P]
+    L.hm_hlld1.argtypes = [P, P, d, d, P]
+    L.hm_prim_vjp_mhd.argtypes = [d, P, P, P]
+    L.hm_floors_mhd.argtypes = [d, d, i, i, P]
+    L.hm_floors_vjp_mhd.argtypes = [d, d, i, i, P, P]
+    L.hm_dt_vjp_mhd.argtypes = [P, i, d, d, d, P]
+    L.hm_hlld_branch.argtypes = [P, P, d, P]
     return L
