@@ -200,8 +200,8 @@ int pk_autotune_all(double dt, double chspeed);
 int pk_list_tiles(int32_t ddim, int32_t ifluid, int64_t cg_id, int32_t out[3]);
 
 /* ---- reverse mode (no counterpart in the reference) ------------------------ */
-/* Adjoint of the fused RK2 sweep for one hydro (HLLC) fluid on blocks that
- * are periodic on themselves: the limiter and wave-speed branches are
+/* Adjoint of the fused RK2 sweep for one fluid (HLLC, or HLLD + GLM) on
+ * blocks that are periodic on themselves: the limiter and wave-speed branches are
  * differentiated along the branch the forward took (exact ties of min/max
  * split evenly), floors included.  The adjoint state lam has the state's
  * layout; each call below moves it backwards through one operation,
@@ -212,19 +212,23 @@ int pk_adjoint_setup(int64_t cg_id, int32_t nslots);
 /* slot <- current state; current state <- slot. */
 int pk_state_save(int64_t cg_id, int32_t slot);
 int pk_state_load(int64_t cg_id, int32_t slot);
-/* lam <- dL/du from a DEVICE array in the reference layout u(nvar, nxg,
- * nyg, nzg); entries of ghost cells are added to their periodic images.
- * pk_adjoint_get writes lam back (ghost entries 0). */
-int pk_adjoint_set(int64_t cg_id, const double *lam);
-int pk_adjoint_get(int64_t cg_id, double *lam);
+/* lam <- dL/d(u, b, psi) from DEVICE arrays in the reference layout
+ * (b and psi of the magnetized fluid; NULL for hydro); entries of ghost
+ * cells are added to their periodic images.  pk_adjoint_get writes lam back
+ * (ghost entries 0). */
+int pk_adjoint_set(int64_t cg_id, const double *lam_u, const double *lam_b, const double *lam_psi);
+int pk_adjoint_get(int64_t cg_id, double *lam_u, double *lam_b, double *lam_psi);
 /* fluid_update_full with the given dt, saving the state before each of its
  * sweeps into slots slot0, slot0+1, ... (6 in 3D); results are those of
  * pk_fluid_update. */
 int pk_fluid_update_record(int64_t cg_id, int32_t slot0, double dt);
-/* lam <- (d sweep / d u)^T lam for the sweep along ddim linearised at the
- * state in slot; *dt_bar += dL/d(dt) of that sweep. */
-int pk_sweep_adjoint(int64_t cg_id, int32_t slot, int32_t ddim, double dt, double *dt_bar);
-/* The sweeps of a step recorded by pk_fluid_update_record, in reverse. */
+/* lam <- (d sweep / d u)^T lam for the sweep pk_sweep(ddim, dt, chspeed,
+ * psi_fac) linearised at the state in slot; *dt_bar += dL/d(dt) at fixed
+ * chspeed, *ch_bar += dL/d(chspeed) (either may be NULL). */
+int pk_sweep_adjoint(int64_t cg_id, int32_t slot, int32_t ddim, double dt, double chspeed, double psi_fac,
+                     double *dt_bar, double *ch_bar);
+/* The sweeps of a step recorded by pk_fluid_update_record, in reverse;
+ * *dt_bar includes the dependence of chspeed on dt (hdc.F90:122). */
 int pk_fluid_update_adjoint(int64_t cg_id, int32_t slot0, double dt, double *dt_bar);
 /* lam += dt_bar * d(dt)/du for dt = cfl * min(dl/c) (timestep_fluid) at
  * the state in slot, through the minimising cell and direction. */

@@ -104,11 +104,11 @@ _PROTOS = {
     "pk_adjoint_setup": [ctypes.c_int64, ctypes.c_int32],
     "pk_state_save": [ctypes.c_int64, ctypes.c_int32],
     "pk_state_load": [ctypes.c_int64, ctypes.c_int32],
-    "pk_adjoint_set": [ctypes.c_int64, ctypes.c_void_p],
-    "pk_adjoint_get": [ctypes.c_int64, ctypes.c_void_p],
+    "pk_adjoint_set": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "pk_adjoint_get": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "pk_fluid_update_record": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double],
-    "pk_sweep_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
-                         ctypes.POINTER(ctypes.c_double)],
+    "pk_sweep_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                         ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_fluid_update_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)],
     "pk_timestep_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double],
     "pk_profile_begin": [],

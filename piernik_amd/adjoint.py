@@ -13,9 +13,12 @@ depends on the state at the start of the step (through the minimising
 cell), except where the growth cap dt <= 2 dt_old binds, where it depends on
 the previous dt.
 
-Tensors use the reference layout u(5, nxg, nyg, nzg) in Fortran order, i.e.
-torch shape (nzg, nyg, nxg, 5), float64, on the GPU; ghost cells included
-(their input values are replaced by periodic images, so their gradient is 0).
+A state is a tuple of tensors in the reference layout: u(5, nxg, nyg, nzg)
+in Fortran order, i.e. torch shape (nzg, nyg, nxg, 5), and for MHD also
+b (nzg, nyg, nxg, 3) and psi (nzg, nyg, nxg); float64, on the GPU, ghost
+cells included (their input values are replaced by periodic images, so their
+gradient is 0).  For MHD the GLM speed ch = cfl_glm*min(dl)/dt is part of the
+differentiated map.
 """
 import ctypes
 
@@ -29,8 +32,8 @@ class RolloutPlan:
     """One registered device block and the slots a checkpointed rollout uses."""
 
     def __init__(self, cfg, n, nsteps, checkpoint_every=8, nb=4, cg_id=900001):
-        if cfg.magnetized or cfg.nfluids != 1:
-            raise PiernikError("the adjoint covers one hydro fluid")
+        if cfg.nfluids != 1:
+            raise PiernikError("the adjoint covers one fluid per block")
         self.cfg = cfg
         self.nsteps = int(nsteps)
         self.k = max(1, int(checkpoint_every))
@@ -48,10 +51,21 @@ class RolloutPlan:
         ng = self.cg.ng
         return (ng[2], ng[1], ng[0], 5)
 
-    def run(self, u0, record=True):
-        """Forward rollout from u0 (device tensor); returns the final state."""
+    def _empty(self, device):
+        ng = self.cg.ng
+        u = torch.empty((ng[2], ng[1], ng[0], 5), dtype=torch.float64, device=device)
+        if not self.cfg.magnetized:
+            return (u,)
+        return (u, torch.empty((ng[2], ng[1], ng[0], 3), dtype=torch.float64, device=device),
+                torch.empty((ng[2], ng[1], ng[0]), dtype=torch.float64, device=device))
+
+    def run(self, state, record=True):
+        """Forward rollout from state (tensor u, or tuple (u, b, psi) for MHD);
+        returns the final state in the same form."""
         cg = self.cg
-        cg.upload_device(u0.contiguous())
+        single = torch.is_tensor(state)
+        state = (state,) if single else tuple(state)
+        cg.upload_device(*[t.contiguous() for t in state])
         update_boundaries(cg)
         self.dts, self.capped = [], []
         dt_old = -1.0
@@ -66,14 +80,17 @@ class RolloutPlan:
             self.dts.append(dt)
             self.capped.append(cap)
             dt_old = dt
-        out = torch.empty(self.shape, dtype=torch.float64, device=u0.device)
-        cg.download_device(out)
-        return out
+        out = self._empty(state[0].device)
+        cg.download_device(*out)
+        return out[0] if single else out
 
     def backward(self, g):
-        """dL/du0 for dL/du_final = g (device tensor)."""
+        """dL/d(initial state) for dL/d(final state) = g, in the same form."""
         cg = self.cg
-        _call("pk_adjoint_set", cg.cg_id, ctypes.c_void_p(_device_ptr(g.contiguous())))
+        single = torch.is_tensor(g)
+        g = (g,) if single else tuple(g)
+        ptrs = [ctypes.c_void_p(_device_ptr(t.contiguous())) for t in g] + [None] * (3 - len(g))
+        _call("pk_adjoint_set", cg.cg_id, *ptrs)
         carry = 0.0
         for seg in reversed(range(self.nck)):
             s0, s1 = seg * self.k, min(seg * self.k + self.k, self.nsteps)
@@ -90,9 +107,10 @@ class RolloutPlan:
                 else:
                     _call("pk_timestep_adjoint", cg.cg_id, slot, total)
                     carry = 0.0
-        out = torch.empty(self.shape, dtype=torch.float64, device=g.device)
-        _call("pk_adjoint_get", cg.cg_id, ctypes.c_void_p(_device_ptr(out)))
-        return out
+        out = self._empty(g[0].device)
+        ptrs = [ctypes.c_void_p(_device_ptr(t)) for t in out] + [None] * (3 - len(out))
+        _call("pk_adjoint_get", cg.cg_id, *ptrs)
+        return out[0] if single else out
 
     def close(self):
         self.cg.unregister()
@@ -100,15 +118,21 @@ class RolloutPlan:
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, plan):
+    def forward(ctx, plan, *state):
         ctx.plan = plan
-        return plan.run(u0.detach(), record=True)
+        ctx.n = len(state)
+        out = plan.run(tuple(t.detach() for t in state), record=True)
+        return out
 
     @staticmethod
-    def backward(ctx, g):
-        return ctx.plan.backward(g.detach()), None
+    def backward(ctx, *g):
+        grads = ctx.plan.backward(tuple(t.detach().contiguous() for t in g))
+        return (None,) + tuple(grads)
 
 
-def rollout(u0, plan):
-    """Final state of `plan.nsteps` steps from u0, differentiable w.r.t. u0."""
-    return _Rollout.apply(u0, plan)
+def rollout(state, plan):
+    """Final state of `plan.nsteps` steps, differentiable w.r.t. the initial
+    state (a tensor u for hydro, a tuple (u, b, psi) for MHD)."""
+    if torch.is_tensor(state):
+        return _Rollout.apply(plan, state)[0]
+    return _Rollout.apply(plan, *state)

@@ -21,65 +21,118 @@
 namespace pk {
 
 template <int DIR>
-__device__ __forceinline__ int plane_of_hd(int k)
+__device__ __forceinline__ int plane_of_adj(int k)
 {
+    // sweep-order variable k -> xyz plane (fluidtypes.F90:318-320, fluidindex.F90:190-192)
     if (DIR == 0) return k;
     if (DIR == 1) {
-        constexpr int p[5] = {0, 2, 1, 3, 4};
+        constexpr int p[9] = {0, 2, 1, 3, 4, 6, 5, 7, 8};
         return p[k];
     }
-    constexpr int p[5] = {0, 3, 2, 1, 4};
+    constexpr int p[9] = {0, 3, 2, 1, 4, 7, 6, 5, 8};
     return p[k];
 }
 
-// limited face states of cell k from q_{k-1}, q_k, q_{k+1}
+// limiter of variable v: fluid variables use 'limiter', B and psi 'limiter_b'
 template <int LIM>
-__device__ __forceinline__ void faces(int lim, const double (&qm)[5], const double (&q)[5], const double (&qp)[5],
-                                      double (&fp)[5], double (&fm)[5])
+__device__ __forceinline__ int lim_of(const AdjArgs &a, int v)
+{
+    return (LIM >= 0) ? LIM : ((v < 5) ? a.lim : a.limb);
+}
+
+// limited face states of cell k from q_{k-1}, q_k, q_{k+1}
+template <class P, int LIM>
+__device__ __forceinline__ void faces(const AdjArgs &a, const double (&qm)[P::NV], const double (&q)[P::NV],
+                                      const double (&qp)[P::NV], double (&fp)[P::NV], double (&fm)[P::NV])
 {
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
-        const double dq = limit((LIM >= 0) ? LIM : lim, q[v] - qm[v], qp[v] - q[v]);
+    for (int v = 0; v < P::NV; v++) {
+        const double dq = limit(lim_of<LIM>(a, v), q[v] - qm[v], qp[v] - q[v]);
         const double h = 0.5 * dq;
         fp[v] = q[v] + h;
         fm[v] = q[v] - h;
     }
 }
 
-__device__ __forceinline__ void lds_get(const double *A, int nt, int t, double (&x)[5])
+template <int NV>
+__device__ __forceinline__ void lds_get(const double *A, int nt, int t, double (&x)[NV])
 {
 #pragma unroll
-    for (int v = 0; v < 5; v++) x[v] = A[v * nt + t];
+    for (int v = 0; v < NV; v++) x[v] = A[v * nt + t];
 }
-__device__ __forceinline__ void lds_put(double *A, int nt, int t, const double (&x)[5])
+template <int NV>
+__device__ __forceinline__ void lds_put(double *A, int nt, int t, const double (&x)[NV])
 {
 #pragma unroll
-    for (int v = 0; v < 5; v++) A[v * nt + t] = x[v];
+    for (int v = 0; v < NV; v++) A[v * nt + t] = x[v];
 }
 
 // q_bar of cell k from its face-state adjoints (fpb: left state of k+1/2,
 // fmb: right state of k-1/2) through the limiter: the own-cell part and the
 // dl/dr adjoints that the neighbours take.
-template <int LIM>
-__device__ __forceinline__ void faces_vjp(int lim, const double (&qm)[5], const double (&q)[5],
-                                          const double (&qp)[5], const double (&fpb)[5], const double (&fmb)[5],
-                                          double (&qb)[5], double (&dlb)[5], double (&drb)[5])
+template <class P, int LIM>
+__device__ __forceinline__ void faces_vjp(const AdjArgs &a, const double (&qm)[P::NV], const double (&q)[P::NV],
+                                          const double (&qp)[P::NV], const double (&fpb)[P::NV],
+                                          const double (&fmb)[P::NV], double (&qb)[P::NV], double (&dlb)[P::NV],
+                                          double (&drb)[P::NV])
 {
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < P::NV; v++) {
         const double dqb = 0.5 * fpb[v] - 0.5 * fmb[v];
-        limit_vjp((LIM >= 0) ? LIM : lim, q[v] - qm[v], qp[v] - q[v], dqb, dlb[v], drb[v]);
+        limit_vjp(lim_of<LIM>(a, v), q[v] - qm[v], qp[v] - q[v], dqb, dlb[v], drb[v]);
         qb[v] = (fpb[v] + fmb[v]) + (dlb[v] - drb[v]);
     }
 }
 
-template <int DIR, int LIM>
+template <class P>
+__device__ __forceinline__ void flux(const AdjArgs &a, const double *l, const double *r, double *F)
+{
+    if constexpr (P::MAG)
+        flux_hlld(l, r, a.gam, a.ch, F);
+    else
+        flux_hllc(l, r, a.gam, F);
+}
+
+// returns dL/dch of this interface
+template <class P>
+__device__ __forceinline__ double flux_vjp(const AdjArgs &a, const double *l, const double *r, const double *fb,
+                                           double *lb, double *rb)
+{
+    if constexpr (P::MAG) {
+        return hlld_vjp(l, r, a.gam, a.ch, fb, lb, rb);
+    } else {
+        hllc_vjp(l, r, a.gam, fb, lb, rb);
+        return 0.0;
+    }
+}
+
+template <class P>
+__device__ __forceinline__ void floors_vjp(const AdjArgs &a, const double (&u)[P::NV], double (&ub)[P::NV])
+{
+    if constexpr (P::MAG)
+        floors_vjp_mhd(u, a.smalld, a.smallei, a.use_smalld, a.use_smallei, ub);
+    else
+        floors_vjp_hd(u, a.smalld, a.smallei, a.use_smalld, a.use_smallei, ub);
+}
+
+template <class P>
+__device__ __forceinline__ void prim_vjp(const AdjArgs &a, const double (&u)[P::NV], const double (&qb)[P::NV],
+                                         double (&ub)[P::NV])
+{
+    if constexpr (P::MAG)
+        prim_vjp_mhd(u, a.gam_1, qb, ub);
+    else
+        prim_vjp_hd(u, a.gam_1, qb, ub);
+}
+
+template <int DIR, class P, int LIM>
 __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
 {
+    constexpr int NV = P::NV;
     extern __shared__ double lds[];
     const int nt = blockDim.x;
     const int t = threadIdx.x;
-    double *Q1 = lds, *Q2 = lds + 5 * nt, *S1 = lds + 10 * nt, *S2 = lds + 15 * nt, *S3 = lds + 20 * nt;
+    double *Q1 = lds, *Q2 = lds + NV * nt, *S1 = lds + 2 * NV * nt, *S2 = lds + 3 * NV * nt, *S3 = lds + 4 * NV * nt;
 
     constexpr int d1 = (DIR == 0) ? 1 : 0;
     constexpr int d2 = (DIR == 2) ? 1 : 2;
@@ -114,181 +167,194 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
     xyz[d2] = c2i;
     const long long idx = (long long)xyz[0] + (long long)a.sy * xyz[1] + (long long)a.sz * xyz[2];
 
-    double U[5];
+    double U[NV];
 #pragma unroll
-    for (int k = 0; k < 5; k++) U[k] = a.u[(long long)plane_of_hd<DIR>(k) * a.sv + idx];
+    for (int k = 0; k < NV; k++) U[k] = a.u[(long long)plane_of_adj<DIR>(k) * a.sv + idx];
 
     // ---- forward: stage 1 ----
-    double q[5];
-    prim<HD>(U, a.gam_1, q);
-    lds_put(Q1, nt, t, q);
+    double q[NV];
+    prim<P>(U, a.gam_1, q);
+    lds_put<NV>(Q1, nt, t, q);
     __syncthreads();
-    double qm[5], qp[5], fp[5], fm[5], F[5], R1[5], Us_pre[5], Us[5];
-    lds_get(Q1, nt, tm, qm);
-    lds_get(Q1, nt, tp, qp);
-    faces<LIM>(a.lim, qm, q, qp, fp, fm);
-    lds_put(S1, nt, t, fm);
+    double qm[NV], qp[NV], fp[NV], fm[NV], F[NV], R1[NV], Us_pre[NV], Us[NV];
+    lds_get<NV>(Q1, nt, tm, qm);
+    lds_get<NV>(Q1, nt, tp, qp);
+    faces<P, LIM>(a, qm, q, qp, fp, fm);
+    lds_put<NV>(S1, nt, t, fm);
     __syncthreads();
     {
-        double r[5];
-        lds_get(S1, nt, tp, r);
-        flux_hllc(fp, r, a.gam, F);
-        lds_put(S2, nt, t, F);
+        double r[NV];
+        lds_get<NV>(S1, nt, tp, r);
+        flux<P>(a, fp, r, F);
+        lds_put<NV>(S2, nt, t, F);
     }
     __syncthreads();
     {
-        double Fm[5];
-        lds_get(S2, nt, tm, Fm);
+        double Fm[NV];
+        lds_get<NV>(S2, nt, tm, Fm);
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
+        for (int v = 0; v < NV; v++) {
             R1[v] = Fm[v] - F[v];
-            Us_pre[v] = (U[v] + a.c1 * R1[v]) + 0.0;
+            const double un = U[v] + a.c1 * R1[v];
+            Us_pre[v] = (v < 5) ? un + 0.0 : un;
             Us[v] = Us_pre[v];
         }
         unsigned fl = 0u;
         double dm = 0.0;
-        floors<HD>(Us, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+        floors<P>(Us, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
     }
     // ---- forward: stage 2 ----
-    double qs[5];
-    prim<HD>(Us, a.gam_1, qs);
-    lds_put(Q2, nt, t, qs);
+    double qs[NV];
+    prim<P>(Us, a.gam_1, qs);
+    lds_put<NV>(Q2, nt, t, qs);
     __syncthreads();
-    lds_get(Q2, nt, tm, qm);
-    lds_get(Q2, nt, tp, qp);
-    faces<LIM>(a.lim, qm, qs, qp, fp, fm);
-    lds_put(S1, nt, t, fm);
+    lds_get<NV>(Q2, nt, tm, qm);
+    lds_get<NV>(Q2, nt, tp, qp);
+    faces<P, LIM>(a, qm, qs, qp, fp, fm);
+    lds_put<NV>(S1, nt, t, fm);
     __syncthreads();
     {
-        double r[5];
-        lds_get(S1, nt, tp, r);
-        flux_hllc(fp, r, a.gam, F);
-        lds_put(S2, nt, t, F);
+        double r[NV];
+        lds_get<NV>(S1, nt, tp, r);
+        flux<P>(a, fp, r, F);
+        lds_put<NV>(S2, nt, t, F);
     }
     __syncthreads();
-    double lp[5];  // lam' through the floors of stage 2
-    double dtb = 0.0;
+    double lp[NV];  // lam' through the psi damping and the floors of stage 2
+    double dtb = 0.0, chb = 0.0;
     {
-        double Fm[5], R2[5], Up[5];
-        lds_get(S2, nt, tm, Fm);
+        double Fm[NV], R2[NV], Up[NV];
+        lds_get<NV>(S2, nt, tm, Fm);
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
+        for (int v = 0; v < NV; v++) {
             R2[v] = Fm[v] - F[v];
-            Up[v] = (U[v] + a.c2 * R2[v]) + 0.0;
+            const double un = U[v] + a.c2 * R2[v];
+            Up[v] = (v < 5) ? un + 0.0 : un;
         }
         // ---- reverse: stage 2 ----
 #pragma unroll
-        for (int k = 0; k < 5; k++) lp[k] = a.lam_out[(long long)plane_of_hd<DIR>(k) * a.sv + idx];
-        floors_vjp_hd(Up, a.smalld, a.smallei, a.use_smalld, a.use_smallei, lp);
+        for (int k = 0; k < NV; k++) lp[k] = a.lam_out[(long long)plane_of_adj<DIR>(k) * a.sv + idx];
+        if constexpr (P::MAG) lp[8] = lp[8] * a.psi_fac;
+        floors_vjp<P>(a, Up, lp);
         if (outl) {
             double acc = 0.0;
 #pragma unroll
-            for (int v = 0; v < 5; v++) acc += lp[v] * R2[v];
+            for (int v = 0; v < NV; v++) acc += lp[v] * R2[v];
             dtb += acc / a.dl;  // c2 = dt/dl
         }
     }
-    lds_put(S1, nt, t, lp);
+    lds_put<NV>(S1, nt, t, lp);
     __syncthreads();
-    double fpb[5];
+    double fpb[NV];
     {
         // interface k+1/2 of stage 2: l = left state of cell k, r = right state of cell k+1
-        double lpn[5], fb[5], q2[5], q3[5], l[5], r[5], junk[5], rb[5];
-        lds_get(S1, nt, tp, lpn);
+        double lpn[NV], fb[NV], q2[NV], q3[NV], l[NV], r[NV], junk[NV], rb[NV];
+        lds_get<NV>(S1, nt, tp, lpn);
 #pragma unroll
-        for (int v = 0; v < 5; v++) fb[v] = a.c2 * (lpn[v] - lp[v]);
-        lds_get(Q2, nt, tm, qm);
-        lds_get(Q2, nt, tp, q2);
-        lds_get(Q2, nt, tpp, q3);
-        faces<LIM>(a.lim, qm, qs, q2, l, junk);
-        faces<LIM>(a.lim, qs, q2, q3, junk, r);
+        for (int v = 0; v < NV; v++) fb[v] = a.c2 * (lpn[v] - lp[v]);
+        lds_get<NV>(Q2, nt, tm, qm);
+        lds_get<NV>(Q2, nt, tp, q2);
+        lds_get<NV>(Q2, nt, tpp, q3);
+        faces<P, LIM>(a, qm, qs, q2, l, junk);
+        faces<P, LIM>(a, qs, q2, q3, junk, r);
 #pragma unroll
-        for (int v = 0; v < 5; v++) fpb[v] = rb[v] = 0.0;
-        hllc_vjp(l, r, a.gam, fb, fpb, rb);
-        lds_put(S2, nt, t, rb);  // belongs to cell k+1
+        for (int v = 0; v < NV; v++) fpb[v] = rb[v] = 0.0;
+        const double cb = flux_vjp<P>(a, l, r, fb, fpb, rb);
+        if (outl) chb += cb;
+        lds_put<NV>(S2, nt, t, rb);  // belongs to cell k+1
     }
     __syncthreads();
-    double lsp[5];  // lam* through the floors of stage 1
+    double lsp[NV];  // lam* through the floors of stage 1
     {
-        double fmb[5], qb[5], dlb[5], drb[5];
-        lds_get(S2, nt, tm, fmb);
-        lds_get(Q2, nt, tm, qm);
-        lds_get(Q2, nt, tp, qp);
-        faces_vjp<LIM>(a.lim, qm, qs, qp, fpb, fmb, qb, dlb, drb);
-        lds_put(S1, nt, t, dlb);
-        lds_put(S3, nt, t, drb);
+        double fmb[NV], qb[NV], dlb[NV], drb[NV];
+        lds_get<NV>(S2, nt, tm, fmb);
+        lds_get<NV>(Q2, nt, tm, qm);
+        lds_get<NV>(Q2, nt, tp, qp);
+        faces_vjp<P, LIM>(a, qm, qs, qp, fpb, fmb, qb, dlb, drb);
+        lds_put<NV>(S1, nt, t, dlb);
+        lds_put<NV>(S3, nt, t, drb);
         __syncthreads();
-        double dln[5], drp[5];
-        lds_get(S1, nt, tp, dln);
-        lds_get(S3, nt, tm, drp);
+        double dln[NV], drp[NV];
+        lds_get<NV>(S1, nt, tp, dln);
+        lds_get<NV>(S3, nt, tm, drp);
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
+        for (int v = 0; v < NV; v++) {
             qb[v] += drp[v] - dln[v];
             lsp[v] = 0.0;
         }
-        prim_vjp_hd(Us, a.gam_1, qb, lsp);
-        floors_vjp_hd(Us_pre, a.smalld, a.smallei, a.use_smalld, a.use_smallei, lsp);
+        prim_vjp<P>(a, Us, qb, lsp);
+        floors_vjp<P>(a, Us_pre, lsp);
         if (outl) {
             double acc = 0.0;
 #pragma unroll
-            for (int v = 0; v < 5; v++) acc += lsp[v] * R1[v];
+            for (int v = 0; v < NV; v++) acc += lsp[v] * R1[v];
             dtb += (0.5 * acc) / a.dl;  // c1 = (dt/2)/dl
         }
-        lds_put(S2, nt, t, lsp);
+        lds_put<NV>(S2, nt, t, lsp);
     }
     __syncthreads();
     // ---- reverse: stage 1 ----
     {
-        double lsn[5], fb[5], q2[5], q3[5], l[5], r[5], junk[5], rb[5];
-        lds_get(S2, nt, tp, lsn);
+        double lsn[NV], fb[NV], q2[NV], q3[NV], l[NV], r[NV], junk[NV], rb[NV];
+        lds_get<NV>(S2, nt, tp, lsn);
 #pragma unroll
-        for (int v = 0; v < 5; v++) fb[v] = a.c1 * (lsn[v] - lsp[v]);
-        lds_get(Q1, nt, tm, qm);
-        lds_get(Q1, nt, tp, q2);
-        lds_get(Q1, nt, tpp, q3);
-        faces<LIM>(a.lim, qm, q, q2, l, junk);
-        faces<LIM>(a.lim, q, q2, q3, junk, r);
+        for (int v = 0; v < NV; v++) fb[v] = a.c1 * (lsn[v] - lsp[v]);
+        lds_get<NV>(Q1, nt, tm, qm);
+        lds_get<NV>(Q1, nt, tp, q2);
+        lds_get<NV>(Q1, nt, tpp, q3);
+        faces<P, LIM>(a, qm, q, q2, l, junk);
+        faces<P, LIM>(a, q, q2, q3, junk, r);
 #pragma unroll
-        for (int v = 0; v < 5; v++) fpb[v] = rb[v] = 0.0;
-        hllc_vjp(l, r, a.gam, fb, fpb, rb);
-        lds_put(S1, nt, t, rb);
+        for (int v = 0; v < NV; v++) fpb[v] = rb[v] = 0.0;
+        const double cb = flux_vjp<P>(a, l, r, fb, fpb, rb);
+        if (outl) chb += cb;
+        lds_put<NV>(S1, nt, t, rb);
     }
     __syncthreads();
-    double lam[5];
+    double lam[NV];
     {
-        double fmb[5], qb[5], dlb[5], drb[5];
-        lds_get(S1, nt, tm, fmb);
-        lds_get(Q1, nt, tm, qm);
-        lds_get(Q1, nt, tp, qp);
-        faces_vjp<LIM>(a.lim, qm, q, qp, fpb, fmb, qb, dlb, drb);
-        lds_put(S2, nt, t, dlb);
-        lds_put(S3, nt, t, drb);
+        double fmb[NV], qb[NV], dlb[NV], drb[NV];
+        lds_get<NV>(S1, nt, tm, fmb);
+        lds_get<NV>(Q1, nt, tm, qm);
+        lds_get<NV>(Q1, nt, tp, qp);
+        faces_vjp<P, LIM>(a, qm, q, qp, fpb, fmb, qb, dlb, drb);
+        lds_put<NV>(S2, nt, t, dlb);
+        lds_put<NV>(S3, nt, t, drb);
         __syncthreads();
-        double dln[5], drp[5];
-        lds_get(S2, nt, tp, dln);
-        lds_get(S3, nt, tm, drp);
+        double dln[NV], drp[NV];
+        lds_get<NV>(S2, nt, tp, dln);
+        lds_get<NV>(S3, nt, tm, drp);
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
+        for (int v = 0; v < NV; v++) {
             qb[v] += drp[v] - dln[v];
             lam[v] = 0.0;
         }
-        prim_vjp_hd(U, a.gam_1, qb, lam);
+        prim_vjp<P>(a, U, qb, lam);
 #pragma unroll
-        for (int v = 0; v < 5; v++) lam[v] = (lam[v] + lsp[v]) + lp[v];
+        for (int v = 0; v < NV; v++) lam[v] = (lam[v] + lsp[v]) + lp[v];
     }
     if (outl) {
 #pragma unroll
-        for (int k = 0; k < 5; k++) a.lam_in[(long long)plane_of_hd<DIR>(k) * a.sv + idx] = lam[k];
+        for (int k = 0; k < NV; k++) a.lam_in[(long long)plane_of_adj<DIR>(k) * a.sv + idx] = lam[k];
     }
-    // dL/d(dt): per-wave sums in a fixed order, one slot per wave of the grid
+    // dL/d(dt) and dL/d(ch): per-wave sums in a fixed order, one slot per wave of the grid
     const int lane = t & 63;
     const int nact = min(64, nt - (t & ~63));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        const double other = __shfl_xor(dtb, o, 64);
-        if ((lane ^ o) < nact && lane < (lane ^ o)) dtb += other;
-        // lanes with the higher index keep their value; lane 0 ends with the sum
+        const double o1 = __shfl_xor(dtb, o, 64);
+        const double o2 = __shfl_xor(chb, o, 64);
+        // the lower lane of each pair accumulates; lane 0 ends with the sum
+        if ((lane ^ o) < nact && lane < (lane ^ o)) {
+            dtb += o1;
+            chb += o2;
+        }
     }
-    if (lane == 0) a.dtb_part[(long long)bid * ((nt + 63) / 64) + t / 64] = dtb;
+    if (lane == 0) {
+        const long long w = (long long)bid * ((nt + 63) / 64) + t / 64;
+        a.dtb_part[w] = dtb;
+        a.chb_part[w] = chb;
+    }
 }
 
 // Sum of the per-wave partials in a fixed order (deterministic).
@@ -360,6 +426,22 @@ __global__ void k_zero_best(unsigned long long *best)
 
 // timestep adjoint: the cell and direction where dl_d/c_d is smallest (the
 // smallest linear index among exact ties), then dL/dU there.
+__device__ __forceinline__ double cell_cs(const DtAdjArgs &a, const double *u)
+{
+    const double d = u[0], e = u[4 * a.sv];
+    const double m0 = u[a.sv], m1 = u[2 * a.sv], m2 = u[3 * a.sv];
+    const double ek = (0.5 * ((m0 * m0 + m1 * m1) + m2 * m2)) / d;
+    if (a.magnetized) {
+        const double bx = u[5 * a.sv], by = u[6 * a.sv], bz = u[7 * a.sv];
+        const double pmag = 0.5 * ((bx * bx + by * by) + bz * bz);
+        const double ps = (e - ek) * a.gam_1 + (2.0 - a.gam) * pmag;
+        const double pr = ps - pmag;
+        return sqrt(fabs((2.0 * pmag + a.gam * pr) / d));
+    }
+    const double pr = (e - ek) * a.gam_1;
+    return sqrt(fabs((a.gam * pr) / d));
+}
+
 __global__ __launch_bounds__(256) void k_dt_argmin(const DtAdjArgs a, unsigned long long *best)
 {
     const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
@@ -372,13 +454,11 @@ __global__ __launch_bounds__(256) void k_dt_argmin(const DtAdjArgs a, unsigned l
         const long long idx =
             (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
         const double *u = a.u + idx;
-        const double d = u[0], m[3] = {u[a.sv], u[2 * a.sv], u[3 * a.sv]}, e = u[4 * a.sv];
-        const double ek = (0.5 * ((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2])) / d;
-        const double pr = (e - ek) * a.gam_1;
-        const double cs = sqrt(fabs((a.gam * pr) / d));
+        const double d = u[0];
+        const double cs = cell_cs(a, u);
         for (int q = 0; q < 3; q++) {
             if (!a.has_dir[q]) continue;
-            const double v = (d > 0.0) ? fabs(m[q] / d) : 0.0;
+            const double v = (d > 0.0) ? fabs(u[(1 + q) * a.sv] / d) : 0.0;
             const double cc = fmx(v + cs, 1.17549435082228750797e-38);
             if (a.dl[q] / cc == a.mn) atomicMin(best, (unsigned long long)(c * 3 + q));
         }
@@ -397,40 +477,60 @@ __global__ void k_dt_apply(const DtAdjArgs a, const unsigned long long *best, do
     const int j = (int)(r % a.n[1]);
     const int k = (int)(r / a.n[1]);
     const long long idx = (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
-    double u[5], ub[5] = {0, 0, 0, 0, 0};
-    for (int v = 0; v < 5; v++) u[v] = a.u[(long long)v * a.sv + idx];
     // dt = cfl * dl_q / c_q
-    dt_vjp_hd(u, q, a.dl[q], a.gam, a.gam_1, a.cfl * a.dtb, ub);
-    for (int v = 0; v < 5; v++) lam[(long long)v * a.sv + idx] += ub[v];
+    if (a.magnetized) {
+        double u[9], ub[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int v = 0; v < 9; v++) u[v] = a.u[(long long)v * a.sv + idx];
+        dt_vjp_mhd(u, q, a.dl[q], a.gam, a.gam_1, a.cfl * a.dtb, ub);
+        for (int v = 0; v < 8; v++) lam[(long long)v * a.sv + idx] += ub[v];
+    } else {
+        double u[5], ub[5] = {0, 0, 0, 0, 0};
+        for (int v = 0; v < 5; v++) u[v] = a.u[(long long)v * a.sv + idx];
+        dt_vjp_hd(u, q, a.dl[q], a.gam, a.gam_1, a.cfl * a.dtb, ub);
+        for (int v = 0; v < 5; v++) lam[(long long)v * a.sv + idx] += ub[v];
+    }
 }
 
 // ---- launchers ------------------------------------------------------------------
 
-template <int DIR, int LIM>
+template <int DIR, class P, int LIM>
 static hipError_t adj_t(const AdjArgs &a, int nt, long long nblk, hipStream_t st)
 {
-    const size_t lds = sizeof(double) * 25 * nt;
+    const size_t lds = sizeof(double) * 5 * P::NV * nt;
     static bool raised = false;
     if (!raised) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_sweep_adjoint<DIR, LIM>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_sweep_adjoint<DIR, P, LIM>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         raised = true;
     }
-    hipLaunchKernelGGL((k_sweep_adjoint<DIR, LIM>), dim3((unsigned)nblk), dim3(nt), lds, st, a);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_sweep_adjoint<DIR, P, LIM>), dim3((unsigned)nblk), dim3(nt), lds, st, a);
     return hipGetLastError();
 }
 
-template <int DIR>
+template <int DIR, class P>
 static hipError_t adj_lim(const AdjArgs &a, int nt, long long nblk, hipStream_t st)
 {
-    switch (a.lim) {
-    case LIM_VANLEER: return adj_t<DIR, LIM_VANLEER>(a, nt, nblk, st);
-    case LIM_MINMOD: return adj_t<DIR, LIM_MINMOD>(a, nt, nblk, st);
-    case LIM_MONCEN: return adj_t<DIR, LIM_MONCEN>(a, nt, nblk, st);
-    case LIM_SUPERBEE: return adj_t<DIR, LIM_SUPERBEE>(a, nt, nblk, st);
-    default: return hipErrorInvalidValue;
+    if (a.lim == a.limb || !P::MAG) {
+        switch (a.lim) {
+        case LIM_VANLEER: return adj_t<DIR, P, LIM_VANLEER>(a, nt, nblk, st);
+        case LIM_MINMOD: return adj_t<DIR, P, LIM_MINMOD>(a, nt, nblk, st);
+        case LIM_MONCEN: return adj_t<DIR, P, LIM_MONCEN>(a, nt, nblk, st);
+        case LIM_SUPERBEE: return adj_t<DIR, P, LIM_SUPERBEE>(a, nt, nblk, st);
+        default: return hipErrorInvalidValue;
+        }
     }
+    return adj_t<DIR, P, -1>(a, nt, nblk, st);
+}
+
+template <class P>
+static hipError_t adj_dir(int dir, const AdjArgs &a, int nt, long long nblk, hipStream_t st)
+{
+    if (dir == 0) return adj_lim<0, P>(a, nt, nblk, st);
+    if (dir == 1) return adj_lim<1, P>(a, nt, nblk, st);
+    if (dir == 2) return adj_lim<2, P>(a, nt, nblk, st);
+    return hipErrorInvalidValue;
 }
 
 long long sweep_adjoint_parts(const AdjArgs &a)
@@ -439,17 +539,15 @@ long long sweep_adjoint_parts(const AdjArgs &a)
     return (long long)a.chunks * a.ptiles * ((nt + 63) / 64);
 }
 
-hipError_t launch_sweep_adjoint(int dir, const AdjArgs &a, double *dtb_out, hipStream_t st)
+hipError_t launch_sweep_adjoint(int dir, bool mhd, const AdjArgs &a, double *dtb_out, hipStream_t st)
 {
     const int nt = a.TP * a.TS;
     if (nt < 1 || nt > 256 || a.out < 1 || a.out != a.TS - 16) return hipErrorInvalidValue;
     const long long nblk = (long long)a.chunks * a.ptiles;
-    hipError_t e = hipErrorInvalidValue;
-    if (dir == 0) e = adj_lim<0>(a, nt, nblk, st);
-    if (dir == 1) e = adj_lim<1>(a, nt, nblk, st);
-    if (dir == 2) e = adj_lim<2>(a, nt, nblk, st);
+    hipError_t e = mhd ? adj_dir<MHD>(dir, a, nt, nblk, st) : adj_dir<HD>(dir, a, nt, nblk, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, a.dtb_part, sweep_adjoint_parts(a), dtb_out);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, a.chb_part, sweep_adjoint_parts(a), dtb_out + 1);
     return hipGetLastError();
 }
 

@@ -5,28 +5,33 @@
 
 namespace pk {
 
-// Reverse mode of one fused RK2 hydro sweep of a periodic block.
+// Reverse mode of one fused RK2 sweep of a periodic block (hydro or MHD+GLM).
 struct AdjArgs {
     const double *u;        // state at the start of the sweep (block layout)
     const double *lam_out;  // dL/d(sweep output), interior cells
     double *lam_in;         // dL/d(sweep input), interior cells (written)
-    double *dtb_part;       // per-wave partial sums of dL/d(dt)
+    double *dtb_part;       // per-wave partial sums of dL/d(dt) at fixed ch
+    double *chb_part;       // per-wave partial sums of dL/d(ch) (MHD)
     long long sv;
     int sy, sz;
     int ng[3], g[3], n[3];
     int TP, TS;             // tile: TP pencils x TS cells; TS - 16 output cells
     int chunks, ptiles, out;
-    int lim;
+    int lim, limb;          // limiter of the fluid variables, of B and psi
     double gam, gam_1;
     double c1, c2;          // (dt/2)/dl and dt/dl
     double dl;              // cell size along the sweep
+    double ch;              // GLM speed of the sweep (MHD)
+    double psi_fac;         // the damping factor the forward sweep applied to psi (MHD; else 1)
     double smalld, smallei;
     int use_smalld, use_smallei;
 };
 
-// Reverse mode of timestep_fluid (one hydro fluid) at its minimising cell.
+// Reverse mode of timestep_fluid (one fluid; neu_cs or ion_cs) at its
+// minimising cell.
 struct DtAdjArgs {
-    const double *u;
+    const double *u;        // rho plane of the fluid; B planes follow at +5 when magnetized
+    int magnetized;
     long long sv;
     int sy, sz;
     int n[3], g[3], has_dir[3];
@@ -38,7 +43,8 @@ struct DtAdjArgs {
 };
 
 long long sweep_adjoint_parts(const AdjArgs &a);
-hipError_t launch_sweep_adjoint(int dir, const AdjArgs &a, double *dtb_out, hipStream_t st);
+// dtb_out[0] = dL/d(dt) at fixed ch, dtb_out[1] = dL/d(ch)
+hipError_t launch_sweep_adjoint(int dir, bool mhd, const AdjArgs &a, double *dtb_out, hipStream_t st);
 hipError_t launch_fold_ghosts(double *lam, long long sv, int sy, int sz, const int ng[3], const int g[3],
                               const int n[3], int nplanes, hipStream_t st);
 hipError_t launch_dt_adjoint(const DtAdjArgs &a, unsigned long long *best, double *lam, hipStream_t st);

@@ -50,6 +50,7 @@ struct Block {
     double *lam[2] = {nullptr, nullptr};
     int lcur = 0;
     double *d_dtb_part = nullptr;
+    double *d_chb_part = nullptr;
     long long dtb_parts = 0;
     double *d_dtb = nullptr;
     unsigned long long *d_best = nullptr;
@@ -550,9 +551,11 @@ void free_adjoint(Block *b)
         b->lam[i] = nullptr;
     }
     if (b->d_dtb_part) (void)hipFree(b->d_dtb_part);
+    if (b->d_chb_part) (void)hipFree(b->d_chb_part);
     if (b->d_dtb) (void)hipFree(b->d_dtb);
     if (b->d_best) (void)hipFree(b->d_best);
     b->d_dtb_part = nullptr;
+    b->d_chb_part = nullptr;
     b->d_dtb = nullptr;
     b->d_best = nullptr;
     b->dtb_parts = 0;
@@ -707,13 +710,17 @@ void adj_tile(const Block *b, int d, int &TP, int &TS, int &out, int &chunks, in
     ptiles = (int)((npen + TP - 1) / TP);
 }
 
-int do_sweep_adjoint(Block *b, const double *u, int d, double dt, double *dtb)
+// lam <- adjoint of the fused sweep along d linearised at state u (a slot);
+// *dtb += dL/d(dt) at fixed ch, *chb += dL/d(ch).
+int do_sweep_adjoint(Block *b, const double *u, int d, double dt, double ch, double psi_fac, double *dtb,
+                     double *chb)
 {
     pk::AdjArgs a{};
     a.u = u + b->xo;
     a.lam_out = b->lam[b->lcur] + b->xo;
     a.lam_in = b->lam[1 - b->lcur] + b->xo;
     a.dtb_part = b->d_dtb_part;
+    a.chb_part = b->d_chb_part;
     a.sv = b->sv;
     a.sy = b->sy;
     a.sz = b->sz;
@@ -725,32 +732,54 @@ int do_sweep_adjoint(Block *b, const double *u, int d, double dt, double *dtb)
     adj_tile(b, d, a.TP, a.TS, a.out, a.chunks, a.ptiles);
     if (pk::sweep_adjoint_parts(a) > b->dtb_parts) return fail(PK_EINVAL, "adjoint partial-sum buffer too small");
     a.lim = g_cfg.limiter;
+    a.limb = g_cfg.limiter_b;
     a.gam = g_cfg.fluid[0].gamma;
     a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
     a.c1 = (0.5 * dt) / b->dl[d];
     a.c2 = (1.0 * dt) / b->dl[d];
     a.dl = b->dl[d];
+    a.ch = ch;
+    a.psi_fac = psi_fac;
     a.smalld = g_cfg.smalld;
     a.smallei = g_cfg.smallei;
     a.use_smalld = g_cfg.use_smalld;
     a.use_smallei = g_cfg.use_smallei;
-    HIPCHK(pk::launch_sweep_adjoint(d, a, b->d_dtb, g_stream));
+    HIPCHK(pk::launch_sweep_adjoint(d, g_mhd, a, b->d_dtb, g_stream));
     b->lcur = 1 - b->lcur;
-    HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtb, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtb, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     *dtb += g_pinned[0];
+    *chb += g_pinned[1];
     return PK_OK;
 }
 
-// The directions of the sweeps of one step in order (XYZ then ZYX).
-std::vector<int> step_dirs()
+// ch and the damping factor fluid_update_full uses on one block for dt
+void block_glm(Block *b, double dt, double *ch, double *fac)
+{
+    *ch = 0.0;
+    *fac = 1.0;
+    if (g_mhd) glm_scalars({min_dl(b)}, dt, ch, fac);
+}
+
+// The directions of the sweeps of one step in order (XYZ then ZYX); damp[i]:
+// sweep i is the last of its substep (it applies the GLM damping).
+std::vector<int> step_dirs(std::vector<bool> *damp = nullptr)
 {
     std::vector<int> v;
-    for (int half = 0; half < 2; half++)
+    if (damp) damp->clear();
+    for (int half = 0; half < 2; half++) {
+        int last = -1;
         for (int s = 0; s < 3; s++) {
             const int d = half == 0 ? s : 2 - s;
-            if (g_cfg.has_dir[d]) v.push_back(d);
+            if (g_cfg.has_dir[d]) last = d;
         }
+        for (int s = 0; s < 3; s++) {
+            const int d = half == 0 ? s : 2 - s;
+            if (!g_cfg.has_dir[d]) continue;
+            v.push_back(d);
+            if (damp) damp->push_back(d == last);
+        }
+    }
     return v;
 }
 
@@ -1660,7 +1689,7 @@ int pk_adjoint_setup(int64_t cg_id, int32_t nslots)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
-    if (g_mhd || g_nfl != 1) return fail(PK_EINVAL, "the adjoint covers one hydro (HLLC) fluid");
+    if (g_nfl != 1) return fail(PK_EINVAL, "the adjoint covers one fluid per block");
     if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "the adjoint covers the RK2 sweep");
     if (nslots < 1) return fail(PK_EINVAL, "nslots = %d", nslots);
     for (int d = 0; d < 3; d++)
@@ -1683,8 +1712,9 @@ int pk_adjoint_setup(int64_t cg_id, int32_t nslots)
         if (p > parts) parts = p;
     }
     HIPCHK(hipMalloc((void **)&b->d_dtb_part, sizeof(double) * parts));
+    HIPCHK(hipMalloc((void **)&b->d_chb_part, sizeof(double) * parts));
     b->dtb_parts = parts;
-    HIPCHK(hipMalloc((void **)&b->d_dtb, sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->d_dtb, 2 * sizeof(double)));
     HIPCHK(hipMalloc((void **)&b->d_best, sizeof(unsigned long long)));
     b->lcur = 0;
     HIPCHK(hipStreamSynchronize(g_stream));
@@ -1716,30 +1746,30 @@ int pk_state_load(int64_t cg_id, int32_t slot)
     return PK_OK;
 }
 
-int pk_adjoint_set(int64_t cg_id, const double *lam)
+int pk_adjoint_set(int64_t cg_id, const double *lam_u, const double *lam_b, const double *lam_psi)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
     RCCHK(adj_check(b));
-    if (!lam) return fail(PK_EINVAL, "null adjoint array");
+    if (!lam_u || (g_mhd && (!lam_b || !lam_psi))) return fail(PK_EINVAL, "missing adjoint array");
     double *dst = b->lam[b->lcur];
     HIPCHK(hipMemsetAsync(dst, 0, state_bytes(b), g_stream));
-    RCCHK(upload_array(b, lam, map_u(), dst + b->xo, true));
+    RCCHK(upload_state(b, lam_u, lam_b, lam_psi, dst + b->xo, true));
     // the ghost cells are periodic copies of interior cells
     HIPCHK(pk::launch_fold_ghosts(dst + b->xo, b->sv, b->sy, b->sz, b->ng, b->g, b->n, g_nplanes, g_stream));
     return PK_OK;
 }
 
-int pk_adjoint_get(int64_t cg_id, double *lam)
+int pk_adjoint_get(int64_t cg_id, double *lam_u, double *lam_b, double *lam_psi)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
     RCCHK(adj_check(b));
-    if (!lam) return fail(PK_EINVAL, "null adjoint array");
-    return download_array(b, lam, map_u(), b->lam[b->lcur] + b->xo, true);
+    return download_state(b, lam_u, lam_b, lam_psi, b->lam[b->lcur] + b->xo, true);
 }
 
-int pk_sweep_adjoint(int64_t cg_id, int32_t slot, int32_t ddim, double dt, double *dt_bar)
+int pk_sweep_adjoint(int64_t cg_id, int32_t slot, int32_t ddim, double dt, double chspeed, double psi_fac,
+                     double *dt_bar, double *ch_bar)
 {
     Block *b;
     double *p;
@@ -1748,9 +1778,10 @@ int pk_sweep_adjoint(int64_t cg_id, int32_t slot, int32_t ddim, double dt, doubl
     RCCHK(slot_ptr(b, slot, &p));
     if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
-    double dtb = 0.0;
-    RCCHK(do_sweep_adjoint(b, p, ddim, dt, &dtb));
+    double dtb = 0.0, chb = 0.0;
+    RCCHK(do_sweep_adjoint(b, p, ddim, dt, chspeed, psi_fac, &dtb, &chb));
     if (dt_bar) *dt_bar += dtb;
+    if (ch_bar) *ch_bar += chb;
     return PK_OK;
 }
 
@@ -1764,11 +1795,15 @@ int pk_fluid_update_record(int64_t cg_id, int32_t slot0, double dt)
     if (slot0 < 0 || slot0 + (int)dirs.size() > (int)b->slots.size())
         return fail(PK_EINVAL, "slots %d..%d do not exist", slot0, slot0 + (int)dirs.size() - 1);
     list_touch();
+    std::vector<bool> damp;
+    step_dirs(&damp);
+    double ch, fac;
+    block_glm(b, dt, &ch, &fac);
     for (size_t i = 0; i < dirs.size(); i++) {
         HIPCHK(hipMemcpyAsync(b->slots[slot0 + i], b->buf[b->cur], state_bytes(b), hipMemcpyDeviceToDevice,
                               g_stream));
         const bool last = i + 1 == dirs.size();
-        RCCHK(do_sweep(b, dirs[i], dt, 0.0, 1.0, true, last));
+        RCCHK(do_sweep(b, dirs[i], dt, ch, damp[i] ? fac : 1.0, true, last));
     }
     return PK_OK;
 }
@@ -1782,8 +1817,16 @@ int pk_fluid_update_adjoint(int64_t cg_id, int32_t slot0, double dt, double *dt_
     const std::vector<int> dirs = step_dirs();
     if (slot0 < 0 || slot0 + (int)dirs.size() > (int)b->slots.size())
         return fail(PK_EINVAL, "slots %d..%d do not exist", slot0, slot0 + (int)dirs.size() - 1);
-    double dtb = 0.0;
-    for (int i = (int)dirs.size() - 1; i >= 0; i--) RCCHK(do_sweep_adjoint(b, b->slots[slot0 + i], dirs[i], dt, &dtb));
+    std::vector<bool> damp;
+    step_dirs(&damp);
+    double ch, fac;
+    block_glm(b, dt, &ch, &fac);
+    double dtb = 0.0, chb = 0.0;
+    for (int i = (int)dirs.size() - 1; i >= 0; i--)
+        RCCHK(do_sweep_adjoint(b, b->slots[slot0 + i], dirs[i], dt, ch, damp[i] ? fac : 1.0, &dtb, &chb));
+    // ch = cfl_glm*min(dl)/dt (hdc.F90:122); the damping factor exp(-glm_alpha*ch*dt/min(dl))
+    // does not depend on dt
+    if (g_mhd) dtb += chb * (-(ch / dt));
     if (dt_bar) *dt_bar += dtb;
     return PK_OK;
 }
@@ -1808,8 +1851,8 @@ int pk_timestep_adjoint(int64_t cg_id, int32_t slot, double dt_bar)
         a.dl[d] = b->dl[d];
     }
     a.plane0 = 0;
-    a.bplane = 0;
-    a.magnetized = 0;
+    a.bplane = 5;
+    a.magnetized = g_mhd ? 1 : 0;
     a.gam = g_cfg.fluid[0].gamma;
     a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
     HIPCHK(pk::launch_dt(a, g_part, b->d_dtout, g_stream));
@@ -1821,6 +1864,7 @@ int pk_timestep_adjoint(int64_t cg_id, int32_t slot, double dt_bar)
     b->dt_pending = false;
     pk::DtAdjArgs q{};
     q.u = a.u;
+    q.magnetized = g_mhd ? 1 : 0;
     q.sv = b->sv;
     q.sy = b->sy;
     q.sz = b->sz;

@@ -22,7 +22,9 @@ module piernik_hip
         &    pk_timestep_fluid, pk_fluid_update, pk_run, pk_get_report, pk_synchronize, &
         &    pk_autotune, pk_profile_begin, pk_profile_end, pk_error_message, &
         &    pk_upload_device, pk_download_device, pk_timestep_all, pk_fluid_update_all, pk_run_all, &
-        &    pk_autotune_all, pk_list_tiles
+        &    pk_autotune_all, pk_list_tiles, pk_adjoint_setup, pk_state_save, pk_state_load, &
+        &    pk_adjoint_set, pk_adjoint_get, pk_fluid_update_record, pk_sweep_adjoint, &
+        &    pk_fluid_update_adjoint, pk_timestep_adjoint
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
 
@@ -227,6 +229,69 @@ module piernik_hip
          integer(c_int64_t), value       :: cg_id
          integer(c_int32_t), intent(out) :: tiles(3)
       end function pk_list_tiles
+
+      ! reverse mode of the fused sweep (one fluid, periodic blocks)
+      integer(c_int32_t) function pk_adjoint_setup(cg_id, nslots) bind(C, name="pk_adjoint_setup")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: nslots
+      end function pk_adjoint_setup
+
+      integer(c_int32_t) function pk_state_save(cg_id, slot) bind(C, name="pk_state_save")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: slot
+      end function pk_state_save
+
+      integer(c_int32_t) function pk_state_load(cg_id, slot) bind(C, name="pk_state_load")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: slot
+      end function pk_state_load
+
+      integer(c_int32_t) function pk_adjoint_set(cg_id, lam_u, lam_b, lam_psi) bind(C, name="pk_adjoint_set")
+         import :: c_int32_t, c_int64_t, c_ptr
+         integer(c_int64_t), value :: cg_id
+         type(c_ptr),        value :: lam_u, lam_b, lam_psi
+      end function pk_adjoint_set
+
+      integer(c_int32_t) function pk_adjoint_get(cg_id, lam_u, lam_b, lam_psi) bind(C, name="pk_adjoint_get")
+         import :: c_int32_t, c_int64_t, c_ptr
+         integer(c_int64_t), value :: cg_id
+         type(c_ptr),        value :: lam_u, lam_b, lam_psi
+      end function pk_adjoint_get
+
+      integer(c_int32_t) function pk_fluid_update_record(cg_id, slot0, dt) bind(C, name="pk_fluid_update_record")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: slot0
+         real(c_double),     value :: dt
+      end function pk_fluid_update_record
+
+      integer(c_int32_t) function pk_sweep_adjoint(cg_id, slot, ddim, dt, chspeed, psi_fac, dt_bar, ch_bar) &
+           &                                       bind(C, name="pk_sweep_adjoint")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value         :: cg_id
+         integer(c_int32_t), value         :: slot, ddim
+         real(c_double),     value         :: dt, chspeed, psi_fac
+         real(c_double),     intent(inout) :: dt_bar, ch_bar
+      end function pk_sweep_adjoint
+
+      integer(c_int32_t) function pk_fluid_update_adjoint(cg_id, slot0, dt, dt_bar) &
+           &                                              bind(C, name="pk_fluid_update_adjoint")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value         :: cg_id
+         integer(c_int32_t), value         :: slot0
+         real(c_double),     value         :: dt
+         real(c_double),     intent(inout) :: dt_bar
+      end function pk_fluid_update_adjoint
+
+      integer(c_int32_t) function pk_timestep_adjoint(cg_id, slot, dt_bar) bind(C, name="pk_timestep_adjoint")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: slot
+         real(c_double),     value :: dt_bar
+      end function pk_timestep_adjoint
 
       integer(c_int32_t) function pk_profile_begin() bind(C, name="pk_profile_begin")
          import :: c_int32_t

@@ -340,13 +340,65 @@ def adjoint_hydro(n=128, seed=0, gamma=5.0 / 3.0, amp=0.1):
     return u, target, cfg
 
 
-def to_device_layout(u, nb=4):
-    """(5, nx, ny, nz) interior array (numpy or torch) -> torch tensor
-    (nzg, nyg, nxg, 5) on the GPU with periodic ghost cells, the reference
-    layout u(5, nxg, nyg, nzg)."""
+def to_device_layout(u, nb=4, has_dir=(True, True, True)):
+    """(C, nx, ny, nz) interior array (numpy or torch; (nx, ny, nz) for one
+    component) -> torch tensor (nzg, nyg, nxg, C) on the GPU with periodic
+    ghost cells in the active directions: the reference layout
+    u(C, nxg, nyg, nzg) (or (nzg, nyg, nxg) for one component)."""
     import torch
 
     t = torch.as_tensor(np.ascontiguousarray(u) if isinstance(u, np.ndarray) else u, dtype=torch.float64)
-    for ax in (1, 2, 3):
-        t = torch.cat([t.narrow(ax, t.shape[ax] - nb, nb), t, t.narrow(ax, 0, nb)], dim=ax)
-    return t.permute(3, 2, 1, 0).contiguous().cuda()
+    scalar = t.dim() == 3
+    if scalar:
+        t = t.unsqueeze(0)
+    for d, ax in enumerate((1, 2, 3)):
+        if has_dir[d]:
+            t = torch.cat([t.narrow(ax, t.shape[ax] - nb, nb), t, t.narrow(ax, 0, nb)], dim=ax)
+    t = t.permute(3, 2, 1, 0).contiguous()
+    if scalar:
+        t = t[..., 0].contiguous()
+    return t.cuda()
+
+
+def interior_of(t, nb=4, has_dir=(True, True, True)):
+    """Inverse view of to_device_layout: (nzg, nyg, nxg[, C]) -> (C, nx, ny, nz) or (nx, ny, nz)."""
+    g = [nb if h else 0 for h in has_dir]
+    sl = tuple(slice(g[d], t.shape[2 - d] - g[d]) for d in (2, 1, 0))
+    v = t[sl]
+    return v.permute(3, 2, 1, 0) if v.dim() == 4 else v.permute(2, 1, 0)
+
+
+def orszag_tang_2d(n, seed=0, amp=1.0, perturb=0.0, gamma=5.0 / 3.0):
+    """Orszag-Tang vortex on the unit square (2D, n x n): rho = 25/(36 pi),
+    p = 5/(12 pi), v = amp*(-sin 2pi y, sin 2pi x), B = amp*B0*(-sin 2pi y,
+    sin 4pi x), B0 = 1/sqrt(4 pi), psi = 0; `perturb` adds seeded smooth
+    relative perturbations to rho and p (a generic state).  Returns interior
+    arrays u (5, n, n, 1), b (3, n, n, 1), psi (n, n, 1) and the Config."""
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    rng = np.random.default_rng(seed)
+
+    def smooth():
+        f = np.zeros((n, n))
+        for _ in range(3):
+            kx, ky = rng.integers(1, 3, 2)
+            f += np.sin(2 * np.pi * (kx * X + ky * Y) + rng.uniform(0, 2 * np.pi))
+        return f / 3.0
+
+    rho = 25.0 / (36.0 * np.pi) * (1.0 + perturb * smooth())
+    p = 5.0 / (12.0 * np.pi) * (1.0 + perturb * smooth())
+    vx, vy = -amp * np.sin(2 * np.pi * Y), amp * np.sin(2 * np.pi * X)
+    b0 = 1.0 / np.sqrt(4.0 * np.pi)
+    bx, by = -amp * b0 * np.sin(2 * np.pi * Y), amp * b0 * np.sin(4 * np.pi * X)
+    u = np.zeros((5, n, n, 1))
+    u[0, ..., 0] = rho
+    u[1, ..., 0] = rho * vx
+    u[2, ..., 0] = rho * vy
+    u[4, ..., 0] = p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy) + 0.5 * (bx * bx + by * by)
+    b = np.zeros((3, n, n, 1))
+    b[0, ..., 0] = bx
+    b[1, ..., 0] = by
+    psi = np.zeros((n, n, 1))
+    cfg = Config(gamma=gamma, magnetized=True, limiter="vanleer", cfl=0.5, cfl_glm=0.5, smalld=1e-10,
+                 smallei=1e-10, has_dir=(True, True, False))
+    return u, b, psi, cfg
