@@ -252,11 +252,13 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
         lds_get<NV>(S1, nt, tp, lpn);
 #pragma unroll
         for (int v = 0; v < NV; v++) fb[v] = a.c2 * (lpn[v] - lp[v]);
+        double qc[NV];
+        lds_get<NV>(Q2, nt, t, qc);
         lds_get<NV>(Q2, nt, tm, qm);
         lds_get<NV>(Q2, nt, tp, q2);
         lds_get<NV>(Q2, nt, tpp, q3);
-        faces<P, LIM>(a, qm, qs, q2, l, junk);
-        faces<P, LIM>(a, qs, q2, q3, junk, r);
+        faces<P, LIM>(a, qm, qc, q2, l, junk);
+        faces<P, LIM>(a, qc, q2, q3, junk, r);
 #pragma unroll
         for (int v = 0; v < NV; v++) fpb[v] = rb[v] = 0.0;
         const double cb = flux_vjp<P>(a, l, r, fb, fpb, rb);
@@ -267,10 +269,12 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
     double lsp[NV];  // lam* through the floors of stage 1
     {
         double fmb[NV], qb[NV], dlb[NV], drb[NV];
+        double qc[NV];
         lds_get<NV>(S2, nt, tm, fmb);
+        lds_get<NV>(Q2, nt, t, qc);
         lds_get<NV>(Q2, nt, tm, qm);
         lds_get<NV>(Q2, nt, tp, qp);
-        faces_vjp<P, LIM>(a, qm, qs, qp, fpb, fmb, qb, dlb, drb);
+        faces_vjp<P, LIM>(a, qm, qc, qp, fpb, fmb, qb, dlb, drb);
         lds_put<NV>(S1, nt, t, dlb);
         lds_put<NV>(S3, nt, t, drb);
         __syncthreads();
@@ -282,7 +286,15 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
             qb[v] += drp[v] - dln[v];
             lsp[v] = 0.0;
         }
-        prim_vjp<P>(a, Us, qb, lsp);
+        {
+            double Usr[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) Usr[v] = Us_pre[v];
+            unsigned fl = 0u;
+            double dm = 0.0;
+            floors<P>(Usr, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+            prim_vjp<P>(a, Usr, qb, lsp);
+        }
         floors_vjp<P>(a, Us_pre, lsp);
         if (outl) {
             double acc = 0.0;
@@ -298,12 +310,17 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
         double lsn[NV], fb[NV], q2[NV], q3[NV], l[NV], r[NV], junk[NV], rb[NV];
         lds_get<NV>(S2, nt, tp, lsn);
 #pragma unroll
-        for (int v = 0; v < NV; v++) fb[v] = a.c1 * (lsn[v] - lsp[v]);
+        for (int v = 0; v < NV; v++) {
+            fb[v] = a.c1 * (lsn[v] - lsp[v]);
+            lsp[v] = lsp[v] + lp[v];  // from here on lam* + lam' (the two direct terms)
+        }
+        double qc[NV];
+        lds_get<NV>(Q1, nt, t, qc);
         lds_get<NV>(Q1, nt, tm, qm);
         lds_get<NV>(Q1, nt, tp, q2);
         lds_get<NV>(Q1, nt, tpp, q3);
-        faces<P, LIM>(a, qm, q, q2, l, junk);
-        faces<P, LIM>(a, q, q2, q3, junk, r);
+        faces<P, LIM>(a, qm, qc, q2, l, junk);
+        faces<P, LIM>(a, qc, q2, q3, junk, r);
 #pragma unroll
         for (int v = 0; v < NV; v++) fpb[v] = rb[v] = 0.0;
         const double cb = flux_vjp<P>(a, l, r, fb, fpb, rb);
@@ -314,10 +331,12 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
     double lam[NV];
     {
         double fmb[NV], qb[NV], dlb[NV], drb[NV];
+        double qc[NV];
         lds_get<NV>(S1, nt, tm, fmb);
+        lds_get<NV>(Q1, nt, t, qc);
         lds_get<NV>(Q1, nt, tm, qm);
         lds_get<NV>(Q1, nt, tp, qp);
-        faces_vjp<P, LIM>(a, qm, q, qp, fpb, fmb, qb, dlb, drb);
+        faces_vjp<P, LIM>(a, qm, qc, qp, fpb, fmb, qb, dlb, drb);
         lds_put<NV>(S2, nt, t, dlb);
         lds_put<NV>(S3, nt, t, drb);
         __syncthreads();
@@ -329,9 +348,12 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
             qb[v] += drp[v] - dln[v];
             lam[v] = 0.0;
         }
-        prim_vjp<P>(a, U, qb, lam);
+        double Ur[NV];  // reloaded: cheaper than holding U across the kernel
 #pragma unroll
-        for (int v = 0; v < NV; v++) lam[v] = (lam[v] + lsp[v]) + lp[v];
+        for (int k = 0; k < NV; k++) Ur[k] = a.u[(long long)plane_of_adj<DIR>(k) * a.sv + idx];
+        prim_vjp<P>(a, Ur, qb, lam);
+#pragma unroll
+        for (int v = 0; v < NV; v++) lam[v] = lam[v] + lsp[v];
     }
     if (outl) {
 #pragma unroll

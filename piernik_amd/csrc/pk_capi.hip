@@ -710,10 +710,11 @@ void adj_tile(const Block *b, int d, int &TP, int &TS, int &out, int &chunks, in
     ptiles = (int)((npen + TP - 1) / TP);
 }
 
-// lam <- adjoint of the fused sweep along d linearised at state u (a slot);
-// *dtb += dL/d(dt) at fixed ch, *chb += dL/d(ch).
+// lam <- adjoint of the fused sweep along d linearised at state u (a slot).
+// dL/d(dt) at fixed ch and dL/d(ch) land in d_dtb[2*out_slot ..]; with dtb
+// set they are also read back and added to *dtb, *chb.
 int do_sweep_adjoint(Block *b, const double *u, int d, double dt, double ch, double psi_fac, double *dtb,
-                     double *chb)
+                     double *chb, int out_slot = 0)
 {
     pk::AdjArgs a{};
     a.u = u + b->xo;
@@ -744,12 +745,15 @@ int do_sweep_adjoint(Block *b, const double *u, int d, double dt, double ch, dou
     a.smallei = g_cfg.smallei;
     a.use_smalld = g_cfg.use_smalld;
     a.use_smallei = g_cfg.use_smallei;
-    HIPCHK(pk::launch_sweep_adjoint(d, g_mhd, a, b->d_dtb, g_stream));
+    HIPCHK(pk::launch_sweep_adjoint(d, g_mhd, a, b->d_dtb + 2 * out_slot, g_stream));
     b->lcur = 1 - b->lcur;
-    HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtb, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
-    *dtb += g_pinned[0];
-    *chb += g_pinned[1];
+    if (dtb) {
+        HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtb + 2 * out_slot, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        *dtb += g_pinned[0];
+        *chb += g_pinned[1];
+    }
     return PK_OK;
 }
 
@@ -1714,7 +1718,7 @@ int pk_adjoint_setup(int64_t cg_id, int32_t nslots)
     HIPCHK(hipMalloc((void **)&b->d_dtb_part, sizeof(double) * parts));
     HIPCHK(hipMalloc((void **)&b->d_chb_part, sizeof(double) * parts));
     b->dtb_parts = parts;
-    HIPCHK(hipMalloc((void **)&b->d_dtb, 2 * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->d_dtb, 2 * 6 * sizeof(double)));
     HIPCHK(hipMalloc((void **)&b->d_best, sizeof(unsigned long long)));
     b->lcur = 0;
     HIPCHK(hipStreamSynchronize(g_stream));
@@ -1823,7 +1827,14 @@ int pk_fluid_update_adjoint(int64_t cg_id, int32_t slot0, double dt, double *dt_
     block_glm(b, dt, &ch, &fac);
     double dtb = 0.0, chb = 0.0;
     for (int i = (int)dirs.size() - 1; i >= 0; i--)
-        RCCHK(do_sweep_adjoint(b, b->slots[slot0 + i], dirs[i], dt, ch, damp[i] ? fac : 1.0, &dtb, &chb));
+        RCCHK(do_sweep_adjoint(b, b->slots[slot0 + i], dirs[i], dt, ch, damp[i] ? fac : 1.0, nullptr, nullptr, i));
+    // the sweeps' sums, read back once
+    HIPCHK(hipMemcpyAsync(g_pinned, b->d_dtb, 2 * dirs.size() * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (size_t i = 0; i < dirs.size(); i++) {
+        dtb += g_pinned[2 * i];
+        chb += g_pinned[2 * i + 1];
+    }
     // ch = cfl_glm*min(dl)/dt (hdc.F90:122); the damping factor exp(-glm_alpha*ch*dt/min(dl))
     // does not depend on dt
     if (g_mhd) dtb += chb * (-(ch / dt));

@@ -372,7 +372,7 @@ def orszag_tang_2d(n, seed=0, amp=1.0, perturb=0.0, gamma=5.0 / 3.0):
     """Orszag-Tang vortex on the unit square (2D, n x n): rho = 25/(36 pi),
     p = 5/(12 pi), v = amp*(-sin 2pi y, sin 2pi x), B = amp*B0*(-sin 2pi y,
     sin 4pi x), B0 = 1/sqrt(4 pi), psi = 0; `perturb` adds seeded smooth
-    relative perturbations to rho and p (a generic state).  Returns interior
+    relative perturbations to rho and p and small v_z, B_z (a generic state).  Returns interior
     arrays u (5, n, n, 1), b (3, n, n, 1), psi (n, n, 1) and the Config."""
     x = (np.arange(n) + 0.5) / n
     X, Y = np.meshgrid(x, x, indexing="ij")
@@ -390,14 +390,19 @@ def orszag_tang_2d(n, seed=0, amp=1.0, perturb=0.0, gamma=5.0 / 3.0):
     vx, vy = -amp * np.sin(2 * np.pi * Y), amp * np.sin(2 * np.pi * X)
     b0 = 1.0 / np.sqrt(4.0 * np.pi)
     bx, by = -amp * b0 * np.sin(2 * np.pi * Y), amp * b0 * np.sin(4 * np.pi * X)
+    vz = perturb * smooth()
+    bz = perturb * b0 * smooth()
     u = np.zeros((5, n, n, 1))
     u[0, ..., 0] = rho
     u[1, ..., 0] = rho * vx
     u[2, ..., 0] = rho * vy
-    u[4, ..., 0] = p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy) + 0.5 * (bx * bx + by * by)
+    u[3, ..., 0] = rho * vz
+    u[4, ..., 0] = (p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz)
+                    + 0.5 * (bx * bx + by * by + bz * bz))
     b = np.zeros((3, n, n, 1))
     b[0, ..., 0] = bx
     b[1, ..., 0] = by
+    b[2, ..., 0] = bz
     psi = np.zeros((n, n, 1))
     cfg = Config(gamma=gamma, magnetized=True, limiter="vanleer", cfl=0.5, cfl_glm=0.5, smalld=1e-10,
                  smallei=1e-10, has_dir=(True, True, False))
