@@ -106,7 +106,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128"], default="mhd256")
+    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128", "ensemble256"],
+                    default="mhd256")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--nblocks", type=int, default=4096, help="blocks of the batched list")
     ap.add_argument("--n", type=int, default=None, help="override the block size (cells per side)")
@@ -129,6 +130,9 @@ def main():
     print(f"[bench] cpu baseline done: {cpu and round(cpu['value'])} cell-updates/s", file=sys.stderr, flush=True)
     if args.workload == "batch4096":
         print(json.dumps(run_batch(args, cpu)))
+        return
+    if args.workload == "ensemble256":
+        print(json.dumps(run_ensemble(args)))
         return
     if args.workload == "adjoint128":
         adj = adjoint_cost()
@@ -285,6 +289,68 @@ def adjoint_cost(n=128, steps=32, every=8):
         "workload": (f"3D hydro, HLLC, van Leer, RK2, CFL 0.5, periodic {n}^3, rho = 1 + 0.1 G (|k| <= 6), "
                      f"p = 1, v = 0, {steps} steps, checkpoint every {every}, loss 0.5*||rho - rho_target||^2"),
         "definition": "(forward with checkpoints + backward with segment recompute) / plain forward",
+    }
+
+
+def run_ensemble(args, ninst=256, steps=40, every=8):
+    """Gradient of the total magnetic energy of 256 independent 2D
+    Orszag-Tang instances (sizes 64^2 .. 256^2, amplitudes 1 +- 0.1, HLLD +
+    GLM, van Leer, CFL 0.5) after 40 steps w.r.t. their initial fields: one
+    persistent launch per sweep for the forward steps of all instances and
+    one per sweep for their adjoint; checkpoint every 8 steps."""
+    import torch
+    import piernik_amd as pk
+    from piernik_amd import adjoint, setups
+
+    cases, cfg = setups.orszag_tang_ensemble(ninst, seed=0)
+    hd = cfg.has_dir
+    pk.init(cfg)
+    ens = adjoint.EnsemblePlan(cfg, [(n, n, 1) for n, *_ in cases], steps, checkpoint_every=every)
+    states = [tuple(setups.to_device_layout(a, 4, hd) for a in (u, b, psi)) for _, u, b, psi in cases]
+    cells = sum(n * n for n, *_ in cases)
+
+    def grad_of(st):
+        g = [torch.zeros_like(st[0]), torch.zeros_like(st[1]), torch.zeros_like(st[2])]
+        setups.interior_of(g[1], 4, hd).copy_(setups.interior_of(st[1], 4, hd))
+        return g
+
+    print(f"[bench] {ninst} instances, {cells} cells", file=sys.stderr, flush=True)
+    fin = ens.run(states)  # warm-up
+    ens.backward([grad_of(f) for f in fin])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(states, record=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fin = ens.run(states, record=True)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    g = [grad_of(f) for f in fin]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads = ens.backward(g)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    energy = sum(0.5 * float(torch.sum(setups.interior_of(f[1], 4, hd) ** 2)) for f in fin)
+    gnorm = sum(float(torch.sum(x * x)) for gr in grads for x in gr) ** 0.5
+    ens.close()
+    pk.finalize()
+    ratio = (t_fwd + t_bwd) / t_plain
+    sizes = {}
+    for n, *_ in cases:
+        sizes[str(n)] = sizes.get(str(n), 0) + 1
+    return {
+        "metric": "adjoint/forward cost ratio, 256-instance 2D MHD ensemble", "value": ratio, "unit": "x",
+        "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": t_plain / steps * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"{ninst} independent 2D Orszag-Tang instances, sizes {sizes}, amplitudes 1 +- 0.1 "
+                                f"(seed 0), HLLD+GLM, van Leer, RK2, CFL 0.5, {steps} steps, checkpoint every {every}, "
+                                "gradient of the total magnetic energy w.r.t. the initial u, b, psi"),
+                   "cells": cells},
+        "ensemble": {"forward_ms": t_plain * 1e3, "forward_checkpointed_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+                     "cell_updates_per_s": cells * 2 * steps / t_plain, "magnetic_energy": energy,
+                     "gradient_norm": gnorm},
     }
 
 

@@ -180,6 +180,29 @@ int pk_fluid_update_all(int32_t mode, double dt);
 /* nsteps main-loop iterations over the list (as pk_run for one block). */
 int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out);
 
+/* ---- ensembles of independent instances (no counterpart in the reference) -- */
+/* The registered blocks (ascending cg_id; pk_ensemble_ids lists them) as
+ * independent problems: each takes its own time_step (its own dt, caps, ch
+ * and GLM damping), all advanced by the same persistent launches as the list
+ * entry points.  Arrays below have one entry per block in that order. */
+int pk_ensemble_ids(int64_t *ids, int32_t n);
+/* One main-loop iteration of every block; first != 0: no growth cap (the
+ * first step of a run).  dt_out, capped_out (may be NULL): each block's dt
+ * and whether dt_max_grow capped it. */
+int pk_ensemble_step(int32_t mode, int32_t first, double *dt_out, int32_t *capped_out);
+int pk_ensemble_run(int32_t mode, int32_t nsteps, double *t_out, double *dt_out);
+/* every block's state <-> its slot (pk_adjoint_setup on every block) */
+int pk_ensemble_state_save(int32_t slot);
+int pk_ensemble_state_load(int32_t slot);
+/* one step of every block with the given dts, keeping the state before each
+ * sweep in slots slot0, slot0+1, ... */
+int pk_ensemble_record_step(const double *dts, int32_t slot0);
+/* the recorded step of every block in reverse (one persistent adjoint launch
+ * per sweep); dt_bar[i] += dL/d(dt_i), the dependence of ch on dt included */
+int pk_ensemble_step_adjoint(const double *dts, int32_t slot0, double *dt_bar);
+/* lam_i += dt_bar[i] * d(dt_i)/du_i at the state in slot (timestep_fluid) */
+int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar);
+
 /* ---- tuning (no counterpart in the reference) ------------------------------ */
 
 /* Time candidate tile shapes of the fused sweep on this block (the state is

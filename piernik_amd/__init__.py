@@ -111,6 +111,16 @@ _PROTOS = {
                          ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_fluid_update_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)],
     "pk_timestep_adjoint": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double],
+    "pk_ensemble_ids": [ctypes.POINTER(ctypes.c_int64), ctypes.c_int32],
+    "pk_ensemble_step": [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_int32)],
+    "pk_ensemble_run": [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                        ctypes.POINTER(ctypes.c_double)],
+    "pk_ensemble_state_save": [ctypes.c_int32],
+    "pk_ensemble_state_load": [ctypes.c_int32],
+    "pk_ensemble_record_step": [ctypes.POINTER(ctypes.c_double), ctypes.c_int32],
+    "pk_ensemble_step_adjoint": [ctypes.POINTER(ctypes.c_double), ctypes.c_int32, ctypes.POINTER(ctypes.c_double)],
+    "pk_ensemble_timestep_adjoint": [ctypes.c_int32, ctypes.POINTER(ctypes.c_double)],
     "pk_profile_begin": [],
     "pk_profile_end": [ctypes.POINTER(_ProfStats)],
 }

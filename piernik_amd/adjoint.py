@@ -136,3 +136,105 @@ def rollout(state, plan):
     if torch.is_tensor(state):
         return _Rollout.apply(plan, state)[0]
     return _Rollout.apply(plan, *state)
+
+
+class EnsemblePlan:
+    """Independent problems, one block each, advanced together.
+
+    Every instance takes its own time_step (own dt, caps, ch, damping); the
+    forward steps are the library's persistent list launches, the backward
+    pass runs the adjoint of all instances in one persistent launch per
+    sweep (pk_ensemble_*).  States are tuples (u, b, psi) for MHD or (u,)
+    for hydro, per instance, in the reference layout (see RolloutPlan)."""
+
+    def __init__(self, cfg, shapes, nsteps, checkpoint_every=8, nb=4, first_id=700001):
+        if cfg.nfluids != 1:
+            raise PiernikError("the adjoint covers one fluid per block")
+        self.cfg = cfg
+        self.nsteps = int(nsteps)
+        self.k = max(1, int(checkpoint_every))
+        self.nsw = sum(1 for h in cfg.has_dir if h) * 2
+        self.nck = (self.nsteps + self.k - 1) // self.k
+        self.blocks = []
+        for i, shape in enumerate(shapes):
+            cg = _DeviceBlock(tuple(shape), nb, cfg, first_id + i)
+            cg.register()
+            _call("pk_adjoint_setup", cg.cg_id, self.nck + self.nsw * self.k)
+            self.blocks.append(cg)
+        n = len(self.blocks)
+        ids = (ctypes.c_int64 * n)()
+        _call("pk_ensemble_ids", ids, n)
+        if [int(x) for x in ids] != [cg.cg_id for cg in self.blocks]:
+            raise PiernikError("the ensemble must hold exactly this plan's blocks")
+        self.dts = []
+        self.capped = []
+
+    def _empty(self, cg, device):
+        ng = cg.ng
+        u = torch.empty((ng[2], ng[1], ng[0], 5), dtype=torch.float64, device=device)
+        if not self.cfg.magnetized:
+            return (u,)
+        return (u, torch.empty((ng[2], ng[1], ng[0], 3), dtype=torch.float64, device=device),
+                torch.empty((ng[2], ng[1], ng[0]), dtype=torch.float64, device=device))
+
+    def run(self, states, record=True, mode=None):
+        """Forward: every instance nsteps main-loop iterations; returns the
+        final states."""
+        from . import BATCH_GRAPH
+        mode = BATCH_GRAPH if mode is None else mode
+        n = len(self.blocks)
+        for cg, st in zip(self.blocks, states):
+            cg.upload_device(*[t.contiguous() for t in st])
+            update_boundaries(cg)
+        dt = (ctypes.c_double * n)()
+        cap = (ctypes.c_int32 * n)()
+        self.dts, self.capped = [], []
+        for s in range(self.nsteps):
+            if record and s % self.k == 0:
+                _call("pk_ensemble_state_save", s // self.k)
+            _call("pk_ensemble_step", int(mode), int(s == 0), dt, cap)
+            self.dts.append([dt[i] for i in range(n)])
+            self.capped.append([bool(cap[i]) for i in range(n)])
+        out = []
+        for cg, st in zip(self.blocks, states):
+            o = self._empty(cg, st[0].device)
+            cg.download_device(*o)
+            out.append(o)
+        return out
+
+    def backward(self, grads):
+        """dL/d(initial states) for dL/d(final states) = grads."""
+        n = len(self.blocks)
+        for cg, g in zip(self.blocks, grads):
+            ptrs = [ctypes.c_void_p(_device_ptr(t.contiguous())) for t in g] + [None] * (3 - len(g))
+            _call("pk_adjoint_set", cg.cg_id, *ptrs)
+        carry = [0.0] * n
+        for seg in reversed(range(self.nck)):
+            s0, s1 = seg * self.k, min(seg * self.k + self.k, self.nsteps)
+            _call("pk_ensemble_state_load", seg)
+            for s in range(s0, s1):
+                _call("pk_ensemble_record_step", (ctypes.c_double * n)(*self.dts[s]), self.nck + (s - s0) * self.nsw)
+            for s in reversed(range(s0, s1)):
+                slot = self.nck + (s - s0) * self.nsw
+                dtb = (ctypes.c_double * n)()
+                _call("pk_ensemble_step_adjoint", (ctypes.c_double * n)(*self.dts[s]), slot, dtb)
+                tb = (ctypes.c_double * n)()
+                for i in range(n):
+                    total = dtb[i] + carry[i]
+                    if self.capped[s][i]:
+                        carry[i] = 2.0 * total  # dt_s = 2 dt_{s-1}
+                    else:
+                        tb[i] = total
+                        carry[i] = 0.0
+                _call("pk_ensemble_timestep_adjoint", slot, tb)
+        out = []
+        for cg, g in zip(self.blocks, grads):
+            o = self._empty(cg, g[0].device)
+            ptrs = [ctypes.c_void_p(_device_ptr(t)) for t in o] + [None] * (3 - len(o))
+            _call("pk_adjoint_get", cg.cg_id, *ptrs)
+            out.append(o)
+        return out
+
+    def close(self):
+        for cg in self.blocks:
+            cg.unregister()

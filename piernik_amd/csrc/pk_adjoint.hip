@@ -125,11 +125,12 @@ __device__ __forceinline__ void prim_vjp(const AdjArgs &a, const double (&u)[P::
         prim_vjp_hd(u, a.gam_1, qb, ub);
 }
 
+// One tile (bid) of the adjoint sweep; lanes t >= TP*TS of a larger
+// workgroup take part in the barriers only.
 template <int DIR, class P, int LIM>
-__global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
+__device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *lds)
 {
     constexpr int NV = P::NV;
-    extern __shared__ double lds[];
     const int nt = blockDim.x;
     const int t = threadIdx.x;
     double *Q1 = lds, *Q2 = lds + NV * nt, *S1 = lds + 2 * NV * nt, *S2 = lds + 3 * NV * nt, *S3 = lds + 4 * NV * nt;
@@ -147,7 +148,6 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
     }
     const int tm = max(t - step, 0), tp = min(t + step, nt - 1), tpp = min(t + 2 * step, nt - 1);
 
-    const int bid = blockIdx.x;
     const int chunk = bid % a.chunks;
     const int ptile = bid / a.chunks;
     const int n1 = a.n[d1], nd = a.n[DIR];
@@ -379,6 +379,58 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
     }
 }
 
+template <int DIR, class P, int LIM>
+__global__ __launch_bounds__(256) void k_sweep_adjoint(const AdjArgs a)
+{
+    extern __shared__ double lds[];
+    adjoint_tile<DIR, P, LIM>(a, blockIdx.x, lds);
+}
+
+// Persistent adjoint sweep over a block list (independent instances): as
+// k_sweep_list, tiles from a device queue, per-block arguments from a table.
+template <int DIR, class P, int LIM>
+__global__ __launch_bounds__(256) void k_sweep_adjoint_list(const AdjListArgs L)
+{
+    extern __shared__ double lds[];
+    __shared__ unsigned s_item;
+    for (;;) {
+        __syncthreads();  // the previous tile is done with LDS and s_item
+        if (threadIdx.x == 0) s_item = atomicAdd(L.queue, 1u);
+        __syncthreads();
+        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)s_item);
+        if (item >= L.nitems) break;
+        typedef const __attribute__((address_space(4))) unsigned cu32;
+        const int blk = __builtin_amdgcn_readfirstlane((int)((cu32 *)L.items)[item]);
+        AdjArgs a;
+        {
+            static_assert(sizeof(AdjArgs) % 4 == 0, "AdjArgs is copied in dwords");
+            cu32 *w = (cu32 *)(L.args + blk);
+            unsigned *aw = reinterpret_cast<unsigned *>(&a);
+#pragma unroll
+            for (int i = 0; i < (int)(sizeof(AdjArgs) / 4); i++) aw[i] = w[i];
+        }
+        const unsigned first = ((cu32 *)L.first)[blk];
+        adjoint_tile<DIR, P, LIM>(a, (int)(item - first), lds);
+    }
+}
+
+// one workgroup per job: the sum of a block's per-wave partials (fixed order)
+__global__ __launch_bounds__(256) void k_sum_parts_jobs(const SumJob *__restrict__ jobs)
+{
+    __shared__ double red[256];
+    const SumJob j = jobs[blockIdx.x];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (long long i = t; i < j.n; i += 256) acc += j.part[i];
+    red[t] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] = red[t] + red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) j.out[0] = red[0];
+}
+
 // Sum of the per-wave partials in a fixed order (deterministic).
 __global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ part, long long n,
                                                    double *__restrict__ out)
@@ -544,6 +596,77 @@ static hipError_t adj_lim(const AdjArgs &a, int nt, long long nblk, hipStream_t 
         }
     }
     return adj_t<DIR, P, -1>(a, nt, nblk, st);
+}
+
+template <int DIR, class P, int LIM>
+static hipError_t adjl_t(const AdjListArgs &L, int grid, hipStream_t st, int *resident)
+{
+    const int nt = 256;
+    const void *fn = (const void *)k_sweep_adjoint_list<DIR, P, LIM>;
+    const size_t lds = sizeof(double) * 5 * P::NV * nt;
+    const size_t lds_max = 160 * 1024 - 256;
+    static bool raised = false;
+    if (!raised) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);
+        if (e != hipSuccess) return e;
+        raised = true;
+    }
+    if (lds > lds_max) return hipErrorInvalidValue;
+    if (resident) {
+        int per = 0, dev = 0, ncu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        *resident = per * ncu;
+        return e;
+    }
+    hipLaunchKernelGGL((k_sweep_adjoint_list<DIR, P, LIM>), dim3((unsigned)grid), dim3(nt), lds, st, L);
+    return hipGetLastError();
+}
+
+template <int DIR, class P>
+static hipError_t adjl_lim(int lim, int limb, const AdjListArgs &L, int grid, hipStream_t st, int *resident)
+{
+    if (lim == limb || !P::MAG) {
+        switch (lim) {
+        case LIM_VANLEER: return adjl_t<DIR, P, LIM_VANLEER>(L, grid, st, resident);
+        case LIM_MINMOD: return adjl_t<DIR, P, LIM_MINMOD>(L, grid, st, resident);
+        case LIM_MONCEN: return adjl_t<DIR, P, LIM_MONCEN>(L, grid, st, resident);
+        case LIM_SUPERBEE: return adjl_t<DIR, P, LIM_SUPERBEE>(L, grid, st, resident);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return adjl_t<DIR, P, -1>(L, grid, st, resident);
+}
+
+static hipError_t adjl_dispatch(int dir, bool mhd, int lim, int limb, const AdjListArgs &L, int grid, hipStream_t st,
+                                int *resident)
+{
+    if (mhd) {
+        if (dir == 0) return adjl_lim<0, MHD>(lim, limb, L, grid, st, resident);
+        if (dir == 1) return adjl_lim<1, MHD>(lim, limb, L, grid, st, resident);
+        if (dir == 2) return adjl_lim<2, MHD>(lim, limb, L, grid, st, resident);
+    } else {
+        if (dir == 0) return adjl_lim<0, HD>(lim, limb, L, grid, st, resident);
+        if (dir == 1) return adjl_lim<1, HD>(lim, limb, L, grid, st, resident);
+        if (dir == 2) return adjl_lim<2, HD>(lim, limb, L, grid, st, resident);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep_adjoint_list(int dir, bool mhd, int lim, int limb, const AdjListArgs &L, hipStream_t st)
+{
+    int r = 0;
+    hipError_t e = adjl_dispatch(dir, mhd, lim, limb, L, 0, st, &r);
+    if (e != hipSuccess) return e;
+    return adjl_dispatch(dir, mhd, lim, limb, L, r > 0 ? r : 1, st, nullptr);
+}
+
+hipError_t launch_sum_parts_jobs(const SumJob *jobs, int njobs, hipStream_t st)
+{
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_parts_jobs, dim3((unsigned)njobs), dim3(256), 0, st, jobs);
+    return hipGetLastError();
 }
 
 template <class P>

@@ -42,7 +42,28 @@ struct DtAdjArgs {
     double dtb;  // dL/d(dt)
 };
 
+// Persistent adjoint sweep over a block list (workgroups of 256 lanes take
+// tiles from a queue; tile item belongs to block items[item] and is its
+// tile item - first[block]).  Each block's AdjArgs has its tile shape with
+// TP*TS <= 256 and its dtb/chb partial arrays sized for 4 waves per tile.
+struct AdjListArgs {
+    const AdjArgs *args;
+    const unsigned *first;
+    const unsigned *items;
+    unsigned nitems;
+    unsigned *queue;
+};
+
+// out[0] = sum of part[0 .. n-1] in a fixed order
+struct SumJob {
+    const double *part;
+    long long n;
+    double *out;
+};
+
 long long sweep_adjoint_parts(const AdjArgs &a);
+hipError_t launch_sweep_adjoint_list(int dir, bool mhd, int lim, int limb, const AdjListArgs &L, hipStream_t st);
+hipError_t launch_sum_parts_jobs(const SumJob *jobs, int njobs, hipStream_t st);
 // dtb_out[0] = dL/d(dt) at fixed ch, dtb_out[1] = dL/d(ch)
 hipError_t launch_sweep_adjoint(int dir, bool mhd, const AdjArgs &a, double *dtb_out, hipStream_t st);
 hipError_t launch_fold_ghosts(double *lam, long long sv, int sy, int sz, const int ng[3], const int g[3],

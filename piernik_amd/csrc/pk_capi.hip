@@ -827,6 +827,32 @@ struct ListState {
 };
 ListState g_list;
 
+// Independent instances (an ensemble): the registered blocks advanced
+// together, each with its own dt, ch and GLM damping factor.
+const int kEnsSlots = 64;  // dt slots per block
+struct EnsembleState {
+    size_t nblk = 0;
+    pk::StepScalars *d_step = nullptr, *h_step = nullptr;  // per block
+    unsigned long long *d_dtslots = nullptr;               // per block kEnsSlots slots
+    double *d_dtout = nullptr, *h_dtout = nullptr;         // per block 4 values
+    std::vector<double> dt_old;
+    bool fresh = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    pk::CopyJob *d_jobs = nullptr;
+    std::vector<pk::CopyJob> h_jobs;
+    // reverse mode
+    pk::AdjArgs *d_adj = nullptr, *h_adj = nullptr;  // per block (pinned host copy)
+    unsigned *d_afirst[3] = {nullptr, nullptr, nullptr}, *d_aitems[3] = {nullptr, nullptr, nullptr};
+    unsigned anitems[3] = {0, 0, 0};
+    bool adj_built = false;
+    pk::SumJob *d_sum = nullptr;
+    std::vector<pk::SumJob> h_sum;
+    double *d_sums = nullptr, *h_sums = nullptr;  // per block, per sweep: dtb, chb
+    unsigned *d_queue = nullptr;
+};
+EnsembleState g_ens;
+
 void destroy_graph()
 {
     if (g_list.exec) (void)hipGraphExecDestroy(g_list.exec);
@@ -835,14 +861,21 @@ void destroy_graph()
     g_list.graph = nullptr;
 }
 
+void ens_free();
+
 void list_invalidate()
 {
     g_list.built = false;
     g_list.fresh = false;
     destroy_graph();
+    ens_free();
 }
 
-void list_touch() { g_list.fresh = false; }
+void list_touch()
+{
+    g_list.fresh = false;
+    g_ens.fresh = false;
+}
 
 void free_tab(LaunchTab &t)
 {
@@ -852,10 +885,35 @@ void free_tab(LaunchTab &t)
     t = LaunchTab();
 }
 
+void ens_free()
+{
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    if (g_ens.exec) (void)hipGraphExecDestroy(g_ens.exec);
+    if (g_ens.graph) (void)hipGraphDestroy(g_ens.graph);
+    if (g_ens.d_step) (void)hipFree(g_ens.d_step);
+    if (g_ens.h_step) (void)hipHostFree(g_ens.h_step);
+    if (g_ens.d_dtslots) (void)hipFree(g_ens.d_dtslots);
+    if (g_ens.d_dtout) (void)hipFree(g_ens.d_dtout);
+    if (g_ens.h_dtout) (void)hipHostFree(g_ens.h_dtout);
+    if (g_ens.d_jobs) (void)hipFree(g_ens.d_jobs);
+    if (g_ens.d_adj) (void)hipFree(g_ens.d_adj);
+    if (g_ens.h_adj) (void)hipHostFree(g_ens.h_adj);
+    for (int d = 0; d < 3; d++) {
+        if (g_ens.d_afirst[d]) (void)hipFree(g_ens.d_afirst[d]);
+        if (g_ens.d_aitems[d]) (void)hipFree(g_ens.d_aitems[d]);
+    }
+    if (g_ens.d_sum) (void)hipFree(g_ens.d_sum);
+    if (g_ens.d_sums) (void)hipFree(g_ens.d_sums);
+    if (g_ens.h_sums) (void)hipHostFree(g_ens.h_sums);
+    if (g_ens.d_queue) (void)hipFree(g_ens.d_queue);
+    g_ens = EnsembleState();
+}
+
 void list_free()
 {
     (void)hipStreamSynchronize(g_stream);
     destroy_graph();
+    ens_free();
     for (auto &row : g_list.tab)
         for (auto &t : row) free_tab(t);
     if (g_list.d_queue) (void)hipFree(g_list.d_queue);
@@ -957,6 +1015,10 @@ int list_build()
         for (size_t i = 0; same && i < g_list.blocks.size(); i++) same = g_list.blocks[i]->cur == g_list.cur0[i];
         if (same) return PK_OK;
         destroy_graph();
+        if (g_ens.exec) (void)hipGraphExecDestroy(g_ens.exec);
+        if (g_ens.graph) (void)hipGraphDestroy(g_ens.graph);
+        g_ens.exec = nullptr;
+        g_ens.graph = nullptr;
     }
     if (g_blocks.empty()) return fail(PK_ESTATE, "no registered blocks");
     if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "the batched list sweep is the fused RK2 sweep");
@@ -1013,13 +1075,15 @@ std::vector<SweepStep> step_sweeps()
 // Enqueue the launches of one step on g_stream.  mode PK_BATCH_PER_BLOCK
 // launches the single-block sweep once per block with the same tile shapes;
 // the persistent modes launch k_sweep_list once per sweep and fluid.
-int enqueue_step(int mode, const pk::StepScalars &sc)
+int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
 {
     const std::vector<SweepStep> sw = step_sweeps();
     const bool persistent = mode != PK_BATCH_PER_BLOCK;
     if (sw.size() * (size_t)g_nfl > (size_t)kQueues) return fail(PK_EINVAL, "too many launches per step");
     if (persistent) HIPCHK(pk::launch_zero_u32(g_list.d_queue, kQueues, g_stream));
-    HIPCHK(pk::launch_dtslots_init(g_list.d_dtslots, kDtSlots, g_stream));
+    unsigned long long *slots = ens ? g_ens.d_dtslots : g_list.d_dtslots;
+    const int nslots = ens ? kEnsSlots : kDtSlots;
+    HIPCHK(pk::launch_dtslots_init(slots, nslots * (ens ? (int)g_ens.nblk : 1), g_stream));
     int q = 0;
     for (size_t k = 0; k < sw.size(); k++) {
         const int d = sw[k].d;
@@ -1037,9 +1101,10 @@ int enqueue_step(int mode, const pk::StepScalars &sc)
                 L.queue = g_list.d_queue + q++;
                 L.parity = parity;
                 L.apply_fac = sw[k].fac && g_fmag[f];
-                L.step = g_list.d_step;
-                L.dtslots = sw[k].dt ? g_list.d_dtslots : nullptr;
-                L.nslots = kDtSlots;
+                L.step = ens ? g_ens.d_step : g_list.d_step;
+                L.dtslots = sw[k].dt ? slots : nullptr;
+                L.nslots = nslots;
+                L.per_block = ens ? 1 : 0;
                 if (mode == PK_BATCH_GRAPH) {
                     HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
                 } else {
@@ -1053,18 +1118,23 @@ int enqueue_step(int mode, const pk::StepScalars &sc)
                         a.src = a.dst;
                         a.dst = const_cast<double *>(s0);
                     }
+                    const pk::StepScalars &bs = ens ? g_ens.h_step[i] : sc;
                     a.base = a.src;
-                    a.c1 = (0.5 * sc.dt) / a.dl[d];
-                    a.c2 = (1.0 * sc.dt) / a.dl[d];
-                    a.ch = sc.ch;
-                    a.psi_fac = (sw[k].fac && g_fmag[f]) ? sc.psi_fac : 1.0;
-                    a.dtslots = sw[k].dt ? g_list.d_dtslots : nullptr;
+                    a.c1 = (0.5 * bs.dt) / a.dl[d];
+                    a.c2 = (1.0 * bs.dt) / a.dl[d];
+                    a.ch = bs.ch;
+                    a.psi_fac = (sw[k].fac && g_fmag[f]) ? bs.psi_fac : 1.0;
+                    a.dtslots = sw[k].dt ? slots + (ens ? 4ll * kEnsSlots * i : 0ll) : nullptr;
+                    a.nslots = nslots;
                     HIPCHK(sweep_kernel(d, g_fmag[f], true, a));
                 }
             }
         }
     }
-    HIPCHK(pk::launch_dtslots_final(g_list.d_dtslots, kDtSlots, g_list.d_dtout, g_stream));
+    if (ens)
+        HIPCHK(pk::launch_dtslots_final_blocks(g_ens.d_dtslots, kEnsSlots, (int)g_ens.nblk, g_ens.d_dtout, g_stream));
+    else
+        HIPCHK(pk::launch_dtslots_final(g_list.d_dtslots, kDtSlots, g_list.d_dtout, g_stream));
     return PK_OK;
 }
 
@@ -1136,6 +1206,7 @@ int list_step(int mode, double dt)
         b->dt_pending = false;
     }
     g_list.fresh = true;
+    g_ens.fresh = false;
     return PK_OK;
 }
 
@@ -1143,6 +1214,141 @@ int check_mode(int mode)
 {
     if (mode != PK_BATCH_GRAPH && mode != PK_BATCH_PERSISTENT && mode != PK_BATCH_PER_BLOCK)
         return fail(PK_EINVAL, "unknown batch mode %d", mode);
+    return PK_OK;
+}
+
+// ------------------------------------------------------------------ ensemble
+
+int ens_build()
+{
+    RCCHK(list_build());
+    const size_t n = g_list.blocks.size();
+    if (g_ens.nblk == n && g_ens.d_step) return PK_OK;
+    ens_free();
+    g_ens.nblk = n;
+    HIPCHK(hipMalloc((void **)&g_ens.d_step, sizeof(pk::StepScalars) * n));
+    HIPCHK(hipHostMalloc((void **)&g_ens.h_step, sizeof(pk::StepScalars) * n, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void **)&g_ens.d_dtslots, sizeof(unsigned long long) * 4 * kEnsSlots * n));
+    HIPCHK(hipMalloc((void **)&g_ens.d_dtout, sizeof(double) * 4 * n));
+    HIPCHK(hipHostMalloc((void **)&g_ens.h_dtout, sizeof(double) * 4 * n, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void **)&g_ens.d_jobs, sizeof(pk::CopyJob) * n));
+    HIPCHK(hipMalloc((void **)&g_ens.d_queue, sizeof(unsigned) * 2));
+    g_ens.h_jobs.resize(n);
+    g_ens.dt_old.assign(n, -1.0);
+    g_ens.fresh = false;
+    return PK_OK;
+}
+
+int ens_graph()
+{
+    if (g_ens.exec) return PK_OK;
+    pk::StepScalars sc{};
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(PK_BATCH_GRAPH, sc, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(g_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIPCHK(e);
+    g_ens.graph = g;
+    HIPCHK(hipGraphInstantiate(&g_ens.exec, g_ens.graph, nullptr, nullptr, 0));
+    return PK_OK;
+}
+
+// per-block dt of the current state (timestep_fluid), before any caps
+int ens_dt_raw(std::vector<double> &dt)
+{
+    const size_t n = g_ens.nblk;
+    dt.assign(n, 0.0);
+    if (g_ens.fresh) {
+        HIPCHK(hipMemcpyAsync(g_ens.h_dtout, g_ens.d_dtout, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        for (size_t i = 0; i < n; i++) {
+            double c;
+            combine_dt(g_ens.h_dtout + 4 * i, 1, &dt[i], &c);
+        }
+        return PK_OK;
+    }
+    for (size_t i = 0; i < n; i++) {
+        Block *b = g_list.blocks[i];
+        RCCHK(do_fill(b));
+        double c;
+        RCCHK(do_timestep(b, &dt[i], &c));
+    }
+    return PK_OK;
+}
+
+// the per-block scalars of a step with per-block dts, uploaded
+int ens_set_step(const double *dts)
+{
+    HIPCHK(hipStreamSynchronize(g_stream));  // the pinned scalars may still feed a copy
+    for (size_t i = 0; i < g_ens.nblk; i++) {
+        pk::StepScalars sc{};
+        sc.dt = dts[i];
+        sc.ch = 0.0;
+        sc.psi_fac = 1.0;
+        if (g_mhd) glm_scalars({g_list.mdl[i]}, dts[i], &sc.ch, &sc.psi_fac);
+        g_ens.h_step[i] = sc;
+    }
+    HIPCHK(hipMemcpyAsync(g_ens.d_step, g_ens.h_step, sizeof(pk::StepScalars) * g_ens.nblk, hipMemcpyHostToDevice,
+                          g_stream));
+    return PK_OK;
+}
+
+// every block's current buffer <-> its slot; parity 1: the data sits in the
+// block's other buffer (between the sweeps of a step)
+int ens_copy_slots(int slot, bool save, int parity = 0)
+{
+    for (size_t i = 0; i < g_ens.nblk; i++) {
+        Block *b = g_list.blocks[i];
+        if (b->lam[0] == nullptr || slot < 0 || slot >= (int)b->slots.size())
+            return fail(PK_EINVAL, "block %lld has no slot %d (pk_adjoint_setup)", (long long)b->id, slot);
+        pk::CopyJob j{};
+        j.src = save ? b->buf[b->cur ^ parity] : b->slots[slot];
+        j.dst = save ? b->slots[slot] : b->buf[b->cur ^ parity];
+        j.n = (long long)b->sv * g_nplanes;
+        g_ens.h_jobs[i] = j;
+    }
+    HIPCHK(hipMemcpyAsync(g_ens.d_jobs, g_ens.h_jobs.data(), sizeof(pk::CopyJob) * g_ens.nblk, hipMemcpyHostToDevice,
+                          g_stream));
+    HIPCHK(pk::launch_copy_blocks(g_ens.d_jobs, (int)g_ens.nblk, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));  // h_jobs is reused by the next call
+    return PK_OK;
+}
+
+// tile tables of the persistent adjoint sweep (per direction) and the sum jobs
+int ens_adj_build()
+{
+    if (g_ens.adj_built) return PK_OK;
+    const size_t n = g_ens.nblk;
+    HIPCHK(hipMalloc((void **)&g_ens.d_adj, sizeof(pk::AdjArgs) * n));
+    HIPCHK(hipHostMalloc((void **)&g_ens.h_adj, sizeof(pk::AdjArgs) * n, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void **)&g_ens.d_sum, sizeof(pk::SumJob) * 2 * n));
+    HIPCHK(hipMalloc((void **)&g_ens.d_sums, sizeof(double) * 12 * n));
+    HIPCHK(hipHostMalloc((void **)&g_ens.h_sums, sizeof(double) * 12 * n, hipHostMallocDefault));
+    g_ens.h_sum.resize(2 * n);
+    for (int d = 0; d < 3; d++) {
+        if (!g_cfg.has_dir[d]) continue;
+        std::vector<unsigned> first(n + 1);
+        unsigned items = 0;
+        for (size_t i = 0; i < n; i++) {
+            int TP = 1, TS = 17, out = 1, chunks = 1, ptiles = 1;
+            adj_tile(g_list.blocks[i], d, TP, TS, out, chunks, ptiles);
+            first[i] = items;
+            items += (unsigned)(chunks * ptiles);
+        }
+        first[n] = items;
+        g_ens.anitems[d] = items;
+        HIPCHK(hipMalloc((void **)&g_ens.d_afirst[d], sizeof(unsigned) * (n + 1)));
+        HIPCHK(hipMalloc((void **)&g_ens.d_aitems[d], sizeof(unsigned) * (items ? items : 1)));
+        HIPCHK(hipMemcpyAsync(g_ens.d_afirst[d], first.data(), sizeof(unsigned) * (n + 1), hipMemcpyHostToDevice,
+                              g_stream));
+        HIPCHK(pk::launch_fill_items(g_ens.d_afirst[d], (int)n, g_ens.d_aitems[d], g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+    }
+    g_ens.adj_built = true;
     return PK_OK;
 }
 
@@ -1712,7 +1918,7 @@ int pk_adjoint_setup(int64_t cg_id, int32_t nslots)
         if (!g_cfg.has_dir[d]) continue;
         int TP = 1, TS = 17, out = 1, chunks = 1, ptiles = 1;
         adj_tile(b, d, TP, TS, out, chunks, ptiles);
-        const long long p = (long long)chunks * ptiles * ((TP * TS + 63) / 64);
+        const long long p = (long long)chunks * ptiles * 4;  // 4 waves per tile: 256-lane list workgroups too
         if (p > parts) parts = p;
     }
     HIPCHK(hipMalloc((void **)&b->d_dtb_part, sizeof(double) * parts));
@@ -1891,6 +2097,231 @@ int pk_timestep_adjoint(int64_t cg_id, int32_t slot, double dt_bar)
     q.cfl = g_cfg.cfl;
     q.dtb = dt_bar;
     HIPCHK(pk::launch_dt_adjoint(q, b->d_best, b->lam[b->lcur] + b->xo, g_stream));
+    return PK_OK;
+}
+
+// ---- ensembles of independent instances -------------------------------------------
+
+int pk_ensemble_step(int32_t mode, int32_t first, double *dt_out, int32_t *capped_out)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    RCCHK(check_mode(mode));
+    RCCHK(ens_build());
+    std::vector<double> dt;
+    RCCHK(ens_dt_raw(dt));
+    for (size_t i = 0; i < g_ens.nblk; i++) {
+        // time_step caps (timestep.F90:200-208) with dt_initial = -1, dt_max_grow = 2
+        int capped = 0;
+        if (!first && g_ens.dt_old[i] > 0.0) {
+            const double cap = g_ens.dt_old[i] * 2.0;
+            if (cap < dt[i]) {
+                dt[i] = cap;
+                capped = 1;
+            }
+        }
+        if (dt_out) dt_out[i] = dt[i];
+        if (capped_out) capped_out[i] = capped;
+        g_ens.dt_old[i] = dt[i];
+    }
+    RCCHK(ens_set_step(dt.data()));
+    if (mode == PK_BATCH_GRAPH) {
+        RCCHK(ens_graph());
+        HIPCHK(hipGraphLaunch(g_ens.exec, g_stream));
+    } else {
+        pk::StepScalars sc{};
+        RCCHK(enqueue_step(mode, sc, true));
+    }
+    for (Block *b : g_list.blocks) {
+        b->uh = -1;
+        b->dt_pending = false;
+    }
+    g_list.fresh = false;
+    g_ens.fresh = true;
+    return PK_OK;
+}
+
+int pk_ensemble_run(int32_t mode, int32_t nsteps, double *t_out, double *dt_out)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (nsteps < 0) return fail(PK_EINVAL, "nsteps < 0");
+    RCCHK(ens_build());
+    const size_t n = g_ens.nblk;
+    std::vector<double> t(n, 0.0), dt(n, 0.0);
+    for (int s = 0; s < nsteps; s++) {
+        RCCHK(pk_ensemble_step(mode, s == 0, dt.data(), nullptr));
+        for (size_t i = 0; i < n; i++) {
+            t[i] = t[i] + dt[i];
+            t[i] = t[i] + dt[i];
+        }
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (size_t i = 0; i < n; i++) {
+        if (t_out) t_out[i] = t[i];
+        if (dt_out) dt_out[i] = dt[i];
+    }
+    return PK_OK;
+}
+
+int pk_ensemble_state_save(int32_t slot)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    RCCHK(ens_build());
+    return ens_copy_slots(slot, true);
+}
+
+int pk_ensemble_state_load(int32_t slot)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    RCCHK(ens_build());
+    RCCHK(ens_copy_slots(slot, false));
+    list_touch();
+    for (Block *b : g_list.blocks) {
+        b->dt_pending = false;
+        b->uh = -1;
+    }
+    return PK_OK;
+}
+
+int pk_ensemble_record_step(const double *dts, int32_t slot0)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (!dts) return fail(PK_EINVAL, "null dts");
+    RCCHK(ens_build());
+    std::vector<bool> damp;
+    const std::vector<int> dirs = step_dirs(&damp);
+    RCCHK(ens_set_step(dts));
+    HIPCHK(pk::launch_zero_u32(g_list.d_queue, kQueues, g_stream));
+    for (size_t k = 0; k < dirs.size(); k++) {
+        RCCHK(ens_copy_slots(slot0 + (int)k, true, (int)(k % 2)));
+        const int d = dirs[k];
+        for (int f = 0; f < g_nfl; f++) {
+            LaunchTab &t = g_list.tab[d][f];
+            pk::ListArgs L{};
+            L.args = t.d_args;
+            L.first = t.d_first;
+            L.items = t.d_items;
+            L.nitems = t.nitems;
+            L.queue = g_list.d_queue + (int)(k * g_nfl + f);
+            L.parity = (int)(k % 2);
+            L.apply_fac = damp[k] && g_fmag[f];
+            L.step = g_ens.d_step;
+            L.dtslots = nullptr;
+            L.nslots = kEnsSlots;
+            L.per_block = 1;
+            HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
+        }
+    }
+    list_touch();
+    for (Block *b : g_list.blocks) {
+        b->dt_pending = false;
+        b->uh = -1;
+    }
+    return PK_OK;
+}
+
+int pk_ensemble_step_adjoint(const double *dts, int32_t slot0, double *dt_bar)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (!dts) return fail(PK_EINVAL, "null dts");
+    if (g_nfl != 1) return fail(PK_EINVAL, "the adjoint covers one fluid per block");
+    RCCHK(ens_build());
+    RCCHK(ens_adj_build());
+    const size_t n = g_ens.nblk;
+    const int lc = g_list.blocks[0]->lcur;
+    for (Block *b : g_list.blocks) {
+        RCCHK(adj_check(b));
+        if (b->lcur != lc) return fail(PK_ESTATE, "the blocks' adjoint states are out of step");
+    }
+    std::vector<bool> damp;
+    const std::vector<int> dirs = step_dirs(&damp);
+    std::vector<double> ch(n, 0.0), fac(n, 1.0);
+    for (size_t i = 0; i < n; i++)
+        if (g_mhd) glm_scalars({g_list.mdl[i]}, dts[i], &ch[i], &fac[i]);
+    for (int k = (int)dirs.size() - 1; k >= 0; k--) {
+        const int d = dirs[k];
+        HIPCHK(hipStreamSynchronize(g_stream));  // the pinned tables may still feed the previous copy
+        for (size_t i = 0; i < n; i++) {
+            Block *b = g_list.blocks[i];
+            if (slot0 + k >= (int)b->slots.size()) return fail(PK_EINVAL, "slot %d does not exist", slot0 + k);
+            pk::AdjArgs a{};
+            a.u = b->slots[slot0 + k] + b->xo;
+            a.lam_out = b->lam[b->lcur] + b->xo;
+            a.lam_in = b->lam[1 - b->lcur] + b->xo;
+            a.dtb_part = b->d_dtb_part;
+            a.chb_part = b->d_chb_part;
+            a.sv = b->sv;
+            a.sy = b->sy;
+            a.sz = b->sz;
+            for (int q = 0; q < 3; q++) {
+                a.ng[q] = b->ng[q];
+                a.g[q] = b->g[q];
+                a.n[q] = b->n[q];
+            }
+            adj_tile(b, d, a.TP, a.TS, a.out, a.chunks, a.ptiles);
+            a.lim = g_cfg.limiter;
+            a.limb = g_cfg.limiter_b;
+            a.gam = g_cfg.fluid[0].gamma;
+            a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
+            a.c1 = (0.5 * dts[i]) / b->dl[d];
+            a.c2 = (1.0 * dts[i]) / b->dl[d];
+            a.dl = b->dl[d];
+            a.ch = ch[i];
+            a.psi_fac = damp[k] ? fac[i] : 1.0;
+            a.smalld = g_cfg.smalld;
+            a.smallei = g_cfg.smallei;
+            a.use_smalld = g_cfg.use_smalld;
+            a.use_smallei = g_cfg.use_smallei;
+            g_ens.h_adj[i] = a;
+            const long long parts = (long long)a.chunks * a.ptiles * 4;
+            g_ens.h_sum[2 * i] = pk::SumJob{b->d_dtb_part, parts, g_ens.d_sums + 12 * i + 2 * k};
+            g_ens.h_sum[2 * i + 1] = pk::SumJob{b->d_chb_part, parts, g_ens.d_sums + 12 * i + 2 * k + 1};
+        }
+        HIPCHK(hipMemcpyAsync(g_ens.d_adj, g_ens.h_adj, sizeof(pk::AdjArgs) * n, hipMemcpyHostToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(g_ens.d_sum, g_ens.h_sum.data(), sizeof(pk::SumJob) * 2 * n, hipMemcpyHostToDevice,
+                              g_stream));
+        HIPCHK(pk::launch_zero_u32(g_ens.d_queue, 1, g_stream));
+        pk::AdjListArgs L{};
+        L.args = g_ens.d_adj;
+        L.first = g_ens.d_afirst[d];
+        L.items = g_ens.d_aitems[d];
+        L.nitems = g_ens.anitems[d];
+        L.queue = g_ens.d_queue;
+        HIPCHK(pk::launch_sweep_adjoint_list(d, g_mhd, g_cfg.limiter, g_cfg.limiter_b, L, g_stream));
+        HIPCHK(pk::launch_sum_parts_jobs(g_ens.d_sum, (int)(2 * n), g_stream));
+        for (Block *b : g_list.blocks) b->lcur = 1 - b->lcur;
+    }
+    HIPCHK(hipMemcpyAsync(g_ens.h_sums, g_ens.d_sums, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (size_t i = 0; i < n; i++) {
+        double dtb = 0.0, chb = 0.0;
+        for (size_t k = 0; k < dirs.size(); k++) {
+            dtb += g_ens.h_sums[12 * i + 2 * k];
+            chb += g_ens.h_sums[12 * i + 2 * k + 1];
+        }
+        // ch = cfl_glm*min(dl)/dt (hdc.F90:122)
+        if (g_mhd) dtb += chb * (-(ch[i] / dts[i]));
+        if (dt_bar) dt_bar[i] += dtb;
+    }
+    return PK_OK;
+}
+
+int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (!dt_bar) return fail(PK_EINVAL, "null dt_bar");
+    RCCHK(ens_build());
+    for (size_t i = 0; i < g_ens.nblk; i++)
+        if (dt_bar[i] != 0.0) RCCHK(pk_timestep_adjoint(g_list.blocks[i]->id, slot, dt_bar[i]));
+    return PK_OK;
+}
+
+int pk_ensemble_ids(int64_t *ids, int32_t n)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    RCCHK(list_build());
+    if (n < (int32_t)g_list.blocks.size()) return fail(PK_EINVAL, "room for %d ids, %d blocks", n,
+                                                      (int)g_list.blocks.size());
+    for (size_t i = 0; i < g_list.blocks.size(); i++) ids[i] = g_list.blocks[i]->id;
     return PK_OK;
 }
 

@@ -309,7 +309,6 @@ __global__ __launch_bounds__(512) void k_sweep_list(const ListArgs L)
 {
     extern __shared__ double lds[];
     __shared__ unsigned s_item;
-    const StepScalars st = *L.step;
     for (;;) {
         __syncthreads();  // the previous tile is done with LDS and s_item
         if (threadIdx.x == 0) s_item = atomicAdd(L.queue, 1u);
@@ -335,11 +334,14 @@ __global__ __launch_bounds__(512) void k_sweep_list(const ListArgs L)
             a.dst = const_cast<double *>(s0);
         }
         a.base = a.src;
-        a.c1 = (0.5 * st.dt) / a.dl[DIR];  // rk_coef * dt / dl, constants.F90:237-238
-        a.c2 = (1.0 * st.dt) / a.dl[DIR];
-        a.ch = st.ch;
-        a.psi_fac = L.apply_fac ? st.psi_fac : 1.0;
-        a.dtslots = L.dtslots;
+        typedef const __attribute__((address_space(4))) double cf64;
+        const cf64 *stp = (const cf64 *)(L.step + (L.per_block ? blk : 0));
+        const double sdt = stp[0], sch = stp[1], sfac = stp[2];
+        a.c1 = (0.5 * sdt) / a.dl[DIR];  // rk_coef * dt / dl, constants.F90:237-238
+        a.c2 = (1.0 * sdt) / a.dl[DIR];
+        a.ch = sch;
+        a.psi_fac = L.apply_fac ? sfac : 1.0;
+        a.dtslots = L.dtslots ? L.dtslots + (L.per_block ? 4ll * L.nslots * blk : 0ll) : nullptr;
         a.nslots = L.nslots;
         a.images = 1;
         sweep_tile<DIR, P, true, LIM>(a, (int)(item - first), item, lds, lds + P::NV * blockDim.x);
@@ -352,6 +354,54 @@ __global__ void k_fill_items(const unsigned *__restrict__ first, int nblocks, un
     const int b = blockIdx.x;
     if (b >= nblocks) return;
     for (unsigned i = first[b] + threadIdx.x; i < first[b + 1]; i += blockDim.x) items[i] = (unsigned)b;
+}
+
+// one workgroup per block: that block's dt slots -> its four results
+__global__ __launch_bounds__(256) void k_dtslots_final_blocks(const unsigned long long *__restrict__ slots, int nslots,
+                                                              double *__restrict__ out)
+{
+    __shared__ double red[4][256];
+    const int t = threadIdx.x;
+    const unsigned long long *s = slots + 4ll * nslots * blockIdx.x;
+    double m0 = 1.7976931348623157e308, m1 = m0, m2 = m0, cmax = 0.0;
+    for (int i = t; i < nslots; i += 256) {
+        m0 = fmn(m0, __longlong_as_double((long long)s[4 * i + 0]));
+        m1 = fmn(m1, __longlong_as_double((long long)s[4 * i + 1]));
+        m2 = fmn(m2, __longlong_as_double((long long)s[4 * i + 2]));
+        cmax = fmx(cmax, __longlong_as_double((long long)s[4 * i + 3]));
+    }
+    red[0][t] = m0;
+    red[1][t] = m1;
+    red[2][t] = m2;
+    red[3][t] = cmax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] = fmn(red[0][t], red[0][t + w]);
+            red[1][t] = fmn(red[1][t], red[1][t + w]);
+            red[2][t] = fmn(red[2][t], red[2][t + w]);
+            red[3][t] = fmx(red[3][t], red[3][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double *o = out + 4 * blockIdx.x;
+        o[0] = red[0][0];
+        o[1] = red[1][0];
+        o[2] = red[2][0];
+        o[3] = red[3][0];
+    }
+}
+
+// a batch of device-to-device copies: workgroups stride over the jobs
+__global__ __launch_bounds__(256) void k_copy_blocks(const pk::CopyJob *__restrict__ jobs, int njobs)
+{
+    for (int j = blockIdx.y; j < njobs; j += gridDim.y) {
+        const pk::CopyJob c = jobs[j];
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < c.n;
+             i += (long long)gridDim.x * blockDim.x)
+            c.dst[i] = c.src[i];
+    }
 }
 
 __global__ void k_zero_u32(unsigned *p, int n)
@@ -778,6 +828,22 @@ hipError_t launch_fill_items(const unsigned *first, int nblocks, unsigned *items
 {
     if (nblocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_items, dim3((unsigned)nblocks), dim3(256), 0, st, first, nblocks, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_dtslots_final_blocks(const unsigned long long *slots, int nslots, int nblocks, double *out,
+                                       hipStream_t st)
+{
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dtslots_final_blocks, dim3((unsigned)nblocks), dim3(256), 0, st, slots, nslots, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_blocks(const pk::CopyJob *jobs, int njobs, hipStream_t st)
+{
+    if (njobs <= 0) return hipSuccess;
+    const unsigned gy = (unsigned)(njobs < 65535 ? njobs : 65535);
+    hipLaunchKernelGGL(k_copy_blocks, dim3(256, gy), dim3(256), 0, st, jobs, njobs);
     return hipGetLastError();
 }
 

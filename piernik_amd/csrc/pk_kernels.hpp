@@ -84,9 +84,18 @@ struct ListArgs {
     unsigned *queue;         // item counter, zero before the launch
     int parity;              // 1: read buffer 1, write buffer 0
     int apply_fac;           // multiply psi by step->psi_fac
-    const StepScalars *step;
+    const StepScalars *step;      // one for the whole list, or one per block (per_block)
     unsigned long long *dtslots;  // if set: fold timestep_fluid of the cells written into these slots
-    int nslots;
+    int nslots;                   // slots (per block when per_block)
+    int per_block;                // independent instances: each block has its own dt, ch, psi factor
+                                  // and its own nslots dt slots
+};
+
+// One device-to-device copy of a batch (k_copy_blocks).
+struct CopyJob {
+    const double *src;
+    double *dst;
+    long long n;  // doubles
 };
 
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st);
@@ -96,6 +105,10 @@ hipError_t launch_sweep_list(int dir, bool mhd, int lim, int limb, const ListArg
                              hipStream_t st);
 int sweep_list_resident(int dir, bool mhd, int lim, int limb, int nt);
 hipError_t launch_fill_items(const unsigned *first, int nblocks, unsigned *items, hipStream_t st);
+// per-block dt slots -> per-block (min dl_x/c, min dl_y/c, min dl_z/c, max c)
+hipError_t launch_dtslots_final_blocks(const unsigned long long *slots, int nslots, int nblocks, double *out,
+                                       hipStream_t st);
+hipError_t launch_copy_blocks(const CopyJob *jobs, int njobs, hipStream_t st);
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st);
 size_t sweep_lds_bytes(bool mhd, int TP, int TS);
 hipError_t launch_fill_ghosts(int dir, const GhostArgs &a, hipStream_t st);

@@ -24,7 +24,9 @@ module piernik_hip
         &    pk_upload_device, pk_download_device, pk_timestep_all, pk_fluid_update_all, pk_run_all, &
         &    pk_autotune_all, pk_list_tiles, pk_adjoint_setup, pk_state_save, pk_state_load, &
         &    pk_adjoint_set, pk_adjoint_get, pk_fluid_update_record, pk_sweep_adjoint, &
-        &    pk_fluid_update_adjoint, pk_timestep_adjoint
+        &    pk_fluid_update_adjoint, pk_timestep_adjoint, pk_ensemble_ids, pk_ensemble_step, pk_ensemble_run, &
+        &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
+        &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
 
@@ -292,6 +294,56 @@ module piernik_hip
          integer(c_int32_t), value :: slot
          real(c_double),     value :: dt_bar
       end function pk_timestep_adjoint
+
+      ! ensembles of independent instances (arrays: one entry per registered block)
+      integer(c_int32_t) function pk_ensemble_ids(ids, n) bind(C, name="pk_ensemble_ids")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), intent(out) :: ids(*)
+         integer(c_int32_t), value       :: n
+      end function pk_ensemble_ids
+
+      integer(c_int32_t) function pk_ensemble_step(mode, first, dt_out, capped_out) bind(C, name="pk_ensemble_step")
+         import :: c_int32_t, c_double
+         integer(c_int32_t), value       :: mode, first
+         real(c_double),     intent(out) :: dt_out(*)
+         integer(c_int32_t), intent(out) :: capped_out(*)
+      end function pk_ensemble_step
+
+      integer(c_int32_t) function pk_ensemble_run(mode, nsteps, t_out, dt_out) bind(C, name="pk_ensemble_run")
+         import :: c_int32_t, c_double
+         integer(c_int32_t), value       :: mode, nsteps
+         real(c_double),     intent(out) :: t_out(*), dt_out(*)
+      end function pk_ensemble_run
+
+      integer(c_int32_t) function pk_ensemble_state_save(slot) bind(C, name="pk_ensemble_state_save")
+         import :: c_int32_t
+         integer(c_int32_t), value :: slot
+      end function pk_ensemble_state_save
+
+      integer(c_int32_t) function pk_ensemble_state_load(slot) bind(C, name="pk_ensemble_state_load")
+         import :: c_int32_t
+         integer(c_int32_t), value :: slot
+      end function pk_ensemble_state_load
+
+      integer(c_int32_t) function pk_ensemble_record_step(dts, slot0) bind(C, name="pk_ensemble_record_step")
+         import :: c_int32_t, c_double
+         real(c_double),     intent(in) :: dts(*)
+         integer(c_int32_t), value      :: slot0
+      end function pk_ensemble_record_step
+
+      integer(c_int32_t) function pk_ensemble_step_adjoint(dts, slot0, dt_bar) bind(C, name="pk_ensemble_step_adjoint")
+         import :: c_int32_t, c_double
+         real(c_double),     intent(in)    :: dts(*)
+         integer(c_int32_t), value         :: slot0
+         real(c_double),     intent(inout) :: dt_bar(*)
+      end function pk_ensemble_step_adjoint
+
+      integer(c_int32_t) function pk_ensemble_timestep_adjoint(slot, dt_bar) &
+           &                                                   bind(C, name="pk_ensemble_timestep_adjoint")
+         import :: c_int32_t, c_double
+         integer(c_int32_t), value      :: slot
+         real(c_double),     intent(in) :: dt_bar(*)
+      end function pk_ensemble_timestep_adjoint
 
       integer(c_int32_t) function pk_profile_begin() bind(C, name="pk_profile_begin")
          import :: c_int32_t

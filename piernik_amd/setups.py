@@ -407,3 +407,21 @@ def orszag_tang_2d(n, seed=0, amp=1.0, perturb=0.0, gamma=5.0 / 3.0):
     cfg = Config(gamma=gamma, magnetized=True, limiter="vanleer", cfl=0.5, cfl_glm=0.5, smalld=1e-10,
                  smallei=1e-10, has_dir=(True, True, False))
     return u, b, psi, cfg
+
+
+OT_SIZES = (64, 96, 128, 192, 256)
+
+
+def orszag_tang_ensemble(ninst=256, seed=0, sizes=OT_SIZES, spread=0.1):
+    """Instances of the 2D Orszag-Tang vortex with sizes drawn from `sizes`
+    and per-instance amplitudes 1 +- spread (seeded).  Returns a list of
+    (n, u, b, psi) interior arrays and the Config."""
+    rng = np.random.default_rng([seed, 5555])
+    out = []
+    cfg = None
+    for i in range(ninst):
+        n = int(rng.choice(np.asarray(sizes)))
+        amp = 1.0 + spread * rng.uniform(-1.0, 1.0)
+        u, b, psi, cfg = orszag_tang_2d(n, seed=i, amp=amp)
+        out.append((n, u, b, psi))
+    return out, cfg
