@@ -539,9 +539,8 @@ __global__ __launch_bounds__(256) void k_dt_argmin(const DtAdjArgs a, unsigned l
     }
 }
 
-__global__ void k_dt_apply(const DtAdjArgs a, const unsigned long long *best, double *lam)
+__device__ void k_dt_apply_body(const DtAdjArgs &a, const unsigned long long *best, double *lam)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const unsigned long long b = *best;
     if (b == ~0ull) return;
     const long long c = (long long)(b / 3);
@@ -563,6 +562,86 @@ __global__ void k_dt_apply(const DtAdjArgs a, const unsigned long long *best, do
         dt_vjp_hd(u, q, a.dl[q], a.gam, a.gam_1, a.cfl * a.dtb, ub);
         for (int v = 0; v < 5; v++) lam[(long long)v * a.sv + idx] += ub[v];
     }
+}
+
+__global__ void k_dt_apply(const DtAdjArgs a, const unsigned long long *best, double *lam)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    k_dt_apply_body(a, best, lam);
+}
+
+// one workgroup per job: min over the job's cells and directions of dl/c
+__global__ __launch_bounds__(256) void k_dt_mn_jobs(const DtBatchJob *__restrict__ jobs)
+{
+    __shared__ double red[256];
+    const DtBatchJob &J = jobs[blockIdx.x];
+    const DtAdjArgs &a = J.a;
+    const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+    double m = 1.7976931348623157e308;
+    for (long long c = threadIdx.x; c < ncell; c += 256) {
+        const int i = (int)(c % a.n[0]);
+        const long long r = c / a.n[0];
+        const int j = (int)(r % a.n[1]);
+        const int k = (int)(r / a.n[1]);
+        const long long idx =
+            (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
+        const double *u = a.u + idx;
+        const double d = u[0];
+        const double cs = cell_cs(a, u);
+        for (int q = 0; q < 3; q++) {
+            if (!a.has_dir[q]) continue;
+            const double v = (d > 0.0) ? fabs(u[(1 + q) * a.sv] / d) : 0.0;
+            m = fmn(m, a.dl[q] / fmx(v + cs, 1.17549435082228750797e-38));
+        }
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmn(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *J.mn = red[0];
+        *J.best = ~0ull;
+    }
+}
+
+// grid (chunks, jobs): the smallest cell/direction index attaining the minimum
+__global__ __launch_bounds__(256) void k_dt_argmin_jobs(const DtBatchJob *__restrict__ jobs)
+{
+    const DtBatchJob &J = jobs[blockIdx.y];
+    const DtAdjArgs &a = J.a;
+    const double mn = *J.mn;
+    const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % a.n[0]);
+        const long long r = c / a.n[0];
+        const int j = (int)(r % a.n[1]);
+        const int k = (int)(r / a.n[1]);
+        const long long idx =
+            (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
+        const double *u = a.u + idx;
+        const double d = u[0];
+        const double cs = cell_cs(a, u);
+        for (int q = 0; q < 3; q++) {
+            if (!a.has_dir[q]) continue;
+            const double v = (d > 0.0) ? fabs(u[(1 + q) * a.sv] / d) : 0.0;
+            if (a.dl[q] / fmx(v + cs, 1.17549435082228750797e-38) == mn)
+                atomicMin(J.best, (unsigned long long)(c * 3 + q));
+        }
+    }
+}
+
+// one thread per job
+__global__ void k_dt_apply_jobs(const DtBatchJob *__restrict__ jobs, int njobs)
+{
+    const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jb >= njobs) return;
+    const DtBatchJob &J = jobs[jb];
+    DtAdjArgs a = J.a;
+    a.dtb = J.dtb;
+    k_dt_apply_body(a, J.best, J.lam);
 }
 
 // ---- launchers ------------------------------------------------------------------
@@ -717,6 +796,15 @@ hipError_t launch_fold_ghosts(double *lam, long long sv, int sy, int sz, const i
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_dt_adjoint_batch(const DtBatchJob *jobs, int njobs, hipStream_t st)
+{
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dt_mn_jobs, dim3((unsigned)njobs), dim3(256), 0, st, jobs);
+    hipLaunchKernelGGL(k_dt_argmin_jobs, dim3(64, (unsigned)njobs), dim3(256), 0, st, jobs);
+    hipLaunchKernelGGL(k_dt_apply_jobs, dim3((unsigned)((njobs + 63) / 64)), dim3(64), 0, st, jobs, njobs);
+    return hipGetLastError();
 }
 
 hipError_t launch_dt_adjoint(const DtAdjArgs &a, unsigned long long *best, double *lam, hipStream_t st)

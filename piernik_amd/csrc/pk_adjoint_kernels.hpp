@@ -61,7 +61,19 @@ struct SumJob {
     double *out;
 };
 
+// Batched timestep adjoint over independent instances: per job the minimum
+// of dl/c (written to *mn), its cell and direction (*best), then
+// lam += dtb * d(cfl*dl/c)/du there.  a.mn and a.dtb are ignored (dtb here).
+struct DtBatchJob {
+    DtAdjArgs a;
+    double dtb;
+    double *lam;
+    double *mn;
+    unsigned long long *best;
+};
+
 long long sweep_adjoint_parts(const AdjArgs &a);
+hipError_t launch_dt_adjoint_batch(const DtBatchJob *jobs, int njobs, hipStream_t st);
 hipError_t launch_sweep_adjoint_list(int dir, bool mhd, int lim, int limb, const AdjListArgs &L, hipStream_t st);
 hipError_t launch_sum_parts_jobs(const SumJob *jobs, int njobs, hipStream_t st);
 // dtb_out[0] = dL/d(dt) at fixed ch, dtb_out[1] = dL/d(ch)

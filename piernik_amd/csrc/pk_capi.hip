@@ -848,6 +848,10 @@ struct EnsembleState {
     bool adj_built = false;
     pk::SumJob *d_sum = nullptr;
     std::vector<pk::SumJob> h_sum;
+    pk::DtBatchJob *d_dtj = nullptr;
+    std::vector<pk::DtBatchJob> h_dtj;
+    double *d_mn = nullptr;
+    unsigned long long *d_best = nullptr;
     double *d_sums = nullptr, *h_sums = nullptr;  // per block, per sweep: dtb, chb
     unsigned *d_queue = nullptr;
 };
@@ -903,6 +907,9 @@ void ens_free()
         if (g_ens.d_aitems[d]) (void)hipFree(g_ens.d_aitems[d]);
     }
     if (g_ens.d_sum) (void)hipFree(g_ens.d_sum);
+    if (g_ens.d_dtj) (void)hipFree(g_ens.d_dtj);
+    if (g_ens.d_mn) (void)hipFree(g_ens.d_mn);
+    if (g_ens.d_best) (void)hipFree(g_ens.d_best);
     if (g_ens.d_sums) (void)hipFree(g_ens.d_sums);
     if (g_ens.h_sums) (void)hipHostFree(g_ens.h_sums);
     if (g_ens.d_queue) (void)hipFree(g_ens.d_queue);
@@ -2309,9 +2316,46 @@ int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar)
 {
     if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
     if (!dt_bar) return fail(PK_EINVAL, "null dt_bar");
+    if (g_nfl != 1) return fail(PK_EINVAL, "the adjoint covers one fluid per block");
     RCCHK(ens_build());
-    for (size_t i = 0; i < g_ens.nblk; i++)
-        if (dt_bar[i] != 0.0) RCCHK(pk_timestep_adjoint(g_list.blocks[i]->id, slot, dt_bar[i]));
+    const size_t n = g_ens.nblk;
+    if (!g_ens.d_dtj) {
+        HIPCHK(hipMalloc((void **)&g_ens.d_dtj, sizeof(pk::DtBatchJob) * n));
+        HIPCHK(hipMalloc((void **)&g_ens.d_mn, sizeof(double) * n));
+        HIPCHK(hipMalloc((void **)&g_ens.d_best, sizeof(unsigned long long) * n));
+        g_ens.h_dtj.resize(n);
+    }
+    // timestep_fluid of each block at the state in slot: all blocks in three launches
+    HIPCHK(hipStreamSynchronize(g_stream));  // h_dtj may still feed the previous copy
+    for (size_t i = 0; i < n; i++) {
+        Block *b = g_list.blocks[i];
+        RCCHK(adj_check(b));
+        if (slot < 0 || slot >= (int)b->slots.size()) return fail(PK_EINVAL, "slot %d does not exist", slot);
+        pk::DtBatchJob j{};
+        j.a.u = b->slots[slot] + b->xo;
+        j.a.magnetized = g_mhd ? 1 : 0;
+        j.a.sv = b->sv;
+        j.a.sy = b->sy;
+        j.a.sz = b->sz;
+        for (int d = 0; d < 3; d++) {
+            j.a.n[d] = b->n[d];
+            j.a.g[d] = b->g[d];
+            j.a.has_dir[d] = g_cfg.has_dir[d];
+            j.a.dl[d] = b->dl[d];
+        }
+        j.a.gam = g_cfg.fluid[0].gamma;
+        j.a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
+        j.a.cfl = g_cfg.cfl;
+        j.dtb = dt_bar[i];
+        j.lam = b->lam[b->lcur] + b->xo;
+        j.mn = g_ens.d_mn + i;
+        j.best = g_ens.d_best + i;
+        g_ens.h_dtj[i] = j;
+    }
+    HIPCHK(hipMemcpyAsync(g_ens.d_dtj, g_ens.h_dtj.data(), sizeof(pk::DtBatchJob) * n, hipMemcpyHostToDevice,
+                          g_stream));
+    HIPCHK(pk::launch_dt_adjoint_batch(g_ens.d_dtj, (int)n, g_stream));
+    for (Block *b : g_list.blocks) b->dt_pending = false;
     return PK_OK;
 }
 

@@ -372,7 +372,7 @@ def orszag_tang_2d(n, seed=0, amp=1.0, perturb=0.0, gamma=5.0 / 3.0):
     """Orszag-Tang vortex on the unit square (2D, n x n): rho = 25/(36 pi),
     p = 5/(12 pi), v = amp*(-sin 2pi y, sin 2pi x), B = amp*B0*(-sin 2pi y,
     sin 4pi x), B0 = 1/sqrt(4 pi), psi = 0; `perturb` adds seeded smooth
-    relative perturbations to rho and p and small v_z, B_z (a generic state).  Returns interior
+    perturbations to every component, psi included (a generic state).  Returns interior
     arrays u (5, n, n, 1), b (3, n, n, 1), psi (n, n, 1) and the Config."""
     x = (np.arange(n) + 0.5) / n
     X, Y = np.meshgrid(x, x, indexing="ij")
@@ -390,6 +390,12 @@ def orszag_tang_2d(n, seed=0, amp=1.0, perturb=0.0, gamma=5.0 / 3.0):
     vx, vy = -amp * np.sin(2 * np.pi * Y), amp * np.sin(2 * np.pi * X)
     b0 = 1.0 / np.sqrt(4.0 * np.pi)
     bx, by = -amp * b0 * np.sin(2 * np.pi * Y), amp * b0 * np.sin(4 * np.pi * X)
+    # a generic state: the plain vortex has v_x, B_x constant along x and
+    # v_y, B_y constant along y (limited slopes exactly 0)
+    vx = vx + perturb * smooth()
+    vy = vy + perturb * smooth()
+    bx = bx + perturb * b0 * smooth()
+    by = by + perturb * b0 * smooth()
     vz = perturb * smooth()
     bz = perturb * b0 * smooth()
     u = np.zeros((5, n, n, 1))
@@ -404,6 +410,7 @@ def orszag_tang_2d(n, seed=0, amp=1.0, perturb=0.0, gamma=5.0 / 3.0):
     b[1, ..., 0] = by
     b[2, ..., 0] = bz
     psi = np.zeros((n, n, 1))
+    psi[..., 0] = perturb * b0 * smooth()
     cfg = Config(gamma=gamma, magnetized=True, limiter="vanleer", cfl=0.5, cfl_glm=0.5, smalld=1e-10,
                  smallei=1e-10, has_dir=(True, True, False))
     return u, b, psi, cfg
