@@ -106,7 +106,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128", "ensemble256"],
+    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128", "ensemble256", "copy"],
                     default="mhd256")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--nblocks", type=int, default=4096, help="blocks of the batched list")
@@ -128,6 +128,15 @@ def main():
     # CPU baseline first, before this process touches the GPU
     cpu = None if args.no_cpu else cpu_baseline(args.workload, args.cpu_budget)
     print(f"[bench] cpu baseline done: {cpu and round(cpu['value'])} cell-updates/s", file=sys.stderr, flush=True)
+    if args.workload == "copy":
+        gbps = copy_ceiling(steps=args.steps)
+        print(json.dumps({
+            "metric": "device-to-device copy bandwidth", "value": gbps, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": 1, "ms_per_step": 2 * COPY_BYTES / gbps / 1e6, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"torch copy_ of {COPY_BYTES >> 30} GiB between two device tensors "
+                                   "(read + write counted)"}}))
+        return
     if args.workload == "batch4096":
         print(json.dumps(run_batch(args, cpu)))
         return
@@ -188,6 +197,7 @@ def main():
 
     rep = pk.get_report(cg)
     pk.finalize()
+    copy_gbps = copy_ceiling()
     adj = None
     if not args.no_adjoint:
         print("[bench] adjoint cost ratio (128^3 hydro rollout)", file=sys.stderr, flush=True)
@@ -231,6 +241,8 @@ def main():
             "algo_bytes_per_launch": algo / launches if launches else None,
             "avg_launch_ms": ms / launches if launches else None,
             "per_dir_gbps": per_dir,
+            "measured_copy_gbps": copy_gbps,
+            "frac_of_measured_copy": achieved / copy_gbps,
         },
         "cpu_baseline": cpu,
         "floors": rep,
@@ -352,6 +364,30 @@ def run_ensemble(args, ninst=256, steps=40, every=8):
                      "cell_updates_per_s": cells * 2 * steps / t_plain, "magnetic_energy": energy,
                      "gradient_norm": gnorm},
     }
+
+
+COPY_BYTES = 1 << 30
+
+
+def copy_ceiling(steps=20):
+    """Measured HBM ceiling: GB/s of a plain device-to-device copy of 1 GiB
+    (bytes read + bytes written over the kernel time).  The same command
+    under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE gives the counters' scale
+    for a kernel of known traffic."""
+    import torch
+    a = torch.full((COPY_BYTES // 8,), 1.0, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(max(1, steps)):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / max(1, steps)
+    del a, b
+    return 2 * COPY_BYTES / (ms * 1e-3) / 1e9
 
 
 def run_batch(args, cpu):

@@ -414,38 +414,58 @@ __global__ __launch_bounds__(256) void k_sweep_adjoint_list(const AdjListArgs L)
     }
 }
 
-// one workgroup per job: the sum of a block's per-wave partials (fixed order)
+// Deterministic sum of n per-wave partials: kRedParts fixed ranges
+// [n*g/kRedParts, n*(g+1)/kRedParts), each summed by one 256-lane workgroup
+// (per-lane strided sums, then a tree), then the range sums added in order g.
+// The one-workgroup-per-job kernel repeats the same ranges and order, so both
+// give the same bits.
+__device__ double wg_range_sum(const double *__restrict__ part, long long lo, long long hi, double *red)
+{
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (long long i = lo + t; i < hi; i += 256) acc += part[i];
+    red[t] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] = red[t] + red[t + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ long long range_lo(long long n, int g) { return n * g / kRedParts; }
+
+// one workgroup per job
 __global__ __launch_bounds__(256) void k_sum_parts_jobs(const SumJob *__restrict__ jobs)
 {
     __shared__ double red[256];
     const SumJob j = jobs[blockIdx.x];
-    const int t = threadIdx.x;
-    double acc = 0.0;
-    for (long long i = t; i < j.n; i += 256) acc += j.part[i];
-    red[t] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) red[t] = red[t] + red[t + w];
-        __syncthreads();
-    }
-    if (t == 0) j.out[0] = red[0];
+    double tot = 0.0;
+    for (int g = 0; g < kRedParts; g++) tot += wg_range_sum(j.part, range_lo(j.n, g), range_lo(j.n, g + 1), red);
+    if (threadIdx.x == 0) j.out[0] = tot;
 }
 
-// Sum of the per-wave partials in a fixed order (deterministic).
-__global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ part, long long n,
-                                                   double *__restrict__ out)
+// grid (kRedParts, 2): range g of partial array y, into part_y[n + g]
+__global__ __launch_bounds__(256) void k_sum_ranges(double *__restrict__ p0, double *__restrict__ p1, long long n)
 {
     __shared__ double red[256];
-    const int t = threadIdx.x;
-    double acc = 0.0;
-    for (long long i = t; i < n; i += 256) acc += part[i];
-    red[t] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) red[t] = red[t] + red[t + w];
-        __syncthreads();
-    }
-    if (t == 0) out[0] = red[0];
+    double *p = blockIdx.y ? p1 : p0;
+    const int g = blockIdx.x;
+    const double r = wg_range_sum(p, range_lo(n, g), range_lo(n, g + 1), red);
+    if (threadIdx.x == 0) p[n + g] = r;
+}
+
+// grid 2: the range sums in order
+__global__ void k_sum_final(const double *__restrict__ p0, const double *__restrict__ p1, long long n,
+                            double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const double *p = blockIdx.x ? p1 : p0;
+    double tot = 0.0;
+    for (int g = 0; g < kRedParts; g++) tot += p[n + g];
+    out[blockIdx.x] = tot;
 }
 
 // Adjoint of the periodic ghost refresh along DIR (k_fill_ghosts): every
@@ -757,21 +777,21 @@ static hipError_t adj_dir(int dir, const AdjArgs &a, int nt, long long nblk, hip
     return hipErrorInvalidValue;
 }
 
-long long sweep_adjoint_parts(const AdjArgs &a)
-{
-    const int nt = a.TP * a.TS;
-    return (long long)a.chunks * a.ptiles * ((nt + 63) / 64);
-}
+// Every adjoint tile runs on a 256-lane workgroup (lanes past TP*TS idle),
+// as in the persistent list kernel: four partial slots per tile in both, so
+// their sums of the same partials agree bit for bit.
+long long sweep_adjoint_parts(const AdjArgs &a) { return (long long)a.chunks * a.ptiles * 4; }
 
 hipError_t launch_sweep_adjoint(int dir, bool mhd, const AdjArgs &a, double *dtb_out, hipStream_t st)
 {
-    const int nt = a.TP * a.TS;
-    if (nt < 1 || nt > 256 || a.out < 1 || a.out != a.TS - 16) return hipErrorInvalidValue;
+    const int nt = 256;
+    if (a.TP * a.TS < 1 || a.TP * a.TS > nt || a.out < 1 || a.out != a.TS - 16) return hipErrorInvalidValue;
     const long long nblk = (long long)a.chunks * a.ptiles;
     hipError_t e = mhd ? adj_dir<MHD>(dir, a, nt, nblk, st) : adj_dir<HD>(dir, a, nt, nblk, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, a.dtb_part, sweep_adjoint_parts(a), dtb_out);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, a.chb_part, sweep_adjoint_parts(a), dtb_out + 1);
+    const long long n = sweep_adjoint_parts(a);
+    hipLaunchKernelGGL(k_sum_ranges, dim3(kRedParts, 2), dim3(256), 0, st, a.dtb_part, a.chb_part, n);
+    hipLaunchKernelGGL(k_sum_final, dim3(2), dim3(64), 0, st, a.dtb_part, a.chb_part, n, dtb_out);
     return hipGetLastError();
 }
 

@@ -72,6 +72,9 @@ struct DtBatchJob {
     unsigned long long *best;
 };
 
+// Partial-sum buffers (dtb_part, chb_part) hold sweep_adjoint_parts(a)
+// partials followed by kRedParts doubles of reduction scratch.
+constexpr int kRedParts = 64;
 long long sweep_adjoint_parts(const AdjArgs &a);
 hipError_t launch_dt_adjoint_batch(const DtBatchJob *jobs, int njobs, hipStream_t st);
 hipError_t launch_sweep_adjoint_list(int dir, bool mhd, int lim, int limb, const AdjListArgs &L, hipStream_t st);

@@ -357,8 +357,11 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
 
 // One directional sweep, both RK2 stages, every fluid.  images: write the
 // periodic ghost images too (the block is periodic on itself); want_dt:
-// reduce the next step's timestep_fluid from the cells written.
-int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool images = false, bool want_dt = false)
+// reduce the next step's timestep_fluid from the cells written.  With
+// src/dst (whole state buffers, one fluid) the sweep reads src and writes
+// dst instead of the block's ping-pong pair, whose state is left alone.
+int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool images = false, bool want_dt = false,
+             double *src = nullptr, double *dst = nullptr)
 {
     if (g_cfg.integration_order == 1) {
         RCCHK(do_stage(b, ddim, PK_EULER, dt, ch));
@@ -377,13 +380,14 @@ int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool imag
         a.c1 = (0.5 * dt) / b->dl[ddim];
         a.c2 = (1.0 * dt) / b->dl[ddim];
         a.psi_fac = psi_fac;
-        a.src = fluid_state(b, b->cur, f);
+        a.src = src ? src + b->xo + (long long)g_fplane[f] * b->sv : fluid_state(b, b->cur, f);
         a.base = a.src;
-        a.dst = fluid_state(b, 1 - b->cur, f);
+        a.dst = dst ? dst + b->xo + (long long)g_fplane[f] * b->sv : fluid_state(b, 1 - b->cur, f);
         a.images = images ? 1 : 0;
         a.dtslots = want_dt ? b->d_dtslots : nullptr;
         RCCHK(launch(ddim, true, a, b, f));
     }
+    if (src || dst) return PK_OK;
     b->cur = 1 - b->cur;
     b->uh = -1;
     b->dt_pending = false;
@@ -1928,8 +1932,8 @@ int pk_adjoint_setup(int64_t cg_id, int32_t nslots)
         const long long p = (long long)chunks * ptiles * 4;  // 4 waves per tile: 256-lane list workgroups too
         if (p > parts) parts = p;
     }
-    HIPCHK(hipMalloc((void **)&b->d_dtb_part, sizeof(double) * parts));
-    HIPCHK(hipMalloc((void **)&b->d_chb_part, sizeof(double) * parts));
+    HIPCHK(hipMalloc((void **)&b->d_dtb_part, sizeof(double) * (parts + pk::kRedParts)));
+    HIPCHK(hipMalloc((void **)&b->d_chb_part, sizeof(double) * (parts + pk::kRedParts)));
     b->dtb_parts = parts;
     HIPCHK(hipMalloc((void **)&b->d_dtb, 2 * 6 * sizeof(double)));
     HIPCHK(hipMalloc((void **)&b->d_best, sizeof(unsigned long long)));
@@ -2016,12 +2020,19 @@ int pk_fluid_update_record(int64_t cg_id, int32_t slot0, double dt)
     step_dirs(&damp);
     double ch, fac;
     block_glm(b, dt, &ch, &fac);
-    for (size_t i = 0; i < dirs.size(); i++) {
-        HIPCHK(hipMemcpyAsync(b->slots[slot0 + i], b->buf[b->cur], state_bytes(b), hipMemcpyDeviceToDevice,
-                              g_stream));
-        const bool last = i + 1 == dirs.size();
-        RCCHK(do_sweep(b, dirs[i], dt, ch, damp[i] ? fac : 1.0, true, last));
-    }
+    // the state before sweep i lands in slot0 + i: one copy, then each sweep
+    // but the last writes the next slot directly (periodic images included)
+    HIPCHK(hipMemcpyAsync(b->slots[slot0], b->buf[b->cur], state_bytes(b), hipMemcpyDeviceToDevice, g_stream));
+    for (size_t i = 0; i + 1 < dirs.size(); i++)
+        RCCHK(do_sweep(b, dirs[i], dt, ch, damp[i] ? fac : 1.0, true, false, b->slots[slot0 + i],
+                       b->slots[slot0 + i + 1]));
+    const size_t l = dirs.size() - 1;
+    RCCHK(do_sweep(b, dirs[l], dt, ch, damp[l] ? fac : 1.0, true, true, b->slots[slot0 + l], b->buf[1 - b->cur]));
+    b->cur = 1 - b->cur;
+    b->uh = -1;
+    b->dt_pending = false;
+    HIPCHK(pk::launch_dtslots_final(b->d_dtslots, kDtSlots, b->d_dtout, g_stream));
+    b->dt_pending = true;
     return PK_OK;
 }
 
