@@ -138,9 +138,26 @@ int pk_solve_cg(int64_t cg_id, int32_t ddim, int32_t istep, double dt, double ch
  * into the last sweep of a substep; 1 otherwise). */
 int pk_sweep(int64_t cg_id, int32_t ddim, double dt, double chspeed, double psi_fac);
 
-/* Periodic ghost refresh of all variables of one block, all directions,
- * corners included (update_boundaries on a single periodic block). */
+/* Ghost refresh of all variables of one block, all directions, corners
+ * included (update_boundaries, sweeps.F90:66-107): periodic on itself, or
+ * for a block with neighbours (pk_set_neighbour) the exchange over the
+ * whole registered list from the neighbours' current state. */
 int pk_fill_ghosts(int64_t cg_id);
+
+/* Connect blocks of the registered list (internal boundaries,
+ * cg_list_bnd.F90:169-322, which the reference finds from the grid's
+ * segment lists, cg%i_bnd / cg%o_bnd): the ghost layers of cg_id on side
+ * `side` (0 low, 1 high) of direction `dir` (0 x, 1 y, 2 z) take the
+ * adjoining nb interior layers of block nb_id (across the domain for a
+ * periodic boundary).  A block registered is its own neighbour on every side
+ * (periodic on itself).  Both blocks must share the face (the same ghosted
+ * extents in the other two directions), nb and the cell size (no fine/coarse
+ * boundaries).  A list with such links is advanced only by the block-list
+ * functions (pk_fluid_update_all, pk_run_all), which exchange the faces each
+ * sweep reads before that sweep; pk_sweep, pk_fluid_update, pk_run and the
+ * reverse-mode and ensemble functions refuse a block with neighbours.
+ * Downloads refresh every ghost cell first. */
+int pk_set_neighbour(int64_t cg_id, int32_t dir, int32_t side, int64_t nb_id);
 
 /* psi *= fac over the whole block (glmdamping, hdc.F90:338). */
 int pk_glm_damp(int64_t cg_id, double fac);

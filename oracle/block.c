@@ -353,8 +353,108 @@ void or_fluid_update_list(const or_params *p, or_block *const *cgs, int nblk, do
  * blocks (timestep.F90:112-225), then fluid_update over the list. */
 double or_run_list(const or_params *p, or_block *const *cgs, int nblk, int nsteps, or_report *reps)
 {
+    return or_run_list_nb(p, cgs, NULL, nblk, nsteps, reps);
+}
+
+/* Ghost exchange between the blocks of a list (internal_boundaries_local,
+ * cg_list_bnd.F90:169-322, as all_fluid_boundaries and all_mag_boundaries
+ * call it, all_boundaries.F90:82-175).  nbr[6*i + 2*d + s] is the index of
+ * the block whose interior adjoins block i's ghost layers on side s (0 low,
+ * 1 high) of direction d; a block periodic on itself is its own neighbour.
+ * The ghost layers of direction d are copied from the neighbour's nb
+ * adjoining interior layers over the whole extent of the other two
+ * directions, ghost cells included, x then y then z, so edges and corners
+ * come out as the corner-complete refresh gives them.  Blocks of one list
+ * share nb and their extents across each face. */
+void or_fill_ghosts_list(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk)
+{
+    int nv = p->nvar, mhd = any_mag(p);
+    for (int d = 0; d < 3; d++) {
+        if (!p->has_dir[d]) continue;
+        for (int i = 0; i < nblk; i++) {
+            or_block *cg = cgs[i];
+            int ng[3];
+            gdims(p, cg, ng);
+            int g = gh(p, cg, d);
+            for (int side = 0; side < 2; side++) {
+                const or_block *o = cgs[nbr[6 * i + 2 * d + side]];
+                int ngo[3];
+                gdims(p, o, ngo);
+                /* own ghost layers and the neighbour's adjoining interior layers */
+                int dst0 = side == 0 ? 0 : g + cg->n[d];
+                int src0 = side == 0 ? o->n[d] : g;
+                for (int k = 0; k < ng[2]; k++)
+                    for (int j = 0; j < ng[1]; j++)
+                        for (int ii = 0; ii < ng[0]; ii++) {
+                            int c[3] = {ii, j, k};
+                            if (c[d] >= g) continue;
+                            int dc3[3] = {ii, j, k}, sc3[3] = {ii, j, k};
+                            dc3[d] = dst0 + c[d];
+                            sc3[d] = src0 + c[d];
+                            size_t dc = (size_t)dc3[0] + (size_t)ng[0] * ((size_t)dc3[1] + (size_t)ng[1] * dc3[2]);
+                            size_t sc = (size_t)sc3[0] + (size_t)ngo[0] * ((size_t)sc3[1] + (size_t)ngo[1] * sc3[2]);
+                            for (int v = 0; v < nv; v++) cg->u[dc * nv + v] = o->u[sc * nv + v];
+                            if (mhd) {
+                                for (int q = 0; q < 3; q++) cg->b[dc * 3 + q] = o->b[sc * 3 + q];
+                                cg->psi[dc] = o->psi[sc];
+                            }
+                        }
+            }
+        }
+    }
+}
+
+static void fill_list(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk)
+{
+    if (nbr)
+        or_fill_ghosts_list(p, cgs, nbr, nblk);
+    else
+        for (int i = 0; i < nblk; i++) or_fill_ghosts_periodic(p, cgs[i]);
+}
+
+/* fluid_update_full over a list of blocks connected by nbr (as in
+ * or_fill_ghosts_list): sweep() runs each stage on every block, then
+ * refreshes the ghosts of the whole list (sweeps.F90:217-272, :66-107). */
+void or_fluid_update_list_nb(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk, double dt,
+                             or_report *reps)
+{
+    int mhd = any_mag(p);
+    double ch = DBL_MAX, fac = 1.0;
+    if (mhd) {
+        for (int i = 0; i < nblk; i++) {
+            double c = or_update_chspeed(p, cgs[i], dt);
+            if (c < ch) ch = c;
+        }
+        double f = 0.0;
+        for (int i = 0; i < nblk; i++) {
+            double r = (p->glm_alpha * ch) / (min_dl(p, cgs[i]) / dt);
+            if (r > f) f = r;
+        }
+        fac = exp(-f);
+    } else {
+        ch = 0.0;
+    }
+    int s0 = first_stage(p), s1 = (p->integration_order == 1) ? EULER : RK2_2;
+    for (int half = 0; half < 2; half++) {
+        for (int s = 0; s < 3; s++) {
+            int d = half == 0 ? s : 2 - s;
+            if (!p->has_dir[d]) continue;
+            for (int istep = s0; istep <= s1; istep++) {
+                for (int i = 0; i < nblk; i++) or_solve_cg(p, cgs[i], d, istep, dt, ch, &reps[i]);
+                fill_list(p, cgs, nbr, nblk);
+            }
+        }
+        if (mhd)
+            for (int i = 0; i < nblk; i++) or_glmdamping(p, cgs[i], fac);
+    }
+}
+
+/* or_run_list over blocks connected by nbr (NULL: each periodic on itself). */
+double or_run_list_nb(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk, int nsteps,
+                      or_report *reps)
+{
     double t = 0.0, dt = -1.0;
-    for (int i = 0; i < nblk; i++) or_fill_ghosts_periodic(p, cgs[i]);
+    fill_list(p, cgs, nbr, nblk);
     for (int s = 0; s < nsteps; s++) {
         double dtn = DBL_MAX;
         for (int i = 0; i < nblk; i++) {
@@ -363,7 +463,10 @@ double or_run_list(const or_params *p, or_block *const *cgs, int nblk, int nstep
             if (d < dtn) dtn = d;
         }
         dt = dtn;
-        or_fluid_update_list(p, cgs, nblk, dt, reps);
+        if (nbr)
+            or_fluid_update_list_nb(p, cgs, nbr, nblk, dt, reps);
+        else
+            or_fluid_update_list(p, cgs, nblk, dt, reps);
         t = t + dt;
         t = t + dt;
     }

@@ -95,6 +95,11 @@ void or_fluid_update(const or_params *p, or_block *cg, double dt, or_report *rep
 double or_run(const or_params *p, or_block *cg, int nsteps, or_report *rep);
 void or_fluid_update_list(const or_params *p, or_block *const *cgs, int nblk, double dt, or_report *reps);
 double or_run_list(const or_params *p, or_block *const *cgs, int nblk, int nsteps, or_report *reps);
+void or_fill_ghosts_list(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk);
+void or_fluid_update_list_nb(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk, double dt,
+                             or_report *reps);
+double or_run_list_nb(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk, int nsteps,
+                      or_report *reps);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,11 @@ def lib():
         L.or_fluid_update_list.argtypes = [P(Params), P(P(Block)), ctypes.c_int, ctypes.c_double, P(Report)]
         L.or_run_list.argtypes = [P(Params), P(P(Block)), ctypes.c_int, ctypes.c_int, P(Report)]
         L.or_run_list.restype = ctypes.c_double
+        ip = P(ctypes.c_int32)
+        L.or_fill_ghosts_list.argtypes = [P(Params), P(P(Block)), ip, ctypes.c_int]
+        L.or_fluid_update_list_nb.argtypes = [P(Params), P(P(Block)), ip, ctypes.c_int, ctypes.c_double, P(Report)]
+        L.or_run_list_nb.argtypes = [P(Params), P(P(Block)), ip, ctypes.c_int, ctypes.c_int, P(Report)]
+        L.or_run_list_nb.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -228,12 +233,41 @@ def fluid_update_list(p, obs, dt):
     _store_reports(obs, reps)
 
 
-def run_list(p, obs, nsteps):
-    """nsteps main-loop iterations over a list of blocks with one dt."""
+def run_list(p, obs, nsteps, nbr=None):
+    """nsteps main-loop iterations over a list of blocks with one dt; nbr
+    (see fill_ghosts_list) connects them, None: each periodic on itself."""
     blks, arr, reps = _block_array(obs)
-    t = lib().or_run_list(ctypes.byref(p), arr, len(obs), nsteps, reps)
+    if nbr is None:
+        t = lib().or_run_list(ctypes.byref(p), arr, len(obs), nsteps, reps)
+    else:
+        nb = _nbr_array(nbr, len(obs))
+        t = lib().or_run_list_nb(ctypes.byref(p), arr, nb.ctypes.data_as(P(ctypes.c_int32)), len(obs), nsteps, reps)
     _store_reports(obs, reps)
     return t
+
+
+def _nbr_array(nbr, nblk):
+    nb = np.ascontiguousarray(nbr, dtype=np.int32).reshape(nblk, 3, 2)
+    if nb.min() < 0 or nb.max() >= nblk:
+        raise ValueError("neighbour index outside the list")
+    return nb
+
+
+def fill_ghosts_list(p, obs, nbr):
+    """Ghost exchange over blocks: nbr[i][d][s] is the index of the block whose
+    interior adjoins block i's ghost layers on side s (0 low, 1 high) of
+    direction d (a block periodic on itself names itself)."""
+    blks, arr, reps = _block_array(obs)
+    nb = _nbr_array(nbr, len(obs))
+    lib().or_fill_ghosts_list(ctypes.byref(p), arr, nb.ctypes.data_as(P(ctypes.c_int32)), len(obs))
+
+
+def fluid_update_list_nb(p, obs, nbr, dt):
+    """fluid_update_full over connected blocks."""
+    blks, arr, reps = _block_array(obs)
+    nb = _nbr_array(nbr, len(obs))
+    lib().or_fluid_update_list_nb(ctypes.byref(p), arr, nb.ctypes.data_as(P(ctypes.c_int32)), len(obs), dt, reps)
+    _store_reports(obs, reps)
 
 
 # ---- pencil-level helpers (column-major (n, nv) arrays) -------------------

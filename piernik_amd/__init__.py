@@ -89,6 +89,7 @@ _PROTOS = {
     "pk_solve_cg": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
     "pk_sweep": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
     "pk_fill_ghosts": [ctypes.c_int64],
+    "pk_set_neighbour": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
     "pk_glm_damp": [ctypes.c_int64, ctypes.c_double],
     "pk_timestep_fluid": [ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_fluid_update": [ctypes.c_int64, ctypes.c_double],
@@ -371,8 +372,28 @@ def sweep(cg, ddim, dt, chspeed=0.0, psi_fac=1.0):
 
 
 def update_boundaries(cg):
-    """Periodic refresh of every ghost cell of the block on the device."""
+    """Refresh of every ghost cell of the block on the device: periodic on
+    itself, or from its neighbours (set_neighbour)."""
     _call("pk_fill_ghosts", cg.cg_id)
+
+
+def set_neighbour(cg, direction, side, nb):
+    """The ghost layers of cg on `side` (0 low, 1 high) of `direction` take the
+    adjoining interior layers of block nb (pk_set_neighbour)."""
+    _call("pk_set_neighbour", cg.cg_id, int(direction), int(side), nb.cg_id)
+
+
+def connect(blocks, nbr):
+    """Register the blocks and link them by a neighbour table nbr[i][d][s]
+    (indices into blocks, as setups.decompose returns)."""
+    for b in blocks:
+        b.register()
+    for i, b in enumerate(blocks):
+        for d in range(3):
+            if not b.has_dir[d]:
+                continue
+            for s in range(2):
+                set_neighbour(b, d, s, blocks[nbr[i][d][s]])
 
 
 def glmdamping(cg, fac):

@@ -54,6 +54,10 @@ struct Block {
     long long dtb_parts = 0;
     double *d_dtb = nullptr;
     unsigned long long *d_best = nullptr;
+    // the block whose interior adjoins each ghost face (pk_set_neighbour):
+    // nbr[d][0] low side, nbr[d][1] high side; the block's own id when it is
+    // periodic on itself
+    int64_t nbr[3][2] = {{0, 0}, {0, 0}, {0, 0}};
 };
 
 const int kDtSlots = 1024;
@@ -813,6 +817,14 @@ struct LaunchTab {
 
 const int kQueues = 16;
 
+// The face jobs of one exchange direction over the list.
+struct FaceTab {
+    std::vector<pk::FaceJob> h;
+    pk::FaceJob *d_jobs = nullptr;
+    size_t cap = 0;
+    long long maxvals = 0;
+};
+
 struct ListState {
     bool built = false;
     std::vector<Block *> blocks;
@@ -828,6 +840,12 @@ struct ListState {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool fresh = false;  // ghosts current and d_dtout holds the timestep of the current state
+    // blocks connected to other blocks (pk_set_neighbour): ghost faces come from
+    // the neighbours through an exchange before every sweep instead of the
+    // sweep's periodic images
+    bool linked = false;
+    bool ghosts_stale = false;  // linked list after a step: the ghosts other than the swept faces are old
+    FaceTab ex[3][2];           // per direction and buffer parity
 };
 ListState g_list;
 
@@ -932,6 +950,9 @@ void list_free()
     if (g_list.h_step) (void)hipHostFree(g_list.h_step);
     if (g_list.d_dtslots) (void)hipFree(g_list.d_dtslots);
     if (g_list.d_dtout) (void)hipFree(g_list.d_dtout);
+    for (auto &row : g_list.ex)
+        for (auto &t : row)
+            if (t.d_jobs) (void)hipFree(t.d_jobs);
     g_list = ListState();
 }
 
@@ -982,7 +1003,7 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
         a.src = fluid_state(b, b->cur, f);
         a.base = a.src;
         a.dst = fluid_state(b, 1 - b->cur, f);
-        a.images = 1;
+        a.images = g_list.linked ? 0 : 1;
         const int ts = ts_override ? (*ts_override)[i] : list_ts(b, d, f, nt);
         tile_geometry(b, d, true, nt / ts, ts, a.TP, a.TS, a.out, a.chunks, a.ptiles);
         if (a.TP * a.TS > nt) return fail(PK_EINVAL, "tile %d x %d exceeds %d lanes", a.TP, a.TS, nt);
@@ -1019,6 +1040,109 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
     return PK_OK;
 }
 
+bool block_linked(const Block *b)
+{
+    for (int d = 0; d < 3; d++)
+        if (b->g[d] > 0 && (b->nbr[d][0] != b->id || b->nbr[d][1] != b->id)) return true;
+    return false;
+}
+
+// o may fill b's ghost faces along d: same faces, ghost width and cell size
+int check_face(const Block *b, const Block *o, int d)
+{
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    if (o->ng[d1] != b->ng[d1] || o->ng[d2] != b->ng[d2])
+        return fail(PK_EINVAL, "blocks %lld and %lld do not share a face along %c", (long long)b->id,
+                    (long long)o->id, "xyz"[d]);
+    if (o->nb != b->nb) return fail(PK_EINVAL, "blocks %lld and %lld differ in nb", (long long)b->id, (long long)o->id);
+    for (int q = 0; q < 3; q++)
+        if (o->dl[q] != b->dl[q])
+            return fail(PK_EINVAL, "blocks %lld and %lld differ in cell size (no fine/coarse boundaries)",
+                        (long long)b->id, (long long)o->id);
+    return PK_OK;
+}
+
+// Face jobs of the exchange along each direction and buffer parity: block b's
+// ghost layers on side s come from the g adjoining interior layers of its
+// neighbour, over the whole ghosted extent of the other two directions.
+int build_faces()
+{
+    for (int d = 0; d < 3; d++) {
+        if (!g_cfg.has_dir[d]) continue;
+        for (int p = 0; p < 2; p++) {
+            FaceTab &t = g_list.ex[d][p];
+            t.h.clear();
+            t.maxvals = 0;
+            for (Block *b : g_list.blocks) {
+                for (int side = 0; side < 2; side++) {
+                    auto it = g_blocks.find(b->nbr[d][side]);
+                    if (it == g_blocks.end())
+                        return fail(PK_ESTATE, "neighbour %lld of block %lld is not registered",
+                                    (long long)b->nbr[d][side], (long long)b->id);
+                    const Block *o = it->second;
+                    RCCHK(check_face(b, o, d));
+                    pk::FaceJob j{};
+                    j.src = state(const_cast<Block *>(o), o->cur ^ p);
+                    j.dst = state(b, b->cur ^ p);
+                    j.ssv = o->sv;
+                    j.dsv = b->sv;
+                    j.ssy = o->sy;
+                    j.ssz = o->sz;
+                    j.dsy = b->sy;
+                    j.dsz = b->sz;
+                    for (int q = 0; q < 3; q++) {
+                        j.ext[q] = b->ng[q];
+                        j.sorg[q] = 0;
+                        j.dorg[q] = 0;
+                    }
+                    j.ext[d] = b->g[d];
+                    j.dorg[d] = side == 0 ? 0 : b->g[d] + b->n[d];
+                    j.sorg[d] = side == 0 ? o->n[d] : o->g[d];
+                    j.nplanes = g_nplanes;
+                    const long long vals = (long long)j.ext[0] * j.ext[1] * j.ext[2] * j.nplanes;
+                    if (vals > t.maxvals) t.maxvals = vals;
+                    t.h.push_back(j);
+                }
+            }
+            if (t.h.size() > 65535) return fail(PK_EINVAL, "too many faces for one exchange launch");
+            if (t.cap < t.h.size()) {
+                if (t.d_jobs) (void)hipFree(t.d_jobs);
+                t.d_jobs = nullptr;
+                HIPCHK(hipMalloc((void **)&t.d_jobs, sizeof(pk::FaceJob) * t.h.size()));
+                t.cap = t.h.size();
+            }
+            HIPCHK(hipMemcpyAsync(t.d_jobs, t.h.data(), sizeof(pk::FaceJob) * t.h.size(), hipMemcpyHostToDevice,
+                                  g_stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return PK_OK;
+}
+
+int exchange(int d, int parity)
+{
+    const FaceTab &t = g_list.ex[d][parity];
+    HIPCHK(pk::launch_exchange_faces(t.d_jobs, (int)t.h.size(), t.maxvals, g_stream));
+    return PK_OK;
+}
+
+// every ghost cell of every block of a linked list, x then y then z
+int exchange_all()
+{
+    for (int d = 0; d < 3; d++)
+        if (g_cfg.has_dir[d]) RCCHK(exchange(d, 0));
+    g_list.ghosts_stale = false;
+    return PK_OK;
+}
+
+// ghost refresh of the whole list: periodic images, or the exchange
+int list_fill()
+{
+    if (g_list.linked) return exchange_all();
+    for (Block *b : g_list.blocks) RCCHK(do_fill(b));
+    return PK_OK;
+}
+
 int list_build()
 {
     if (g_list.built) {
@@ -1045,6 +1169,9 @@ int list_build()
         g_list.cur0.push_back(b->cur);
         g_list.mdl.push_back(min_dl(b));
     }
+    g_list.linked = false;
+    for (Block *b : g_list.blocks) g_list.linked = g_list.linked || block_linked(b);
+    if (g_list.linked) RCCHK(build_faces());
     for (int d = 0; d < 3; d++) {
         if (!g_cfg.has_dir[d]) continue;
         for (int f = 0; f < g_nfl; f++) {
@@ -1099,6 +1226,7 @@ int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
     for (size_t k = 0; k < sw.size(); k++) {
         const int d = sw[k].d;
         const int parity = (int)(k % 2);
+        if (g_list.linked && !ens) RCCHK(exchange(d, parity));
         for (int f = 0; f < g_nfl; f++) {
             LaunchTab &t = g_list.tab[d][f];
             double bytes = 0.0;
@@ -1172,8 +1300,8 @@ int ensure_graph()
 int list_prepare(double *dt_out, double *c_out)
 {
     double dt = DBL_MAX, c = 0.0;
+    RCCHK(list_fill());
     for (Block *b : g_list.blocks) {
-        RCCHK(do_fill(b));
         double dtb, cb;
         RCCHK(do_timestep(b, &dtb, &cb));
         if (dtb < dt) dt = dtb;
@@ -1217,6 +1345,7 @@ int list_step(int mode, double dt)
         b->dt_pending = false;
     }
     g_list.fresh = true;
+    g_list.ghosts_stale = g_list.linked;
     g_ens.fresh = false;
     return PK_OK;
 }
@@ -1233,6 +1362,7 @@ int check_mode(int mode)
 int ens_build()
 {
     RCCHK(list_build());
+    if (g_list.linked) return fail(PK_EINVAL, "ensemble instances must be periodic on themselves (no neighbours)");
     const size_t n = g_list.blocks.size();
     if (g_ens.nblk == n && g_ens.d_step) return PK_OK;
     ens_free();
@@ -1363,6 +1493,25 @@ int ens_adj_build()
     return PK_OK;
 }
 
+// device-resident stepping of one block: only for blocks periodic on themselves
+int solo(const Block *b)
+{
+    if (block_linked(b))
+        return fail(PK_EINVAL, "block %lld has neighbours: advance it with the block-list functions",
+                    (long long)b->id);
+    return PK_OK;
+}
+
+// Before a download: the ghosts a connected list's steps left old (the
+// steps exchange only the faces each sweep reads) are refreshed from the
+// neighbours, so a download returns what update_boundaries would leave.
+int ghosts_for_download()
+{
+    if (!g_list.ghosts_stale) return PK_OK;
+    RCCHK(list_build());
+    return g_list.linked ? exchange_all() : PK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1445,6 +1594,7 @@ int pk_register_block(int64_t cg_id, const int32_t n[3], int32_t nb, const doubl
     Block *blk = new Block();
     blk->id = cg_id;
     blk->nb = nb;
+    for (int d = 0; d < 3; d++) blk->nbr[d][0] = blk->nbr[d][1] = cg_id;
     for (int d = 0; d < 3; d++) {
         if (n[d] < 1) {
             delete blk;
@@ -1509,6 +1659,7 @@ int pk_download(int64_t cg_id)
     Block *b;
     RCCHK(get_block(cg_id, &b));
     if (!b->h_u) return fail(PK_EINVAL, "block %lld has no host arrays", (long long)cg_id);
+    RCCHK(ghosts_for_download());
     RCCHK(download_state(b, b->h_u, b->h_b, b->h_psi, state(b, b->cur)));
     if (b->uh >= 0 && b->h_uh) return download_state(b, b->h_uh, b->h_bh, b->h_psih, state(b, b->uh));
     return PK_OK;
@@ -1527,6 +1678,7 @@ int pk_download_to(int64_t cg_id, double *u, double *bb, double *psi)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
+    RCCHK(ghosts_for_download());
     return download_state(b, u, bb, psi, state(b, b->cur));
 }
 
@@ -1543,6 +1695,7 @@ int pk_download_device(int64_t cg_id, double *u, double *bb, double *psi)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
+    RCCHK(ghosts_for_download());
     return download_state(b, u, bb, psi, state(b, b->cur), true);
 }
 
@@ -1562,6 +1715,7 @@ int pk_sweep(int64_t cg_id, int32_t ddim, double dt, double chspeed, double psi_
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
+    RCCHK(solo(b));
     if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
     list_touch();
@@ -1572,7 +1726,27 @@ int pk_fill_ghosts(int64_t cg_id)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
-    return do_fill(b);
+    if (!block_linked(b)) return do_fill(b);
+    // a connected block takes its ghosts from the neighbours' current state
+    RCCHK(list_build());
+    return exchange_all();
+}
+
+int pk_set_neighbour(int64_t cg_id, int32_t dir, int32_t side, int64_t nb_id)
+{
+    Block *b, *o;
+    RCCHK(get_block(cg_id, &b));
+    if (dir < 0 || dir > 2 || !g_cfg.has_dir[dir]) return fail(PK_EINVAL, "inactive or bad direction %d", dir);
+    if (side != 0 && side != 1) return fail(PK_EINVAL, "side must be 0 (low) or 1 (high), got %d", side);
+    RCCHK(get_block(nb_id, &o));
+    RCCHK(check_face(b, o, dir));
+    if (nb_id != cg_id && b->lam[0])
+        return fail(PK_EINVAL, "block %lld is set up for the adjoint, which needs it periodic on itself",
+                    (long long)cg_id);
+    (void)hipStreamSynchronize(g_stream);
+    b->nbr[dir][side] = nb_id;
+    list_invalidate();
+    return PK_OK;
 }
 
 int pk_glm_damp(int64_t cg_id, double fac)
@@ -1598,6 +1772,7 @@ int pk_fluid_update(int64_t cg_id, double dt)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
+    RCCHK(solo(b));
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
     list_touch();
     return do_fluid_update(b, dt);
@@ -1607,6 +1782,7 @@ int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
+    RCCHK(solo(b));
     if (nsteps < 0) return fail(PK_EINVAL, "nsteps < 0");
     list_touch();
     double t = 0.0, dt = -1.0, dt_old = -1.0;
@@ -1744,9 +1920,7 @@ int pk_fluid_update_all(int32_t mode, double dt)
     RCCHK(check_mode(mode));
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
     RCCHK(list_build());
-    if (!g_list.fresh) {
-        for (Block *b : g_list.blocks) RCCHK(do_fill(b));
-    }
+    if (!g_list.fresh) RCCHK(list_fill());
     return list_step(mode, dt);
 }
 
@@ -1912,6 +2086,7 @@ int pk_adjoint_setup(int64_t cg_id, int32_t nslots)
     RCCHK(get_block(cg_id, &b));
     if (g_nfl != 1) return fail(PK_EINVAL, "the adjoint covers one fluid per block");
     if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "the adjoint covers the RK2 sweep");
+    if (block_linked(b)) return fail(PK_EINVAL, "the adjoint covers blocks periodic on themselves");
     if (nslots < 1) return fail(PK_EINVAL, "nslots = %d", nslots);
     for (int d = 0; d < 3; d++)
         if (g_cfg.has_dir[d] && b->g[d] < 4) return fail(PK_EINVAL, "the fused RK2 sweep needs nb >= 4");

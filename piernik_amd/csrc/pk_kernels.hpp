@@ -98,6 +98,19 @@ struct CopyJob {
     long long n;  // doubles
 };
 
+// One face of a ghost exchange (k_exchange_faces): the box ext[0] x ext[1]
+// x ext[2] of every plane, from the neighbour's state at sorg to this
+// block's at dorg (ghosted coordinates, x fastest; src and dst point at
+// cell (0, 0, 0) of plane 0).
+struct FaceJob {
+    const double *src;
+    double *dst;
+    long long ssv, dsv;  // plane strides
+    int ssy, ssz, dsy, dsz;
+    int sorg[3], dorg[3], ext[3];
+    int nplanes;
+};
+
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st);
 // Persistent batched fused sweep with workgroups of nt lanes; grid = 0 picks
 // the number of workgroups the device holds at once.
@@ -109,6 +122,8 @@ hipError_t launch_fill_items(const unsigned *first, int nblocks, unsigned *items
 hipError_t launch_dtslots_final_blocks(const unsigned long long *slots, int nslots, int nblocks, double *out,
                                        hipStream_t st);
 hipError_t launch_copy_blocks(const CopyJob *jobs, int njobs, hipStream_t st);
+// the faces of one exchange; maxvals = the largest ext product times nplanes
+hipError_t launch_exchange_faces(const FaceJob *jobs, int njobs, long long maxvals, hipStream_t st);
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st);
 size_t sweep_lds_bytes(bool mhd, int TP, int TS);
 hipError_t launch_fill_ghosts(int dir, const GhostArgs &a, hipStream_t st);

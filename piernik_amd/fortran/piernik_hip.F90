@@ -18,7 +18,7 @@ module piernik_hip
    private
    public :: pk_fluid, pk_config, pk_stage_report, pk_profile_stats
    public :: pk_init, pk_finalize, pk_register_block, pk_unregister_block, pk_upload, pk_download, &
-        &    pk_upload_from, pk_download_to, pk_solve_cg, pk_sweep, pk_fill_ghosts, pk_glm_damp, &
+        &    pk_upload_from, pk_download_to, pk_solve_cg, pk_sweep, pk_fill_ghosts, pk_set_neighbour, pk_glm_damp, &
         &    pk_timestep_fluid, pk_fluid_update, pk_run, pk_get_report, pk_synchronize, &
         &    pk_autotune, pk_profile_begin, pk_profile_end, pk_error_message, &
         &    pk_upload_device, pk_download_device, pk_timestep_all, pk_fluid_update_all, pk_run_all, &
@@ -145,6 +145,12 @@ module piernik_hip
          import :: c_int32_t, c_int64_t
          integer(c_int64_t), value :: cg_id
       end function pk_fill_ghosts
+
+      integer(c_int32_t) function pk_set_neighbour(cg_id, dir, side, nb_id) bind(C, name="pk_set_neighbour")
+         import :: c_int32_t, c_int64_t
+         integer(c_int64_t), value :: cg_id, nb_id
+         integer(c_int32_t), value :: dir, side
+      end function pk_set_neighbour
 
       integer(c_int32_t) function pk_glm_damp(cg_id, fac) bind(C, name="pk_glm_damp")
          import :: c_int32_t, c_int64_t, c_double

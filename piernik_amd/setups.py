@@ -313,6 +313,72 @@ class _DeviceBlock(GridContainer):
 # ---- differentiable hydro rollout ------------------------------------------------
 
 
+def decompose(cg, cuts, first_id=None):
+    """Split the periodic domain of `cg` into blocks at the interior cut
+    positions cuts[d] (sorted, 0 < c < n[d]; ignored in inactive
+    directions; every block at least nb cells wide, so a ghost layer never
+    reaches past the adjoining block), blocks numbered x fastest.  Each block has cg's cell size
+    and nb, and its state (ghost cells included) is cg's interior with
+    periodic wrap.  Returns (blocks, nbr, boxes): GridContainers, nbr[i][d][s]
+    the index of the block whose interior adjoins block i's ghost layers on
+    side s (0 low, 1 high) of direction d (periodic across the domain), and
+    boxes[i] = (offset, size) in cells."""
+    edges = []
+    for d in range(3):
+        c = sorted(int(x) for x in cuts[d]) if cg.has_dir[d] else []
+        if any(x <= 0 or x >= cg.n[d] for x in c):
+            raise ValueError(f"cut outside 1..{cg.n[d] - 1} along {'xyz'[d]}")
+        edges.append([0] + c + [cg.n[d]])
+        if cg.has_dir[d] and min(np.diff(edges[-1])) < cg.nb:
+            raise ValueError(f"blocks along {'xyz'[d]} must be at least nb = {cg.nb} cells wide")
+    cnt = [len(e) - 1 for e in edges]
+    arrays = [("u", cg.u), ("b", cg.b), ("psi", cg.psi)]
+    blocks, boxes, nbr = [], [], []
+    for k in range(cnt[2]):
+        for j in range(cnt[1]):
+            for i in range(cnt[0]):
+                ijk = (i, j, k)
+                off = tuple(edges[d][ijk[d]] for d in range(3))
+                size = tuple(edges[d][ijk[d] + 1] - off[d] for d in range(3))
+                kw = {} if first_id is None else {"cg_id": first_id + len(blocks)}
+                blk = GridContainer(size, nb=cg.nb, dl=cg.dl, magnetized=cg.magnetized, has_dir=cg.has_dir,
+                                    nfluids=cg.nfluids, **kw)
+                idx = [(np.arange(-blk.g[d], size[d] + blk.g[d]) + off[d]) % cg.n[d] + cg.g[d] for d in range(3)]
+                for name, a in arrays:
+                    if a is None:
+                        continue
+                    dst = getattr(blk, name)
+                    if a.ndim == 4:
+                        dst[...] = a[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]
+                    else:
+                        dst[...] = a[idx[0]][:, idx[1]][:, :, idx[2]]
+                row = []
+                for d in range(3):
+                    lo, hi = list(ijk), list(ijk)
+                    lo[d] = (ijk[d] - 1) % cnt[d]
+                    hi[d] = (ijk[d] + 1) % cnt[d]
+                    row.append([lo[0] + cnt[0] * (lo[1] + cnt[1] * lo[2]), hi[0] + cnt[0] * (hi[1] + cnt[1] * hi[2])])
+                nbr.append(row)
+                blocks.append(blk)
+                boxes.append((off, size))
+    return blocks, nbr, boxes
+
+
+def assemble(blocks, boxes, name, n):
+    """The interior of array `name` ("u", "b" or "psi") over the whole domain
+    of n cells from the blocks of decompose()."""
+    first = getattr(blocks[0], name)
+    out = np.zeros(first.shape[:-3] + tuple(n), dtype=first.dtype) if first.ndim == 4 else np.zeros(tuple(n))
+    for blk, (off, size) in zip(blocks, boxes):
+        sl = tuple(slice(off[d], off[d] + size[d]) for d in range(3))
+        a = blk.interior(getattr(blk, name))
+        if a.ndim == 4:
+            out[(slice(None),) + sl] = a
+        else:
+            out[sl] = a
+    return out
+
+
 def unit_gaussian(n, seed, kmax=6.0):
     """Seeded periodic Gaussian random field on n^3 with 0 < |k| <= kmax,
     normalised to zero mean and unit variance."""
