@@ -74,7 +74,8 @@ typedef struct pk_config {
     double cfl_glm;            /* GLM speed factor, ch = cfl_glm*min(dl)/dt (hdc.F90:122) */
     double glm_alpha;          /* GLM damping (hdc.F90:325) */
     int32_t has_dir[3];        /* dom%has_dir: active directions */
-    int32_t tile_x[2];         /* sweep tile shape (pencils, cells) along x; 0 = autotuned default */
+    int32_t tile_x[2];         /* sweep tile shape (pencils, cells) along x; 0 = autotuned default;
+                                  cells = -1: one lane per whole pencil (march kernel) */
     int32_t tile_yz[2];        /* sweep tile shape (pencils, cells) along y and z; 0 = default */
     int32_t fast_math;         /* 0: every operation rounds as the reference's FP64 code (bit-exact);
                                   1: FMA contraction and fast division/sqrt in the sweep (~1e-15 per op) */
@@ -226,7 +227,11 @@ int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar);
  * read, never advanced) and keep the fastest per direction for every block
  * of the same shape and physics.  dt and chspeed should be representative.
  * tiles_out (may be NULL) receives (pencils, cells) for x, y, z; 0 for an
- * inactive direction.  A shape given in pk_config overrides the tuned one. */
+ * inactive direction; cells = -1 is the march kernel (one lane per whole
+ * pencil, 64 pencils per workgroup), a candidate for y and z sweeps of
+ * blocks with at least 65536 pencils.  A shape given in pk_config
+ * (tile_x, tile_yz; (64, -1) selects the march kernel) overrides the tuned
+ * one. */
 int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6]);
 
 /* Tile shapes of the persistent list sweep: per direction and fluid kind the

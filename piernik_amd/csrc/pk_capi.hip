@@ -215,18 +215,33 @@ void tile_geometry(const Block *b, int d, bool fused, int tp, int ts, int &TP, i
     ptiles = (npen + tp - 1) / tp;
 }
 
+// Tile length that selects the march kernel: one lane per whole pencil.
+const int kMarch = -1;
+
 // Precedence: the shape given in pk_config, then the one pk_autotune
 // measured for this block shape, then 3 x 72 (4-wave workgroups, two per CU).
+// A tile length of kMarch selects the march kernel for the fused sweep.
 void choose_tile(const Block *b, int d, bool fused, bool mag, int &TP, int &TS, int &out, int &chunks, int &ptiles)
 {
     const int *ov = (d == 0) ? g_cfg.tile_x : g_cfg.tile_yz;
     int tp = ov[0], ts = ov[1];
-    if (ts <= 0 && fused) {
+    if (ts == kMarch && !fused) ts = 0;
+    if (ts <= 0 && ts != kMarch && fused) {
         auto it = g_tuned.find(tune_key(b, d, mag));
         if (it != g_tuned.end()) {
             tp = it->second.first;
             ts = it->second.second;
         }
+    }
+    if (ts == kMarch && fused) {
+        // whole pencils, one lane each, 64-lane workgroups (k_march)
+        const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+        TP = 64;
+        TS = kMarch;
+        out = b->n[d];
+        chunks = 1;
+        ptiles = (b->n[d1] * b->n[d2] + 63) / 64;
+        return;
     }
     if (ts <= 0) {
         ts = 72;
@@ -1858,6 +1873,10 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
             std::vector<std::pair<int, int>> seen;
             float best = 1e30f;
             std::pair<int, int> best_t(0, 0);
+            // the march kernel is a candidate for y and z sweeps with enough
+            // pencils to give every SIMD a wave (x pencils are rows: uncoalesced)
+            const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+            const bool try_march = d > 0 && (long long)b->n[d1] * b->n[d2] >= 65536;
             for (int ts : ts_list) {
                 for (int w : waves_list) {
                     int TP, TS, out, chunks, ptiles;
@@ -1885,6 +1904,24 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
                         best = ms;
                         best_t = {TP, TS};
                     }
+                }
+            }
+            if (try_march) {
+                a.TP = 64;
+                a.TS = kMarch;
+                a.out = b->n[d];
+                a.chunks = 1;
+                a.ptiles = (b->n[d1] * b->n[d2] + 63) / 64;
+                HIPCHK(sweep_kernel(d, g_fmag[f], true, a));  // warm-up
+                HIPCHK(hipEventRecord(e0, g_stream));
+                for (int rep = 0; rep < 3; rep++) HIPCHK(sweep_kernel(d, g_fmag[f], true, a));
+                HIPCHK(hipEventRecord(e1, g_stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_t = {64, kMarch};
                 }
             }
             g_tuned[tune_key(b, d, g_fmag[f])] = best_t;

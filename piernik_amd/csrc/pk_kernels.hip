@@ -92,46 +92,46 @@ __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)
     }
 }
 
-// timestep_fluid (timestep.F90:460-490, ion_cs / neu_cs) of the cell a lane
-// has just written, reduced over the wave and folded into one slot with
-// 64-bit atomics (the values are positive, so their bit patterns order as
-// the numbers do; min and max do not depend on the order of reduction).
+// timestep_fluid (timestep.F90:460-490, ion_cs / neu_cs) of one written
+// cell, folded into a lane's running min of dl/c per direction and max of c.
 template <int DIR, class P>
-__device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double (&Wn)[P::NV], bool valid,
-                                                unsigned slot_id)
+__device__ __forceinline__ void dt_cell(const SweepArgs &a, const double (&Wn)[P::NV], double (&mn)[3], double &cmax)
 {
     const double small = 1.17549435082228750797e-38;
-    const double huge = 1.7976931348623157e308;
-    double mn[3] = {huge, huge, huge}, cmax = 0.0;
-    if (valid) {
-        const double d = Wn[0];
-        const double mx = Wn[plane_of<DIR>(1)], my = Wn[plane_of<DIR>(2)], mz = Wn[plane_of<DIR>(3)];
-        const double e = Wn[4];
-        const double m[3] = {mx, my, mz};
-        double v[3] = {0.0, 0.0, 0.0};
-        if (d > 0.0) {
+    const double d = Wn[0];
+    const double mx = Wn[plane_of<DIR>(1)], my = Wn[plane_of<DIR>(2)], mz = Wn[plane_of<DIR>(3)];
+    const double e = Wn[4];
+    const double m[3] = {mx, my, mz};
+    double v[3] = {0.0, 0.0, 0.0};
+    if (d > 0.0) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) v[q] = fabs(m[q] / d);
-        }
-        const double ek = (0.5 * ((mx * mx + my * my) + mz * mz)) / d;
-        double cs;
-        if constexpr (P::MAG) {
-            const double bx = Wn[plane_of<DIR>(5)], by = Wn[plane_of<DIR>(6)], bz = Wn[plane_of<DIR>(7)];
-            const double pmag = 0.5 * ((bx * bx + by * by) + bz * bz);
-            const double ps = (e - ek) * a.gam_1 + (2.0 - a.gam) * pmag;
-            const double pr = ps - pmag;
-            cs = sqrt(fabs((2.0 * pmag + a.gam * pr) / d));
-        } else {
-            const double pr = (e - ek) * a.gam_1;
-            cs = sqrt(fabs((a.gam * pr) / d));
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const double c = fmx(v[q] + cs, small);
-            cmax = fmx(cmax, c);
-            if (a.has_dir[q]) mn[q] = a.dl[q] / c;
-        }
+        for (int q = 0; q < 3; q++) v[q] = fabs(m[q] / d);
     }
+    const double ek = (0.5 * ((mx * mx + my * my) + mz * mz)) / d;
+    double cs;
+    if constexpr (P::MAG) {
+        const double bx = Wn[plane_of<DIR>(5)], by = Wn[plane_of<DIR>(6)], bz = Wn[plane_of<DIR>(7)];
+        const double pmag = 0.5 * ((bx * bx + by * by) + bz * bz);
+        const double ps = (e - ek) * a.gam_1 + (2.0 - a.gam) * pmag;
+        const double pr = ps - pmag;
+        cs = sqrt(fabs((2.0 * pmag + a.gam * pr) / d));
+    } else {
+        const double pr = (e - ek) * a.gam_1;
+        cs = sqrt(fabs((a.gam * pr) / d));
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double c = fmx(v[q] + cs, small);
+        cmax = fmx(cmax, c);
+        if (a.has_dir[q]) mn[q] = fmn(mn[q], a.dl[q] / c);
+    }
+}
+
+// The lanes' running values reduced over the wave and folded into one slot
+// with 64-bit atomics (the values are positive, so their bit patterns order
+// as the numbers do; min and max do not depend on the order of reduction).
+__device__ __forceinline__ void dt_reduce(const SweepArgs &a, double (&mn)[3], double &cmax, unsigned slot_id)
+{
     // xor butterfly over the lanes that exist (the last wave of a workgroup
     // may be partial); partners beyond it are skipped, lane 0 ends with all.
     const int lane = threadIdx.x & 63;
@@ -156,6 +156,45 @@ __device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double
         for (int q = 0; q < 3; q++) atomicMin(s + q, (unsigned long long)__double_as_longlong(mn[q]));
         atomicMax(s + 3, (unsigned long long)__double_as_longlong(cmax));
     }
+}
+
+// timestep_fluid of the cell a lane has just written (valid lanes), reduced
+// over the wave into one slot.
+template <int DIR, class P>
+__device__ __forceinline__ void dt_contribution(const SweepArgs &a, const double (&Wn)[P::NV], bool valid,
+                                                unsigned slot_id)
+{
+    const double huge = 1.7976931348623157e308;
+    double mn[3] = {huge, huge, huge}, cmax = 0.0;
+    if (valid) dt_cell<DIR, P>(a, Wn, mn, cmax);
+    dt_reduce(a, mn, cmax, slot_id);
+}
+
+// Periodic images of a written cell: each ghost cell has exactly one interior
+// pre-image, so writing every output cell's images refreshes all ghosts.
+template <int DIR, class P>
+__device__ __forceinline__ void write_images(const SweepArgs &a, const int (&xyz)[3], long long idx,
+                                             const double (&Wn)[P::NV])
+{
+    int off[3][3], cnt[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int ic = xyz[d] - a.g[d];
+        cnt[d] = 1;
+        off[d][0] = 0;
+        if (a.g[d] > 0) {
+            if (ic < a.g[d]) off[d][cnt[d]++] = a.n[d];
+            if (ic >= a.n[d] - a.g[d]) off[d][cnt[d]++] = -a.n[d];
+        }
+    }
+    for (int i0 = 0; i0 < cnt[0]; i0++)
+        for (int i1 = 0; i1 < cnt[1]; i1++)
+            for (int i2 = 0; i2 < cnt[2]; i2++) {
+                if ((i0 | i1 | i2) == 0) continue;
+                const long long j = idx + off[0][i0] + (long long)a.sy * off[1][i1] + (long long)a.sz * off[2][i2];
+#pragma unroll
+                for (int k = 0; k < P::NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + j] = Wn[k];
+            }
 }
 
 // One tile of a sweep: chunk `bid % chunks` along the sweep direction times
@@ -259,33 +298,171 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int bid, unsigned
     if (outl && svalid) {
 #pragma unroll
         for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + idx] = Wn[k];
-        if (FUSED && a.images) {
-            // periodic images: each ghost cell has exactly one interior pre-image
-            int off[3][3], cnt[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const int ic = xyz[d] - a.g[d];
-                cnt[d] = 1;
-                off[d][0] = 0;
-                if (a.g[d] > 0) {
-                    if (ic < a.g[d]) off[d][cnt[d]++] = a.n[d];
-                    if (ic >= a.n[d] - a.g[d]) off[d][cnt[d]++] = -a.n[d];
-                }
-            }
-            for (int i0 = 0; i0 < cnt[0]; i0++)
-                for (int i1 = 0; i1 < cnt[1]; i1++)
-                    for (int i2 = 0; i2 < cnt[2]; i2++) {
-                        if ((i0 | i1 | i2) == 0) continue;
-                        const long long j =
-                            idx + off[0][i0] + (long long)a.sy * off[1][i1] + (long long)a.sz * off[2][i2];
-#pragma unroll
-                        for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + j] = Wn[k];
-                    }
-        }
+        if (FUSED && a.images) write_images<DIR, P>(a, xyz, idx, Wn);
     }
     // floor bookkeeping: rare, one atomic per lane that saw something
     if (flags) atomicOr(a.flags, flags);
     if (dmass != 0.0) atomicAdd(a.dmass, dmass);
+}
+
+// ---------------------------------------------------------------- march ----
+// The fused RK2 sweep with one lane per pencil.  A lane walks its pencil
+// cell by cell and carries a register window instead of sharing neighbours
+// through LDS: the primitives of the last two cells and the last face state
+// and flux of each stage.  Reading cell t it completes the stage-1 flux at
+// (t-2)+1/2, the stage-1 state of cell t-2, the stage-2 flux at (t-4)+1/2 and
+// the new state of cell t-4.  No cell is computed twice, no barrier is needed,
+// and for y and z sweeps consecutive lanes are consecutive x, so every load
+// and store is coalesced.  Each cell sees the operands and operations of the
+// tile kernel in the same order: the results are bit-identical.
+
+template <class P, int LIM>
+__device__ __forceinline__ void cell_faces(const SweepArgs &a, const double (&qm)[P::NV], const double (&q)[P::NV],
+                                           const double (&qp)[P::NV], double (&fp)[P::NV], double (&fm)[P::NV])
+{
+#pragma unroll
+    for (int v = 0; v < P::NV; v++) {
+        const int lim = (LIM >= 0) ? LIM : ((v < 5) ? a.lim : a.limb);
+        const double dl = q[v] - qm[v];
+        const double dr = qp[v] - q[v];
+        const double h = 0.5 * limit(lim, dl, dr);
+        fp[v] = q[v] + h;
+        fm[v] = q[v] - h;
+    }
+}
+
+template <class P>
+__device__ __forceinline__ void riemann(const SweepArgs &a, const double (&l)[P::NV], const double (&r)[P::NV],
+                                        double (&F)[P::NV])
+{
+    if constexpr (P::MAG)
+        flux_hlld(l, r, a.gam, a.ch, F);
+    else
+        flux_hllc(l, r, a.gam, F);
+}
+
+template <class P>
+__device__ __forceinline__ void cell_update(const double (&W0)[P::NV], double coef, const double (&Fm)[P::NV],
+                                            const double (&Fp)[P::NV], double (&Wn)[P::NV])
+{
+#pragma unroll
+    for (int v = 0; v < P::NV; v++) {
+        const double un = W0[v] + coef * (Fm[v] - Fp[v]);
+        Wn[v] = (v < 5) ? un + 0.0 : un;  // internal_sources adds +0 to the fluid part
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void copy_nv(double (&d)[NV], const double (&s)[NV])
+{
+#pragma unroll
+    for (int v = 0; v < NV; v++) d[v] = s[v];
+}
+
+template <int DIR, class P, int LIM>
+__global__ __launch_bounds__(64) void k_march(const SweepArgs a)
+{
+    constexpr int NV = P::NV;
+    constexpr int ds = DIR;
+    constexpr int d1 = (DIR == 0) ? 1 : 0;
+    constexpr int d2 = (DIR == 2) ? 1 : 2;
+    const int n1 = a.n[d1];
+    const int npen = n1 * a.n[d2];
+    const int pf = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool pvalid = pf < npen;
+    const int pfc = pvalid ? pf : npen - 1;
+    int xyz[3];
+    xyz[d1] = a.g[d1] + pfc % n1;
+    xyz[d2] = a.g[d2] + pfc / n1;
+    xyz[ds] = 0;
+    const long long step = (DIR == 0) ? 1ll : ((DIR == 1) ? (long long)a.sy : (long long)a.sz);
+    const long long base = (long long)xyz[0] + (long long)a.sy * xyz[1] + (long long)a.sz * xyz[2];
+    const int g = a.g[ds], n = a.n[ds];
+    const int t0 = g - 4, t1 = g + n + 4;  // cells read: [t0, t1)
+
+    auto load = [&](int c, double (&W)[NV]) {
+        c = c < t0 ? t0 : (c >= t1 ? t1 - 1 : c);
+        const long long o = base + step * c;
+#pragma unroll
+        for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + o];
+    };
+
+    // the window (values before the pipeline fills are never stored)
+    double qa[NV], qb[NV];    // primitives of cells t-2, t-1
+    double fpp[NV], F1m[NV];  // stage 1: face state on the right of cell t-2; flux at (t-2)-1/2
+    double sa[NV], sb[NV];    // stage-1 primitives of cells t-4, t-3
+    double fps[NV], F2m[NV];  // stage 2: face state on the right of cell t-4; flux at (t-4)-1/2
+#pragma unroll
+    for (int v = 0; v < NV; v++) qa[v] = qb[v] = fpp[v] = F1m[v] = sa[v] = sb[v] = fps[v] = F2m[v] = 1.0;
+
+    unsigned flags = 0u;
+    double dmass = 0.0;
+    const double huge = 1.7976931348623157e308;
+    double mn[3] = {huge, huge, huge}, cmax = 0.0;
+    double Wnext[NV];
+    load(t0, Wnext);
+    for (int t = t0; t < t1; t++) {
+        double W[NV], W2[NV], W4[NV];
+        copy_nv(W, Wnext);
+        load(t + 1, Wnext);
+        load(t - 2, W2);
+        load(t - 4, W4);
+        double q[NV];
+        prim<P>(W, a.gam_1, q);
+        // stage 1: face states of cell t-1, flux at (t-2)+1/2, state of cell t-2
+        double fp[NV], fm[NV], F1[NV];
+        cell_faces<P, LIM>(a, qa, qb, q, fp, fm);
+        riemann<P>(a, fpp, fm, F1);
+        double Ws[NV];
+        cell_update<P>(W2, a.c1, F1m, F1, Ws);
+        {
+            unsigned fl = 0u;
+            double dm = 0.0;
+            floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+            const int c = t - 2;
+            if (c >= g && c < g + n) {
+                flags |= fl;
+                dmass += dm;
+            }
+        }
+        double qs[NV];
+        prim<P>(Ws, a.gam_1, qs);
+        // stage 2: face states of cell t-3, flux at (t-4)+1/2, new state of cell t-4
+        double gp[NV], gm[NV], F2[NV];
+        cell_faces<P, LIM>(a, sa, sb, qs, gp, gm);
+        riemann<P>(a, fps, gm, F2);
+        double Wn[NV];
+        cell_update<P>(W4, a.c2, F2m, F2, Wn);
+        const int c = t - 4;
+        if (c >= g) {
+            unsigned fl = 0u;
+            double dm = 0.0;
+            floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+            flags |= fl;
+            dmass += dm;
+            if (a.dtslots && pvalid) dt_cell<DIR, P>(a, Wn, mn, cmax);
+            if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
+            if (pvalid) {
+                xyz[ds] = c;
+                const long long idx = base + step * c;
+#pragma unroll
+                for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + idx] = Wn[k];
+                if (a.images) write_images<DIR, P>(a, xyz, idx, Wn);
+            }
+        }
+        // advance the window
+        copy_nv(qa, qb);
+        copy_nv(qb, q);
+        copy_nv(fpp, fp);
+        copy_nv(F1m, F1);
+        copy_nv(sa, sb);
+        copy_nv(sb, qs);
+        copy_nv(fps, gp);
+        copy_nv(F2m, F2);
+    }
+    if (a.dtslots) dt_reduce(a, mn, cmax, blockIdx.x);
+    if (pvalid && flags) atomicOr(a.flags, flags);
+    if (pvalid && dmass != 0.0) atomicAdd(a.dmass, dmass);
 }
 
 template <int DIR, class P, bool FUSED, int LIM>
@@ -684,9 +861,34 @@ static hipError_t launch_sweep_t(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
+template <int DIR, class P, int LIM>
+static hipError_t launch_march_t(const SweepArgs &a, hipStream_t st)
+{
+    const int d1 = (DIR == 0) ? 1 : 0, d2 = (DIR == 2) ? 1 : 2;
+    const long long npen = (long long)a.n[d1] * a.n[d2];
+    const long long nblk = (npen + 63) / 64;
+    if (a.g[DIR] < 4 || nblk < 1 || nblk > 0x7fffffffll) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_march<DIR, P, LIM>), dim3((unsigned)nblk), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
 template <int DIR, class P, bool FUSED>
 static hipError_t launch_lim(const SweepArgs &a, hipStream_t st)
 {
+    if constexpr (FUSED) {
+        if (a.TS < 0) {  // whole pencils, one lane each (k_march)
+            if (a.lim == a.limb) {
+                switch (a.lim) {
+                case LIM_VANLEER: return launch_march_t<DIR, P, LIM_VANLEER>(a, st);
+                case LIM_MINMOD: return launch_march_t<DIR, P, LIM_MINMOD>(a, st);
+                case LIM_MONCEN: return launch_march_t<DIR, P, LIM_MONCEN>(a, st);
+                case LIM_SUPERBEE: return launch_march_t<DIR, P, LIM_SUPERBEE>(a, st);
+                default: break;
+                }
+            }
+            return launch_march_t<DIR, P, -1>(a, st);
+        }
+    }
     if (a.lim == a.limb) {
         switch (a.lim) {
         case LIM_VANLEER: return launch_sweep_t<DIR, P, FUSED, LIM_VANLEER>(a, st);

@@ -160,3 +160,44 @@ def test_autotuned_tiles_keep_bit_exactness(oracle):
     assert t_g == t_o
     assert_same(cg, ob, "autotuned 24^3")
     pk.finalize()
+
+
+MARCH = (64, -1)  # tile length -1: one lane per whole pencil (k_march)
+
+
+@pytest.mark.parametrize("case,tile_x", [("mhd", MARCH), ("mhd", (0, 0)), ("sedov", MARCH), ("mhd2d", MARCH),
+                                          ("mhd_vanleer_moncen", MARCH)])
+def test_march_kernel_bit_exact(oracle, case, tile_x):
+    """The march kernel (y and z, and x too when tile_x selects it) gives
+    the oracle's bits over whole stepping loops, periodic images and the fused
+    timestep included."""
+    if case == "mhd":
+        cg, cfg = setups.mhd_turbulence(n=24, seed=17)
+        steps = 3
+    elif case == "sedov":
+        cg, cfg = setups.sedov_hd(n=24)
+        steps = 6
+    elif case == "mhd_vanleer_moncen":
+        cg, cfg = setups.mhd_turbulence(n=20, seed=18)
+        cfg.limiter, cfg.limiter_b = "vanleer", "moncen"
+        steps = 3
+    else:
+        cfg = pk.Config(magnetized=True, limiter="vanleer", has_dir=(True, True, False), smalld=1e-3, smallei=1e-4)
+        cg = None
+        steps = 3
+    cfg.tile_x = tile_x
+    cfg.tile_yz = MARCH
+    print(f"march {case}, tile_x {tile_x}", flush=True)
+    setup_lib(cfg)
+    p = oracle.params_from_config(cfg)
+    if cg is None:
+        cg, ob = make_pair(oracle, (16, 12, 1), True, seed=19, cfg=cfg, has_dir=(True, True, False))
+    else:
+        ob = oracle.OBlock.from_cg(cg)
+        cg.upload()
+    t_g, _ = pk.run(cg, steps)
+    t_o = oracle.run(p, ob, steps)
+    cg.download()
+    assert t_g == t_o
+    assert_same(cg, ob, f"march {case}")
+    pk.finalize()
