@@ -67,6 +67,15 @@ __device__ __forceinline__ void lds_put(double *A, int nt, int t, const double (
     for (int v = 0; v < NV; v++) A[v * nt + t] = x[v];
 }
 
+// LDS arrays of NV doubles per lane: five shared neighbour buffers, and for
+// MHD three lane-private ones that hold values live across both flux VJPs
+// (registers for hydro, whose occupancy the larger LDS would halve).
+template <class P>
+constexpr int adj_lds_arrays()
+{
+    return P::MAG ? 8 : 5;
+}
+
 // q_bar of cell k from its face-state adjoints (fpb: left state of k+1/2,
 // fmb: right state of k-1/2) through the limiter: the own-cell part and the
 // dl/dr adjoints that the neighbours take.
@@ -134,6 +143,14 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
     const int nt = blockDim.x;
     const int t = threadIdx.x;
     double *Q1 = lds, *Q2 = lds + NV * nt, *S1 = lds + 2 * NV * nt, *S2 = lds + 3 * NV * nt, *S3 = lds + 4 * NV * nt;
+    double *K1 = lds + 5 * NV * nt, *K2 = lds + 6 * NV * nt, *K3 = lds + 7 * NV * nt;  // MHD: lane-private
+    // park x in a lane-private LDS array (MHD) until fetch() reloads it
+    auto park = [&](double *K, const double (&x)[NV]) {
+        if constexpr (P::MAG) lds_put<NV>(K, nt, t, x);
+    };
+    auto fetch = [&](double *K, double (&x)[NV]) {
+        if constexpr (P::MAG) lds_get<NV>(K, nt, t, x);
+    };
 
     constexpr int d1 = (DIR == 0) ? 1 : 0;
     constexpr int d2 = (DIR == 2) ? 1 : 2;
@@ -203,6 +220,8 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
         double dm = 0.0;
         floors<P>(Us, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
     }
+    park(K1, R1);
+    park(K2, Us_pre);
     // ---- forward: stage 2 ----
     double qs[NV];
     prim<P>(Us, a.gam_1, qs);
@@ -244,6 +263,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
         }
     }
     lds_put<NV>(S1, nt, t, lp);
+    park(K3, lp);
     __syncthreads();
     double fpb[NV];
     {
@@ -286,6 +306,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
             qb[v] += drp[v] - dln[v];
             lsp[v] = 0.0;
         }
+        fetch(K2, Us_pre);
         {
             double Usr[NV];
 #pragma unroll
@@ -296,6 +317,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
             prim_vjp<P>(a, Usr, qb, lsp);
         }
         floors_vjp<P>(a, Us_pre, lsp);
+        fetch(K1, R1);
         if (outl) {
             double acc = 0.0;
 #pragma unroll
@@ -309,6 +331,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
     {
         double lsn[NV], fb[NV], q2[NV], q3[NV], l[NV], r[NV], junk[NV], rb[NV];
         lds_get<NV>(S2, nt, tp, lsn);
+        fetch(K3, lp);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             fb[v] = a.c1 * (lsn[v] - lsp[v]);
@@ -669,7 +692,7 @@ __global__ void k_dt_apply_jobs(const DtBatchJob *__restrict__ jobs, int njobs)
 template <int DIR, class P, int LIM>
 static hipError_t adj_t(const AdjArgs &a, int nt, long long nblk, hipStream_t st)
 {
-    const size_t lds = sizeof(double) * 5 * P::NV * nt;
+    const size_t lds = sizeof(double) * adj_lds_arrays<P>() * P::NV * nt;
     static bool raised = false;
     if (!raised) {
         hipError_t e = hipFuncSetAttribute((const void *)k_sweep_adjoint<DIR, P, LIM>,
@@ -702,7 +725,7 @@ static hipError_t adjl_t(const AdjListArgs &L, int grid, hipStream_t st, int *re
 {
     const int nt = 256;
     const void *fn = (const void *)k_sweep_adjoint_list<DIR, P, LIM>;
-    const size_t lds = sizeof(double) * 5 * P::NV * nt;
+    const size_t lds = sizeof(double) * adj_lds_arrays<P>() * P::NV * nt;
     const size_t lds_max = 160 * 1024 - 256;
     static bool raised = false;
     if (!raised) {
