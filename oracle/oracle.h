@@ -80,6 +80,9 @@ void or_pencil_stage(const or_params *p, int n, double dtodx, double chspeed,
                      const double *b, double *u1, double *b1, int nbp, double dvol,
                      or_report *rep);
 
+void or_pencil_flux(const or_params *p, int n, double chspeed, const double *u, const double *b, double *flx,
+                    double *bflx);
+
 /* ---- block-level routines ---------------------------------------------- */
 void or_solve_cg(const or_params *p, or_block *cg, int ddim, int istep,
                  double dt, double chspeed, or_report *rep);
@@ -96,6 +99,11 @@ double or_run(const or_params *p, or_block *cg, int nsteps, or_report *rep);
 void or_fluid_update_list(const or_params *p, or_block *const *cgs, int nblk, double dt, or_report *reps);
 double or_run_list(const or_params *p, or_block *const *cgs, int nblk, int nsteps, or_report *reps);
 void or_fill_ghosts_list(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk);
+
+/* ---- the unsplit Riemann driver (unsplit.c) ------------------------------ */
+void or_unsplit_stage(const or_params *p, or_block *cg, int istep, double dt, double chspeed, or_report *rep);
+void or_fluid_update_unsplit(const or_params *p, or_block *cg, double dt, or_report *rep);
+double or_run_unsplit(const or_params *p, or_block *cg, double tend, int nmax, int32_t *nsteps, or_report *rep);
 void or_fluid_update_list_nb(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk, double dt,
                              or_report *reps);
 double or_run_list_nb(const or_params *p, or_block *const *cgs, const int32_t *nbr, int nblk, int nsteps,

@@ -89,6 +89,12 @@ def lib():
         L.or_fluid_update_list_nb.argtypes = [P(Params), P(P(Block)), ip, ctypes.c_int, ctypes.c_double, P(Report)]
         L.or_run_list_nb.argtypes = [P(Params), P(P(Block)), ip, ctypes.c_int, ctypes.c_int, P(Report)]
         L.or_run_list_nb.restype = ctypes.c_double
+        L.or_unsplit_stage.argtypes = [P(Params), P(Block), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       P(Report)]
+        L.or_fluid_update_unsplit.argtypes = [P(Params), P(Block), ctypes.c_double, P(Report)]
+        L.or_run_unsplit.argtypes = [P(Params), P(Block), ctypes.c_double, ctypes.c_int, P(ctypes.c_int32),
+                                     P(Report)]
+        L.or_run_unsplit.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -319,3 +325,17 @@ def hlld(ql, qr, bl, br, gam, ch):
 
 def spacing(x):
     return lib().or_spacing(float(x))
+
+
+def fluid_update_unsplit(p, ob, dt):
+    """fluid_update_unsplit on one periodic block."""
+    blk = ob.c()
+    lib().or_fluid_update_unsplit(ctypes.byref(p), ctypes.byref(blk), dt, ctypes.byref(ob.rep))
+
+
+def run_unsplit(p, ob, tend, nmax=1 << 30):
+    """The main loop with the unsplit solver until t >= tend; returns (t, nsteps)."""
+    blk = ob.c()
+    ns = ctypes.c_int32()
+    t = lib().or_run_unsplit(ctypes.byref(p), ctypes.byref(blk), tend, nmax, ctypes.byref(ns), ctypes.byref(ob.rep))
+    return t, ns.value

@@ -495,6 +495,39 @@ void or_floors(const or_params *p, int n, double *u1, const double *b1, int nbp,
 }
 
 /*
+ * The interface fluxes of one pencil (solve / solve_u of the unsplit driver,
+ * unsplit_mag_modules.F90:166-209, solve_cg_unsplit.F90:157-187):
+ * interpol -> riemann_wrap.  u (n, nvar), b (n, 4) = [B_n, B_t1, B_t2, psi]
+ * or NULL; flx (n-1, nvar), bflx (n-1, 4): interface i+1/2 at row i.
+ */
+void or_pencil_flux(const or_params *p, int n, double chspeed, const double *u, const double *b, double *flx,
+                    double *bflx)
+{
+    int nv = p->nvar, m = n - 1;
+    int mhd = any_magnetized(p);
+    double *q = (double *)malloc(sizeof(double) * (size_t)n * nv);
+    double *ql = (double *)malloc(sizeof(double) * (size_t)m * nv);
+    double *qr = (double *)malloc(sizeof(double) * (size_t)m * nv);
+    double *bl = NULL, *br = NULL;
+    or_utoq(p, n, u, mhd ? b : NULL, q);
+    or_linear(p->limiter, n, nv, q, ql, qr);
+    if (mhd) {
+        bl = (double *)malloc(sizeof(double) * (size_t)m * 4);
+        br = (double *)malloc(sizeof(double) * (size_t)m * 4);
+        or_linear(p->limiter_b, n, 4, b, bl, br);
+    }
+    for (int f = 0; f < p->nfluids; f++) {
+        int o = p->fl[f].beg;
+        if (p->fl[f].magnetized)
+            or_hlld(m, ql + (size_t)o * m, qr + (size_t)o * m, bl, br, p->fl[f].gam, chspeed, flx + (size_t)o * m,
+                    bflx);
+        else
+            or_hllc(m, ql + (size_t)o * m, qr + (size_t)o * m, p->fl[f].gam, flx + (size_t)o * m);
+    }
+    free(q); free(ql); free(qr); free(bl); free(br);
+}
+
+/*
  * One stage of the pencil operator as run for one pencil by solve_cg_ub /
  * solve_cg_u (solve_cg_riemann.F90:202-222 and :290-298):
  *   u1 = u; b1 = b

@@ -379,6 +379,66 @@ def assemble(blocks, boxes, name, n):
     return out
 
 
+# problems/cpaw/problem.par: the wave the reference's CPAW test measures
+CPAW_PAR = dict(d0=1.0, p0=0.1, vx0=0.0, vy0=0.0, vz0=0.0, A=0.1, vA=1.0, kx=6.283185307, ky=12.566370614, kz=0.0)
+CPAW_TEND = 0.447213595
+# L2 error norm of B after one period, problems/cpaw/problem.par:2-13
+CPAW_L2 = {(32, 16): 0.00996214, (48, 24): 0.00559514, (64, 32): 0.00347388, (96, 48): 0.00171573,
+           (128, 64): 0.00102726, (192, 96): 0.00049221, (256, 128): 0.00029084}
+
+
+def cpaw(nx=128, ny=64, nb=4, gamma=5.0 / 3.0, **par):
+    """Circularly polarised Alfven wave (problems/cpaw/initproblem.F90:137-205,
+    problem.par): 2D periodic domain [0, 1] x [0, 0.5] of nx x ny cells,
+    the wave vector (kx, ky, 0) rotated from the x axis, v' = (vx0, vy0 +
+    A sin phi, vz0 + A cos phi), B' = (vA sqrt(d0), -sqrt(d0) A sin phi,
+    -sqrt(d0) A cos phi), p = p0, every cell ghosts included from the
+    formula.  Returns (GridContainer, Config, ini_B) with ini_B the initial
+    interior B (3, nx, ny, 1) the L2 error is measured against; the unsplit
+    solver's settings: HLLD + GLM, van Leer, CFL 0.3, cfl_glm 0.3,
+    smalld = smallei = 1e-5, ionized gamma 5/3."""
+    q = dict(CPAW_PAR)
+    q.update(par)
+    d0, p0, A, vA = q["d0"], q["p0"], q["A"], q["vA"]
+    kx, ky, kz = q["kx"], q["ky"], q["kz"]
+    hd = (True, True, False)
+    dl = (1.0 / nx, 0.5 / ny, 1.0)
+    cg = GridContainer((nx, ny, 1), nb=nb, dl=dl, magnetized=True, has_dir=hd)
+    x = (np.arange(cg.ng[0]) - nb + 0.5) * dl[0]
+    y = (np.arange(cg.ng[1]) - nb + 0.5) * dl[1]
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    Z = np.full_like(X, 0.5)
+    aa = np.arctan(ky / kx)
+    g = np.arctan(np.cos(aa) * (kz / kx))
+    kp = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    rot = np.array([[np.cos(aa) * np.cos(g), -np.sin(aa), -np.cos(aa) * np.sin(g)],
+                    [np.sin(aa) * np.cos(g), np.cos(aa), -np.sin(aa) * np.sin(g)],
+                    [np.sin(g), 0.0, np.cos(g)]])
+    phi = kp * ((rot[0, 0] * X + rot[1, 0] * Y) + rot[2, 0] * Z)
+    vp = [q["vx0"] + 0.0 * phi, q["vy0"] + A * np.sin(phi), q["vz0"] + A * np.cos(phi)]
+    bp = [vA * np.sqrt(d0) + 0.0 * phi, -np.sqrt(d0) * A * np.sin(phi), -np.sqrt(d0) * A * np.cos(phi)]
+    v = [(rot[i, 0] * vp[0] + rot[i, 1] * vp[1]) + rot[i, 2] * vp[2] for i in range(3)]
+    b = [(rot[i, 0] * bp[0] + rot[i, 1] * bp[1]) + rot[i, 2] * bp[2] for i in range(3)]
+    cg.u[0, ..., 0] = d0
+    for i in range(3):
+        cg.u[1 + i, ..., 0] = v[i]
+        cg.b[i, ..., 0] = b[i]
+    ekin = 0.5 * ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) * d0
+    emag = 0.5 * ((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2])
+    cg.u[4, ..., 0] = (p0 / (gamma - 1.0) + ekin) + emag
+    cfg = Config(gamma=gamma, magnetized=True, limiter="vanleer", cfl=0.3, cfl_glm=0.3, smalld=1e-5, smallei=1e-5,
+                 has_dir=hd)
+    return cg, cfg, np.array(cg.interior(cg.b))
+
+
+def cpaw_l2(b, ini_b):
+    """The CPAW error norm (problems/cpaw/initproblem.F90:268-282): over the
+    active directions (x, y), sqrt(sum (B_ini - B)^2 / sum B_ini^2)."""
+    num = sum(float(np.sum((ini_b[d] - b[d]) ** 2)) for d in range(2))
+    den = sum(float(np.sum(ini_b[d] ** 2)) for d in range(2))
+    return float(np.sqrt(num / den))
+
+
 def unit_gaussian(n, seed, kmax=6.0):
     """Seeded periodic Gaussian random field on n^3 with 0 < |k| <= kmax,
     normalised to zero mean and unit variance."""
