@@ -221,6 +221,28 @@ int pk_ensemble_step_adjoint(const double *dts, int32_t slot0, double *dt_bar);
 /* lam_i += dt_bar[i] * d(dt_i)/du_i at the state in slot (timestep_fluid) */
 int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar);
 
+/* ---- the unsplit Riemann solver (RIEMANN_UNSPLIT) --------------------------- */
+
+/* One RK stage of the unsplit solver on a block periodic on itself:
+ * solve_cg_unsplit(cg, istep) (src/scheme/unsplit_schemes/solve_cg_unsplit.F90:14-45,
+ * called at unsplit_sweeps.F90:161) -- the interface fluxes of every active
+ * direction from one state (u at RK2_1, uh at RK2_2), their differences added
+ * x, y, z in turn to uh (a copy of u) or to u, B and psi included, the
+ * vanishing sources and, at the last stage, care_for_positives once per
+ * direction in its sweep order -- followed by update_boundaries of the
+ * stage's arrays (unsplit_sweeps.F90:47-82, periodic).  One fluid; nb >= 2. */
+int pk_unsplit_stage(int64_t cg_id, int32_t istep, double dt, double chspeed);
+
+/* fluid_update_unsplit (unsplit_fluidupdate.F90:45-126) on a periodic block:
+ * c_h = cfl_glm*min(dl)/dt, psi damped by exp(-glm_alpha*c_h*(dt/2)/min(dl))
+ * before and after the stages. */
+int pk_fluid_update_unsplit(int64_t cg_id, double dt);
+
+/* The main loop with the unsplit solver until t >= tend or nmax steps:
+ * time_step with its caps and the unsplit clamp dt <= (tend - t)*(1 + 2 eps)
+ * (timestep.F90:198-222), then pk_fluid_update_unsplit; t += dt. */
+int pk_run_unsplit(int64_t cg_id, double tend, int32_t nmax, double *t_out, int32_t *nsteps_out);
+
 /* ---- tuning (no counterpart in the reference) ------------------------------ */
 
 /* Time candidate tile shapes of the fused sweep on this block (the state is

@@ -90,6 +90,10 @@ _PROTOS = {
     "pk_sweep": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
     "pk_fill_ghosts": [ctypes.c_int64],
     "pk_set_neighbour": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
+    "pk_unsplit_stage": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
+    "pk_fluid_update_unsplit": [ctypes.c_int64, ctypes.c_double],
+    "pk_run_unsplit": [ctypes.c_int64, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                       ctypes.POINTER(ctypes.c_int32)],
     "pk_glm_damp": [ctypes.c_int64, ctypes.c_double],
     "pk_timestep_fluid": [ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_fluid_update": [ctypes.c_int64, ctypes.c_double],
@@ -375,6 +379,25 @@ def update_boundaries(cg):
     """Refresh of every ghost cell of the block on the device: periodic on
     itself, or from its neighbours (set_neighbour)."""
     _call("pk_fill_ghosts", cg.cg_id)
+
+
+def unsplit_stage(cg, istep, dt, chspeed=0.0):
+    """One RK stage of the unsplit solver on a periodic block
+    (solve_cg_unsplit + update_boundaries)."""
+    _call("pk_unsplit_stage", cg.cg_id, int(istep), float(dt), float(chspeed))
+
+
+def fluid_update_unsplit(cg, dt):
+    """fluid_update_unsplit on a periodic block."""
+    _call("pk_fluid_update_unsplit", cg.cg_id, float(dt))
+
+
+def run_unsplit(cg, tend, nmax=1 << 30):
+    """The main loop with the unsplit solver until t >= tend (or nmax steps);
+    returns (t, nsteps)."""
+    t, ns = ctypes.c_double(), ctypes.c_int32()
+    _call("pk_run_unsplit", cg.cg_id, float(tend), int(nmax), ctypes.byref(t), ctypes.byref(ns))
+    return t.value, ns.value
 
 
 def set_neighbour(cg, direction, side, nb):

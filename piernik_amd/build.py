@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpiernik_hip.so")
 
-SOURCES = ["pk_kernels.hip", "pk_kernels_fast.hip", "pk_adjoint.hip", "pk_capi.hip"]
+SOURCES = ["pk_kernels.hip", "pk_kernels_fast.hip", "pk_adjoint.hip", "pk_unsplit.hip", "pk_capi.hip"]
 
 # -ffp-contract=off: every a*b+c rounds twice, as the reference's FP64 code.
 # -packed-fp32-ops off: no packed FP32 instructions are generated.

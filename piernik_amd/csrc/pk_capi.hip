@@ -25,6 +25,7 @@
 #include "../../include/piernik_hip.h"
 #include "pk_adjoint_kernels.hpp"
 #include "pk_kernels.hpp"
+#include "pk_unsplit.hpp"
 
 namespace {
 
@@ -58,6 +59,9 @@ struct Block {
     // nbr[d][0] low side, nbr[d][1] high side; the block's own id when it is
     // periodic on itself
     int64_t nbr[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    // unsplit stages (pk_unsplit_stage): interface fluxes of x, y, z, each
+    // with the state's planes and strides
+    double *d_flux = nullptr;
 };
 
 const int kDtSlots = 1024;
@@ -593,6 +597,7 @@ void free_block(Block *b)
     if (b->d_dmass) (void)hipFree(b->d_dmass);
     if (b->d_dtout) (void)hipFree(b->d_dtout);
     if (b->d_dtslots) (void)hipFree(b->d_dtslots);
+    if (b->d_flux) (void)hipFree(b->d_flux);
     delete b;
 }
 
@@ -1508,6 +1513,130 @@ int ens_adj_build()
     return PK_OK;
 }
 
+// ---- the unsplit Riemann driver (RIEMANN_UNSPLIT) ----
+
+// periodic ghost refresh of one of the block's two buffers
+int fill_buffer(Block *b, int i)
+{
+    pk::GhostArgs a{};
+    a.u = state(b, i);
+    a.sv = b->sv;
+    a.sy = b->sy;
+    a.sz = b->sz;
+    for (int d = 0; d < 3; d++) {
+        a.ng[d] = b->ng[d];
+        a.g[d] = b->g[d];
+        a.n[d] = b->n[d];
+    }
+    a.plane0 = 0;
+    a.nplanes = g_nplanes;
+    for (int d = 0; d < 3; d++)
+        if (b->g[d] > 0) HIPCHK(pk::launch_fill_ghosts(d, a, g_stream));
+    return PK_OK;
+}
+
+// One RK stage of unsplit_sweep on a periodic block (solve_cg_ub / solve_cg_u,
+// then update_boundaries of the stage's arrays, unsplit_sweeps.F90:126-190):
+// the first stage reads u and writes uh (the other buffer), the last reads
+// uh and adds its fluxes to u in place.
+int unsplit_stage(Block *b, int istep, double dt, double ch)
+{
+    if (g_nfl != 1) return fail(PK_EINVAL, "the unsplit solver covers one fluid");
+    if (block_linked(b)) return fail(PK_EINVAL, "the unsplit solver covers blocks periodic on themselves");
+    const bool euler = g_cfg.integration_order == 1;
+    const bool first = !euler && istep == PK_RK2_1;
+    const bool last = euler ? istep == PK_EULER : istep == PK_RK2_2;
+    if (!first && !last) return fail(PK_EINVAL, "istep %d does not match integration_order %d", istep,
+                                     g_cfg.integration_order);
+    for (int d = 0; d < 3; d++)
+        if (g_cfg.has_dir[d] && b->g[d] < 2) return fail(PK_EINVAL, "the unsplit stage needs nb >= 2");
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    if (!b->d_flux) HIPCHK(hipMalloc((void **)&b->d_flux, sizeof(double) * 3 * (size_t)g_nplanes * b->sv));
+    int src, dst;
+    if (first) {
+        src = b->cur;
+        dst = 1 - b->cur;
+    } else if (euler) {
+        src = dst = b->cur;
+    } else {
+        if (b->uh < 0) return fail(PK_ESTATE, "the first stage has not run on block %lld", (long long)b->id);
+        src = b->uh;
+        dst = b->cur;
+    }
+    pk::UnsplitArgs a{};
+    a.src = state(b, src);
+    a.base = state(b, b->cur);
+    a.dst = state(b, dst);
+    a.sv = b->sv;
+    a.sy = b->sy;
+    a.sz = b->sz;
+    for (int d = 0; d < 3; d++) {
+        a.ng[d] = b->ng[d];
+        a.g[d] = b->g[d];
+        a.n[d] = b->n[d];
+        a.has_dir[d] = g_cfg.has_dir[d];
+        a.dl[d] = b->dl[d];
+    }
+    a.lim = g_cfg.limiter;
+    a.limb = g_cfg.limiter_b;
+    a.gam = g_cfg.fluid[0].gamma;
+    a.gam_1 = g_cfg.fluid[0].gamma - 1.0;
+    a.ch = ch;
+    a.dt = dt;
+    a.rk = rk_coef(istep);
+    a.last = last ? 1 : 0;
+    a.smalld = g_cfg.smalld;
+    a.smallei = g_cfg.smallei;
+    a.use_smalld = g_cfg.use_smalld;
+    a.use_smallei = g_cfg.use_smallei;
+    a.flags = b->d_flags;
+    a.dmass = b->d_dmass;
+    const double *F[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < 3; d++) {
+        if (!g_cfg.has_dir[d]) continue;
+        double *Fd = b->d_flux + (size_t)d * g_nplanes * b->sv;
+        HIPCHK(pk::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream));
+        F[d] = Fd;
+    }
+    HIPCHK(pk::launch_unsplit_update(g_mhd, a, F, g_stream));
+    RCCHK(fill_buffer(b, dst));
+    b->uh = euler ? -1 : (first ? dst : b->uh);
+    b->dt_pending = false;
+    return PK_OK;
+}
+
+// psi *= fac over the block's current buffer, ghosts included (hdc.F90:338)
+int damp_psi(Block *b, double fac)
+{
+    HIPCHK(pk::launch_scale(fluid_state(b, b->cur, g_magf) + 8 * b->sv, (long long)b->sz * b->ng[2], fac, g_stream));
+    return PK_OK;
+}
+
+// fluid_update_unsplit (unsplit_fluidupdate.F90:45-126) on one periodic block:
+// update_chspeed, glmdamping(half), the stages, glmdamping(half)
+int fluid_update_unsplit(Block *b, double dt)
+{
+    double ch = 0.0, fac = 1.0;
+    if (g_mhd) {
+        const double mdl = min_dl(b);
+        ch = (g_cfg.cfl_glm * mdl) / dt;  // hdc.F90:122
+        // hdc.F90:315-331 with dt_eff = dt/2
+        double f = 0.0;
+        const double r = (g_cfg.glm_alpha * ch) / (mdl / (dt / 2.0));
+        if (r > f) f = r;
+        fac = exp(-f);
+        RCCHK(damp_psi(b, fac));
+    }
+    if (g_cfg.integration_order == 1) {
+        RCCHK(unsplit_stage(b, PK_EULER, dt, ch));
+    } else {
+        RCCHK(unsplit_stage(b, PK_RK2_1, dt, ch));
+        RCCHK(unsplit_stage(b, PK_RK2_2, dt, ch));
+    }
+    if (g_mhd) RCCHK(damp_psi(b, fac));
+    return PK_OK;
+}
+
 // device-resident stepping of one block: only for blocks periodic on themselves
 int solo(const Block *b)
 {
@@ -1818,6 +1947,56 @@ int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out)
     HIPCHK(hipStreamSynchronize(g_stream));
     if (t_out) *t_out = t;
     if (dt_out) *dt_out = dt;
+    return PK_OK;
+}
+
+int pk_unsplit_stage(int64_t cg_id, int32_t istep, double dt, double chspeed)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    list_touch();
+    return unsplit_stage(b, istep, dt, chspeed);
+}
+
+int pk_fluid_update_unsplit(int64_t cg_id, double dt)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    RCCHK(solo(b));
+    list_touch();
+    return fluid_update_unsplit(b, dt);
+}
+
+int pk_run_unsplit(int64_t cg_id, double tend, int32_t nmax, double *t_out, int32_t *nsteps_out)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    RCCHK(solo(b));
+    if (nmax < 0) return fail(PK_EINVAL, "nmax < 0");
+    list_touch();
+    double t = 0.0, dt = -1.0, dt_old = -1.0;
+    int s = 0;
+    RCCHK(do_fill(b));
+    while (t < tend && s < nmax) {
+        double c;
+        RCCHK(do_timestep(b, &dt, &c));
+        // time_step caps (timestep.F90:198-207), then the unsplit clamp to tend (:221-222)
+        if (s >= 1 && dt_old > 0.0) {
+            const double cap = dt_old * 2.0;
+            if (cap < dt) dt = cap;
+        }
+        const double rem = tend - t;
+        const double cl = rem + (2.0 * DBL_EPSILON) * rem;
+        if (cl < dt) dt = cl;
+        RCCHK(fluid_update_unsplit(b, dt));
+        t = t + dt;
+        dt_old = dt;
+        s++;
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (t_out) *t_out = t;
+    if (nsteps_out) *nsteps_out = s;
     return PK_OK;
 }
 

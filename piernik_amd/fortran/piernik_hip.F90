@@ -26,7 +26,8 @@ module piernik_hip
         &    pk_adjoint_set, pk_adjoint_get, pk_fluid_update_record, pk_sweep_adjoint, &
         &    pk_fluid_update_adjoint, pk_timestep_adjoint, pk_ensemble_ids, pk_ensemble_step, pk_ensemble_run, &
         &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
-        &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint
+        &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint, &
+        &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
 
@@ -145,6 +146,28 @@ module piernik_hip
          import :: c_int32_t, c_int64_t
          integer(c_int64_t), value :: cg_id
       end function pk_fill_ghosts
+
+      integer(c_int32_t) function pk_unsplit_stage(cg_id, istep, dt, chspeed) bind(C, name="pk_unsplit_stage")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         integer(c_int32_t), value :: istep
+         real(c_double),     value :: dt, chspeed
+      end function pk_unsplit_stage
+
+      integer(c_int32_t) function pk_fluid_update_unsplit(cg_id, dt) bind(C, name="pk_fluid_update_unsplit")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         real(c_double),     value :: dt
+      end function pk_fluid_update_unsplit
+
+      integer(c_int32_t) function pk_run_unsplit(cg_id, tend, nmax, t_out, nsteps_out) bind(C, name="pk_run_unsplit")
+         import :: c_int32_t, c_int64_t, c_double
+         integer(c_int64_t), value :: cg_id
+         real(c_double),     value :: tend
+         integer(c_int32_t), value :: nmax
+         real(c_double),     intent(out) :: t_out
+         integer(c_int32_t), intent(out) :: nsteps_out
+      end function pk_run_unsplit
 
       integer(c_int32_t) function pk_set_neighbour(cg_id, dir, side, nb_id) bind(C, name="pk_set_neighbour")
          import :: c_int32_t, c_int64_t

@@ -1,0 +1,104 @@
+"""The unsplit Riemann solver on the GPU (pk_unsplit_stage,
+pk_fluid_update_unsplit, pk_run_unsplit) against the oracle's unsplit
+driver, bit for bit over whole runs, and against the reference's own known
+answer: the circularly polarised Alfven wave at 128 x 64 gives
+L2 = 0.00102726 (Makefile:351-366), which the oracle reproduces at every
+resolution of problems/cpaw/problem.par (tests/test_oracle_cpaw.py).
+"""
+import numpy as np
+import pytest
+
+import piernik_amd as pk
+from piernik_amd import setups
+
+pytestmark = pytest.mark.gpu
+
+
+def start(cfg):
+    pk.load_library()
+    pk.finalize()
+    pk.init(cfg)
+
+
+def assert_same(cg, ob, where):
+    for name in ("u", "b", "psi"):
+        a, o = getattr(cg, name), getattr(ob, name)
+        if a is None:
+            continue
+        bad = np.argwhere(~(a == o))
+        assert bad.size == 0, f"{where} {name}: {len(bad)} mismatches, first {tuple(bad[0])}"
+
+
+def test_cpaw_run_bit_exact(oracle):
+    cg, cfg, ini = setups.cpaw(32, 16)
+    print("unsplit CPAW 32x16 to tend", flush=True)
+    start(cfg)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    cg.upload()
+    t_g, n_g = pk.run_unsplit(cg, setups.CPAW_TEND)
+    t_o, n_o = oracle.run_unsplit(p, ob, setups.CPAW_TEND)
+    cg.download()
+    assert (t_g, n_g) == (t_o, n_o)
+    assert_same(cg, ob, "cpaw 32x16")
+    assert f"{setups.cpaw_l2(cg.interior(cg.b), ini):.8f}" == "0.00996214"
+    pk.finalize()
+
+
+@pytest.mark.parametrize("mag,lim,limb", [(True, "minmod", "moncen"), (True, "vanleer", "vanleer"),
+                                          (False, "superbee", "superbee")])
+def test_3d_steps_bit_exact(oracle, mag, lim, limb):
+    n = (12, 10, 8)
+    cg = setups.smooth_random(n, seed=31, magnetized=mag)
+    if mag:
+        cg.psi[...] = np.random.default_rng(4).normal(0, 0.01, cg.psi.shape)
+    setups.fill_ghosts_periodic(cg)
+    cfg = pk.Config(magnetized=mag, limiter=lim, limiter_b=limb, cfl=0.4, cfl_glm=0.4, smalld=1e-10, smallei=1e-10)
+    print(f"unsplit 3D {n} mag={mag} {lim}/{limb}", flush=True)
+    start(cfg)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    cg.upload()
+    t_g, n_g = pk.run_unsplit(cg, 1.0, nmax=3)
+    t_o, n_o = oracle.run_unsplit(p, ob, 1.0, nmax=3)
+    cg.download()
+    assert (t_g, n_g) == (t_o, n_o) and n_g == 3
+    assert_same(cg, ob, "3d")
+    pk.finalize()
+
+
+def test_stages_bit_exact(oracle):
+    """The two stages one at a time: uh after the first, u after the second."""
+    cg = setups.smooth_random((10, 12, 6), seed=32, magnetized=True)
+    cg.psi[...] = np.random.default_rng(5).normal(0, 0.01, cg.psi.shape)
+    setups.fill_ghosts_periodic(cg)
+    cfg = pk.Config(magnetized=True, limiter="vanleer", cfl=0.4, cfl_glm=0.4, smalld=1e-10, smallei=1e-10)
+    start(cfg)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    cg.upload()
+    dt, ch = 2.0e-3, 0.8
+    for istep in (pk.RK2_1, pk.RK2_2):
+        pk.unsplit_stage(cg, istep, dt, ch)
+        oracle.lib().or_unsplit_stage(oracle.ctypes.byref(p), oracle.ctypes.byref(ob.c()), istep, dt, ch,
+                                      oracle.ctypes.byref(ob.rep))
+        cg.download()
+        if istep == pk.RK2_1:
+            for name in ("uh", "bh", "psih"):
+                assert np.array_equal(getattr(cg, name), getattr(ob, name)), name
+    assert_same(cg, ob, "two stages")
+    pk.finalize()
+
+
+def test_cpaw_128x64_known_answer():
+    """The reference's CI value, on the GPU."""
+    cg, cfg, ini = setups.cpaw(128, 64)
+    print("unsplit CPAW 128x64 to tend", flush=True)
+    start(cfg)
+    cg.upload()
+    t, n = pk.run_unsplit(cg, setups.CPAW_TEND)
+    cg.download()
+    l2 = setups.cpaw_l2(cg.interior(cg.b), ini)
+    print(f"steps {n}, L2 {l2:.8f}")
+    assert n == 224 and f"{l2:.8f}" == "0.00102726"
+    pk.finalize()
