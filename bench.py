@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "FP64 MHD cell-updates/s at 256^3"
+UNSPLIT_CFL = 0.3  # problems/cpaw/problem.par: 0.41 is near the unsplit solver's limit
 
 
 def _cpu_worker(args):
@@ -45,7 +46,7 @@ def _cpu_worker(args):
     workload, n, seed, budget = args
     from oracle import pyoracle as po
     from piernik_amd import setups
-    if workload == "mhd256":
+    if workload in ("mhd256", "unsplit256"):
         cg, cfg = setups.mhd_turbulence(n=n, seed=seed)
     elif workload == "batch4096":
         cfg = setups.batch_config()
@@ -56,6 +57,17 @@ def _cpu_worker(args):
     ob = po.OBlock.from_cg(cg)
     t0 = time.perf_counter()
     steps = 0
+    if workload == "unsplit256":
+        p.cfl = UNSPLIT_CFL
+        tt = 0.0
+        while True:
+            dt, _ = po.time_step(p, ob, steps, -1.0)
+            po.fluid_update_unsplit(p, ob, dt)
+            tt += dt
+            steps += 1
+            if time.perf_counter() - t0 >= budget:
+                break
+        return steps * n ** 3, time.perf_counter() - t0
     while True:
         po.run(p, ob, 1)
         steps += 1
@@ -86,7 +98,7 @@ def cpu_baseline(workload, budget=10.0):
         "kind": "port",
         "sample": (f"oracle (C restatement of the reference sweep, one thread per process) on {cores} "
                    f"independent periodic {nblk}^3 blocks of the same "
-                   f"{ {'mhd256': 'MHD HLLD+GLM', 'batch4096': 'two-fluid (MHD HLLD+GLM + HD HLLC)'}.get(workload, 'HD HLLC')} "
+                   f"{ {'mhd256': 'MHD HLLD+GLM', 'unsplit256': 'MHD HLLD+GLM, unsplit solver', 'batch4096': 'two-fluid (MHD HLLD+GLM + HD HLLC)'}.get(workload, 'HD HLLC')} "
                    f"problem, whole PIERNIK steps for ~{budget:.0f} s each; {updates} cell-updates in {slowest:.2f} s "
                    f"(wall {wall:.2f} s)"),
     }
@@ -106,7 +118,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128", "ensemble256", "copy"],
+    ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128", "ensemble256", "copy",
+                                           "unsplit256"],
                     default="mhd256")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--nblocks", type=int, default=4096, help="blocks of the batched list")
@@ -141,6 +154,9 @@ def main():
         return
     if args.workload == "batch4096":
         print(json.dumps(run_batch(args, cpu)))
+        return
+    if args.workload == "unsplit256":
+        print(json.dumps(run_unsplit(args, cpu)))
         return
     if args.workload == "mhd256" and args.blocks:
         print(json.dumps(run_blocks(args)))
@@ -393,6 +409,67 @@ def copy_ceiling(steps=20):
     ms = e0.elapsed_time(e1) / max(1, steps)
     del a, b
     return 2 * COPY_BYTES / (ms * 1e-3) / 1e9
+
+
+def run_unsplit(args, cpu):
+    """The 256^3 MHD turbulence block advanced by the unsplit solver
+    (RIEMANN_UNSPLIT): one step is one dt, so one cell-update per cell and
+    step.  Algorithmic bytes: each of the two RK stages reads the state and
+    writes it once, 2 x 144 B per cell-update."""
+    import piernik_amd as pk
+    from piernik_amd import setups
+
+    n = args.n or 256
+    cg, cfg = setups.mhd_turbulence(n=n, seed=1)
+    cfg.cfl = UNSPLIT_CFL
+    pk.init(cfg)
+    cg.upload()
+    pk.run_unsplit(cg, 1e30, nmax=max(1, args.warmup))
+    pk.synchronize()
+    t0 = time.perf_counter()
+    t_sim, nst = pk.run_unsplit(cg, 1e30, nmax=args.steps)
+    pk.synchronize()
+    el = time.perf_counter() - t0
+    rep = pk.get_report(cg)
+    pk.finalize()
+    cells = n ** 3
+    algo = 2 * 144.0 * cells * args.steps
+    achieved = algo / el / 1e9
+    return {
+        "metric": "FP64 MHD cell-updates/s at 256^3, unsplit solver",
+        "value": cells * args.steps / el,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3D ideal MHD, HLLD+GLM, minmod, unsplit RK2 (solver_str = 'unsplit'), CFL {UNSPLIT_CFL}, "
+                         f"single periodic {n}^3 block, nb=4, Gaussian random v and B=curl A, beta=1"),
+            "cells": cells,
+            "step": "time_step + fluid_update_unsplit (2 stages: fluxes of x, y, z, then the update)",
+            "sim_time": t_sim,
+            "arithmetic": "FP64, every operation rounded as the reference's code (bit-exact)",
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "whole step (k_unsplit_flux x3 + k_unsplit_update per stage)",
+            "algo_bytes_per_step": algo / args.steps,
+        },
+        "cpu_baseline": cpu,
+        "floors": rep,
+    }
 
 
 def run_blocks(args):
