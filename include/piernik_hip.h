@@ -221,6 +221,29 @@ int pk_ensemble_step_adjoint(const double *dts, int32_t slot0, double *dt_bar);
 /* lam_i += dt_bar[i] * d(dt_i)/du_i at the state in slot (timestep_fluid) */
 int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar);
 
+/* ---- step retries (cflcontrol) --------------------------------------------- */
+
+#define PK_CFL_WARN 0      /* 'warn' / 'none': steps are never redone */
+#define PK_CFL_REDO 1      /* 'redo': redo with dt_full * dt_shrink**attempt */
+#define PK_CFL_FLEXIBLE 2  /* 'flexible': redo with dt_full * cfl_max/stepcfl (times dt_shrink from attempt 2) */
+
+/* cflcontrol, cfl_max, dt_shrink, max_redostep_attempts and
+ * disallow_negatives of &NUMERICAL_SETUP (global.F90:118-120,216-258;
+ * timestep.F90:73-90).  With PK_CFL_REDO or PK_CFL_FLEXIBLE, pk_run and
+ * pk_run_all save the state before every step and redo the step as
+ * repeat_fluidstep / check_cfl_violation / cfl_warn do
+ * (timestep_retry.F90:86-273, timestep.F90:247-361): when c_max of the new
+ * state over c_max of the old, times cfl and the current shrink factor,
+ * exceeds cfl_max (sanitized as global.F90:304 does), or when the floors met
+ * negative values and disallow_negatives is set.  Default PK_CFL_WARN.  The
+ * backups take one extra state per block. */
+int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t max_attempts,
+                       int32_t disallow_negatives);
+
+/* Steps redone by the last pk_run / pk_run_all (no counterpart in the reference;
+ * its log prints "Redoing previous step"). */
+int pk_redone_steps(int32_t *count);
+
 /* ---- the unsplit Riemann solver (RIEMANN_UNSPLIT) --------------------------- */
 
 /* One RK stage of the unsplit solver on a block periodic on itself:

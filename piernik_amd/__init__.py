@@ -91,6 +91,8 @@ _PROTOS = {
     "pk_fill_ghosts": [ctypes.c_int64],
     "pk_set_neighbour": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
     "pk_unsplit_stage": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
+    "pk_set_cfl_control": [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32],
+    "pk_redone_steps": [ctypes.POINTER(ctypes.c_int32)],
     "pk_fluid_update_unsplit": [ctypes.c_int64, ctypes.c_double],
     "pk_run_unsplit": [ctypes.c_int64, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                        ctypes.POINTER(ctypes.c_int32)],
@@ -379,6 +381,23 @@ def update_boundaries(cg):
     """Refresh of every ghost cell of the block on the device: periodic on
     itself, or from its neighbours (set_neighbour)."""
     _call("pk_fill_ghosts", cg.cg_id)
+
+
+CFL_CONTROL = {"warn": 0, "none": 0, "redo": 1, "repeat": 1, "flex": 2, "flexible": 2}
+
+
+def set_cfl_control(cflcontrol="warn", cfl_max=0.9, dt_shrink=0.5, max_redostep_attempts=10,
+                    disallow_negatives=True):
+    """&NUMERICAL_SETUP's step-retry knobs for run / run_all (timestep.F90:73-90)."""
+    _call("pk_set_cfl_control", CFL_CONTROL[cflcontrol], float(cfl_max), float(dt_shrink),
+          int(max_redostep_attempts), int(bool(disallow_negatives)))
+
+
+def redone_steps():
+    """How many steps the last run / run_all redid."""
+    n = ctypes.c_int32()
+    _call("pk_redone_steps", ctypes.byref(n))
+    return n.value
 
 
 def unsplit_stage(cg, istep, dt, chspeed=0.0):

@@ -62,6 +62,10 @@ struct Block {
     // unsplit stages (pk_unsplit_stage): interface fluxes of x, y, z, each
     // with the state's planes and strides
     double *d_flux = nullptr;
+    // the state at the start of a step that may be redone (cflcontrol 'redo')
+    double *backup = nullptr;
+    unsigned saved_flags = 0u;
+    double saved_dmass[2] = {0.0, 0.0};
 };
 
 const int kDtSlots = 1024;
@@ -598,6 +602,7 @@ void free_block(Block *b)
     if (b->d_dtout) (void)hipFree(b->d_dtout);
     if (b->d_dtslots) (void)hipFree(b->d_dtslots);
     if (b->d_flux) (void)hipFree(b->d_flux);
+    if (b->backup) (void)hipFree(b->backup);
     delete b;
 }
 
@@ -1637,6 +1642,158 @@ int fluid_update_unsplit(Block *b, double dt)
     return PK_OK;
 }
 
+// ---- step retries (cflcontrol = 'redo' / 'flexible') ----
+
+// timestep.F90:73-90 with repetitive_steps; the other knobs of &NUMERICAL_SETUP
+// (global.F90:233,237,245,258)
+struct CflControl {
+    int mode = PK_CFL_WARN;
+    double cfl_max = 0.9;
+    double dt_shrink = 0.5;
+    int max_attempts = 10;
+    int disallow_negatives = 1;
+};
+CflControl g_cflc;
+
+// cfl_max as init_global sanitizes it (global.F90:304)
+double cfl_max_sane()
+{
+    const double c = g_cfg.cfl;
+    return fmin(fmax(g_cflc.cfl_max, fmin(fmin(c * 1.1, c + 0.05), (1.0 + c) / 2.0)), 1.0);
+}
+
+// repeat_fluidstep's backup of every block (timestep_retry.F90:86-273), and
+// the floor bookkeeping so that a redone step is accounted once
+// (downgrade_magic_mass)
+int step_save(const std::vector<Block *> &bl)
+{
+    for (Block *b : bl) {
+        if (!b->backup) HIPCHK(hipMalloc((void **)&b->backup, state_bytes(b)));
+        HIPCHK(hipMemcpyAsync(b->backup, b->buf[b->cur], state_bytes(b), hipMemcpyDeviceToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(&b->saved_flags, b->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipMemcpyAsync(b->saved_dmass, b->d_dmass, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return PK_OK;
+}
+
+int step_restore(const std::vector<Block *> &bl)
+{
+    for (Block *b : bl) {
+        HIPCHK(hipMemcpyAsync(b->buf[b->cur], b->backup, state_bytes(b), hipMemcpyDeviceToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(b->d_dmass, b->saved_dmass, 2 * sizeof(double), hipMemcpyHostToDevice, g_stream));
+        b->dt_pending = false;
+        b->uh = -1;
+    }
+    list_touch();  // the list's pending dt and ghosts belong to the discarded step
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return PK_OK;
+}
+
+// this step's negative-value flags only: cleared before the step, read after
+// and merged into the accumulated report (check_cfl_violation resets
+// dn_negative / ei_negative every step, timestep.F90:291-300)
+int step_clear_flags(const std::vector<Block *> &bl)
+{
+    for (Block *b : bl) HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
+    return PK_OK;
+}
+
+int step_flags(const std::vector<Block *> &bl, unsigned *neg)
+{
+    std::vector<unsigned> f(bl.size(), 0u);
+    for (size_t i = 0; i < bl.size(); i++)
+        HIPCHK(hipMemcpyAsync(&f[i], bl[i]->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    *neg = 0u;
+    for (size_t i = 0; i < bl.size(); i++) {
+        *neg |= f[i];
+        const unsigned merged = f[i] | bl[i]->saved_flags;
+        HIPCHK(hipMemcpyAsync(bl[i]->d_flags, &merged, sizeof(unsigned), hipMemcpyHostToDevice, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+    }
+    return PK_OK;
+}
+
+// The main loop (piernik.F90:125-170) for nsteps steps: time_step with its
+// caps, one step, and with repetitive steps check_cfl_violation after it --
+// stepcfl = c_new/c_old*cfl*dt_cur_shrink > cfl_max, or negatives seen with
+// disallow_negatives, redoes the step from its saved state with
+// dt = dt_full*dt_cur_shrink (cfl_warn, timestep.F90:327-361; 'redo':
+// dt_shrink**attempt, 'flexible': cfl_max/stepcfl, times dt_shrink from the
+// second attempt).  A PIERNIK step is two substeps: t += 2 dt.
+struct StepLoop {
+    std::vector<Block *> blocks;
+    std::function<int(double *, double *)> timestep;
+    std::function<int(double)> update;
+};
+
+int run_loop(StepLoop &L, int nsteps, double *t_out, double *dt_out, int *redone)
+{
+    const bool redo = g_cflc.mode != PK_CFL_WARN;
+    const double cfl_max = cfl_max_sane();
+    double t = 0.0, dt = -1.0, dt_old = -1.0, dt_full = 0.0, c_old = 0.0, shrink = 1.0;
+    int s = 0, attempt = 0, nredo = 0;
+    bool repeat = false;
+    while (s < nsteps) {
+        if (!repeat) {
+            double c;
+            RCCHK(L.timestep(&dt, &c));
+            // time_step caps (timestep.F90:198-207) with dt_initial = -1, dt_max_grow = 2
+            if (s >= 1 && dt_old > 0.0) {
+                const double cap = dt_old * 2.0;
+                if (cap < dt) dt = cap;
+            }
+            dt_full = dt;
+            c_old = c;
+            if (redo) RCCHK(step_save(L.blocks));
+        } else {
+            RCCHK(step_restore(L.blocks));
+            dt = dt_full * shrink;
+        }
+        if (redo) RCCHK(step_clear_flags(L.blocks));
+        RCCHK(L.update(dt));
+        dt_old = dt;
+        if (redo) {
+            double dtn, c_new;
+            unsigned neg;
+            RCCHK(L.timestep(&dtn, &c_new));
+            RCCHK(step_flags(L.blocks, &neg));
+            double stepcfl = g_cfg.cfl * shrink;
+            if (c_old > 0.0) stepcfl = c_new / c_old * g_cfg.cfl * shrink;
+            repeat = stepcfl > cfl_max || (g_cflc.disallow_negatives && neg != 0u);
+            if (repeat) {
+                if (g_cflc.mode == PK_CFL_FLEXIBLE) {
+                    shrink = cfl_max / stepcfl;
+                    if (attempt >= 1) shrink = shrink * g_cflc.dt_shrink;
+                } else {
+                    double f = 1.0;
+                    for (int k = 0; k <= attempt; k++) f = f * g_cflc.dt_shrink;
+                    shrink = f;
+                }
+                attempt++;
+                nredo++;
+                if (attempt > g_cflc.max_attempts)
+                    return fail(PK_EINVAL, "step %d redone more than max_redostep_attempts = %d times", s,
+                                g_cflc.max_attempts);
+                continue;
+            }
+            shrink = 1.0;
+            attempt = 0;
+        }
+        t = t + dt;
+        t = t + dt;
+        s++;
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (t_out) *t_out = t;
+    if (dt_out) *dt_out = dt;
+    if (redone) *redone = nredo;
+    return PK_OK;
+}
+
+int g_last_redone = 0;
+
 // device-resident stepping of one block: only for blocks periodic on themselves
 int solo(const Block *b)
 {
@@ -1701,6 +1858,7 @@ int pk_init(const pk_config *cfg)
     }
     g_nplanes = p;
     g_mhd = g_magf >= 0;
+    g_cflc = CflControl();
     g_init = true;
     return PK_OK;
 }
@@ -1929,24 +2087,33 @@ int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out)
     RCCHK(solo(b));
     if (nsteps < 0) return fail(PK_EINVAL, "nsteps < 0");
     list_touch();
-    double t = 0.0, dt = -1.0, dt_old = -1.0;
     RCCHK(do_fill(b));
-    for (int s = 0; s < nsteps; s++) {
-        double c;
-        RCCHK(do_timestep(b, &dt, &c));
-        // time_step caps (timestep.F90:200-208) with dt_initial = -1, dt_max_grow = 2
-        if (s >= 1 && dt_old > 0.0) {
-            const double cap = dt_old * 2.0;
-            if (cap < dt) dt = cap;
-        }
-        RCCHK(do_fluid_update(b, dt));
-        t = t + dt;
-        t = t + dt;
-        dt_old = dt;
-    }
-    HIPCHK(hipStreamSynchronize(g_stream));
-    if (t_out) *t_out = t;
-    if (dt_out) *dt_out = dt;
+    StepLoop L;
+    L.blocks = {b};
+    L.timestep = [b](double *dt, double *c) { return do_timestep(b, dt, c); };
+    L.update = [b](double dt) { return do_fluid_update(b, dt); };
+    return run_loop(L, nsteps, t_out, dt_out, &g_last_redone);
+}
+
+int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t max_attempts,
+                       int32_t disallow_negatives)
+{
+    if (mode != PK_CFL_WARN && mode != PK_CFL_REDO && mode != PK_CFL_FLEXIBLE)
+        return fail(PK_EINVAL, "unknown cfl control mode %d", mode);
+    if (!(cfl_max > 0.0) || !(dt_shrink > 0.0 && dt_shrink < 1.0) || max_attempts < 0)
+        return fail(PK_EINVAL, "cfl_max > 0, 0 < dt_shrink < 1 and max_attempts >= 0 required");
+    g_cflc.mode = mode;
+    g_cflc.cfl_max = cfl_max;
+    g_cflc.dt_shrink = dt_shrink;
+    g_cflc.max_attempts = max_attempts;
+    g_cflc.disallow_negatives = disallow_negatives ? 1 : 0;
+    return PK_OK;
+}
+
+int pk_redone_steps(int32_t *count)
+{
+    if (!count) return fail(PK_EINVAL, "null output");
+    *count = g_last_redone;
     return PK_OK;
 }
 
@@ -2146,23 +2313,15 @@ int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out)
     RCCHK(check_mode(mode));
     if (nsteps < 0) return fail(PK_EINVAL, "nsteps < 0");
     RCCHK(list_build());
-    double t = 0.0, dt = -1.0, dt_old = -1.0;
-    for (int s = 0; s < nsteps; s++) {
-        double c;
-        RCCHK(list_dt(&dt, &c));
-        if (s >= 1 && dt_old > 0.0) {
-            const double cap = dt_old * 2.0;
-            if (cap < dt) dt = cap;
-        }
-        RCCHK(list_step(mode, dt));
-        t = t + dt;
-        t = t + dt;
-        dt_old = dt;
-    }
-    HIPCHK(hipStreamSynchronize(g_stream));
-    if (t_out) *t_out = t;
-    if (dt_out) *dt_out = dt;
-    return PK_OK;
+    StepLoop L;
+    L.blocks = g_list.blocks;
+    L.timestep = [](double *dt, double *c) { return list_dt(dt, c); };
+    L.update = [mode](double dt) {
+        // a redone step starts from restored states: ghosts and dt are recomputed
+        if (!g_list.fresh) RCCHK(list_fill());
+        return list_step(mode, dt);
+    };
+    return run_loop(L, nsteps, t_out, dt_out, &g_last_redone);
 }
 
 int pk_autotune_all(double dt, double chspeed)

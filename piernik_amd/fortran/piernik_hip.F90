@@ -27,7 +27,7 @@ module piernik_hip
         &    pk_fluid_update_adjoint, pk_timestep_adjoint, pk_ensemble_ids, pk_ensemble_step, pk_ensemble_run, &
         &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
         &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint, &
-        &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit
+        &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit, pk_set_cfl_control, pk_redone_steps
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
 
@@ -146,6 +146,18 @@ module piernik_hip
          import :: c_int32_t, c_int64_t
          integer(c_int64_t), value :: cg_id
       end function pk_fill_ghosts
+
+      integer(c_int32_t) function pk_set_cfl_control(mode, cfl_max, dt_shrink, max_attempts, disallow_negatives) &
+           &                                         bind(C, name="pk_set_cfl_control")
+         import :: c_int32_t, c_double
+         integer(c_int32_t), value :: mode, max_attempts, disallow_negatives
+         real(c_double),     value :: cfl_max, dt_shrink
+      end function pk_set_cfl_control
+
+      integer(c_int32_t) function pk_redone_steps(count) bind(C, name="pk_redone_steps")
+         import :: c_int32_t
+         integer(c_int32_t), intent(out) :: count
+      end function pk_redone_steps
 
       integer(c_int32_t) function pk_unsplit_stage(cg_id, istep, dt, chspeed) bind(C, name="pk_unsplit_stage")
          import :: c_int32_t, c_int64_t, c_double

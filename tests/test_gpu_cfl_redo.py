@@ -1,0 +1,103 @@
+"""Step retries (cflcontrol = 'redo' / 'flexible', pk_set_cfl_control) on the
+GPU against the reference's control flow driving the oracle.
+
+The reference's main loop (piernik.F90:125-170) saves the state before each
+step (repeat_fluidstep, timestep_retry.F90:86-273), and after the step
+check_cfl_violation (timestep.F90:247-310) recomputes the timestep of the
+new state; cfl_warn (:317-361) flags a redo when c_max(new)/c_max(old) * cfl
+* dt_cur_shrink exceeds cfl_max, and the step is redone from the saved state
+with dt = dt_full * dt_cur_shrink.  The Sedov blast at CFL 0.7 has a
+stepcfl of 0.859 in its first step: with cfl_max = 0.8 that step is redone
+once.  Everything else must stay bit-exact.
+"""
+import numpy as np
+import pytest
+
+import piernik_amd as pk
+from piernik_amd import setups
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_run_redo(po, p, ob, nsteps, mode, cfl_max, dt_shrink=0.5):
+    """The reference's main loop with repetitive steps, over the oracle."""
+    c = p.cfl
+    cmax = min(max(cfl_max, min(c * 1.1, c + 0.05, (1.0 + c) / 2.0)), 1.0)  # global.F90:304
+    t, dt_old, dt_full, c_old, shrink = 0.0, -1.0, 0.0, 0.0, 1.0
+    s, attempt, redone, repeat = 0, 0, 0, False
+    po.fill_ghosts(p, ob)
+    saved = None
+    while s < nsteps:
+        if not repeat:
+            dt, c_old = po.time_step(p, ob, s, dt_old)
+            dt_full = dt
+            saved = [a.copy() for a in (ob.u, ob.b, ob.psi) if a is not None]
+            mm = list(ob.rep.magic_mass)
+            neg0 = (ob.rep.dn_negative, ob.rep.ei_negative)
+        else:
+            for a, sv in zip([a for a in (ob.u, ob.b, ob.psi) if a is not None], saved):
+                a[...] = sv
+            ob.rep.magic_mass[0], ob.rep.magic_mass[1] = mm
+            dt = dt_full * shrink
+        ob.rep.dn_negative = ob.rep.ei_negative = 0
+        po.fluid_update(p, ob, dt)
+        dt_old = dt
+        _, c_new = po.time_step(p, ob, s + 1, dt_old)
+        neg = ob.rep.dn_negative or ob.rep.ei_negative
+        ob.rep.dn_negative, ob.rep.ei_negative = neg0[0] | ob.rep.dn_negative, neg0[1] | ob.rep.ei_negative
+        stepcfl = p.cfl * shrink
+        if c_old > 0.0:
+            stepcfl = c_new / c_old * p.cfl * shrink
+        repeat = stepcfl > cmax or bool(neg)
+        if repeat:
+            if mode == "flexible":
+                shrink = cmax / stepcfl
+                if attempt >= 1:
+                    shrink = shrink * dt_shrink
+            else:
+                shrink = dt_shrink ** (attempt + 1)
+            attempt += 1
+            redone += 1
+            continue
+        shrink, attempt = 1.0, 0
+        t = t + dt
+        t = t + dt
+        s += 1
+    return t, redone
+
+
+@pytest.mark.parametrize("mode,driver", [("redo", "block"), ("flexible", "block"), ("redo", "list")])
+def test_redo_matches_reference_control_flow(oracle, mode, driver):
+    cg, cfg = setups.sedov_hd(n=24)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    t_o, redone_o = oracle_run_redo(oracle, p, ob, 6, mode, 0.8)
+    assert redone_o >= 1
+    print(f"cflcontrol {mode} via {driver}: oracle redid {redone_o} step(s)", flush=True)
+    pk.load_library()
+    pk.finalize()
+    pk.init(cfg)
+    pk.set_cfl_control(mode, cfl_max=0.8)
+    cg.upload()
+    if driver == "block":
+        t_g, _ = pk.run(cg, 6)
+    else:
+        t_g, _ = pk.run_all(6, pk.BATCH_GRAPH)
+    assert pk.redone_steps() == redone_o
+    assert t_g == t_o
+    cg.download()
+    bad = np.argwhere(~(cg.u == ob.u))
+    assert bad.size == 0, f"{len(bad)} mismatches, first {tuple(bad[0])}"
+    pk.finalize()
+
+
+def test_warn_never_redoes():
+    cg, cfg = setups.sedov_hd(n=16)
+    pk.load_library()
+    pk.finalize()
+    pk.init(cfg)
+    pk.set_cfl_control("warn", cfl_max=0.8)
+    cg.upload()
+    pk.run(cg, 4)
+    assert pk.redone_steps() == 0
+    pk.finalize()
