@@ -4,6 +4,8 @@
  * Restates, for one grid block ("cg") with periodic boundaries:
  *   solve_cg_riemann / solve_cg_ub / solve_cg_u
  *                       src/scheme/Riemann_Solver/solve_cg_riemann.F90:68-307
+ *   set_fluxpointers / save_outfluxes (fine/coarse flux points)
+ *                       src/grid/grid_container_fcflx.F90:148-235
  *   get_sweep + sweep permutations
  *                       src/base/named_array.F90:327-348,
  *                       src/fluids/fluidtypes.F90:318-320, fluidindex.F90:190-192
@@ -65,6 +67,47 @@ static int first_stage(const or_params *p) { return p->integration_order == 1 ? 
 void or_solve_cg(const or_params *p, or_block *cg, int ddim, int istep,
                  double dt, double chspeed, or_report *rep)
 {
+    or_solve_cg_fc(p, cg, ddim, istep, dt, chspeed, rep, NULL);
+}
+
+/*
+ * set_fluxpointers for one pencil (grid_container_fcflx.F90:148-209, the
+ * cartesian case): a side is active where its index lies inside the
+ * interior, ijkse(ddim, LO) for LO sides and ijkse(ddim, HI) for HI sides.
+ * With c the pencil's local cell of that index, li replaces the flux at
+ * c+1/2, ri at c-1/2, lo saves the flux at c-1/2 and ro at c+1/2.
+ */
+static void fc_pencil(const or_params *p, const or_block *cg, int ddim, const or_fcflx *fc, int o1, int o2,
+                      or_peflx *e)
+{
+    static const int ortho[3][2] = {{1, 2}, {2, 0}, {0, 1}}; /* pdims(:, ORTHO1:ORTHO2) */
+    const int n1 = cg->n[ortho[ddim][0]];
+    const size_t at = (size_t)o1 + (size_t)n1 * o2;
+    const int g = gh(p, cg, ddim), n = cg->n[ddim];
+    const int mhd = any_mag(p);
+    for (int k = 0; k < 4; k++) {
+        const or_fcface *f = (k < 2) ? &fc->fine[k] : &fc->coarse[k - 2];
+        const int side = k & 1; /* 0 LO, 1 HI */
+        e->row[k] = -1;
+        e->u[k] = e->b[k] = NULL;
+        if (!f->index) continue;
+        const int64_t idx = f->index[at];
+        const int active = side == 0 ? idx >= fc->first : idx <= fc->first + n - 1;
+        if (!active) continue;
+        const int c = (int)(idx - fc->first) + g;
+        const int right = (k == 0 || k == 3); /* li and ro act on the right face of c */
+        e->row[k] = right ? c : c - 1;
+        e->u[k] = f->uflx + at * (size_t)p->nvar;
+        e->b[k] = mhd ? f->bflx + at * 4 : NULL;
+    }
+}
+
+/* or_solve_cg with the block's fine/coarse flux arrays (fc may be NULL):
+ * the pencils' external fluxes as set_fluxpointers / save_outfluxes give
+ * them (solve_cg_riemann.F90:201,224,289,300). */
+void or_solve_cg_fc(const or_params *p, or_block *cg, int ddim, int istep,
+                    double dt, double chspeed, or_report *rep, const or_fcflx *fc)
+{
     int ng[3];
     gdims(p, cg, ng);
     int nv = p->nvar, mhd = any_mag(p);
@@ -113,7 +156,15 @@ void or_solve_cg(const or_params *p, or_block *cg, int ddim, int istep,
                     b[(size_t)3 * n + i] = cg->psi[cell];
                 }
             }
-            or_pencil_stage(p, n, dtodx, chspeed, u0, u, b0, b, u1, b1, nbp, dvol, rep);
+            or_peflx e, *pe = NULL;
+            if (fc) {
+                const int i1 = c1 - gh(p, cg, t1), i2 = c2 - gh(p, cg, t2);
+                /* (ORTHO1, ORTHO2) is (t1, t2) for x and z sweeps, (t2, t1) for y */
+                if (ddim == 1) fc_pencil(p, cg, ddim, fc, i2, i1, &e);
+                else fc_pencil(p, cg, ddim, fc, i1, i2, &e);
+                pe = &e;
+            }
+            or_pencil_stage_ext(p, n, dtodx, chspeed, u0, u, b0, b, u1, b1, nbp, dvol, rep, pe);
             for (int i = 0; i < n; i++) {
                 size_t cell = base + (size_t)i * sd[ddim];
                 for (int f = 0; f < p->nfluids; f++) {

@@ -63,6 +63,34 @@ typedef struct or_block {
     double *psi, *psih;         /* (nxg, nyg, nzg) or NULL */
 } or_block;
 
+/* External fluxes of one pencil (ext_fluxes, fluxtypes.F90:44-51): flux
+ * row (interface i+1/2 at row i) of li, ri (replaced by a finer neighbour's
+ * fluxes) and lo, ro (saved for a coarser neighbour), -1 where absent;
+ * u[k] holds nvar u-fluxes in sweep order, b[k] the 4 [B, psi] fluxes. */
+typedef struct or_peflx {
+    int32_t row[4];
+    double *u[4];
+    double *b[4];
+} or_peflx;
+
+/* One side of cg%finebnd(ddim, side) or cg%coarsebnd(ddim, side)
+ * (flx_arr.F90:38-42) in the reference layout over the block's interior
+ * pencils, (i1, i2) = the two directions pdims(ddim, ORTHO1:ORTHO2)
+ * (constants.F90:127-128: x -> (y, z), y -> (z, x), z -> (x, y)):
+ * index(n1, n2), uflx(nvar, n1, n2), bflx(4, n1, n2) or NULL.  index NULL:
+ * the side has no fine/coarse face. */
+typedef struct or_fcface {
+    const int32_t *index;
+    double *uflx;
+    double *bflx;
+} or_fcface;
+
+typedef struct or_fcflx {
+    int64_t first;       /* cg%ijkse(ddim, LO): the index value of the first interior cell */
+    or_fcface fine[2];   /* finebnd(ddim, LO:HI) */
+    or_fcface coarse[2]; /* coarsebnd(ddim, LO:HI) */
+} or_fcflx;
+
 /* ---- scalar helpers ---------------------------------------------------- */
 double or_spacing(double x);
 
@@ -80,12 +108,19 @@ void or_pencil_stage(const or_params *p, int n, double dtodx, double chspeed,
                      const double *b, double *u1, double *b1, int nbp, double dvol,
                      or_report *rep);
 
+void or_pencil_stage_ext(const or_params *p, int n, double dtodx, double chspeed,
+                         const double *u0, const double *u, const double *b0,
+                         const double *b, double *u1, double *b1, int nbp, double dvol,
+                         or_report *rep, const or_peflx *e);
+
 void or_pencil_flux(const or_params *p, int n, double chspeed, const double *u, const double *b, double *flx,
                     double *bflx);
 
 /* ---- block-level routines ---------------------------------------------- */
 void or_solve_cg(const or_params *p, or_block *cg, int ddim, int istep,
                  double dt, double chspeed, or_report *rep);
+void or_solve_cg_fc(const or_params *p, or_block *cg, int ddim, int istep,
+                    double dt, double chspeed, or_report *rep, const or_fcflx *fc);
 void or_fill_ghosts_periodic(const or_params *p, or_block *cg);
 double or_update_chspeed(const or_params *p, const or_block *cg, double dt);
 double or_glm_factor(const or_params *p, const or_block *cg, double chspeed, double dt);

@@ -46,6 +46,14 @@ class Block(ctypes.Structure):
         (nm, ctypes.c_void_p) for nm in ("u", "uh", "b", "bh", "psi", "psih")]
 
 
+class FcFace(ctypes.Structure):
+    _fields_ = [("index", ctypes.c_void_p), ("uflx", ctypes.c_void_p), ("bflx", ctypes.c_void_p)]
+
+
+class FcFlx(ctypes.Structure):
+    _fields_ = [("first", ctypes.c_int64), ("fine", FcFace * 2), ("coarse", FcFace * 2)]
+
+
 _lib = None
 P = ctypes.POINTER
 
@@ -69,6 +77,9 @@ def lib():
                                       dp, dp, ctypes.c_int, ctypes.c_double, P(Report)]
         L.or_solve_cg.argtypes = [P(Params), P(Block), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_double, P(Report)]
+        L.or_pencil_flux.argtypes = [P(Params), ctypes.c_int, ctypes.c_double, dp, dp, dp, dp]
+        L.or_solve_cg_fc.argtypes = [P(Params), P(Block), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, P(Report), P(FcFlx)]
         L.or_fill_ghosts_periodic.argtypes = [P(Params), P(Block)]
         L.or_update_chspeed.argtypes = [P(Params), P(Block), ctypes.c_double]
         L.or_update_chspeed.restype = ctypes.c_double
@@ -173,9 +184,64 @@ class OBlock:
         return blk
 
 
-def solve_cg(p, ob, ddim, istep, dt, chspeed=0.0):
+# pdims(ddim, ORTHO1:ORTHO2) (constants.F90:127-128)
+ORTHO = ((1, 2), (2, 0), (0, 1))
+
+
+class FcFaces:
+    """cg%finebnd(ddim, LO:HI) and cg%coarsebnd(ddim, LO:HI) of one block and
+    direction, in the reference layout (flx_arr.F90:38-42): index(n1, n2),
+    uflx(nvar, n1, n2), bflx(4, n1, n2), Fortran order, (n1, n2) the interior
+    extents along pdims(ddim, ORTHO1:ORTHO2).  `first` is cg%ijkse(ddim, LO);
+    an index below it (LO) or above first + n - 1 (HI) marks no face, as
+    init_gc_fcflx leaves them (grid_container_fcflx.F90:60-65)."""
+
+    def __init__(self, n, ddim, nvar, magnetized, first=0):
+        self.ddim, self.first = ddim, first
+        n1, n2 = n[ORTHO[ddim][0]], n[ORTHO[ddim][1]]
+        none = (first - 1, first + n[ddim])  # outside the interior on each side
+        self.index = {(k, s): np.full((n1, n2), none[s], dtype=np.int32, order="F")
+                      for k in ("fine", "coarse") for s in (0, 1)}
+        self.uflx = {(k, s): np.zeros((nvar, n1, n2), order="F") for k in ("fine", "coarse") for s in (0, 1)}
+        self.bflx = {(k, s): (np.zeros((4, n1, n2), order="F") if magnetized else None)
+                     for k in ("fine", "coarse") for s in (0, 1)}
+
+    def c(self):
+        fc = FcFlx()
+        fc.first = self.first
+        for kind, arr in (("fine", fc.fine), ("coarse", fc.coarse)):
+            for s in (0, 1):
+                arr[s].index = self.index[kind, s].ctypes.data
+                arr[s].uflx = self.uflx[kind, s].ctypes.data
+                b = self.bflx[kind, s]
+                arr[s].bflx = b.ctypes.data if b is not None else None
+        return fc
+
+
+def solve_cg(p, ob, ddim, istep, dt, chspeed=0.0, fc=None):
     blk = ob.c()
-    lib().or_solve_cg(ctypes.byref(p), ctypes.byref(blk), ddim, istep, dt, chspeed, ctypes.byref(ob.rep))
+    if fc is None:
+        lib().or_solve_cg(ctypes.byref(p), ctypes.byref(blk), ddim, istep, dt, chspeed, ctypes.byref(ob.rep))
+    else:
+        assert fc.ddim == ddim
+        f = fc.c()
+        lib().or_solve_cg_fc(ctypes.byref(p), ctypes.byref(blk), ddim, istep, dt, chspeed, ctypes.byref(ob.rep),
+                             ctypes.byref(f))
+
+
+def pencil_flux(p, u, b=None, chspeed=0.0):
+    """Interface fluxes of one pencil: u (n, nvar) and b (n, 4) in sweep
+    order; returns flx (n-1, nvar), bflx (n-1, 4) (row i: interface i+1/2)."""
+    dp = P(ctypes.c_double)
+    u = np.asfortranarray(u, dtype=np.float64)
+    n, nv = u.shape
+    flx = np.zeros((n - 1, nv), order="F")
+    bflx = np.zeros((n - 1, 4), order="F")
+    bb = np.asfortranarray(b, dtype=np.float64) if b is not None else None
+    lib().or_pencil_flux(ctypes.byref(p), n, chspeed, u.ctypes.data_as(dp),
+                         bb.ctypes.data_as(dp) if bb is not None else None, flx.ctypes.data_as(dp),
+                         bflx.ctypes.data_as(dp))
+    return flx, (bflx if b is not None else None)
 
 
 def fill_ghosts(p, ob):

@@ -541,6 +541,21 @@ void or_pencil_stage(const or_params *p, int n, double dtodx, double chspeed,
                      const double *b, double *u1, double *b1, int nbp, double dvol,
                      or_report *rep)
 {
+    or_pencil_stage_ext(p, n, dtodx, chspeed, u0, u, b0, b, u1, b1, nbp, dvol, rep, NULL);
+}
+
+/*
+ * or_pencil_stage with the pencil's external fluxes (ext_fluxes,
+ * fluxtypes.F90:44-51; e may be NULL): after riemann_wrap the flux rows li
+ * and ri are replaced by the fluxes a finer neighbour sent, then rows lo and
+ * ro are saved for a coarser neighbour, u-fluxes first, then with HDC the
+ * [B, psi] fluxes (solve_cg_riemann.F90:353-363, solve_u :405-408).
+ */
+void or_pencil_stage_ext(const or_params *p, int n, double dtodx, double chspeed,
+                         const double *u0, const double *u, const double *b0,
+                         const double *b, double *u1, double *b1, int nbp, double dvol,
+                         or_report *rep, const or_peflx *e)
+{
     int nv = p->nvar, m = n - 1;
     int mhd = any_magnetized(p);
     memcpy(u1, u, sizeof(double) * (size_t)n * nv);
@@ -568,6 +583,25 @@ void or_pencil_stage(const or_params *p, int n, double dtodx, double chspeed,
                     flx + (size_t)o * m, bflx);
         else
             or_hllc(m, ql + (size_t)o * m, qr + (size_t)o * m, p->fl[f].gam, flx + (size_t)o * m);
+    }
+    if (e) {
+        for (int k = 0; k < 4; k++) {
+            int r = e->row[k];
+            if (r < 0) continue;
+            for (int v = 0; v < nv; v++) {
+                if (k < 2) flx[(size_t)v * m + r] = e->u[k][v];
+                else e->u[k][v] = flx[(size_t)v * m + r];
+            }
+        }
+        if (mhd)
+            for (int k = 0; k < 4; k++) {
+                int r = e->row[k];
+                if (r < 0) continue;
+                for (int v = 0; v < 4; v++) {
+                    if (k < 2) bflx[(size_t)v * m + r] = e->b[k][v];
+                    else e->b[k][v] = bflx[(size_t)v * m + r];
+                }
+            }
     }
     /* solve_cg_riemann.F90:366,368,411 */
     for (int v = 0; v < nv; v++)
