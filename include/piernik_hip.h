@@ -133,6 +133,30 @@ int pk_download_device(int64_t cg_id, double *u, double *b, double *psi);
  * the GLM speed of hdc.F90:90-140. */
 int pk_solve_cg(int64_t cg_id, int32_t ddim, int32_t istep, double dt, double chspeed);
 
+/* Fine/coarse flux points of pk_solve_cg on a refined level: one side of
+ * cg%finebnd(ddim, side) (kind PK_FC_FINEBND: fluxes a finer neighbour sent,
+ * which replace the block's flux at their face) or cg%coarsebnd(ddim, side)
+ * (PK_FC_COARSEBND: the block's own flux at a face bordering a coarser
+ * neighbour, saved for it), as set_fluxpointers and save_outfluxes use them
+ * (grid_container_fcflx.F90:148-235; flx_arr.F90:38-42 for the arrays;
+ * applied in solve, solve_cg_riemann.F90:353-363 and :405-408).  The host
+ * arrays stay the caller's, in the reference layout over the block's
+ * interior pencils, (i1, i2) along pdims(ddim, ORTHO1:ORTHO2)
+ * (constants.F90:127-128: x -> (y, z), y -> (z, x), z -> (x, y)):
+ * index(n1, n2) (int32, the reference's cell index), uflx(nvar, n1, n2) and,
+ * with a magnetic field, bflx(4, n1, n2) = [B, psi] fluxes in sweep order.
+ * first is cg%ijkse(ddim, LO), the index of the block's first interior cell:
+ * a LO side is active on a pencil where index >= first, a HI side where
+ * index <= first + n - 1.  Each pk_solve_cg along ddim reads the finebnd
+ * arrays and, on return, has written the coarsebnd fluxes of the active
+ * pencils.  index == NULL removes the side.  The fused and batched paths
+ * (pk_sweep, pk_fluid_update, pk_run, the block-list and unsplit functions)
+ * refuse a block with flux points. */
+#define PK_FC_FINEBND 0
+#define PK_FC_COARSEBND 1
+int pk_register_fcflx(int64_t cg_id, int32_t ddim, int32_t side, int32_t kind, int64_t first,
+                      const int32_t *index, double *uflx, double *bflx);
+
 /* Both RK2 stages of one directional sweep in one pass (requires nb >= 4
  * and ghost cells consistent with the neighbours along ddim, as after
  * update_boundaries).  psi_fac multiplies psi on output (GLM damping fused

@@ -90,6 +90,8 @@ _PROTOS = {
     "pk_sweep": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
     "pk_fill_ghosts": [ctypes.c_int64],
     "pk_set_neighbour": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
+    "pk_register_fcflx": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "pk_unsplit_stage": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
     "pk_set_cfl_control": [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32],
     "pk_redone_steps": [ctypes.POINTER(ctypes.c_int32)],
@@ -417,6 +419,38 @@ def run_unsplit(cg, tend, nmax=1 << 30):
     t, ns = ctypes.c_double(), ctypes.c_int32()
     _call("pk_run_unsplit", cg.cg_id, float(tend), int(nmax), ctypes.byref(t), ctypes.byref(ns))
     return t.value, ns.value
+
+
+FC_FINEBND, FC_COARSEBND = 0, 1
+
+
+def register_fcflx(cg, ddim, side, kind, first, index, uflx, bflx=None):
+    """Fine/coarse flux points of cg along ddim (pk_register_fcflx): one side
+    of cg%finebnd (kind FC_FINEBND) or cg%coarsebnd (FC_COARSEBND) in the
+    reference layout, index (n1, n2) int32, uflx (nvar, n1, n2), bflx (4, n1,
+    n2), Fortran order over pdims(ddim, ORTHO1:ORTHO2); first is
+    cg%ijkse(ddim, LO).  pk_solve_cg reads finebnd and writes coarsebnd in
+    place, so the arrays are kept referenced here.  index None removes."""
+    cg.register()
+    refs = cg.__dict__.setdefault("_fc_refs", {})
+    if index is None:
+        _call("pk_register_fcflx", cg.cg_id, int(ddim), int(side), int(kind), int(first), None, None, None)
+        refs.pop((ddim, side, kind), None)
+        return
+    d1, d2 = ((1, 2), (2, 0), (0, 1))[ddim]
+    n1, n2 = cg.n[d1], cg.n[d2]
+    ok = (index.dtype == np.int32 and index.shape == (n1, n2) and index.flags.f_contiguous
+          and uflx.dtype == np.float64 and uflx.shape == (cg.nvar, n1, n2) and uflx.flags.f_contiguous)
+    if cg.magnetized:
+        ok = ok and bflx is not None and bflx.dtype == np.float64 and bflx.shape == (4, n1, n2) \
+            and bflx.flags.f_contiguous
+    if not ok:
+        raise ValueError("fine/coarse flux arrays must be Fortran-ordered index (n1, n2) int32, "
+                         f"uflx ({cg.nvar}, n1, n2) and bflx (4, n1, n2) float64 with (n1, n2) = {(n1, n2)}")
+    ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    _call("pk_register_fcflx", cg.cg_id, int(ddim), int(side), int(kind), int(first), ptr(index), ptr(uflx),
+          ptr(bflx if cg.magnetized else None))
+    refs[(ddim, side, kind)] = (index, uflx, bflx)
 
 
 def set_neighbour(cg, direction, side, nb):

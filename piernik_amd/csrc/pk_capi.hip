@@ -66,6 +66,17 @@ struct Block {
     double *backup = nullptr;
     unsigned saved_flags = 0u;
     double saved_dmass[2] = {0.0, 0.0};
+    // fine/coarse flux points (pk_register_fcflx): host arrays per direction,
+    // kind (0 finebnd, 1 coarsebnd) and side, and their device staging
+    struct FcReg {
+        const int32_t *index = nullptr;
+        double *uflx = nullptr, *bflx = nullptr;
+        int64_t first = 0;
+    } fc[3][2][2];
+    int *d_fc_at[3] = {nullptr, nullptr, nullptr};
+    double *d_fc_in[3] = {nullptr, nullptr, nullptr}, *d_fc_out[3] = {nullptr, nullptr, nullptr};
+    std::vector<int> h_fc_at;
+    std::vector<double> h_fc_in, h_fc_out;
 };
 
 const int kDtSlots = 1024;
@@ -353,6 +364,136 @@ int copy_transverse_ghosts(Block *b, int d, const double *src, double *dst)
     return PK_OK;
 }
 
+// ---- fine/coarse flux points (set_fluxpointers / save_outfluxes,
+// grid_container_fcflx.F90:148-235) ------------------------------------
+
+bool has_fc(const Block *b)
+{
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k < 2; k++)
+            for (int s = 0; s < 2; s++)
+                if (b->fc[d][k][s].index) return true;
+    return false;
+}
+
+int no_fc(const Block *b, const char *what)
+{
+    if (has_fc(b))
+        return fail(PK_EINVAL, "block %lld has fine/coarse flux points, which only pk_solve_cg applies (%s)",
+                    (long long)b->id, what);
+    return PK_OK;
+}
+
+// u-fluxes of every fluid (flind%all), then [B, psi] with a magnetic field
+int fc_rec() { return 5 * g_nfl + (g_mhd ? 4 : 0); }
+
+// Kernel pencil pf = i1 + n1*i2 over (d1, d2) -> position in the reference's
+// face arrays over (pdims(d, ORTHO1), pdims(d, ORTHO2)) (constants.F90:127-128):
+// the same pair for x and z sweeps, (z, x) for y sweeps.
+long long fc_ref_at(int d, int i1, int i2, int n1, int n2)
+{
+    return d == 1 ? (long long)i2 + (long long)n2 * i1 : (long long)i1 + (long long)n1 * i2;
+}
+
+// Stage the points of direction d for the kernels: per pencil the cell whose
+// right-face flux each point concerns (li: c, ri: c-1, lo: c-1, ro: c for the
+// local cell c of its index, active where set_fluxpointers finds the index
+// inside the interior) and the replacement fluxes.  *active: some point is.
+int fc_begin(Block *b, int d, bool *active)
+{
+    *active = false;
+    bool reg = false;
+    for (int k = 0; k < 2; k++)
+        for (int s = 0; s < 2; s++) reg = reg || b->fc[d][k][s].index != nullptr;
+    if (!reg) return PK_OK;
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    const int n1 = b->n[d1], n2 = b->n[d2];
+    const long long np = (long long)n1 * n2;
+    const int rec = fc_rec(), nvar = 5 * g_nfl;
+    const int g = b->g[d], n = b->n[d], ng = b->ng[d];
+    HIPCHK(hipStreamSynchronize(g_stream));  // the previous call's copies from the host buffers are done
+    b->h_fc_at.assign((size_t)(4 * np), pk::kFcNone);
+    b->h_fc_in.assign((size_t)(2 * np * rec), 0.0);
+    bool any = false;
+    for (int kind = 0; kind < 2; kind++)
+        for (int side = 0; side < 2; side++) {
+            const Block::FcReg &r = b->fc[d][kind][side];
+            if (!r.index) continue;
+            const int k = 2 * kind + side;  // 0 li, 1 ri, 2 lo, 3 ro
+            for (int i2 = 0; i2 < n2; i2++)
+                for (int i1 = 0; i1 < n1; i1++) {
+                    const long long pf = i1 + (long long)n1 * i2, at = fc_ref_at(d, i1, i2, n1, n2);
+                    const long long idx = r.index[at];
+                    const bool act = (side == 0) ? idx >= r.first : idx <= r.first + n - 1;
+                    if (!act) continue;
+                    const long long c = idx - r.first + g;
+                    const long long cs = (k == 0 || k == 3) ? c : c - 1;
+                    if (cs < 0 || cs > ng - 2)
+                        return fail(PK_EINVAL, "block %lld: fine/coarse face at index %lld along %d is outside the pencil",
+                                    (long long)b->id, idx, d);
+                    b->h_fc_at[(size_t)(k * np + pf)] = (int)cs;
+                    any = true;
+                    if (kind == 1) continue;
+                    double *dst = &b->h_fc_in[(size_t)((side * np + pf) * rec)];
+                    for (int v = 0; v < nvar; v++) dst[v] = r.uflx[at * nvar + v];
+                    if (g_mhd)
+                        for (int v = 0; v < 4; v++) dst[nvar + v] = r.bflx[at * 4 + v];
+                }
+        }
+    if (!any) return PK_OK;
+    if (!b->d_fc_at[d]) {
+        HIPCHK(hipMalloc((void **)&b->d_fc_at[d], sizeof(int) * 4 * np));
+        HIPCHK(hipMalloc((void **)&b->d_fc_in[d], sizeof(double) * 2 * np * rec));
+        HIPCHK(hipMalloc((void **)&b->d_fc_out[d], sizeof(double) * 2 * np * rec));
+    }
+    HIPCHK(hipMemcpyAsync(b->d_fc_at[d], b->h_fc_at.data(), sizeof(int) * 4 * np, hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(b->d_fc_in[d], b->h_fc_in.data(), sizeof(double) * 2 * np * rec, hipMemcpyHostToDevice,
+                          g_stream));
+    *active = true;
+    return PK_OK;
+}
+
+void fc_args(const Block *b, int d, int f, pk::SweepArgs &a)
+{
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    a.fc_at = b->d_fc_at[d];
+    a.fc_in = b->d_fc_in[d];
+    a.fc_out = b->d_fc_out[d];
+    a.fc_npen = b->n[d1] * b->n[d2];
+    a.fc_rec = fc_rec();
+    a.fc_beg = 5 * f;  // flind order: fluid f's u-fluxes at 5f..5f+4
+    a.fc_boff = 5 * g_nfl;
+}
+
+// save_outfluxes: the saved fluxes into the registered coarsebnd arrays, at
+// the pencils whose point is active.
+int fc_end(Block *b, int d)
+{
+    if (!b->fc[d][1][0].index && !b->fc[d][1][1].index) return PK_OK;
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    const int n1 = b->n[d1], n2 = b->n[d2];
+    const long long np = (long long)n1 * n2;
+    const int rec = fc_rec(), nvar = 5 * g_nfl;
+    b->h_fc_out.resize((size_t)(2 * np * rec));
+    HIPCHK(hipMemcpyAsync(b->h_fc_out.data(), b->d_fc_out[d], sizeof(double) * 2 * np * rec, hipMemcpyDeviceToHost,
+                          g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (int side = 0; side < 2; side++) {
+        const Block::FcReg &r = b->fc[d][1][side];
+        if (!r.index) continue;
+        for (int i2 = 0; i2 < n2; i2++)
+            for (int i1 = 0; i1 < n1; i1++) {
+                const long long pf = i1 + (long long)n1 * i2, at = fc_ref_at(d, i1, i2, n1, n2);
+                if (b->h_fc_at[(size_t)((2 + side) * np + pf)] == pk::kFcNone) continue;
+                const double *src = &b->h_fc_out[(size_t)((side * np + pf) * rec)];
+                for (int v = 0; v < nvar; v++) r.uflx[at * nvar + v] = src[v];
+                if (g_mhd)
+                    for (int v = 0; v < 4; v++) r.bflx[at * 4 + v] = src[nvar + v];
+            }
+    }
+    return PK_OK;
+}
+
 int do_stage(Block *b, int ddim, int istep, double dt, double ch)
 {
     b->dt_pending = false;
@@ -360,6 +501,8 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
     if (!first && (b->uh < 0 || b->uh == b->cur))
         return fail(PK_ESTATE, "stage %d called without a preceding first stage", istep);
     const int src_i = b->cur, dst_i = first ? 1 - b->cur : b->uh;
+    bool fc = false;
+    RCCHK(fc_begin(b, ddim, &fc));
     for (int f = 0; f < g_nfl; f++) {
         pk::SweepArgs a{};
         fill_common(b, f, a);
@@ -369,9 +512,11 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
         a.src = fluid_state(b, src_i, f);
         a.base = first ? a.src : fluid_state(b, dst_i, f);
         a.dst = fluid_state(b, dst_i, f);
+        if (fc) fc_args(b, ddim, f, a);
         RCCHK(launch(ddim, false, a, b, f));
     }
     RCCHK(copy_transverse_ghosts(b, ddim, state(b, src_i), state(b, dst_i)));
+    if (fc) RCCHK(fc_end(b, ddim));
     if (first) {
         b->uh = b->cur;
         b->cur = 1 - b->cur;
@@ -390,6 +535,7 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
 int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool images = false, bool want_dt = false,
              double *src = nullptr, double *dst = nullptr)
 {
+    RCCHK(no_fc(b, "a whole sweep runs both stages in one pass"));
     if (g_cfg.integration_order == 1) {
         RCCHK(do_stage(b, ddim, PK_EULER, dt, ch));
         if (psi_fac != 1.0 && g_mhd) {
@@ -603,6 +749,11 @@ void free_block(Block *b)
     if (b->d_dtslots) (void)hipFree(b->d_dtslots);
     if (b->d_flux) (void)hipFree(b->d_flux);
     if (b->backup) (void)hipFree(b->backup);
+    for (int d = 0; d < 3; d++) {
+        if (b->d_fc_at[d]) (void)hipFree(b->d_fc_at[d]);
+        if (b->d_fc_in[d]) (void)hipFree(b->d_fc_in[d]);
+        if (b->d_fc_out[d]) (void)hipFree(b->d_fc_out[d]);
+    }
     delete b;
 }
 
@@ -1190,6 +1341,7 @@ int list_build()
         for (int d = 0; d < 3; d++)
             if (g_cfg.has_dir[d] && b->g[d] < 4)
                 return fail(PK_EINVAL, "block %lld: the fused RK2 sweep needs nb >= 4", (long long)b->id);
+        RCCHK(no_fc(b, "the batched list sweep"));
         g_list.blocks.push_back(b);
         g_list.cur0.push_back(b->cur);
         g_list.mdl.push_back(min_dl(b));
@@ -1546,6 +1698,7 @@ int fill_buffer(Block *b, int i)
 // uh and adds its fluxes to u in place.
 int unsplit_stage(Block *b, int istep, double dt, double ch)
 {
+    RCCHK(no_fc(b, "the unsplit solver"));
     if (g_nfl != 1) return fail(PK_EINVAL, "the unsplit solver covers one fluid");
     if (block_linked(b)) return fail(PK_EINVAL, "the unsplit solver covers blocks periodic on themselves");
     const bool euler = g_cfg.integration_order == 1;
@@ -2032,6 +2185,27 @@ int pk_fill_ghosts(int64_t cg_id)
     // a connected block takes its ghosts from the neighbours' current state
     RCCHK(list_build());
     return exchange_all();
+}
+
+int pk_register_fcflx(int64_t cg_id, int32_t ddim, int32_t side, int32_t kind, int64_t first, const int32_t *index,
+                      double *uflx, double *bflx)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
+    if (side != 0 && side != 1) return fail(PK_EINVAL, "side must be 0 (low) or 1 (high), got %d", side);
+    if (kind != PK_FC_FINEBND && kind != PK_FC_COARSEBND)
+        return fail(PK_EINVAL, "kind must be PK_FC_FINEBND or PK_FC_COARSEBND, got %d", kind);
+    if (index && !uflx) return fail(PK_EINVAL, "uflx is required with index");
+    if (index && g_mhd && !bflx) return fail(PK_EINVAL, "bflx (B and psi fluxes) is required with a magnetic field");
+    (void)hipStreamSynchronize(g_stream);
+    Block::FcReg &r = b->fc[ddim][kind][side];
+    r.index = index;
+    r.uflx = index ? uflx : nullptr;
+    r.bflx = (index && g_mhd) ? bflx : nullptr;
+    r.first = first;
+    list_invalidate();
+    return PK_OK;
 }
 
 int pk_set_neighbour(int64_t cg_id, int32_t dir, int32_t side, int64_t nb_id)

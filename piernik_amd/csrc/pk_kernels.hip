@@ -38,14 +38,43 @@ __device__ __forceinline__ int plane_of(int k)
     return p[k];
 }
 
+// Fine/coarse flux points of pencil pf at the right face of cell cs, in
+// solve's order (solve_cg_riemann.F90:353-363): a flux sent by a finer
+// neighbour replaces the computed one (li, ri), then the flux at a face that
+// borders a coarser neighbour is saved (lo, ro), by the one lane that owns
+// the cell.  Sweep-order u-fluxes of this fluid, then [B, psi].
+template <class P>
+__device__ __forceinline__ void fc_points(const SweepArgs &a, int pf, int cs, bool own, double (&F)[P::NV])
+{
+    const long long np = a.fc_npen;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        if (a.fc_at[k * np + pf] != cs) continue;
+        const double *r = a.fc_in + ((long long)k * np + pf) * a.fc_rec;
+#pragma unroll
+        for (int v = 0; v < P::NV; v++) F[v] = r[v < 5 ? a.fc_beg + v : a.fc_boff + (v - 5)];
+    }
+    if (!own) return;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        if (a.fc_at[(2 + k) * np + pf] != cs) continue;
+        double *r = a.fc_out + ((long long)k * np + pf) * a.fc_rec;
+#pragma unroll
+        for (int v = 0; v < P::NV; v++) r[v < 5 ? a.fc_beg + v : a.fc_boff + (v - 5)] = F[v];
+    }
+}
+
 // One RK stage of the tile.  A holds primitives then fluxes, B face states.
 // LIM >= 0 fixes both limiters at compile time; LIM < 0 reads them from a.
-template <class P, int LIM>
+// FC: apply the fine/coarse flux points of pencil fc_pf (-1: none) at the
+// lane's cell fc_cs; fc_own marks the lane that writes saved fluxes.
+template <class P, int LIM, bool FC = false>
 __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)[P::NV],
                                            const double (&W0)[P::NV], double coef, bool upd,
                                            bool lo_end, bool hi_end, double *__restrict__ A,
                                            double *__restrict__ B, int t, int tm, int tp, int nt,
-                                           double (&Wn)[P::NV])
+                                           double (&Wn)[P::NV], int fc_pf = -1, int fc_cs = 0,
+                                           bool fc_own = false)
 {
     constexpr int NV = P::NV;
     double q[NV];
@@ -80,6 +109,9 @@ __device__ __forceinline__ void tile_stage(const SweepArgs &a, const double (&W)
         flux_hlld(fp, fr, a.gam, a.ch, F);
     else
         flux_hllc(fp, fr, a.gam, F);
+    if constexpr (FC) {
+        if (a.fc_at && fc_pf >= 0) fc_points<P>(a, fc_pf, fc_cs, fc_own, F);
+    }
 #pragma unroll
     for (int v = 0; v < NV; v++) A[v * nt + t] = F[v];
     __syncthreads();
@@ -278,7 +310,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int bid, unsigned
 #pragma unroll
             for (int k = 0; k < NV; k++) W0[k] = W[k];
         }
-        tile_stage<P, LIM>(a, W, W0, a.c1, !(lo || hi), lo, hi, LA, LB, t, tm, tp, nt, Wn);
+        tile_stage<P, LIM, true>(a, W, W0, a.c1, !(lo || hi), lo, hi, LA, LB, t, tm, tp, nt, Wn, pvalid ? pf : -1,
+                                 cs, outl && svalid);
     }
     {
         double dm = 0.0;

@@ -5,6 +5,10 @@
 
 namespace pk {
 
+// fc_at value of a pencil without that flux point: below every cell a lane
+// of a tile can hold (tiles start at most a halo before the pencil).
+constexpr int kFcNone = -(1 << 20);
+
 struct SweepArgs {
     const double *src;   // state read with halo (current stage input)
     const double *base;  // U0 of the update (start-of-step state); == src at the first stage
@@ -35,6 +39,15 @@ struct SweepArgs {
     int nslots;
     int has_dir[3];
     double dl[3];
+    // stage mode only: fine/coarse flux points of the pencils (null when none)
+    const int *fc_at;    // [4][fc_npen]: the cell whose right-face flux is replaced (li, ri) or saved (lo, ro);
+                         // kFcNone (below every lane's cell) where absent
+    const double *fc_in; // [2][fc_npen][fc_rec]: replacement fluxes of li, ri
+    double *fc_out;      // [2][fc_npen][fc_rec]: saved fluxes of lo, ro
+    int fc_npen;         // pencils of the transverse interior (d1 fastest)
+    int fc_rec;          // record length: the u-fluxes of every fluid, then [B, psi] with a magnetic field
+    int fc_beg;          // this fluid's first u-flux in a record
+    int fc_boff;         // first [B, psi] flux in a record
 };
 
 struct GhostArgs {

@@ -27,9 +27,14 @@ module piernik_hip
         &    pk_fluid_update_adjoint, pk_timestep_adjoint, pk_ensemble_ids, pk_ensemble_step, pk_ensemble_run, &
         &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
         &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint, &
-        &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit, pk_set_cfl_control, pk_redone_steps
+        &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit, pk_set_cfl_control, pk_redone_steps, &
+        &    pk_register_fcflx
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
+   public :: PK_FC_FINEBND, PK_FC_COARSEBND
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
+
+   ! kinds of pk_register_fcflx: cg%finebnd, cg%coarsebnd (grid_container_fcflx.F90:40-41)
+   integer(c_int32_t), parameter :: PK_FC_FINEBND = 0, PK_FC_COARSEBND = 1
 
    ! the reference's stage enumerators (src/base/constants.F90:230-232)
    integer(c_int32_t), parameter :: PK_EULER = 10000, PK_RK2_1 = 10001, PK_RK2_2 = 10002
@@ -180,6 +185,14 @@ module piernik_hip
          real(c_double),     intent(out) :: t_out
          integer(c_int32_t), intent(out) :: nsteps_out
       end function pk_run_unsplit
+
+      integer(c_int32_t) function pk_register_fcflx(cg_id, ddim, side, kind, first, index, uflx, bflx) &
+           &                                       bind(C, name="pk_register_fcflx")
+         import :: c_int32_t, c_int64_t, c_ptr
+         integer(c_int64_t), value :: cg_id, first
+         integer(c_int32_t), value :: ddim, side, kind
+         type(c_ptr),        value :: index, uflx, bflx
+      end function pk_register_fcflx
 
       integer(c_int32_t) function pk_set_neighbour(cg_id, dir, side, nb_id) bind(C, name="pk_set_neighbour")
          import :: c_int32_t, c_int64_t

@@ -185,3 +185,12 @@ def oracle_stage(po, p, d, istep, dt, ch, C, F, fc_c, fc_f, correct=True):
     po.solve_cg(p, C, d, istep, dt, ch, fc=fc_c if correct else None)
     po.fill_ghosts(p, C)
     prolong_ghosts(d, F, C)
+
+
+class Absolute:
+    """|u|, |b|, |psi| of a block: composite_sums of it is the scale of the sums."""
+
+    def __init__(self, o):
+        self.u = np.abs(o.u)
+        self.b = np.abs(o.b) if o.b is not None else None
+        self.psi = np.abs(o.psi) if o.psi is not None else None

@@ -37,20 +37,13 @@ def run_composite(po, case, d, order, correct=True, limiter="vanleer"):
     p = po.params_from_config(cfg)
     C, F, fc_c, fc_f = fx.build(po, p, cfg, d)
     before = fx.composite_sums(d, C, F)
-    scale = fx.composite_sums(d, _abs(C), _abs(F))
+    scale = fx.composite_sums(d, fx.Absolute(C), fx.Absolute(F))
     dt, ch = fx.stage_dt(d), 1.0
     steps = (EULER,) if order == 1 else (RK2_1, RK2_2)
     for istep in steps:
         fx.oracle_stage(po, p, d, istep, dt, ch, C, F, fc_c, fc_f, correct)
     after = fx.composite_sums(d, C, F)
     return before, after, scale, C, F
-
-
-class _abs:
-    def __init__(self, o):
-        self.u = np.abs(o.u)
-        self.b = np.abs(o.b) if o.b is not None else None
-        self.psi = np.abs(o.psi) if o.psi is not None else None
 
 
 @pytest.mark.parametrize("d", [0, 1, 2])
