@@ -39,6 +39,7 @@ extern "C" {
 #define PK_EHIP 3        /* HIP runtime error */
 #define PK_ESTATE 4      /* call out of order (e.g. before pk_init) */
 #define PK_ENOMEM 5
+#define PK_EIO 6         /* file I/O (HDF5, libpiernik_gdf) */
 
 #define PK_EULER 10000
 #define PK_RK2_1 10001
@@ -122,6 +123,19 @@ int pk_download_to(int64_t cg_id, double *u, double *b, double *psi);
  * tensors of shape (nzg, nyg, nxg, nvar) on this GPU); no host staging. */
 int pk_upload_device(int64_t cg_id, const double *u, const double *b, const double *psi);
 int pk_download_device(int64_t cg_id, double *u, double *b, double *psi);
+
+/* One output field of the data files over the block's interior, computed on
+ * the device from its current state as datafields_hdf5 computes it
+ * (data_hdf5.F90:345-691; ekin, emag, sq_sum3 of func.F90:55-88), into
+ * out[n_b(1)*n_b(2)*n_b(3)] in the reference's (nx, ny, nz) order.  Names are
+ * the reference's short names (hdf_vars): den?, vlx? vly? vlz?, momx? momy?
+ * momz?, ene?, pre?, eth? for ? = i (the ionized fluid: the magnetized one,
+ * else the first) or n (the neutral one: the other of two fluids, the only
+ * one of a hydro run), v (|m|/density of the ionized fluid, else the
+ * neutral), and with a magnetic field magx magy magz, magB and psi.  pre?
+ * and eth? of the ionized fluid subtract the magnetic energy (cell-centred
+ * B).  The GDF writer (piernik_gdf.h) calls this per block and field. */
+int pk_datafield(int64_t cg_id, const char *name, double *out);
 
 /* ---- the operator --------------------------------------------------------- */
 

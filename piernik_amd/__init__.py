@@ -90,6 +90,7 @@ _PROTOS = {
     "pk_sweep": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
     "pk_fill_ghosts": [ctypes.c_int64],
     "pk_set_neighbour": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
+    "pk_datafield": [ctypes.c_int64, ctypes.c_char_p, ctypes.c_void_p],
     "pk_register_fcflx": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "pk_unsplit_stage": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
@@ -419,6 +420,15 @@ def run_unsplit(cg, tend, nmax=1 << 30):
     t, ns = ctypes.c_double(), ctypes.c_int32()
     _call("pk_run_unsplit", cg.cg_id, float(tend), int(nmax), ctypes.byref(t), ctypes.byref(ns))
     return t.value, ns.value
+
+
+def datafield(cg, name):
+    """Output field `name` (the reference's short names: deni, vlxi, prei,
+    magB, psi, ...) of cg's interior, computed on the device from its current
+    state (pk_datafield); float64 array of shape n (x, y, z)."""
+    out = np.zeros(cg.n, dtype=np.float64, order="F")
+    _call("pk_datafield", cg.cg_id, name.encode(), out.ctypes.data_as(ctypes.c_void_p))
+    return out
 
 
 FC_FINEBND, FC_COARSEBND = 0, 1

@@ -13,8 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpiernik_hip.so")
+GDF_LIB = os.path.join(HERE, "libpiernik_gdf.so")
+HDF5 = os.environ.get("PK_HDF5_ROOT", "/opt/conda")  # HDF5 1.10 C library and headers of this image
 
-SOURCES = ["pk_kernels.hip", "pk_kernels_fast.hip", "pk_adjoint.hip", "pk_unsplit.hip", "pk_capi.hip"]
+SOURCES = ["pk_kernels.hip", "pk_kernels_fast.hip", "pk_adjoint.hip", "pk_unsplit.hip", "pk_datafields.hip",
+           "pk_capi.hip"]
 
 # -ffp-contract=off: every a*b+c rounds twice, as the reference's FP64 code.
 # -packed-fp32-ops off: no packed FP32 instructions are generated.
@@ -68,7 +71,26 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    build_gdf(verbose)
     return LIB
+
+
+def build_gdf(verbose=False):
+    """libpiernik_gdf.so: the GDF data files, host code over the HDF5 C
+    library; pk_datafield comes from libpiernik_hip.so beside it."""
+    if not os.path.exists(os.path.join(HDF5, "include", "hdf5.h")):
+        sys.stderr.write("HDF5 headers not found under %s: libpiernik_gdf.so not built\n" % HDF5)
+        return None
+    # the run path lists the system's C++ runtime before the HDF5 directory,
+    # which also holds an older libstdc++ that ROCm's libraries cannot use
+    cxx = subprocess.run(["g++", "-print-file-name=libstdc++.so"], capture_output=True, text=True).stdout.strip()
+    rpath = ":".join(["$ORIGIN", os.path.dirname(os.path.realpath(cxx)), os.path.join(HDF5, "lib")])
+    tmp = GDF_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(HDF5, "include"),
+          os.path.join(CSRC, "pk_gdf.cpp"), "-o", tmp, "-L" + HERE, "-l:libpiernik_hip.so",
+          "-L" + os.path.join(HDF5, "lib"), "-lhdf5", "-Wl,-rpath," + rpath], verbose)
+    os.replace(tmp, GDF_LIB)
+    return GDF_LIB
 
 
 if __name__ == "__main__":

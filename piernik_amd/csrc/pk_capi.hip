@@ -26,6 +26,7 @@
 #include "pk_adjoint_kernels.hpp"
 #include "pk_kernels.hpp"
 #include "pk_unsplit.hpp"
+#include "pk_datafields.hpp"
 
 namespace {
 
@@ -2185,6 +2186,100 @@ int pk_fill_ghosts(int64_t cg_id)
     // a connected block takes its ghosts from the neighbours' current state
     RCCHK(list_build());
     return exchange_all();
+}
+
+}  // extern "C"
+
+namespace {
+
+// The fluid a field name's last letter selects (common_shortcuts,
+// common_hdf5.F90:418-542): 'i' the ionized fluid (the magnetized one, else
+// the first), 'n' the neutral one (the other of two fluids; the only fluid
+// of a hydro run).  -1 if there is none.
+int named_fluid(char c)
+{
+    if (c == 'i') return g_magf >= 0 ? g_magf : 0;
+    if (c != 'n') return -1;
+    if (g_nfl == 2) return g_magf >= 0 ? 1 - g_magf : 1;
+    return g_magf >= 0 ? -1 : 0;
+}
+
+// datafields_hdf5 (data_hdf5.F90:496-580) for the fields computed from u, b
+// and psi; psi is the qna array the fallback of get_data_from_cg finds.
+int datafield_args(Block *b, const char *name, pk::DatafieldArgs &a)
+{
+    a = pk::DatafieldArgs{};
+    a.state = state(b, b->cur);
+    a.sv = b->sv;
+    a.sy = b->sy;
+    a.sz = b->sz;
+    for (int d = 0; d < 3; d++) {
+        a.g[d] = b->g[d];
+        a.n[d] = b->n[d];
+    }
+    a.mag = -1;
+    const std::string v = name ? name : "";
+    const int mplane = g_mhd ? g_fplane[g_magf] + 5 : -1;
+    auto fluid = [&](char c, int *f) {
+        *f = named_fluid(c);
+        if (*f < 0) return fail(PK_EINVAL, "field '%s': no such fluid in this configuration", v.c_str());
+        a.fluid = g_fplane[*f];
+        a.gam_1 = g_cfg.fluid[*f].gamma - 1.0;
+        return PK_OK;
+    };
+    int f = -1;
+    if (v.size() == 4 && (v.compare(0, 3, "den") == 0 || v.compare(0, 3, "ene") == 0)) {
+        RCCHK(fluid(v[3], &f));
+        a.kind = pk::DF_COPY;
+        a.plane = a.fluid + (v[0] == 'd' ? 0 : 4);
+    } else if (v.size() == 4 && v.compare(0, 2, "vl") == 0 && v[2] >= 'x' && v[2] <= 'z') {
+        RCCHK(fluid(v[3], &f));
+        a.kind = pk::DF_VEL;
+        a.plane = a.fluid + 1 + (v[2] - 'x');
+    } else if (v.size() == 5 && v.compare(0, 3, "mom") == 0 && v[3] >= 'x' && v[3] <= 'z') {
+        RCCHK(fluid(v[4], &f));
+        a.kind = pk::DF_COPY;
+        a.plane = a.fluid + 1 + (v[3] - 'x');
+    } else if (v.size() == 4 && (v.compare(0, 3, "pre") == 0 || v.compare(0, 3, "eth") == 0)) {
+        RCCHK(fluid(v[3], &f));
+        a.kind = v[0] == 'p' ? pk::DF_PRE : pk::DF_ETH;
+        if (v[3] == 'i') a.mag = mplane;  // emag_c enters the ionized fluid's pressure
+    } else if (v == "v") {
+        RCCHK(fluid(g_nfl == 2 || g_magf >= 0 ? 'i' : 'n', &f));
+        a.kind = pk::DF_VTOT;
+    } else if (g_mhd && v.size() == 4 && v.compare(0, 3, "mag") == 0 && v[3] >= 'x' && v[3] <= 'z') {
+        a.kind = pk::DF_COPY;
+        a.plane = mplane + (v[3] - 'x');
+    } else if (g_mhd && v == "magB") {
+        a.kind = pk::DF_MAGB;
+        a.mag = mplane;
+    } else if (g_mhd && v == "psi") {
+        a.kind = pk::DF_COPY;
+        a.plane = mplane + 3;
+    } else {
+        return fail(PK_EINVAL, "'%s' is not an output field of this configuration", v.c_str());
+    }
+    return PK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pk_datafield(int64_t cg_id, const char *name, double *out)
+{
+    Block *b;
+    RCCHK(get_block(cg_id, &b));
+    if (!out) return fail(PK_EINVAL, "no output array");
+    pk::DatafieldArgs a;
+    RCCHK(datafield_args(b, name, a));
+    const size_t bytes = sizeof(double) * (size_t)b->n[0] * b->n[1] * b->n[2];
+    RCCHK(ensure_stage(bytes));
+    a.out = g_stage;
+    HIPCHK(pk::launch_datafield(a, g_stream));
+    HIPCHK(hipMemcpyAsync(out, g_stage, bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return PK_OK;
 }
 
 int pk_register_fcflx(int64_t cg_id, int32_t ddim, int32_t side, int32_t kind, int64_t first, const int32_t *index,

@@ -28,7 +28,7 @@ module piernik_hip
         &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
         &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint, &
         &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit, pk_set_cfl_control, pk_redone_steps, &
-        &    pk_register_fcflx
+        &    pk_register_fcflx, pk_datafield
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_FC_FINEBND, PK_FC_COARSEBND
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
@@ -185,6 +185,14 @@ module piernik_hip
          real(c_double),     intent(out) :: t_out
          integer(c_int32_t), intent(out) :: nsteps_out
       end function pk_run_unsplit
+
+      ! name: NUL-terminated short name (hdf_vars); out: n_b(1)*n_b(2)*n_b(3) values
+      integer(c_int32_t) function pk_datafield(cg_id, name, out) bind(C, name="pk_datafield")
+         import :: c_int64_t, c_int32_t, c_char, c_double
+         integer(c_int64_t), value                         :: cg_id
+         character(kind=c_char), dimension(*), intent(in)  :: name
+         real(c_double),         dimension(*), intent(out) :: out
+      end function pk_datafield
 
       integer(c_int32_t) function pk_register_fcflx(cg_id, ddim, side, kind, first, index, uflx, bflx) &
            &                                       bind(C, name="pk_register_fcflx")
