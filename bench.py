@@ -39,6 +39,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 METRIC = "FP64 MHD cell-updates/s at 256^3"
 UNSPLIT_CFL = 0.3  # problems/cpaw/problem.par: 0.41 is near the unsplit solver's limit
+# the data-file fields of an MHD run (the reference's short names, hdf_vars)
+GDF_FIELDS_MHD = ["deni", "vlxi", "vlyi", "vlzi", "momxi", "momyi", "momzi", "enei", "prei", "ethi", "v",
+                  "magx", "magy", "magz", "magB", "psi"]
 
 
 def _cpu_worker(args):
@@ -46,7 +49,7 @@ def _cpu_worker(args):
     workload, n, seed, budget = args
     from oracle import pyoracle as po
     from piernik_amd import setups
-    if workload in ("mhd256", "unsplit256"):
+    if workload in ("mhd256", "unsplit256", "stage256", "gdf256"):
         cg, cfg = setups.mhd_turbulence(n=n, seed=seed)
     elif workload == "batch4096":
         cfg = setups.batch_config()
@@ -57,6 +60,36 @@ def _cpu_worker(args):
     ob = po.OBlock.from_cg(cg)
     t0 = time.perf_counter()
     steps = 0
+    if workload == "stage256":
+        # the drop-in operator: RK2 stages of solve_cg along x, y, z
+        while True:
+            for d in range(3):
+                for istep in (10001, 10002):
+                    po.solve_cg(p, ob, d, istep, 1e-4, 1.0)
+                    steps += 1
+            if time.perf_counter() - t0 >= budget:
+                break
+        return steps * n ** 3, time.perf_counter() - t0
+    if workload == "gdf256":
+        # output fields in numpy (the oracle's datafields_hdf5) and the file
+        # from host arrays, into a scratch directory of this host
+        import tempfile
+        from oracle import datafields as df
+        from piernik_amd import gdf
+        written = 0
+        with tempfile.TemporaryDirectory() as tmp:
+            meta = gdf.Meta(n_d=(n, n, n), nfluids=1)
+            place = gdf.Placement((n, n, n), dl=cg.dl)
+            while True:
+                data = [df.datafield(nm, ob.u, ob.b, ob.psi, cfg.fluids, cg.nb) for nm in GDF_FIELDS_MHD]
+                path = os.path.join(tmp, f"f{steps}.h5")
+                gdf.write_host(path, [place], GDF_FIELDS_MHD, [data], meta)
+                written += os.path.getsize(path)
+                os.remove(path)
+                steps += 1
+                if time.perf_counter() - t0 >= budget:
+                    break
+        return written, time.perf_counter() - t0
     if workload == "unsplit256":
         p.cfl = UNSPLIT_CFL
         tt = 0.0
@@ -85,6 +118,20 @@ def cpu_baseline(workload, budget=10.0):
     cores = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
     nblk = 32 if workload == "batch4096" else 64
     ctx = mp.get_context("fork")
+    if workload == "gdf256":
+        # one writer process and one file, as the device path writes it:
+        # the whole 256^3 block, numpy fields and the HDF5 write on one core
+        with ctx.Pool(1) as pool:
+            written, el = pool.map(_cpu_worker, [(workload, 256, 100, budget)])[0]
+        return {
+            "value": written / el / 1e9,
+            "unit": "GB/s of data file",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"oracle output fields (numpy restatement of datafields_hdf5) of the {len(GDF_FIELDS_MHD)} "
+                       f"fields of a 256^3 MHD block and its GDF-v2 file written from host arrays, one process, "
+                       f"files written until ~{budget:.0f} s had passed; {written} file bytes in {el:.2f} s"),
+        }
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
         res = pool.map(_cpu_worker, [(workload, nblk, 100 + i, budget) for i in range(cores)])
@@ -93,13 +140,13 @@ def cpu_baseline(workload, budget=10.0):
     slowest = max(r[1] for r in res)
     return {
         "value": updates / slowest,
-        "unit": "cell-updates/s",
+        "unit": "cell-stages/s" if workload == "stage256" else "cell-updates/s",
         "cores": cores,
         "kind": "port",
         "sample": (f"oracle (C restatement of the reference sweep, one thread per process) on {cores} "
                    f"independent periodic {nblk}^3 blocks of the same "
-                   f"{ {'mhd256': 'MHD HLLD+GLM', 'unsplit256': 'MHD HLLD+GLM, unsplit solver', 'batch4096': 'two-fluid (MHD HLLD+GLM + HD HLLC)'}.get(workload, 'HD HLLC')} "
-                   f"problem, whole PIERNIK steps for ~{budget:.0f} s each; {updates} cell-updates in {slowest:.2f} s "
+                   f"{ {'mhd256': 'MHD HLLD+GLM', 'unsplit256': 'MHD HLLD+GLM, unsplit solver', 'stage256': 'MHD HLLD+GLM, solve_cg stages', 'batch4096': 'two-fluid (MHD HLLD+GLM + HD HLLC)'}.get(workload, 'HD HLLC')} "
+                   f"problem, {'RK2 stages along x, y, z' if workload == 'stage256' else 'whole PIERNIK steps'} for ~{budget:.0f} s each; {updates} cell-updates in {slowest:.2f} s "
                    f"(wall {wall:.2f} s)"),
     }
 
@@ -119,7 +166,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128", "ensemble256", "copy",
-                                           "unsplit256"],
+                                           "unsplit256", "stage256", "gdf256"],
                     default="mhd256")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--nblocks", type=int, default=4096, help="blocks of the batched list")
@@ -131,6 +178,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--blocks", type=int, nargs=3, default=None,
                     help="mhd256: cut the domain into BX x BY x BZ linked blocks advanced as one list")
+    ap.add_argument("--no-fc", action="store_true",
+                    help="stage256: time only the plain stages (no flux points, no PCIe round trips)")
     ap.add_argument("--exact", action="store_true",
                     help="time the bit-exact build of the sweep instead of the FMA-contracted one")
     args = ap.parse_args()
@@ -157,6 +206,12 @@ def main():
         return
     if args.workload == "unsplit256":
         print(json.dumps(run_unsplit(args, cpu)))
+        return
+    if args.workload == "stage256":
+        print(json.dumps(run_stage(args, cpu)))
+        return
+    if args.workload == "gdf256":
+        print(json.dumps(run_gdf(args, cpu)))
         return
     if args.workload == "mhd256" and args.blocks:
         print(json.dumps(run_blocks(args)))
@@ -469,6 +524,224 @@ def run_unsplit(args, cpu):
         },
         "cpu_baseline": cpu,
         "floors": rep,
+    }
+
+
+def run_stage(args, cpu):
+    """The drop-in operator: pk_solve_cg (one RK stage of one direction, the
+    solve_cg_sub call of sweeps.F90:245) on the 256^3 MHD block, RK2 stages
+    along x, y and z with the periodic ghost refresh between them, as sweep()
+    runs them.  Timed three ways: device-resident without fine/coarse flux
+    points; device-resident with flux points on every pencil end (every
+    pencil has li, lo and ro: the worst case of an AMR level); and as the
+    Fortran adapter drives it, with the block uploaded and downloaded around
+    every call.  Algorithmic bytes of a stage: the state read and written
+    once per interior cell (2 x 144 B; the second stage also reads U0, which
+    is not counted)."""
+    import numpy as np
+    import piernik_amd as pk
+    from piernik_amd import setups
+
+    n = args.n or 256
+    cg, cfg = setups.mhd_turbulence(n=n, seed=1)
+    pk.init(cfg)
+    cg.upload()
+    pk.run(cg, 1)
+    dt, ch = 1e-4, 1.0
+    print(f"[bench] stage {n}^3 ready", file=sys.stderr, flush=True)
+
+    def stages(k):
+        for _ in range(k):
+            for d in range(3):
+                for istep in (pk.RK2_1, pk.RK2_2):
+                    pk.solve_cg(cg, d, istep, dt, ch)
+                    pk.update_boundaries(cg)
+
+    def timed(k):
+        stages(1)
+        pk.synchronize()
+        pk.profile_begin()
+        t0 = time.perf_counter()
+        stages(k)
+        pk.synchronize()
+        el = time.perf_counter() - t0
+        return el, pk.profile_end()
+
+    k = max(1, args.steps)
+    el_plain, prof = timed(k)
+
+    def with_points(every):
+        """Flux points (lo, ro and li) on the pencils whose transverse
+        indices are both multiples of `every` (1: every pencil, the worst case
+        of a refined level), every direction; li at the last cell replaces its
+        right face by the flux ro saved there in the previous call."""
+        refs = []
+        for d in range(3):
+            o1, o2 = ((1, 2), (2, 0), (0, 1))[d]
+            n1, n2 = cg.n[o1], cg.n[o2]
+            on = np.zeros((n1, n2), dtype=bool)
+            on[::every, ::every] = True
+            saved = [np.zeros((5, n1, n2), order="F") for _ in range(2)]
+            bsaved = [np.zeros((4, n1, n2), order="F") for _ in range(2)]
+            first = np.asfortranarray(np.where(on, 0, -1).astype(np.int32))
+            last = np.asfortranarray(np.where(on, n - 1, n).astype(np.int32))
+            last_lo = np.asfortranarray(np.where(on, n - 1, -1).astype(np.int32))
+            pk.register_fcflx(cg, d, 0, pk.FC_COARSEBND, 0, first, saved[0], bsaved[0])
+            pk.register_fcflx(cg, d, 1, pk.FC_COARSEBND, 0, last, saved[1], bsaved[1])
+            pk.register_fcflx(cg, d, 0, pk.FC_FINEBND, 0, last_lo, saved[1], bsaved[1])
+            refs.append((saved, bsaved, first, last, last_lo))
+        el, pr = timed(k)
+        for d in range(3):
+            for s_ in (0, 1):
+                for kind in (pk.FC_FINEBND, pk.FC_COARSEBND):
+                    pk.register_fcflx(cg, d, s_, kind, 0, None, None, None)
+        return el, pr
+
+    el_fc = el_fc16 = None
+    prof_fc = {"algo_bytes": [0.0] * 4, "ms": [1.0] * 4}
+    if not args.no_fc:
+        el_fc, prof_fc = with_points(1)
+        el_fc16, _ = with_points(4)
+    # as the adapter drives it: upload, one stage, download
+    kp = 0 if args.no_fc else max(1, k // 4)
+    t0 = time.perf_counter()
+    for _ in range(kp):
+        for d in range(3):
+            for istep in (pk.RK2_1, pk.RK2_2):
+                cg.upload()
+                pk.solve_cg(cg, d, istep, dt, ch)
+                cg.download()
+    el_pcie = time.perf_counter() - t0
+    pk.finalize()
+
+    cells = n ** 3
+    nst = 6 * k
+    algo = sum(prof["algo_bytes"][:3])
+    kms = sum(prof["ms"][:3])
+    launches = sum(prof["launches"][:3])
+    achieved = algo / (kms * 1e-3) / 1e9
+    return {
+        "metric": "FP64 MHD cell-stages/s at 256^3, drop-in solve_cg (one RK stage of one direction per call)",
+        "value": cells * nst / el_plain,
+        "unit": "cell-stages/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": 1,
+        "ms_per_step": el_plain / k * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3D ideal MHD, HLLD+GLM, minmod, single periodic {n}^3 block, nb=4; a step is the 6 "
+                         "solve_cg calls of one dt-substep (RK2_1, RK2_2 for x, y, z) with update_boundaries"),
+            "cells": cells,
+            "with_fc_points_cell_stages_per_s": cells * nst / el_fc if el_fc else None,
+            "fc_overhead": el_fc / el_plain if el_fc else None,
+            "fc_overhead_1_in_16_pencils": el_fc16 / el_plain if el_fc16 else None,
+            "fc_kernel_gbps": (sum(prof_fc["algo_bytes"][:3]) / (sum(prof_fc["ms"][:3]) * 1e-3) / 1e9
+                               if el_fc else None),
+            "pcie_inclusive_cell_stages_per_s": cells * 6 * kp / el_pcie if kp else None,
+            "arithmetic": "FP64, every operation rounded as the reference's code (bit-exact)",
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "k_sweep<dir, MHD, false, minmod> (stage mode, whole pencils), x/y/z",
+            "algo_bytes_per_launch": algo / launches if launches else None,
+            "avg_launch_ms": kms / launches if launches else None,
+        },
+        "cpu_baseline": cpu,
+    }
+
+
+def run_gdf(args, cpu):
+    """Data files of the 256^3 MHD block: the 16 output fields of an MHD run
+    derived on the device (k_datafield, one launch per field; the field then
+    copied to the host) and the GDF-v2 file written from them
+    (libpiernik_gdf), FP32 datasets as the reference writes by default, the
+    rounding done on the device.  value: file bytes written per second.
+    Algorithmic bytes of a k_datafield launch: the planes the field reads
+    (8 B each) plus the FP32 field written (4 B), per interior cell."""
+    import tempfile
+    import piernik_amd as pk
+    from piernik_amd import gdf, setups
+
+    n = args.n or 256
+    cg, cfg = setups.mhd_turbulence(n=n, seed=1)
+    pk.init(cfg)
+    cg.upload()
+    pk.run(cg, 1)
+    names = GDF_FIELDS_MHD
+    for nm in names:
+        pk.datafield_f32(cg, nm)
+    pk.synchronize()
+    print(f"[bench] gdf {n}^3 ready", file=sys.stderr, flush=True)
+    k = max(1, args.steps // 4)
+    # the fields as the file takes them: FP32, rounded on the device
+    pk.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        for nm in names:
+            pk.datafield_f32(cg, nm)
+    el_fields = time.perf_counter() - t0
+    prof = pk.profile_end()
+    meta = gdf.Meta(time=0.0, n_d=(n, n, n), nfluids=1)
+    place = gdf.Placement((n, n, n), dl=cg.dl)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "gdf.h5")
+        gdf.write(path, [(cg, place)], names, meta)  # warm the page cache and the writer
+        t0 = time.perf_counter()
+        for _ in range(k):
+            gdf.write(path, [(cg, place)], names, meta)
+        el_file = time.perf_counter() - t0
+        size = os.path.getsize(path)
+    pk.finalize()
+    launches = prof["launches"][3]
+    kms = prof["ms"][3]
+    algo = prof["algo_bytes"][3]
+    achieved = algo / (kms * 1e-3) / 1e9
+    field_bytes = 4.0 * n ** 3 * len(names)
+    return {
+        "metric": "GDF-v2 data file of the 256^3 MHD block (16 fields, FP32) written per second",
+        "value": size * k / el_file / 1e9,
+        "unit": "GB/s of data file",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": 1,
+        "ms_per_step": el_file / k * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"output fields {','.join(names)} of a 3D MHD {n}^3 block (one solver step from the "
+                         "Gaussian random state) derived on the device, then one GDF-v2 file per step into a "
+                         "local scratch directory"),
+            "file_bytes": size,
+            "fields_to_host_gbps": field_bytes * k / el_fields / 1e9,
+            "arithmetic": "FP64 fields, bit-exact with datafields_hdf5; FP32 datasets (h5_64bit = .false.)",
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "k_datafield (one launch per field, FP32 output)",
+            "algo_bytes_per_launch": algo / launches if launches else None,
+            "avg_launch_ms": kms / launches if launches else None,
+        },
+        "cpu_baseline": cpu,
     }
 
 

@@ -136,6 +136,9 @@ int pk_download_device(int64_t cg_id, double *u, double *b, double *psi);
  * and eth? of the ionized fluid subtract the magnetic energy (cell-centred
  * B).  The GDF writer (piernik_gdf.h) calls this per block and field. */
 int pk_datafield(int64_t cg_id, const char *name, double *out);
+/* The same rounded to nearest FP32 on the device (the conversion HDF5 applies
+ * when FP64 values are written to the reference's default FP32 datasets). */
+int pk_datafield_f32(int64_t cg_id, const char *name, float *out);
 
 /* ---- the operator --------------------------------------------------------- */
 
@@ -364,14 +367,16 @@ int pk_timestep_adjoint(int64_t cg_id, int32_t slot, double dt_bar);
 
 /* ---- measurement (no counterpart in the reference) ------------------------ */
 
-/* Per-direction totals of the sweep kernels launched between
- * pk_profile_begin() and pk_profile_end(), timed with HIP events recorded on
- * the library's stream around each launch.  algo_bytes counts what the
- * sweep must move: every interior cell read once and written once. */
+/* Totals of the kernels launched between pk_profile_begin() and
+ * pk_profile_end(), timed with HIP events recorded on the library's stream
+ * around each launch: entries 0-2 the sweep kernels of x, y, z (algo_bytes:
+ * every interior cell read once and written once), entry 3 the output-field
+ * kernel of pk_datafield (algo_bytes: the planes the field reads and the
+ * field written, once per interior cell). */
 typedef struct pk_profile_stats {
-    int64_t launches[3];
-    double ms[3];
-    double algo_bytes[3];
+    int64_t launches[4];
+    double ms[4];
+    double algo_bytes[4];
 } pk_profile_stats;
 
 int pk_profile_begin(void);

@@ -64,7 +64,7 @@ class _Config(ctypes.Structure):
 
 
 class _ProfStats(ctypes.Structure):
-    _fields_ = [("launches", ctypes.c_int64 * 3), ("ms", ctypes.c_double * 3), ("algo_bytes", ctypes.c_double * 3)]
+    _fields_ = [("launches", ctypes.c_int64 * 4), ("ms", ctypes.c_double * 4), ("algo_bytes", ctypes.c_double * 4)]
 
 
 class _Report(ctypes.Structure):
@@ -91,6 +91,7 @@ _PROTOS = {
     "pk_fill_ghosts": [ctypes.c_int64],
     "pk_set_neighbour": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
     "pk_datafield": [ctypes.c_int64, ctypes.c_char_p, ctypes.c_void_p],
+    "pk_datafield_f32": [ctypes.c_int64, ctypes.c_char_p, ctypes.c_void_p],
     "pk_register_fcflx": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "pk_unsplit_stage": [ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double],
@@ -431,6 +432,13 @@ def datafield(cg, name):
     return out
 
 
+def datafield_f32(cg, name):
+    """datafield rounded to nearest FP32 on the device (pk_datafield_f32)."""
+    out = np.zeros(cg.n, dtype=np.float32, order="F")
+    _call("pk_datafield_f32", cg.cg_id, name.encode(), out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
 FC_FINEBND, FC_COARSEBND = 0, 1
 
 
@@ -594,7 +602,8 @@ def profile_begin():
 
 
 def profile_end():
-    """Stop timing; per direction: launches, total ms, algorithmic bytes."""
+    """Stop timing; per sweep direction (entries 0-2) and for the output-field
+    kernel (entry 3): launches, total ms, algorithmic bytes."""
     st = _ProfStats()
     _call("pk_profile_end", ctypes.byref(st))
     return {"launches": list(st.launches), "ms": list(st.ms), "algo_bytes": list(st.algo_bytes)}

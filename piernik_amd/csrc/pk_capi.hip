@@ -74,10 +74,16 @@ struct Block {
         double *uflx = nullptr, *bflx = nullptr;
         int64_t first = 0;
     } fc[3][2][2];
-    int *d_fc_at[3] = {nullptr, nullptr, nullptr};
-    double *d_fc_in[3] = {nullptr, nullptr, nullptr}, *d_fc_out[3] = {nullptr, nullptr, nullptr};
-    std::vector<int> h_fc_at;
-    std::vector<double> h_fc_in, h_fc_out;
+    struct FcStage {
+        int *d_tab = nullptr;                      // [2][4][npen]: cell (fc_at), then record slot
+        double *d_in = nullptr, *d_out = nullptr;  // records of the active points
+        size_t in_cap = 0, out_cap = 0;            // records
+        double *h_in = nullptr, *h_out = nullptr;  // pinned staging of the records
+        size_t h_in_cap = 0, h_out_cap = 0;
+        std::vector<int> tab;                      // the table last uploaded
+        std::vector<long long> out_at;             // per saved record: side * 2^40 + reference position
+        hipEvent_t in_done = nullptr;              // the last upload from h_in has been read
+    } fcs[3];
 };
 
 const int kDtSlots = 1024;
@@ -396,10 +402,34 @@ long long fc_ref_at(int d, int i1, int i2, int n1, int n2)
     return d == 1 ? (long long)i2 + (long long)n2 * i1 : (long long)i1 + (long long)n1 * i2;
 }
 
-// Stage the points of direction d for the kernels: per pencil the cell whose
-// right-face flux each point concerns (li: c, ri: c-1, lo: c-1, ro: c for the
-// local cell c of its index, active where set_fluxpointers finds the index
-// inside the interior) and the replacement fluxes.  *active: some point is.
+// Grow a device or pinned record buffer to hold n records of rec values.
+int fc_grow(double **p, size_t *cap, size_t n, int rec, bool pinned)
+{
+    if (n <= *cap && *p) return PK_OK;
+    const size_t want = std::max<size_t>(n, 64);
+    if (*p) {
+        HIPCHK(hipStreamSynchronize(g_stream));  // no kernel or copy still uses the old buffer
+        if (pinned)
+            (void)hipHostFree(*p);
+        else
+            (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (pinned)
+        HIPCHK(hipHostMalloc((void **)p, sizeof(double) * want * rec, hipHostMallocDefault));
+    else
+        HIPCHK(hipMalloc((void **)p, sizeof(double) * want * rec));
+    *cap = want;
+    return PK_OK;
+}
+
+// Stage the points of direction d for the kernels.  Per pencil and point the
+// table holds the cell whose right-face flux the point concerns (li: c, ri:
+// c-1, lo: c-1, ro: c for the local cell c of its index, active where
+// set_fluxpointers finds the index inside the interior) and the point's
+// record; only active points have records.  The table goes to the device
+// only when it changed (the indices change at regrids); the replacement
+// fluxes go every call.  *active: some point is.
 int fc_begin(Block *b, int d, bool *active)
 {
     *active = false;
@@ -407,15 +437,19 @@ int fc_begin(Block *b, int d, bool *active)
     for (int k = 0; k < 2; k++)
         for (int s = 0; s < 2; s++) reg = reg || b->fc[d][k][s].index != nullptr;
     if (!reg) return PK_OK;
+    Block::FcStage &F = b->fcs[d];
     const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
     const int n1 = b->n[d1], n2 = b->n[d2];
     const long long np = (long long)n1 * n2;
     const int rec = fc_rec(), nvar = 5 * g_nfl;
     const int g = b->g[d], n = b->n[d], ng = b->ng[d];
-    HIPCHK(hipStreamSynchronize(g_stream));  // the previous call's copies from the host buffers are done
-    b->h_fc_at.assign((size_t)(4 * np), pk::kFcNone);
-    b->h_fc_in.assign((size_t)(2 * np * rec), 0.0);
-    bool any = false;
+    std::vector<int> tab((size_t)(8 * np), pk::kFcNone);
+    int *at = tab.data(), *slot = tab.data() + 4 * np;
+    // count the active points, then fill the records
+    long long nin = 0, nout = 0;
+    F.out_at.clear();
+    std::vector<std::pair<long long, const double *>> ins;  // replacement record sources
+    std::vector<const double *> insb;
     for (int kind = 0; kind < 2; kind++)
         for (int side = 0; side < 2; side++) {
             const Block::FcReg &r = b->fc[d][kind][side];
@@ -423,8 +457,8 @@ int fc_begin(Block *b, int d, bool *active)
             const int k = 2 * kind + side;  // 0 li, 1 ri, 2 lo, 3 ro
             for (int i2 = 0; i2 < n2; i2++)
                 for (int i1 = 0; i1 < n1; i1++) {
-                    const long long pf = i1 + (long long)n1 * i2, at = fc_ref_at(d, i1, i2, n1, n2);
-                    const long long idx = r.index[at];
+                    const long long pf = i1 + (long long)n1 * i2, ra = fc_ref_at(d, i1, i2, n1, n2);
+                    const long long idx = r.index[ra];
                     const bool act = (side == 0) ? idx >= r.first : idx <= r.first + n - 1;
                     if (!act) continue;
                     const long long c = idx - r.first + g;
@@ -432,24 +466,38 @@ int fc_begin(Block *b, int d, bool *active)
                     if (cs < 0 || cs > ng - 2)
                         return fail(PK_EINVAL, "block %lld: fine/coarse face at index %lld along %d is outside the pencil",
                                     (long long)b->id, idx, d);
-                    b->h_fc_at[(size_t)(k * np + pf)] = (int)cs;
-                    any = true;
-                    if (kind == 1) continue;
-                    double *dst = &b->h_fc_in[(size_t)((side * np + pf) * rec)];
-                    for (int v = 0; v < nvar; v++) dst[v] = r.uflx[at * nvar + v];
-                    if (g_mhd)
-                        for (int v = 0; v < 4; v++) dst[nvar + v] = r.bflx[at * 4 + v];
+                    at[k * np + pf] = (int)cs;
+                    if (kind == 0) {
+                        slot[k * np + pf] = (int)nin++;
+                        ins.push_back({ra, r.uflx});
+                        insb.push_back(r.bflx);
+                    } else {
+                        slot[k * np + pf] = (int)nout++;
+                        F.out_at.push_back(((long long)side << 40) + ra);
+                    }
                 }
         }
-    if (!any) return PK_OK;
-    if (!b->d_fc_at[d]) {
-        HIPCHK(hipMalloc((void **)&b->d_fc_at[d], sizeof(int) * 4 * np));
-        HIPCHK(hipMalloc((void **)&b->d_fc_in[d], sizeof(double) * 2 * np * rec));
-        HIPCHK(hipMalloc((void **)&b->d_fc_out[d], sizeof(double) * 2 * np * rec));
+    if (nin + nout == 0) return PK_OK;
+    if (nin > 0x7fffffffll || nout > 0x7fffffffll) return fail(PK_EINVAL, "too many fine/coarse points");
+    if (!F.d_tab) HIPCHK(hipMalloc((void **)&F.d_tab, sizeof(int) * 8 * np));
+    if (!F.in_done) HIPCHK(hipEventCreateWithFlags(&F.in_done, hipEventDisableTiming));
+    RCCHK(fc_grow(&F.d_in, &F.in_cap, (size_t)nin, rec, false));
+    RCCHK(fc_grow(&F.d_out, &F.out_cap, (size_t)nout, rec, false));
+    HIPCHK(hipEventSynchronize(F.in_done));  // the previous call's upload from h_in has been read
+    RCCHK(fc_grow(&F.h_in, &F.h_in_cap, (size_t)nin, rec, true));
+    RCCHK(fc_grow(&F.h_out, &F.h_out_cap, (size_t)nout, rec, true));
+    for (long long s = 0; s < nin; s++) {
+        double *dst = F.h_in + s * rec;
+        const long long ra = ins[(size_t)s].first;
+        memcpy(dst, ins[(size_t)s].second + ra * nvar, sizeof(double) * nvar);
+        if (g_mhd) memcpy(dst + nvar, insb[(size_t)s] + ra * 4, sizeof(double) * 4);
     }
-    HIPCHK(hipMemcpyAsync(b->d_fc_at[d], b->h_fc_at.data(), sizeof(int) * 4 * np, hipMemcpyHostToDevice, g_stream));
-    HIPCHK(hipMemcpyAsync(b->d_fc_in[d], b->h_fc_in.data(), sizeof(double) * 2 * np * rec, hipMemcpyHostToDevice,
-                          g_stream));
+    if (F.tab != tab) {
+        F.tab.swap(tab);
+        HIPCHK(hipMemcpyAsync(F.d_tab, F.tab.data(), sizeof(int) * 8 * np, hipMemcpyHostToDevice, g_stream));
+    }
+    if (nin) HIPCHK(hipMemcpyAsync(F.d_in, F.h_in, sizeof(double) * nin * rec, hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipEventRecord(F.in_done, g_stream));
     *active = true;
     return PK_OK;
 }
@@ -457,40 +505,34 @@ int fc_begin(Block *b, int d, bool *active)
 void fc_args(const Block *b, int d, int f, pk::SweepArgs &a)
 {
     const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
-    a.fc_at = b->d_fc_at[d];
-    a.fc_in = b->d_fc_in[d];
-    a.fc_out = b->d_fc_out[d];
-    a.fc_npen = b->n[d1] * b->n[d2];
+    const long long np = (long long)b->n[d1] * b->n[d2];
+    a.fc_at = b->fcs[d].d_tab;
+    a.fc_slot = b->fcs[d].d_tab + 4 * np;
+    a.fc_in = b->fcs[d].d_in;
+    a.fc_out = b->fcs[d].d_out;
+    a.fc_npen = (int)np;
     a.fc_rec = fc_rec();
     a.fc_beg = 5 * f;  // flind order: fluid f's u-fluxes at 5f..5f+4
     a.fc_boff = 5 * g_nfl;
 }
 
-// save_outfluxes: the saved fluxes into the registered coarsebnd arrays, at
-// the pencils whose point is active.
+// save_outfluxes: the saved fluxes of the active points into the
+// registered coarsebnd arrays.
 int fc_end(Block *b, int d)
 {
-    if (!b->fc[d][1][0].index && !b->fc[d][1][1].index) return PK_OK;
-    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
-    const int n1 = b->n[d1], n2 = b->n[d2];
-    const long long np = (long long)n1 * n2;
+    Block::FcStage &F = b->fcs[d];
+    const size_t nout = F.out_at.size();
+    if (!nout) return PK_OK;
     const int rec = fc_rec(), nvar = 5 * g_nfl;
-    b->h_fc_out.resize((size_t)(2 * np * rec));
-    HIPCHK(hipMemcpyAsync(b->h_fc_out.data(), b->d_fc_out[d], sizeof(double) * 2 * np * rec, hipMemcpyDeviceToHost,
-                          g_stream));
+    HIPCHK(hipMemcpyAsync(F.h_out, F.d_out, sizeof(double) * nout * rec, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
-    for (int side = 0; side < 2; side++) {
+    for (size_t s = 0; s < nout; s++) {
+        const int side = (int)(F.out_at[s] >> 40);
+        const long long ra = F.out_at[s] & ((1ll << 40) - 1);
         const Block::FcReg &r = b->fc[d][1][side];
-        if (!r.index) continue;
-        for (int i2 = 0; i2 < n2; i2++)
-            for (int i1 = 0; i1 < n1; i1++) {
-                const long long pf = i1 + (long long)n1 * i2, at = fc_ref_at(d, i1, i2, n1, n2);
-                if (b->h_fc_at[(size_t)((2 + side) * np + pf)] == pk::kFcNone) continue;
-                const double *src = &b->h_fc_out[(size_t)((side * np + pf) * rec)];
-                for (int v = 0; v < nvar; v++) r.uflx[at * nvar + v] = src[v];
-                if (g_mhd)
-                    for (int v = 0; v < 4; v++) r.bflx[at * 4 + v] = src[nvar + v];
-            }
+        const double *src = F.h_out + s * rec;
+        memcpy(r.uflx + ra * nvar, src, sizeof(double) * nvar);
+        if (g_mhd) memcpy(r.bflx + ra * 4, src + nvar, sizeof(double) * 4);
     }
     return PK_OK;
 }
@@ -751,9 +793,13 @@ void free_block(Block *b)
     if (b->d_flux) (void)hipFree(b->d_flux);
     if (b->backup) (void)hipFree(b->backup);
     for (int d = 0; d < 3; d++) {
-        if (b->d_fc_at[d]) (void)hipFree(b->d_fc_at[d]);
-        if (b->d_fc_in[d]) (void)hipFree(b->d_fc_in[d]);
-        if (b->d_fc_out[d]) (void)hipFree(b->d_fc_out[d]);
+        Block::FcStage &f = b->fcs[d];
+        if (f.d_tab) (void)hipFree(f.d_tab);
+        if (f.d_in) (void)hipFree(f.d_in);
+        if (f.d_out) (void)hipFree(f.d_out);
+        if (f.h_in) (void)hipHostFree(f.h_in);
+        if (f.h_out) (void)hipHostFree(f.h_out);
+        if (f.in_done) (void)hipEventDestroy(f.in_done);
     }
     delete b;
 }
@@ -2266,21 +2312,44 @@ int datafield_args(Block *b, const char *name, pk::DatafieldArgs &a)
 
 extern "C" {
 
-int pk_datafield(int64_t cg_id, const char *name, double *out)
+}  // extern "C"
+
+namespace {
+
+int datafield(int64_t cg_id, const char *name, void *out, bool fp32)
 {
     Block *b;
     RCCHK(get_block(cg_id, &b));
     if (!out) return fail(PK_EINVAL, "no output array");
     pk::DatafieldArgs a;
     RCCHK(datafield_args(b, name, a));
-    const size_t bytes = sizeof(double) * (size_t)b->n[0] * b->n[1] * b->n[2];
+    const size_t cells = (size_t)b->n[0] * b->n[1] * b->n[2];
+    const size_t bytes = (fp32 ? sizeof(float) : sizeof(double)) * cells;
     RCCHK(ensure_stage(bytes));
-    a.out = g_stage;
-    HIPCHK(pk::launch_datafield(a, g_stream));
+    if (fp32)
+        a.out32 = (float *)g_stage;
+    else
+        a.out = g_stage;
+    // planes read per cell: the field's inputs, then the field written
+    int reads = 1;
+    if (a.kind == pk::DF_VEL) reads = 2;
+    if (a.kind == pk::DF_VTOT) reads = 4;
+    if (a.kind == pk::DF_PRE || a.kind == pk::DF_ETH) reads = 5 + (a.mag >= 0 ? 3 : 0);
+    if (a.kind == pk::DF_MAGB) reads = 3;
+    const double algo = 8.0 * cells * reads + (double)bytes;
+    RCCHK(launch_timed(3, algo, [&]() { return pk::launch_datafield(a, g_stream); }));
     HIPCHK(hipMemcpyAsync(out, g_stage, bytes, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int pk_datafield(int64_t cg_id, const char *name, double *out) { return datafield(cg_id, name, out, false); }
+
+int pk_datafield_f32(int64_t cg_id, const char *name, float *out) { return datafield(cg_id, name, out, true); }
 
 int pk_register_fcflx(int64_t cg_id, int32_t ddim, int32_t side, int32_t kind, int64_t first, const int32_t *index,
                       double *uflx, double *bflx)

@@ -48,7 +48,10 @@ __global__ __launch_bounds__(256) void k_datafield(const DatafieldArgs a)
             }
         }
     }
-    a.out[e] = r;
+    if (a.out32)
+        a.out32[e] = (float)r;  // round to nearest even, as HDF5's FP64 -> FP32 conversion
+    else
+        a.out[e] = r;
 }
 
 }  // namespace

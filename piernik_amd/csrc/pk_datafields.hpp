@@ -25,6 +25,7 @@ struct DatafieldArgs {
     int mag;              // first B plane, -1 without a magnetic field for this field
     double gam_1;
     double *out;          // n[0]*n[1]*n[2] values, x fastest (the reference's (nx, ny, nz))
+    float *out32;         // if set instead: the values rounded to nearest FP32
 };
 
 hipError_t launch_datafield(const DatafieldArgs &a, hipStream_t st);

@@ -268,13 +268,19 @@ void unit_of(const std::string &f, const pk_gdf_meta &m, double *val, std::strin
 
 const size_t kCbuff = 32, kUnits = 5 * 32, kFmax = 60;  // constants.F90:138-140, grid_data_format.F90
 
-// Field values of block c, field f into buf (n_b cells, x fastest).
-typedef int (*FieldSource)(const void *ctx, int c, int f, const pk_gdf_cg &cg, const char *name, double *buf);
+// Field values of block c, field f into buf (n_b cells, x fastest): FP64,
+// or FP32 when fp32 is set and the source can round on its side (*rounded
+// tells which it wrote).
+typedef int (*FieldSource)(const void *ctx, int c, int f, const pk_gdf_cg &cg, const char *name, void *buf,
+                           bool fp32, bool *rounded);
 
-int src_device(const void *, int, int, const pk_gdf_cg &cg, const char *name, double *buf)
+int src_device(const void *, int, int, const pk_gdf_cg &cg, const char *name, void *buf, bool fp32, bool *rounded)
 {
-    const int rc = pk_datafield(cg.cg_id, name, buf);
+    // FP32 datasets: rounded on the device, half the bytes over PCIe and no
+    // conversion in HDF5 (the same round-to-nearest)
+    const int rc = fp32 ? pk_datafield_f32(cg.cg_id, name, (float *)buf) : pk_datafield(cg.cg_id, name, (double *)buf);
     if (rc != PK_OK) return fail(rc, "pk_datafield(%lld, %s): %s", (long long)cg.cg_id, name, pk_last_error());
+    *rounded = fp32;
     return PK_OK;
 }
 
@@ -283,12 +289,13 @@ struct HostData {
     int nfields;
 };
 
-int src_host(const void *ctx, int c, int f, const pk_gdf_cg &cg, const char *, double *buf)
+int src_host(const void *ctx, int c, int f, const pk_gdf_cg &cg, const char *, void *buf, bool, bool *rounded)
 {
     const HostData *h = (const HostData *)ctx;
     const double *p = h->data[(size_t)c * h->nfields + f];
     if (!p) return fail(PK_EINVAL, "no data for block %d field %d", c, f);
     memcpy(buf, p, sizeof(double) * (size_t)cg.n_b[0] * cg.n_b[1] * cg.n_b[2]);
+    *rounded = false;
     return PK_OK;
 }
 
@@ -392,12 +399,15 @@ int write_file(const char *path, const pk_gdf_meta *mp, int32_t ncg, const pk_gd
             const hsize_t dims[3] = {(hsize_t)cg.n_b[2], (hsize_t)cg.n_b[1], (hsize_t)cg.n_b[0]};
             buf.resize((size_t)cg.n_b[0] * cg.n_b[1] * cg.n_b[2]);
             for (int f = 0; f < nfields; f++) {
-                if ((rc = src(ctx, c, f, cg, names[f].c_str(), buf.data()))) return rc;
+                bool rounded = false;
+                if ((rc = src(ctx, c, f, cg, names[f].c_str(), buf.data(), !m.h5_64bit, &rounded))) return rc;
                 Hid sp(H5Screate_simple(3, dims, nullptr), H5Sclose);
                 Hid d(H5Dcreate2(g.id, dnames[f].c_str(), ftype, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
                       H5Dclose);
                 if (!d.ok()) return fail(PK_EIO, "HDF5: dataset %s of %s", dnames[f].c_str(), gname);
-                H5TRY(H5Dwrite(d.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf.data()), "field write");
+                H5TRY(H5Dwrite(d.id, rounded ? H5T_NATIVE_FLOAT : H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT,
+                               buf.data()),
+                      "field write");
             }
         }
     }

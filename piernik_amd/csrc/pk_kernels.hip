@@ -50,7 +50,7 @@ __device__ __forceinline__ void fc_points(const SweepArgs &a, int pf, int cs, bo
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         if (a.fc_at[k * np + pf] != cs) continue;
-        const double *r = a.fc_in + ((long long)k * np + pf) * a.fc_rec;
+        const double *r = a.fc_in + (long long)a.fc_slot[k * np + pf] * a.fc_rec;
 #pragma unroll
         for (int v = 0; v < P::NV; v++) F[v] = r[v < 5 ? a.fc_beg + v : a.fc_boff + (v - 5)];
     }
@@ -58,7 +58,7 @@ __device__ __forceinline__ void fc_points(const SweepArgs &a, int pf, int cs, bo
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         if (a.fc_at[(2 + k) * np + pf] != cs) continue;
-        double *r = a.fc_out + ((long long)k * np + pf) * a.fc_rec;
+        double *r = a.fc_out + (long long)a.fc_slot[(2 + k) * np + pf] * a.fc_rec;
 #pragma unroll
         for (int v = 0; v < P::NV; v++) r[v < 5 ? a.fc_beg + v : a.fc_boff + (v - 5)] = F[v];
     }

@@ -42,8 +42,9 @@ struct SweepArgs {
     // stage mode only: fine/coarse flux points of the pencils (null when none)
     const int *fc_at;    // [4][fc_npen]: the cell whose right-face flux is replaced (li, ri) or saved (lo, ro);
                          // kFcNone (below every lane's cell) where absent
-    const double *fc_in; // [2][fc_npen][fc_rec]: replacement fluxes of li, ri
-    double *fc_out;      // [2][fc_npen][fc_rec]: saved fluxes of lo, ro
+    const int *fc_slot;  // [4][fc_npen]: the record of each active point in fc_in (li, ri) or fc_out (lo, ro)
+    const double *fc_in; // records of fc_rec values: replacement fluxes of the active li, ri points
+    double *fc_out;      // records: the saved fluxes of the active lo, ro points
     int fc_npen;         // pencils of the transverse interior (d1 fastest)
     int fc_rec;          // record length: the u-fluxes of every fluid, then [B, psi] with a magnetic field
     int fc_beg;          // this fluid's first u-flux in a record

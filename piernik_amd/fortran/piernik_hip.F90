@@ -10,7 +10,7 @@
 !
 module piernik_hip
 
-   use, intrinsic :: iso_c_binding, only: c_int32_t, c_int64_t, c_double, c_ptr, c_char, c_null_char, &
+   use, intrinsic :: iso_c_binding, only: c_int32_t, c_int64_t, c_double, c_float, c_ptr, c_char, c_null_char, &
         &                                  c_f_pointer, c_associated
 
    implicit none
@@ -28,7 +28,7 @@ module piernik_hip
         &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
         &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint, &
         &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit, pk_set_cfl_control, pk_redone_steps, &
-        &    pk_register_fcflx, pk_datafield
+        &    pk_register_fcflx, pk_datafield, pk_datafield_f32
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_FC_FINEBND, PK_FC_COARSEBND
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
@@ -74,9 +74,9 @@ module piernik_hip
    end type pk_stage_report
 
    type, bind(C) :: pk_profile_stats
-      integer(c_int64_t) :: launches(3)
-      real(c_double)     :: ms(3)
-      real(c_double)     :: algo_bytes(3)
+      integer(c_int64_t) :: launches(4)
+      real(c_double)     :: ms(4)
+      real(c_double)     :: algo_bytes(4)
    end type pk_profile_stats
 
    interface
@@ -193,6 +193,13 @@ module piernik_hip
          character(kind=c_char), dimension(*), intent(in)  :: name
          real(c_double),         dimension(*), intent(out) :: out
       end function pk_datafield
+
+      integer(c_int32_t) function pk_datafield_f32(cg_id, name, out) bind(C, name="pk_datafield_f32")
+         import :: c_int64_t, c_int32_t, c_char, c_float
+         integer(c_int64_t), value                         :: cg_id
+         character(kind=c_char), dimension(*), intent(in)  :: name
+         real(c_float),          dimension(*), intent(out) :: out
+      end function pk_datafield_f32
 
       integer(c_int32_t) function pk_register_fcflx(cg_id, ddim, side, kind, first, index, uflx, bflx) &
            &                                       bind(C, name="pk_register_fcflx")

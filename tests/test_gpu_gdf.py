@@ -73,6 +73,8 @@ def test_datafields_bit_exact(case):
             want = df.datafield(name, cg.u, cg.b, cg.psi, cfg.fluids, cg.nb, cg.has_dir)
             bad = np.argwhere(~(got == want))
             assert bad.size == 0, f"{case} {name}: {len(bad)} cells differ, first {tuple(bad[0])}"
+            # FP32 on the device: the round-to-nearest of the FP64 value
+            assert np.array_equal(pk.datafield_f32(cg, name), want.astype(np.float32)), name
     with pytest.raises(pk.PiernikError, match="not an output field"):
         pk.datafield(cgs[0], "nonsense")
     pk.finalize()
