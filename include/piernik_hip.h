@@ -267,6 +267,7 @@ int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar);
 #define PK_CFL_WARN 0      /* 'warn' / 'none': steps are never redone */
 #define PK_CFL_REDO 1      /* 'redo': redo with dt_full * dt_shrink**attempt */
 #define PK_CFL_FLEXIBLE 2  /* 'flexible': redo with dt_full * cfl_max/stepcfl (times dt_shrink from attempt 2) */
+#define PK_CFL_AUTO 3      /* 'auto' / 'adaptive': dt scaled by cfl_c from the change of c_max (cfl_auto) */
 
 /* cflcontrol, cfl_max, dt_shrink, max_redostep_attempts and
  * disallow_negatives of &NUMERICAL_SETUP (global.F90:118-120,216-258;
@@ -276,8 +277,11 @@ int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar);
  * (timestep_retry.F90:86-273, timestep.F90:247-361): when c_max of the new
  * state over c_max of the old, times cfl and the current shrink factor,
  * exceeds cfl_max (sanitized as global.F90:304 does), or when the floors met
- * negative values and disallow_negatives is set.  Default PK_CFL_WARN.  The
- * backups take one extra state per block. */
+ * negative values and disallow_negatives is set.  With PK_CFL_AUTO no step
+ * is redone: each step's dt is multiplied by cfl_c = min(1, (cfl_c +
+ * min(1, stepcfl_old/stepcfl*dt/dt_old))/2), stepcfl = c_max/c_max_old*cfl
+ * (cfl_auto, timestep.F90:368-405; cfl_c starts at 1 with each pk_run).
+ * Default PK_CFL_WARN.  The backups take one extra state per block. */
 int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t max_attempts,
                        int32_t disallow_negatives);
 

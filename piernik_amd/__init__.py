@@ -387,7 +387,7 @@ def update_boundaries(cg):
     _call("pk_fill_ghosts", cg.cg_id)
 
 
-CFL_CONTROL = {"warn": 0, "none": 0, "redo": 1, "repeat": 1, "flex": 2, "flexible": 2}
+CFL_CONTROL = {"warn": 0, "none": 0, "redo": 1, "repeat": 1, "flex": 2, "flexible": 2, "auto": 3, "adaptive": 3}
 
 
 def set_cfl_control(cflcontrol="warn", cfl_max=0.9, dt_shrink=0.5, max_redostep_attempts=10,

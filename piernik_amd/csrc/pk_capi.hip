@@ -1930,9 +1930,11 @@ struct StepLoop {
 
 int run_loop(StepLoop &L, int nsteps, double *t_out, double *dt_out, int *redone)
 {
-    const bool redo = g_cflc.mode != PK_CFL_WARN;
+    const bool redo = g_cflc.mode == PK_CFL_REDO || g_cflc.mode == PK_CFL_FLEXIBLE;
+    const bool aut = g_cflc.mode == PK_CFL_AUTO;
     const double cfl_max = cfl_max_sane();
     double t = 0.0, dt = -1.0, dt_old = -1.0, dt_full = 0.0, c_old = 0.0, shrink = 1.0;
+    double c_main_old = 0.0, stepcfl = 0.0, cfl_c = 1.0;  // timestep_pub.F90:42
     int s = 0, attempt = 0, nredo = 0;
     bool repeat = false;
     while (s < nsteps) {
@@ -1944,6 +1946,22 @@ int run_loop(StepLoop &L, int nsteps, double *t_out, double *dt_out, int *redone
                 const double cap = dt_old * 2.0;
                 if (cap < dt) dt = cap;
             }
+            if (aut) {
+                // cfl_auto (timestep.F90:368-405), called from the main time_step
+                double stepcfl_old = stepcfl;
+                stepcfl = g_cfg.cfl;
+                if (c_main_old > 0.0)
+                    stepcfl = c / c_main_old * g_cfg.cfl;
+                else
+                    stepcfl_old = g_cfg.cfl;
+                if (stepcfl > 0.0 && dt_old > 0.0) {
+                    cfl_c = fmin(1.0, 0.5 * (cfl_c + fmin(1.0, stepcfl_old / stepcfl * dt / dt_old)));
+                    dt = dt * cfl_c;
+                } else {
+                    cfl_c = 1.0;
+                }
+            }
+            c_main_old = c;
             dt_full = dt;
             c_old = c;
             if (redo) RCCHK(step_save(L.blocks));
@@ -2436,7 +2454,7 @@ int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out)
 int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t max_attempts,
                        int32_t disallow_negatives)
 {
-    if (mode != PK_CFL_WARN && mode != PK_CFL_REDO && mode != PK_CFL_FLEXIBLE)
+    if (mode != PK_CFL_WARN && mode != PK_CFL_REDO && mode != PK_CFL_FLEXIBLE && mode != PK_CFL_AUTO)
         return fail(PK_EINVAL, "unknown cfl control mode %d", mode);
     if (!(cfl_max > 0.0) || !(dt_shrink > 0.0 && dt_shrink < 1.0) || max_attempts < 0)
         return fail(PK_EINVAL, "cfl_max > 0, 0 < dt_shrink < 1 and max_attempts >= 0 required");

@@ -31,7 +31,11 @@ module piernik_hip
         &    pk_register_fcflx, pk_datafield, pk_datafield_f32
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_FC_FINEBND, PK_FC_COARSEBND
+   public :: PK_CFL_WARN, PK_CFL_REDO, PK_CFL_FLEXIBLE, PK_CFL_AUTO
    public :: PK_BATCH_GRAPH, PK_BATCH_PERSISTENT, PK_BATCH_PER_BLOCK
+
+   ! cflcontrol of pk_set_cfl_control: 'warn'/'none', 'redo', 'flexible', 'auto' (timestep.F90:73-90)
+   integer(c_int32_t), parameter :: PK_CFL_WARN = 0, PK_CFL_REDO = 1, PK_CFL_FLEXIBLE = 2, PK_CFL_AUTO = 3
 
    ! kinds of pk_register_fcflx: cg%finebnd, cg%coarsebnd (grid_container_fcflx.F90:40-41)
    integer(c_int32_t), parameter :: PK_FC_FINEBND = 0, PK_FC_COARSEBND = 1
