@@ -285,6 +285,17 @@ int pk_ensemble_timestep_adjoint(int32_t slot, const double *dt_bar);
 int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t max_attempts,
                        int32_t disallow_negatives);
 
+/* dt_initial, dt_max_grow, dt_min, dt_max of &NUMERICAL_SETUP (global.F90:
+ * 243-247, defaults -1, 2, tiny, huge) as time_step applies them to pk_run
+ * and pk_run_all (timestep.F90:198-224).  A call is one main loop from
+ * nstep = 0: dt_initial > 0 caps its first dt, dt_initial < 0 scales it by
+ * -dt_initial, later steps grow by at most dt_max_grow.  A dt below dt_min
+ * fails the call (the reference writes a crash dump).  tend is the simulated
+ * time left at the start of the call: dt <= (tend - t)/2 (+2 epsilon) as
+ * the split scheme clamps it, and the loop ends once t >= tend
+ * (piernik.F90:125).  pk_init restores the defaults (tend = huge). */
+int pk_set_time_control(double dt_initial, double dt_max_grow, double dt_min, double dt_max, double tend);
+
 /* Steps redone by the last pk_run / pk_run_all (no counterpart in the reference;
  * its log prints "Redoing previous step"). */
 int pk_redone_steps(int32_t *count);

@@ -91,6 +91,7 @@ _PROTOS = {
     "pk_fill_ghosts": [ctypes.c_int64],
     "pk_set_neighbour": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
     "pk_datafield": [ctypes.c_int64, ctypes.c_char_p, ctypes.c_void_p],
+    "pk_set_time_control": [ctypes.c_double] * 5,
     "pk_datafield_f32": [ctypes.c_int64, ctypes.c_char_p, ctypes.c_void_p],
     "pk_register_fcflx": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
@@ -395,6 +396,13 @@ def set_cfl_control(cflcontrol="warn", cfl_max=0.9, dt_shrink=0.5, max_redostep_
     """&NUMERICAL_SETUP's step-retry knobs for run / run_all (timestep.F90:73-90)."""
     _call("pk_set_cfl_control", CFL_CONTROL[cflcontrol], float(cfl_max), float(dt_shrink),
           int(max_redostep_attempts), int(bool(disallow_negatives)))
+
+
+def set_time_control(dt_initial=-1.0, dt_max_grow=2.0, dt_min=2.2250738585072014e-308, dt_max=1.7976931348623157e308,
+                     tend=1.7976931348623157e308):
+    """&NUMERICAL_SETUP's dt_initial, dt_max_grow, dt_min, dt_max and the time
+    left (tend) for run / run_all (pk_set_time_control)."""
+    _call("pk_set_time_control", float(dt_initial), float(dt_max_grow), float(dt_min), float(dt_max), float(tend))
 
 
 def redone_steps():

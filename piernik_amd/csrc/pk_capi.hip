@@ -1855,6 +1855,17 @@ struct CflControl {
 };
 CflControl g_cflc;
 
+// dt_initial, dt_max_grow, dt_min, dt_max of &NUMERICAL_SETUP
+// (global.F90:243-247) and tend, as time_step applies them (timestep.F90:198-224)
+struct TimeControl {
+    double dt_initial = -1.0;
+    double dt_max_grow = 2.0;
+    double dt_min = DBL_MIN;  // tiny(1.)
+    double dt_max = DBL_MAX;  // huge(1.)
+    double tend = DBL_MAX;    // simulated time left at the start of a pk_run / pk_run_all
+};
+TimeControl g_tc;
+
 // cfl_max as init_global sanitizes it (global.F90:304)
 double cfl_max_sane()
 {
@@ -1937,13 +1948,20 @@ int run_loop(StepLoop &L, int nsteps, double *t_out, double *dt_out, int *redone
     double c_main_old = 0.0, stepcfl = 0.0, cfl_c = 1.0;  // timestep_pub.F90:42
     int s = 0, attempt = 0, nredo = 0;
     bool repeat = false;
-    while (s < nsteps) {
+    while (s < nsteps && t < g_tc.tend) {
         if (!repeat) {
             double c;
             RCCHK(L.timestep(&dt, &c));
-            // time_step caps (timestep.F90:198-207) with dt_initial = -1, dt_max_grow = 2
-            if (s >= 1 && dt_old > 0.0) {
-                const double cap = dt_old * 2.0;
+            // time_step caps (timestep.F90:198-207); a call is a main loop from nstep = 0
+            if (s < 1) {
+                if (g_tc.dt_initial > 0.0) {
+                    if (g_tc.dt_initial < dt) dt = g_tc.dt_initial;
+                } else if (g_tc.dt_initial < 0.0) {
+                    const double cap = -g_tc.dt_initial * dt;
+                    if (cap < dt) dt = cap;
+                }
+            } else if (dt_old > 0.0) {
+                const double cap = dt_old * g_tc.dt_max_grow;
                 if (cap < dt) dt = cap;
             }
             if (aut) {
@@ -1962,6 +1980,14 @@ int run_loop(StepLoop &L, int nsteps, double *t_out, double *dt_out, int *redone
                 }
             }
             c_main_old = c;
+            if (dt < g_tc.dt_min)  // time_step writes a crash dump here (timestep.F90:212-219)
+                return fail(PK_EINVAL, "step %d: dt = %.6e is below dt_min = %.6e", s, dt, g_tc.dt_min);
+            {
+                // the split scheme's clamp to the time left (timestep.F90:224)
+                const double left = g_tc.tend - t;
+                const double lim = (0.5 * left) + ((2.0 * DBL_EPSILON) * left);
+                dt = fmin(fmin(dt, g_tc.dt_max), lim);
+            }
             dt_full = dt;
             c_old = c;
             if (redo) RCCHK(step_save(L.blocks));
@@ -2077,6 +2103,7 @@ int pk_init(const pk_config *cfg)
     g_nplanes = p;
     g_mhd = g_magf >= 0;
     g_cflc = CflControl();
+    g_tc = TimeControl();
     g_init = true;
     return PK_OK;
 }
@@ -2463,6 +2490,18 @@ int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t m
     g_cflc.dt_shrink = dt_shrink;
     g_cflc.max_attempts = max_attempts;
     g_cflc.disallow_negatives = disallow_negatives ? 1 : 0;
+    return PK_OK;
+}
+
+int pk_set_time_control(double dt_initial, double dt_max_grow, double dt_min, double dt_max, double tend)
+{
+    if (!(dt_max_grow > 0.0) || !(dt_min >= 0.0) || !(dt_max > 0.0) || !(tend > 0.0))
+        return fail(PK_EINVAL, "dt_max_grow, dt_max and tend must be positive, dt_min not negative");
+    g_tc.dt_initial = dt_initial;
+    g_tc.dt_max_grow = dt_max_grow;
+    g_tc.dt_min = dt_min;
+    g_tc.dt_max = dt_max;
+    g_tc.tend = tend;
     return PK_OK;
 }
 

@@ -28,7 +28,7 @@ module piernik_hip
         &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
         &    pk_ensemble_step_adjoint, pk_ensemble_timestep_adjoint, &
         &    pk_unsplit_stage, pk_fluid_update_unsplit, pk_run_unsplit, pk_set_cfl_control, pk_redone_steps, &
-        &    pk_register_fcflx, pk_datafield, pk_datafield_f32
+        &    pk_register_fcflx, pk_datafield, pk_datafield_f32, pk_set_time_control
    public :: PK_EULER, PK_RK2_1, PK_RK2_2, PK_LIM_VANLEER, PK_LIM_MINMOD, PK_LIM_MONCEN, PK_LIM_SUPERBEE
    public :: PK_FC_FINEBND, PK_FC_COARSEBND
    public :: PK_CFL_WARN, PK_CFL_REDO, PK_CFL_FLEXIBLE, PK_CFL_AUTO
@@ -191,6 +191,12 @@ module piernik_hip
       end function pk_run_unsplit
 
       ! name: NUL-terminated short name (hdf_vars); out: n_b(1)*n_b(2)*n_b(3) values
+      integer(c_int32_t) function pk_set_time_control(dt_initial, dt_max_grow, dt_min, dt_max, tend) &
+           &                                        bind(C, name="pk_set_time_control")
+         import :: c_int32_t, c_double
+         real(c_double), value :: dt_initial, dt_max_grow, dt_min, dt_max, tend
+      end function pk_set_time_control
+
       integer(c_int32_t) function pk_datafield(cg_id, name, out) bind(C, name="pk_datafield")
          import :: c_int64_t, c_int32_t, c_char, c_double
          integer(c_int64_t), value                         :: cg_id

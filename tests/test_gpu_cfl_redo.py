@@ -120,3 +120,42 @@ def test_warn_never_redoes():
     pk.run(cg, 4)
     assert pk.redone_steps() == 0
     pk.finalize()
+
+
+@pytest.mark.parametrize("driver", ["block", "list"])
+def test_time_control_matches_reference(oracle, driver):
+    """dt_initial > 0, dt_max_grow, dt_max and tend applied as time_step does
+    (timestep.F90:198-224), the loop ending at tend (piernik.F90:125).  The
+    oracle's time_step applies the default caps (dt_initial = -1,
+    dt_max_grow = 2); the tighter ones here are applied on top of them."""
+    dt_initial, grow, dt_max, tend = 2.0e-4, 1.2, 1.5e-3, 0.012
+    eps = np.finfo(np.float64).eps
+    cg, cfg = setups.sedov_hd(n=24)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    oracle.fill_ghosts(p, ob)
+    t, s, dt_old = 0.0, 0, -1.0
+    while s < 100 and t < tend:
+        dt, _ = oracle.time_step(p, ob, s, dt_old)
+        dt = min(dt, dt_initial) if s < 1 else min(dt, dt_old * grow)
+        dt = min(min(dt, dt_max), (0.5 * (tend - t)) + ((2.0 * eps) * (tend - t)))
+        oracle.fluid_update(p, ob, dt)
+        dt_old = dt
+        t = t + dt
+        t = t + dt
+        s += 1
+    assert 5 < s < 100
+    print(f"time control via {driver}: {s} steps", flush=True)
+    pk.load_library()
+    pk.finalize()
+    pk.init(cfg)
+    pk.set_time_control(dt_initial=dt_initial, dt_max_grow=grow, dt_max=dt_max, tend=tend)
+    cg.upload()
+    t_g, _ = pk.run(cg, 100) if driver == "block" else pk.run_all(100, pk.BATCH_GRAPH)
+    assert t_g == t and t_g >= tend
+    cg.download()
+    assert np.array_equal(cg.u, ob.u)
+    with pytest.raises(pk.PiernikError, match="below dt_min"):
+        pk.set_time_control(dt_min=1.0)
+        pk.run(cg, 1)
+    pk.finalize()
