@@ -185,8 +185,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path to run "
+              "(the domain would shard as z slabs exchanging nb ghost layers); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     # CPU baseline first, before this process touches the GPU
