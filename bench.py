@@ -160,6 +160,37 @@ def load_traffic(name):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # 256 CUs x 4 SIMDs, peak engine clock
+
+
+def fp64_view(prof, cells, name="sweep_mhd256_fp64_flops.json"):
+    """The sweep seen as the FP64 arithmetic it issues: committed per-cell
+    instruction counts (rocprofv3 --pmc SQ_INSTS_VALU*, one launch per
+    direction) times the cells of each timed launch, over the launches' HIP
+    event time.  valu_issue_frac: wave-level VALU instructions at 4 cycles
+    each (64 lanes on a 16-lane SIMD) over the SIMD cycles of that time."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        per = json.load(f)["per_dir"]
+    flops = sum(per[d]["fp64_flops_per_cell"] * cells * prof["launches"][d] for d in range(3))
+    insts = sum(per[d]["valu_insts_per_cell"] * cells / 64.0 * prof["launches"][d] for d in range(3))
+    sec = sum(prof["ms"][d] for d in range(3)) * 1e-3
+    if sec <= 0:
+        return None
+    achieved = flops / sec / 1e12
+    return {
+        "achieved_tflops": achieved,
+        "peak_tflops": FP64_PEAK_TFLOPS,
+        "frac": achieved / FP64_PEAK_TFLOPS,
+        "valu_issue_frac": insts * 4.0 / (sec * SIMDS * CLOCK_HZ),
+        "fp64_flops_per_cell": [per[d]["fp64_flops_per_cell"] for d in range(3)],
+        "counts": f"profiles/{name}",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,6 +351,8 @@ def main():
             "measured_copy_gbps": copy_gbps,
             "frac_of_measured_copy": achieved / copy_gbps,
         },
+        # the same launches as FP64 arithmetic: the bound the sweep actually meets
+        "fp64_valu": fp64_view(prof, cells) if (n == 256 and nv == 9 and not args.exact) else None,
         "cpu_baseline": cpu,
         "floors": rep,
         "adjoint_forward_cost_ratio": adj and adj["ratio"],
