@@ -25,7 +25,7 @@ struct UnsplitArgs {
     double *dmass;       // device: floor-added mass / cell volume
 };
 
-// Interface fluxes along dir of the transverse-interior pencils into F
+// Interface fluxes along dir of the transverse-interior pencils into F (tiled)
 // (the state's planes and strides; row c holds the flux at c-1/2, rows
 // g..g+n written).
 hipError_t launch_unsplit_flux(int dir, bool mhd, const UnsplitArgs &a, double *F, hipStream_t st);
