@@ -163,7 +163,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
         p = t % a.TP;
         s = t / a.TP;
     }
-    const int tm = max(t - step, 0), tp = min(t + step, nt - 1), tpp = min(t + 2 * step, nt - 1);
+    const int tm = max(t - step, 0), tp = min(t + step, nt - 1);
 
     const int chunk = bid % a.chunks;
     const int ptile = bid / a.chunks;
@@ -262,23 +262,23 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
             dtb += acc / a.dl;  // c2 = dt/dl
         }
     }
-    lds_put<NV>(S1, nt, t, lp);
+    lds_put<NV>(S3, nt, t, lp);  // S1 still holds every cell's stage-2 right-face state fm
     park(K3, lp);
     __syncthreads();
     double fpb[NV];
     {
-        // interface k+1/2 of stage 2: l = left state of cell k, r = right state of cell k+1
-        double lpn[NV], fb[NV], q2[NV], q3[NV], l[NV], r[NV], junk[NV], rb[NV];
-        lds_get<NV>(S1, nt, tp, lpn);
+        // interface k+1/2 of stage 2: l = left state of cell k (recomputed),
+        // r = right state of cell k+1 (the value the forward flux took)
+        double lpn[NV], fb[NV], q2[NV], l[NV], r[NV], junk[NV], rb[NV];
+        lds_get<NV>(S3, nt, tp, lpn);
 #pragma unroll
         for (int v = 0; v < NV; v++) fb[v] = a.c2 * (lpn[v] - lp[v]);
         double qc[NV];
         lds_get<NV>(Q2, nt, t, qc);
         lds_get<NV>(Q2, nt, tm, qm);
         lds_get<NV>(Q2, nt, tp, q2);
-        lds_get<NV>(Q2, nt, tpp, q3);
         faces<P, LIM>(a, qm, qc, q2, l, junk);
-        faces<P, LIM>(a, qc, q2, q3, junk, r);
+        lds_get<NV>(S1, nt, tp, r);
 #pragma unroll
         for (int v = 0; v < NV; v++) fpb[v] = rb[v] = 0.0;
         const double cb = flux_vjp<P>(a, l, r, fb, fpb, rb);
@@ -287,6 +287,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
     }
     __syncthreads();
     double lsp[NV];  // lam* through the floors of stage 1
+    double l1[NV];   // stage-1 left state of interface k+1/2
     {
         double fmb[NV], qb[NV], dlb[NV], drb[NV];
         double qc[NV];
@@ -325,11 +326,19 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
             dtb += (0.5 * acc) / a.dl;  // c1 = (dt/2)/dl
         }
         lds_put<NV>(S2, nt, t, lsp);
+        // stage-1 face states of cell k: l1 stays here, fm goes to Q2 (whose
+        // reads all came before the last barrier) for the left neighbour
+        double qc1[NV], qp1[NV], fm1[NV];
+        lds_get<NV>(Q1, nt, t, qc1);
+        lds_get<NV>(Q1, nt, tm, qm);
+        lds_get<NV>(Q1, nt, tp, qp1);
+        faces<P, LIM>(a, qm, qc1, qp1, l1, fm1);
+        lds_put<NV>(Q2, nt, t, fm1);
     }
     __syncthreads();
     // ---- reverse: stage 1 ----
     {
-        double lsn[NV], fb[NV], q2[NV], q3[NV], l[NV], r[NV], junk[NV], rb[NV];
+        double lsn[NV], fb[NV], r[NV], rb[NV];
         lds_get<NV>(S2, nt, tp, lsn);
         fetch(K3, lp);
 #pragma unroll
@@ -337,16 +346,10 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
             fb[v] = a.c1 * (lsn[v] - lsp[v]);
             lsp[v] = lsp[v] + lp[v];  // from here on lam* + lam' (the two direct terms)
         }
-        double qc[NV];
-        lds_get<NV>(Q1, nt, t, qc);
-        lds_get<NV>(Q1, nt, tm, qm);
-        lds_get<NV>(Q1, nt, tp, q2);
-        lds_get<NV>(Q1, nt, tpp, q3);
-        faces<P, LIM>(a, qm, qc, q2, l, junk);
-        faces<P, LIM>(a, qc, q2, q3, junk, r);
+        lds_get<NV>(Q2, nt, tp, r);  // stage-1 right state of cell k+1
 #pragma unroll
         for (int v = 0; v < NV; v++) fpb[v] = rb[v] = 0.0;
-        const double cb = flux_vjp<P>(a, l, r, fb, fpb, rb);
+        const double cb = flux_vjp<P>(a, l1, r, fb, fpb, rb);
         if (outl) chb += cb;
         lds_put<NV>(S1, nt, t, rb);
     }

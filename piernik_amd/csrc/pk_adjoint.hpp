@@ -9,9 +9,13 @@
 //   dt_vjp_hd(), dt_vjp_mhd()           timestep_fluid at one cell
 //                   (timestep.F90:460-490 with neu_cs / ion_cs)
 //
-// Each function re-evaluates the forward operations in the forward's order
-// (so the linearisation point is the forward's value bit for bit) and then
-// runs the chain rule backwards through the branch the forward took.
+// Each function re-evaluates the forward values that select a branch (wave
+// speeds and their signs, the star-state tests, limiter cases, min/max
+// choices, floors) in the forward's order, bit for bit, and then runs the
+// chain rule backwards through the branch the forward took.  Quotients that
+// only scale adjoints are formed with one reciprocal per divisor: they
+// differ from a division by at most an ulp, far below the derivative's own
+// truncation, and an IEEE division costs about ten FP64 instructions.
 // Where a min or max has two exactly equal arguments the derivative is split
 // evenly between them (the forward value does not depend on the choice).
 #pragma once
@@ -35,24 +39,24 @@ PK_HD void dmax(double a, double b, double &wa, double &wb)
 // q = prim(u); given qb = dL/dq, add dL/du to ub.
 PK_HD void prim_vjp_hd(const double (&u)[5], double gam_1, const double (&qb)[5], double (&ub)[5])
 {
-    const double d = u[0];
-    const double q1 = u[1] / d, q2 = u[2] / d, q3 = u[3] / d;
-    const double ek = (0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3])) / d;
+    const double rd = 1.0 / u[0];
+    const double q1 = u[1] * rd, q2 = u[2] * rd, q3 = u[3] * rd;
+    const double ek = (0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3])) * rd;
     // p = gam_1 * (E - ek)
     double db = qb[0];
     ub[4] += gam_1 * qb[4];
     const double ekb = -(gam_1 * qb[4]);
     // ek = S / d, S = 0.5*(m1^2 + m2^2 + m3^2)
-    const double sb = ekb / d;
-    db -= ekb * ek / d;
+    const double sb = ekb * rd;
+    db -= sb * ek;
     ub[1] += sb * u[1];
     ub[2] += sb * u[2];
     ub[3] += sb * u[3];
     // v_k = m_k / d
-    ub[1] += qb[1] / d;
-    ub[2] += qb[2] / d;
-    ub[3] += qb[3] / d;
-    db -= (qb[1] * q1 + qb[2] * q2 + qb[3] * q3) / d;
+    ub[1] += qb[1] * rd;
+    ub[2] += qb[2] * rd;
+    ub[3] += qb[3] * rd;
+    db -= (qb[1] * q1 + qb[2] * q2 + qb[3] * q3) * rd;
     ub[0] += db;
 }
 
@@ -67,9 +71,9 @@ PK_HD void limit_vjp(int lim, double dl, double dr, double dqb, double &dlb, dou
         if (dcen > 0.0) {
             // 2*dl*dr/(dl+dr)
             const double s = dl + dr;
-            const double s2 = s * s;
-            dlb = dqb * (2.0 * dr * dr) / s2;
-            drb = dqb * (2.0 * dl * dl) / s2;
+            const double g = dqb / (s * s);
+            dlb = g * (2.0 * dr * dr);
+            drb = g * (2.0 * dl * dl);
         }
         return;
     }
@@ -128,9 +132,10 @@ PK_HD void hllc_side_vjp(const double *q, double s_, double svx, double en, doub
                          double prt, double w, const double *fb, HllcSideBar &o)
 {
     const double ssm = s_ - sm;
-    const double d = dn / ssm;
+    const double rs = 1.0 / ssm;
+    const double d = dn * rs;
     const double num = (((svx * en) - (pr * q[1])) + (prt * sm));
-    const double us[5] = {d, d * sm, d * q[2], d * q[3], num / ssm};
+    const double us[5] = {d, d * sm, d * q[2], d * q[3], num * rs};
     const double U[5] = {q[0], q[0] * q[1], q[0] * q[2], q[0] * q[3], en};
     double usb[5], Ub[5];
     o.s = 0.0;
@@ -154,8 +159,8 @@ PK_HD void hllc_side_vjp(const double *q, double s_, double svx, double en, doub
     o.sm = usb[1] * d;
     o.q[2] += usb[2] * d;
     o.q[3] += usb[3] * d;
-    const double numb = usb[4] / ssm;
-    double ssmb = -(usb[4] * us[4]) / ssm;
+    const double numb = usb[4] * rs;
+    double ssmb = -(numb * us[4]);
     o.svx = numb * en;
     o.en += numb * svx;
     o.pr = -(numb * q[1]);
@@ -163,8 +168,8 @@ PK_HD void hllc_side_vjp(const double *q, double s_, double svx, double en, doub
     o.prt = numb * sm;
     o.sm += numb * prt;
     // d = dn / ssm
-    o.dn = db / ssm;
-    ssmb -= db * d / ssm;
+    o.dn = db * rs;
+    ssmb -= o.dn * d;
     // ssm = s_ - sm
     o.s += ssmb;
     o.sm -= ssmb;
@@ -174,10 +179,11 @@ PK_HD void hllc_side_vjp(const double *q, double s_, double svx, double en, doub
 PK_HD void hllc_vjp(const double *l, const double *r, double gam, const double *fb, double *lb, double *rb)
 {
     const double prl = l[4], prr = r[4];
+    const double rg1 = 1.0 / (gam - 1.0);
     const double kl = (l[1] * l[1] + l[2] * l[2]) + l[3] * l[3];
     const double kr = (r[1] * r[1] + r[2] * r[2]) + r[3] * r[3];
-    const double enl = (l[4] / (gam - 1.0)) + (0.5 * l[0]) * kl;
-    const double enr = (r[4] / (gam - 1.0)) + (0.5 * r[0]) * kr;
+    const double enl = (l[4] * rg1) + (0.5 * l[0]) * kl;
+    const double enr = (r[4] * rg1) + (0.5 * r[0]) * kr;
     const double cl2 = (gam * l[4]) / l[0], cr2 = (gam * r[4]) / r[0];
     const double c = sqrt(fmx(cl2, cr2));
     const double sl = fmn(l[1], r[1]) - c;
@@ -260,8 +266,7 @@ PK_HD void hllc_vjp(const double *l, const double *r, double gam, const double *
             slb += 0.5 * smb;
             srb += 0.5 * smb;
         } else {
-            const double den = auxr - auxl;
-            const double nb = smb / den, denb = -(smb * sm) / den;
+            const double nb = smb / (auxr - auxl), denb = -(nb * sm);
             double auxrb = nb * r[1] + denb;
             double auxlb = -(nb * l[1]) - denb;
             rq[1] += nb * auxr;
@@ -287,11 +292,11 @@ PK_HD void hllc_vjp(const double *l, const double *r, double gam, const double *
         const double cb = srb - slb;
         const double cmb = cb / (2.0 * c);
         dmax(cl2, cr2, wl, wr);
-        const double cl2b = cmb * wl, cr2b = cmb * wr;
-        lq[4] += cl2b * gam / l[0];
-        lq[0] -= cl2b * cl2 / l[0];
-        rq[4] += cr2b * gam / r[0];
-        rq[0] -= cr2b * cr2 / r[0];
+        const double gl = (cmb * wl) / l[0], gr = (cmb * wr) / r[0];
+        lq[4] += gl * gam;
+        lq[0] -= gl * cl2;
+        rq[4] += gr * gam;
+        rq[0] -= gr * cr2;
     }
     // physical fluxes of both sides
     const double *qs[2] = {l, r};
@@ -314,7 +319,7 @@ PK_HD void hllc_vjp(const double *l, const double *r, double gam, const double *
         prb[side] += g[1] + g[4] * q[1];
         enb[side] += g[4] * q[1];
         // en = p/(gam-1) + (0.5*q0) * K, K = q1^2 + q2^2 + q3^2
-        o[4] += enb[side] / (gam - 1.0) + prb[side];
+        o[4] += enb[side] * rg1 + prb[side];
         o[0] += enb[side] * 0.5 * ks[side];
         const double kb = enb[side] * 0.5 * q[0];
         o[1] += 2.0 * q[1] * kb;
@@ -337,7 +342,8 @@ PK_HD void hlld_side_vjp(const double *q, double b2, double pr, double gam, cons
                          double fbzb, const double *Ub, double enb, double *qb, double &b2b, double &prb)
 {
     const double K = (q[1] * q[1] + q[2] * q[2]) + q[3] * q[3];
-    const double en = ((q[4] / (gam - 1.0)) + (0.5 * q[0]) * K) + 0.5 * b2;
+    const double rg1 = 1.0 / (gam - 1.0);
+    const double en = ((q[4] * rg1) + (0.5 * q[0]) * K) + 0.5 * b2;
     const double W = (q[1] * q[5] + q[2] * q[6]) + q[3] * q[7];
     const double ent = enb + Ub[4] + fb[4] * q[1];
     const double Wb = -(fb[4] * q[5]);
@@ -348,7 +354,7 @@ PK_HD void hlld_side_vjp(const double *q, double b2, double pr, double gam, cons
              fb[4] * (en + pr) + Wb * q[5] + fbyb * q[6] + fbzb * q[7] + Ub[1] * q[0] + ent * q[0] * q[1];
     qb[2] += fb[2] * q[0] * q[1] + Wb * q[6] - fbyb * q[5] + Ub[2] * q[0] + ent * q[0] * q[2];
     qb[3] += fb[3] * q[0] * q[1] + Wb * q[7] - fbzb * q[5] + Ub[3] * q[0] + ent * q[0] * q[3];
-    qb[4] += ent / (gam - 1.0);
+    qb[4] += ent * rg1;
     qb[5] += -(2.0 * q[5] * fb[1]) - q[6] * fb[2] - q[7] * fb[3] - fb[4] * W + Wb * q[1] - fbyb * q[2] - fbzb * q[3];
     qb[6] += -(q[5] * fb[2]) + Wb * q[2] + fbyb * q[1];
     qb[7] += -(q[5] * fb[3]) + Wb * q[3] + fbzb * q[1];
@@ -362,12 +368,14 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
 {
     // GLM fluxes (hlld.F90:229-230)
     const double kch = (ch * ch) * 0.5;
-    const double A = (r[5] + l[5]) - (r[8] - l[8]) / ch;
+    const double rch = 1.0 / ch;
+    const double A = (r[5] + l[5]) - (r[8] - l[8]) * rch;
     double chb = fb[8] * (ch * A + 0.5 * (r[8] - l[8])) - fb[5] * 0.5 * (r[5] - l[5]);
     lb[5] += fb[8] * kch + fb[5] * 0.5 * ch;
     rb[5] += fb[8] * kch - fb[5] * 0.5 * ch;
-    lb[8] += fb[8] * kch / ch + fb[5] * 0.5;
-    rb[8] += -(fb[8] * kch / ch) + fb[5] * 0.5;
+    const double g8 = fb[8] * kch * rch;
+    lb[8] += g8 + fb[5] * 0.5;
+    rb[8] += -g8 + fb[5] * 0.5;
 
     // ---- forward values ----
     const double bxl = l[5], byl = l[6], bzl = l[7];
@@ -454,10 +462,9 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
         const double vb_r = (r[1] * bxr + r[2] * byr) + r[3] * bzr;
         const double vb_sl = (sm * bxl + vsly * bsly) + vslz * bslz;
         const double vb_sr = (sm * bxr + vsry * bsry) + vsrz * bsrz;
-        const double enl = ((l[4] / (gam - 1.0)) + (0.5 * l[0]) * ((l[1] * l[1] + l[2] * l[2]) + l[3] * l[3])) +
-                           0.5 * b2l;
-        const double enr = ((r[4] / (gam - 1.0)) + (0.5 * r[0]) * ((r[1] * r[1] + r[2] * r[2]) + r[3] * r[3])) +
-                           0.5 * b2r;
+        const double rg1 = 1.0 / (gam - 1.0);
+        const double enl = ((l[4] * rg1) + (0.5 * l[0]) * ((l[1] * l[1] + l[2] * l[2]) + l[3] * l[3])) + 0.5 * b2l;
+        const double enr = ((r[4] * rg1) + (0.5 * r[0]) * ((r[1] * r[1] + r[2] * r[2]) + r[3] * r[3])) + 0.5 * b2r;
         const double nl = (((slvxl * enl) - (prl * l[1])) + (prt * sm)) + bxl * (vb_l - vb_sl);
         const double nr = (((srvxr * enr) - (prr * r[1])) + (prt * sm)) + bxr * (vb_r - vb_sr);
         const double usl4 = nl / slsm, usr4 = nr / srsm;
@@ -579,6 +586,7 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
         }
         // us = [ds, ds*sm, ds*vsy, ds*vsz, n/ssm]
         double slsmb = 0.0, srsmb = 0.0;
+        const double rsl = 1.0 / slsm, rsr = 1.0 / srsm;
         {
             dslb += usl_b[0] + usl_b[1] * sm + usl_b[2] * vsly + usl_b[3] * vslz;
             dsrb += usr_b[0] + usr_b[1] * sm + usr_b[2] * vsry + usr_b[3] * vsrz;
@@ -587,9 +595,9 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
             vslzb += usl_b[3] * dsl;
             vsryb += usr_b[2] * dsr;
             vsrzb += usr_b[3] * dsr;
-            const double nlb = usl_b[4] / slsm, nrb = usr_b[4] / srsm;
-            slsmb -= usl_b[4] * usl4 / slsm;
-            srsmb -= usr_b[4] * usr4 / srsm;
+            const double nlb = usl_b[4] * rsl, nrb = usr_b[4] * rsr;
+            slsmb -= nlb * usl4;
+            srsmb -= nrb * usr4;
             // n = (((svx*en) - (pr*q1)) + (prt*sm)) + bx*(vb - vbs)
             slvxlb += nlb * enl;
             enlb += nlb * slvxl;
@@ -650,8 +658,9 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
             ql[5] += 0.5 * b2xb;
             qr[5] += 0.5 * b2xb;
             // b2y = ((sqL*bsry + sqR*bsly) + (b_sig*mul)*(vsry - vsly)) / add
-            double addb = -(b2y_b * b2y + b2z_b * b2z + v2yb * v2y + v2zb * v2z) / add;
-            const double ny = b2y_b / add, nz = b2z_b / add;
+            const double radd = 1.0 / add;
+            double addb = -(b2y_b * b2y + b2z_b * b2z + v2yb * v2y + v2zb * v2z) * radd;
+            const double ny = b2y_b * radd, nz = b2z_b * radd;
             dlsb += ny * bsry + nz * bsrz;
             bsry_b += ny * dn_lsqt;
             bsrz_b += nz * dn_lsqt;
@@ -664,7 +673,7 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
             vsrzb += nz * (b_sig * mul);
             vslzb -= nz * (b_sig * mul);
             // v2y = ((sqL*vsly + sqR*vsry) + b_sig*(bsry - bsly)) / add
-            const double my = v2yb / add, mz = v2zb / add;
+            const double my = v2yb * radd, mz = v2zb * radd;
             dlsb += my * vsly + mz * vslz;
             vslyb += my * dn_lsqt;
             vslzb += mz * dn_lsqt;
@@ -684,21 +693,23 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
             smb += all_b + alr_b;
             const double sgl = (bxl > 0.0) ? 1.0 : -1.0;
             const double sgr = (bxr > 0.0) ? 1.0 : ((bxr < 0.0) ? -1.0 : 0.0);
-            ql[5] -= all_b * sgl / dn_lsqt;
-            dlsb += all_b * fabs(bxl) / (dn_lsqt * dn_lsqt);
-            qr[5] += alr_b * sgr / dn_rsqt;
-            drsb -= alr_b * fabs(bxr) / (dn_rsqt * dn_rsqt);
+            const double rql = 1.0 / dn_lsqt, rqr = 1.0 / dn_rsqt;
+            ql[5] -= all_b * sgl * rql;
+            dlsb += all_b * fabs(bxl) * (rql * rql);
+            qr[5] += alr_b * sgr * rqr;
+            drsb -= alr_b * fabs(bxr) * (rqr * rqr);
             // sq = sqrt(ds)
-            dslb += dlsb / (2.0 * dn_lsqt);
-            dsrb += drsb / (2.0 * dn_rsqt);
+            dslb += dlsb * (0.5 * rql);
+            dsrb += drsb * (0.5 * rqr);
         }
         // ds = dn / ssm
-        double dnlb = dslb / slsm, dnrb = dsrb / srsm;
-        slsmb -= dslb * dsl / slsm;
-        srsmb -= dsrb * dsr / srsm;
+        double dnlb = dslb * rsl, dnrb = dsrb * rsr;
+        slsmb -= dnlb * dsl;
+        srsmb -= dnrb * dsr;
         // vs = q2 + (bx/dn)*(by - bsy)
         {
-            const double ol = bxl / dn_l, orr = bxr / dn_r;
+            const double rdl = 1.0 / dn_l, rdr = 1.0 / dn_r;
+            const double ol = bxl * rdl, orr = bxr * rdr;
             ql[2] += vslyb;
             ql[3] += vslzb;
             qr[2] += vsryb;
@@ -713,28 +724,30 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
             bsry_b -= vsryb * orr;
             qr[7] += vsrzb * orr;
             bsrz_b -= vsrzb * orr;
-            ql[5] += olb / dn_l;
-            dnlb -= olb * ol / dn_l;
-            qr[5] += orb / dn_r;
-            dnrb -= orb * orr / dn_r;
+            ql[5] += olb * rdl;
+            dnlb -= olb * ol * rdl;
+            qr[5] += orb * rdr;
+            dnrb -= orb * orr * rdr;
         }
         // star transverse B
         double hyb = 0.0, hzb = 0.0, c1b = 0.0, c2b = 0.0, klb = 0.0, krb = 0.0, b_lrb = 0.0;
         if (okl) {
             // bsy = (by*k)/c1
-            ql[6] += bsly_b * kl / c1;
-            ql[7] += bslz_b * kl / c1;
-            klb += (bsly_b * byl + bslz_b * bzl) / c1;
-            c1b -= (bsly_b * bsly + bslz_b * bslz) / c1;
+            const double rc = 1.0 / c1;
+            ql[6] += bsly_b * kl * rc;
+            ql[7] += bslz_b * kl * rc;
+            klb += (bsly_b * byl + bslz_b * bzl) * rc;
+            c1b -= (bsly_b * bsly + bslz_b * bslz) * rc;
         } else {
             hyb += bsly_b;
             hzb += bslz_b;
         }
         if (okr) {
-            qr[6] += bsry_b * kr / c2;
-            qr[7] += bsrz_b * kr / c2;
-            krb += (bsry_b * byr + bsrz_b * bzr) / c2;
-            c2b -= (bsry_b * bsry + bsrz_b * bsrz) / c2;
+            const double rc = 1.0 / c2;
+            qr[6] += bsry_b * kr * rc;
+            qr[7] += bsrz_b * kr * rc;
+            krb += (bsry_b * byr + bsrz_b * bzr) * rc;
+            c2b -= (bsry_b * bsry + bsrz_b * bsrz) * rc;
         } else {
             hyb += bsry_b;
             hzb += bsrz_b;
@@ -754,8 +767,9 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
         double srmslb = 0.0;
         if (hll) {
             // h = ((sr*byr - sl*byl) - (fbyr - fbyl)) / srmsl
-            const double ny = hyb / srmsl, nz = hzb / srmsl;
-            srmslb -= (hyb * hy + hzb * hz) / srmsl;
+            const double rh = 1.0 / srmsl;
+            const double ny = hyb * rh, nz = hzb * rh;
+            srmslb -= (hyb * hy + hzb * hz) * rh;
             srb += ny * byr + nz * bzr;
             qr[6] += ny * sr;
             qr[7] += nz * sr;
@@ -810,8 +824,7 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
             slb += 0.5 * smb;
             srb += 0.5 * smb;
         } else {
-            const double den = auxr - auxl;
-            const double nb = smb / den, denb = -(smb * sm) / den;
+            const double nb = smb / (auxr - auxl), denb = -(nb * sm);
             const double auxrb = nb * r[1] + denb, auxlb = -(nb * l[1]) - denb;
             qr[1] += nb * auxr;
             ql[1] -= nb * auxl;
@@ -849,7 +862,7 @@ PK_HD double hlld_vjp(const double *l, const double *r, double gam, double ch, c
             double *qb = qb_[side];
             if (cb_[side] == 0.0) continue;
             const double nb = cb_[side] / q[0];
-            qb[0] -= cb_[side] * c_[side] / q[0];
+            qb[0] -= nb * c_[side];
             const double sqb = nb;
             double tb = nb;
             // sq = sqrt(t^2 - 4 gp bx^2)
@@ -917,7 +930,7 @@ PK_HD void floors_vjp_mhd(const double (&u)[9], double smalld, double smallei, i
     double db = ub[0];
     ub[4] = eb;
     const double sb = kb / d;
-    db -= kb * ek / d;
+    db -= sb * ek;
     ub[1] += sb * u[1];
     ub[2] += sb * u[2];
     ub[3] += sb * u[3];
@@ -996,7 +1009,7 @@ PK_HD void floors_vjp_hd(const double (&u)[5], double smalld, double smallei, in
     double db = ub[0];
     ub[4] = eb;
     const double sb = ekb / d;
-    db -= ekb * ek / d;
+    db -= sb * ek;
     ub[1] += sb * u[1];
     ub[2] += sb * u[2];
     ub[3] += sb * u[3];
