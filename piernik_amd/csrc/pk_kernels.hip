@@ -511,20 +511,26 @@ __global__ __launch_bounds__(512) void k_sweep(const SweepArgs a)
 }
 
 // Persistent fused sweep over a block list.  Each workgroup takes the next
-// tile from the queue (one atomic per tile), loads that block's arguments
-// and runs the tile; it leaves when the queue is empty, so every workgroup
-// ends after at most nitems + gridDim.x fetches in total.
+// `grab` tiles from the queue (one atomic per fetch), and for each loads
+// that block's arguments and runs the tile; it leaves when the queue is
+// empty, so every workgroup ends after at most nitems / grab + gridDim.x
+// fetches in total.
 template <int DIR, class P, int LIM>
 __global__ __launch_bounds__(512) void k_sweep_list(const ListArgs L)
 {
     extern __shared__ double lds[];
     __shared__ unsigned s_item;
+    const unsigned grab = L.grab > 1u ? L.grab : 1u;
+    unsigned item = 0u, end = 0u;
     for (;;) {
         __syncthreads();  // the previous tile is done with LDS and s_item
-        if (threadIdx.x == 0) s_item = atomicAdd(L.queue, 1u);
-        __syncthreads();
-        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)s_item);
-        if (item >= L.nitems) break;
+        if (item >= end) {
+            if (threadIdx.x == 0) s_item = atomicAdd(L.queue, grab);
+            __syncthreads();
+            item = (unsigned)__builtin_amdgcn_readfirstlane((int)s_item);
+            if (item >= L.nitems) break;
+            end = min(item + grab, L.nitems);
+        }
         // the tables are read-only while the kernel runs: read them through the
         // constant address space so the uniform loads go to scalar registers
         typedef const __attribute__((address_space(4))) unsigned cu32;
@@ -555,6 +561,7 @@ __global__ __launch_bounds__(512) void k_sweep_list(const ListArgs L)
         a.nslots = L.nslots;
         a.images = 1;
         sweep_tile<DIR, P, true, LIM>(a, (int)(item - first), item, lds, lds + P::NV * blockDim.x);
+        item++;
     }
 }
 

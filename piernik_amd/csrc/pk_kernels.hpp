@@ -103,6 +103,7 @@ struct ListArgs {
     int nslots;                   // slots (per block when per_block)
     int per_block;                // independent instances: each block has its own dt, ch, psi factor
                                   // and its own nslots dt slots
+    unsigned grab;                // consecutive items taken per queue fetch (0 or 1: one)
 };
 
 // One device-to-device copy of a batch (k_copy_blocks).
