@@ -16,8 +16,8 @@ LIB = os.path.join(HERE, "libpiernik_hip.so")
 GDF_LIB = os.path.join(HERE, "libpiernik_gdf.so")
 HDF5 = os.environ.get("PK_HDF5_ROOT", "/opt/conda")  # HDF5 1.10 C library and headers of this image
 
-SOURCES = ["pk_kernels.hip", "pk_kernels_fast.hip", "pk_adjoint.hip", "pk_unsplit.hip", "pk_datafields.hip",
-           "pk_capi.hip"]
+SOURCES = ["pk_kernels.hip", "pk_kernels_fast.hip", "pk_adjoint.hip", "pk_unsplit.hip", "pk_unsplit_fast.hip",
+           "pk_datafields.hip", "pk_capi.hip"]
 
 # -ffp-contract=off: every a*b+c rounds twice, as the reference's FP64 code.
 # -packed-fp32-ops off: no packed FP32 instructions are generated.

@@ -1807,10 +1807,12 @@ int unsplit_stage(Block *b, int istep, double dt, double ch)
     for (int d = 0; d < 3; d++) {
         if (!g_cfg.has_dir[d]) continue;
         double *Fd = b->d_flux + (size_t)d * g_nplanes * b->sv;
-        HIPCHK(pk::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream));
+        HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream)
+                                  : pk::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream));
         F[d] = Fd;
     }
-    HIPCHK(pk::launch_unsplit_update(g_mhd, a, F, g_stream));
+    HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_update(g_mhd, a, F, g_stream)
+                              : pk::launch_unsplit_update(g_mhd, a, F, g_stream));
     RCCHK(fill_buffer(b, dst));
     b->uh = euler ? -1 : (first ? dst : b->uh);
     b->dt_pending = false;

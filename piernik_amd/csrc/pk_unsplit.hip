@@ -17,12 +17,16 @@
 // direction's sweep order.  Ghost cells are not written: update_boundaries
 // replaces them after every stage.
 //
-// Built without FMA contraction: bit-identical to the oracle's unsplit driver.
+// Built without FMA contraction (namespace pk): bit-identical to the
+// oracle's unsplit driver.  pk_unsplit_fast.hip builds the same source with
+// contraction and fast division into namespace pk_fast.
 #include <hip/hip_runtime.h>
 #include "pk_math.hpp"
 #include "pk_unsplit.hpp"
 
-namespace pk {
+namespace PK_KNS {
+
+using pk::UnsplitArgs;
 
 namespace {
 
@@ -238,4 +242,4 @@ hipError_t launch_unsplit_update(bool mhd, const UnsplitArgs &a, const double *c
     return hipGetLastError();
 }
 
-}  // namespace pk
+}  // namespace PK_KNS

@@ -34,3 +34,10 @@ hipError_t launch_unsplit_flux(int dir, bool mhd, const UnsplitArgs &a, double *
 hipError_t launch_unsplit_update(bool mhd, const UnsplitArgs &a, const double *const F[3], hipStream_t st);
 
 }  // namespace pk
+
+// The same kernels built with FMA contraction and fast division and square
+// root (pk_unsplit_fast.hip).
+namespace pk_fast {
+hipError_t launch_unsplit_flux(int dir, bool mhd, const pk::UnsplitArgs &a, double *F, hipStream_t st);
+hipError_t launch_unsplit_update(bool mhd, const pk::UnsplitArgs &a, const double *const F[3], hipStream_t st);
+}  // namespace pk_fast

@@ -102,3 +102,42 @@ def test_cpaw_128x64_known_answer():
     print(f"steps {n}, L2 {l2:.8f}")
     assert n == 224 and f"{l2:.8f}" == "0.00102726"
     pk.finalize()
+
+
+def rel_l1(a, b):
+    return float(np.sum(np.abs(a - b)) / max(float(np.sum(np.abs(b))), 1e-300))
+
+
+@pytest.mark.parametrize("mag", [True, False])
+def test_relaxed_build_within_1e10_of_oracle(oracle, mag):
+    """The FMA-contracted build (pk_config.fast_math, the bench's default)
+    against the oracle after 20 steps: rel-L1 <= 1e-10 per conserved
+    variable, the bar for floating-point parity, with positive density."""
+    n = (24, 20, 16)
+    cg = setups.smooth_random(n, seed=33, magnetized=mag)
+    if mag:
+        cg.psi[...] = np.random.default_rng(6).normal(0, 0.01, cg.psi.shape)
+    setups.fill_ghosts_periodic(cg)
+    cfg = pk.Config(magnetized=mag, limiter="minmod", cfl=0.3, cfl_glm=0.3, smalld=1e-10, smallei=1e-10,
+                    fast_math=True)
+    print(f"unsplit relaxed build {n} mag={mag}, 20 steps", flush=True)
+    start(cfg)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    cg.upload()
+    t_g, n_g = pk.run_unsplit(cg, 1.0, nmax=20)
+    t_o, n_o = oracle.run_unsplit(p, ob, 1.0, nmax=20)
+    cg.download()
+    assert n_g == n_o == 20 and abs(t_g - t_o) <= 1e-12 * t_o
+    worst = 0.0
+    for name in ("u", "b"):
+        a, o = getattr(cg, name), getattr(ob, name)
+        if a is None:
+            continue
+        for v in range(a.shape[0]):
+            err = rel_l1(cg.interior(a)[v], cg.interior(o)[v])
+            worst = max(worst, err)
+            assert err <= 1e-10, (name, v, err)
+    assert cg.interior(cg.u)[0].min() > 0.0
+    print(f"worst rel-L1 {worst:.3e}")
+    pk.finalize()
