@@ -1763,7 +1763,6 @@ int unsplit_stage(Block *b, int istep, double dt, double ch)
     for (int d = 0; d < 3; d++)
         if (g_cfg.has_dir[d] && b->g[d] < 2) return fail(PK_EINVAL, "the unsplit stage needs nb >= 2");
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
-    if (!b->d_flux) HIPCHK(hipMalloc((void **)&b->d_flux, sizeof(double) * 3 * (size_t)g_nplanes * b->sv));
     int src, dst;
     if (first) {
         src = b->cur;
@@ -1803,16 +1802,34 @@ int unsplit_stage(Block *b, int istep, double dt, double ch)
     a.use_smallei = g_cfg.use_smallei;
     a.flags = b->d_flags;
     a.dmass = b->d_dmass;
-    const double *F[3] = {nullptr, nullptr, nullptr};
-    for (int d = 0; d < 3; d++) {
-        if (!g_cfg.has_dir[d]) continue;
-        double *Fd = b->d_flux + (size_t)d * g_nplanes * b->sv;
-        HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream)
-                                  : pk::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream));
-        F[d] = Fd;
+    if (src != dst) {
+        // one pass per direction: flux and flux difference, no flux arrays
+        int d_first = -1, d_last = -1;
+        for (int d = 0; d < 3; d++)
+            if (g_cfg.has_dir[d]) {
+                if (d_first < 0) d_first = d;
+                d_last = d;
+            }
+        for (int d = 0; d < 3; d++) {
+            if (!g_cfg.has_dir[d]) continue;
+            const int fi = d == d_first, la = d == d_last;
+            HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_fused(d, g_mhd, a, fi, la, g_stream)
+                                   : pk::launch_unsplit_fused(d, g_mhd, a, fi, la, g_stream));
+        }
+    } else {
+        // in place (the Euler stage): every flux before any update
+        if (!b->d_flux) HIPCHK(hipMalloc((void **)&b->d_flux, sizeof(double) * 3 * (size_t)g_nplanes * b->sv));
+        const double *F[3] = {nullptr, nullptr, nullptr};
+        for (int d = 0; d < 3; d++) {
+            if (!g_cfg.has_dir[d]) continue;
+            double *Fd = b->d_flux + (size_t)d * g_nplanes * b->sv;
+            HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream)
+                                   : pk::launch_unsplit_flux(d, g_mhd, a, Fd, g_stream));
+            F[d] = Fd;
+        }
+        HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_update(g_mhd, a, F, g_stream)
+                               : pk::launch_unsplit_update(g_mhd, a, F, g_stream));
     }
-    HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_update(g_mhd, a, F, g_stream)
-                              : pk::launch_unsplit_update(g_mhd, a, F, g_stream));
     RCCHK(fill_buffer(b, dst));
     b->uh = euler ? -1 : (first ? dst : b->uh);
     b->dt_pending = false;

@@ -17,6 +17,12 @@
 // direction's sweep order.  Ghost cells are not written: update_boundaries
 // replaces them after every stage.
 //
+// k_unsplit_fused_tile: both in one pass per direction, for stages whose
+// output is not their input (the two RK2 stages): the flux tile adds each
+// cell's flux difference to a running sum kept in the output buffer, and the
+// last direction's pass runs the sources.  Same operands in the same order,
+// so the same bits, without writing and re-reading the flux arrays.
+//
 // Built without FMA contraction (namespace pk): bit-identical to the
 // oracle's unsplit driver.  pk_unsplit_fast.hip builds the same source with
 // contraction and fast division into namespace pk_fast.
@@ -125,6 +131,136 @@ __global__ __launch_bounds__(256) void k_unsplit_flux_tile(const UnsplitArgs a, 
     for (int k = 0; k < NV; k++) F[(long long)uplane(DIR, k) * a.sv + row] = f[k];
 }
 
+// Flux along DIR and its difference in one tile (dst != src).  The tile
+// starts two cells before its first updated cell (TS-4 per tile); lanes 1 ..
+// TS-3 evaluate the flux at their cell's right face as k_unsplit_flux_tile
+// does, and the lane of cell c adds coef*(F(c-1/2) - F(c+1/2)), the left
+// flux from its neighbour through LDS, to the cell's running sum.  The sum
+// starts from base at the first active direction and waits in dst between
+// directions, each cell read and written by its own lane only; after the
+// last active direction the source passes run and dst holds the stage's
+// result.  The operands and their order are those of k_unsplit_update, so
+// the result is the same bit for bit, without the flux arrays.
+template <int DIR, class P, int LIM>
+__global__ __launch_bounds__(256) void k_unsplit_fused_tile(const UnsplitArgs a, int TP, int TS, int chunks,
+                                                            int first, int last)
+{
+    constexpr int NV = P::NV;
+    constexpr int d1 = (DIR == 0) ? 1 : 0;
+    constexpr int d2 = (DIR == 2) ? 1 : 2;
+    extern __shared__ double lds[];
+    double *A = lds;                    // primitives, then fluxes
+    double *B = lds + NV * blockDim.x;  // left face states
+    const int nt = blockDim.x;
+    const int t = threadIdx.x;
+    int bid = blockIdx.x;
+    const int nfull = (int)(gridDim.x / 8u) * 8;
+    if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
+    int p, sl, tm, tp;
+    if (DIR == 0) {
+        sl = t % TS;
+        p = t / TS;
+        tm = t - 1;
+        tp = t + 1;
+    } else {
+        p = t % TP;
+        sl = t / TP;
+        tm = t - TP;
+        tp = t + TP;
+    }
+    tm = tm < 0 ? 0 : tm;
+    tp = tp >= nt ? nt - 1 : tp;
+    const int chunk = bid % chunks;
+    const int ptile = bid / chunks;
+    const int n1 = a.n[d1];
+    const int npen = n1 * a.n[d2];
+    const int pf = ptile * TP + p;
+    const bool pvalid = t < TP * TS && pf < npen;
+    const int pfc = pf < npen ? pf : npen - 1;
+    const int out = TS - 4;
+    const int c0 = a.g[DIR] + chunk * out;  // the tile's first updated cell
+    const int cs = c0 - 2 + sl;             // this lane's cell
+    const int ng = a.ng[DIR];
+    const int csc = cs < 0 ? 0 : (cs >= ng ? ng - 1 : cs);
+    int xyz[3];
+    xyz[DIR] = csc;
+    xyz[d1] = a.g[d1] + pfc % n1;
+    xyz[d2] = a.g[d2] + pfc / n1;
+    const long long idx = (long long)xyz[0] + (long long)a.sy * xyz[1] + (long long)a.sz * xyz[2];
+
+    double W[NV], q[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) W[k] = a.src[(long long)uplane(DIR, k) * a.sv + idx];
+    prim<P>(W, a.gam_1, q);
+#pragma unroll
+    for (int v = 0; v < NV; v++) A[v * nt + t] = q[v];
+    __syncthreads();
+    double fp[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const int lim = (LIM >= 0) ? LIM : ((v < 5) ? a.lim : a.limb);
+        const double h = 0.5 * limit(lim, q[v] - A[v * nt + tm], A[v * nt + tp] - q[v]);
+        fp[v] = q[v] + h;
+        B[v * nt + t] = q[v] - h;
+    }
+    __syncthreads();
+    double f[NV];
+    if (sl >= 1 && sl < TS - 2) {
+        double fr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) fr[v] = B[v * nt + tp];
+        if constexpr (P::MAG)
+            flux_hlld(fp, fr, a.gam, a.ch, f);
+        else
+            flux_hllc(fp, fr, a.gam, f);
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) f[v] = 0.0;
+    }
+    // every read of A (the primitives) came before the last barrier
+#pragma unroll
+    for (int v = 0; v < NV; v++) A[v * nt + t] = f[v];
+    __syncthreads();
+    const bool upd = pvalid && sl >= 2 && sl < 2 + out && cs < a.g[DIR] + a.n[DIR];
+    if (!upd) return;
+    const long long ci = idx;  // an interior cell: csc == cs
+    const double *S = first ? a.base : a.dst;
+    double T[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) T[v] = S[(long long)v * a.sv + ci];
+    const double coef = (a.dt / a.dl[DIR]) * a.rk;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const int v = uplane(DIR, k);
+        T[v] = T[v] + coef * (A[k * nt + tm] - f[k]);
+    }
+    if (last) {
+        unsigned flags = 0u;
+        double dmass = 0.0;
+        for (int d = 0; d < 3; d++) {
+            if (!a.has_dir[d]) continue;
+            double Ws[NV];
+#pragma unroll
+            for (int kk = 0; kk < NV; kk++) Ws[kk] = T[uplane(d, kk)];
+#pragma unroll
+            for (int v = 0; v < 5; v++) Ws[v] = Ws[v] + 0.0;
+            if (a.last) {
+                unsigned fl = 0u;
+                double dm = 0.0;
+                floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+                flags |= fl;
+                dmass += dm;
+            }
+#pragma unroll
+            for (int kk = 0; kk < NV; kk++) T[uplane(d, kk)] = Ws[kk];
+        }
+        if (flags) atomicOr(a.flags, flags);
+        if (dmass != 0.0) atomicAdd(a.dmass, dmass);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + ci] = T[v];
+}
+
 template <class P>
 __global__ __launch_bounds__(256) void k_unsplit_update(const UnsplitArgs a, const double *__restrict__ F0,
                                                         const double *__restrict__ F1,
@@ -182,6 +318,36 @@ __global__ __launch_bounds__(256) void k_unsplit_update(const UnsplitArgs a, con
 constexpr int kTile[3][2] = {{4, 64}, {8, 32}, {8, 32}};
 
 template <int DIR, class P, int LIM>
+hipError_t fused_tile(const UnsplitArgs &a, int first, int last, hipStream_t st)
+{
+    const int d1 = (DIR == 0) ? 1 : 0, d2 = (DIR == 2) ? 1 : 2;
+    const long long npen = (long long)a.n[d1] * a.n[d2];
+    const int TP = kTile[DIR][0], TS = kTile[DIR][1];
+    const int chunks = (a.n[DIR] + (TS - 4) - 1) / (TS - 4);
+    const long long tiles = (npen + TP - 1) / TP * chunks;
+    if (tiles < 1 || tiles > 0x7fffffffll) return hipErrorInvalidValue;
+    const size_t shm = sizeof(double) * 2 * P::NV * TP * TS;
+    hipLaunchKernelGGL((k_unsplit_fused_tile<DIR, P, LIM>), dim3((unsigned)tiles), dim3(TP * TS), shm, st, a, TP,
+                       TS, chunks, first, last);
+    return hipGetLastError();
+}
+
+template <int DIR, class P>
+hipError_t fused_lim(const UnsplitArgs &a, int first, int last, hipStream_t st)
+{
+    if (a.lim == a.limb) {
+        switch (a.lim) {
+        case LIM_VANLEER: return fused_tile<DIR, P, LIM_VANLEER>(a, first, last, st);
+        case LIM_MINMOD: return fused_tile<DIR, P, LIM_MINMOD>(a, first, last, st);
+        case LIM_MONCEN: return fused_tile<DIR, P, LIM_MONCEN>(a, first, last, st);
+        case LIM_SUPERBEE: return fused_tile<DIR, P, LIM_SUPERBEE>(a, first, last, st);
+        default: break;
+        }
+    }
+    return fused_tile<DIR, P, -1>(a, first, last, st);
+}
+
+template <int DIR, class P, int LIM>
 hipError_t flux_tile(const UnsplitArgs &a, double *F, hipStream_t st)
 {
     const int d1 = (DIR == 0) ? 1 : 0, d2 = (DIR == 2) ? 1 : 2;
@@ -228,6 +394,19 @@ hipError_t launch_unsplit_flux(int dir, bool mhd, const UnsplitArgs &a, double *
     if (dir == 0) return flux_lim<0, HD>(a, F, (unsigned)blocks, st);
     if (dir == 1) return flux_lim<1, HD>(a, F, (unsigned)blocks, st);
     return flux_lim<2, HD>(a, F, (unsigned)blocks, st);
+}
+
+hipError_t launch_unsplit_fused(int dir, bool mhd, const UnsplitArgs &a, int first, int last, hipStream_t st)
+{
+    if (dir < 0 || dir > 2 || !a.has_dir[dir] || a.g[dir] < 2 || a.src == a.dst) return hipErrorInvalidValue;
+    if (mhd) {
+        if (dir == 0) return fused_lim<0, MHD>(a, first, last, st);
+        if (dir == 1) return fused_lim<1, MHD>(a, first, last, st);
+        return fused_lim<2, MHD>(a, first, last, st);
+    }
+    if (dir == 0) return fused_lim<0, HD>(a, first, last, st);
+    if (dir == 1) return fused_lim<1, HD>(a, first, last, st);
+    return fused_lim<2, HD>(a, first, last, st);
 }
 
 hipError_t launch_unsplit_update(bool mhd, const UnsplitArgs &a, const double *const F[3], hipStream_t st)

@@ -32,6 +32,11 @@ hipError_t launch_unsplit_flux(int dir, bool mhd, const UnsplitArgs &a, double *
 // The flux differences of every active direction and the source passes,
 // over the interior cells; F[d] from launch_unsplit_flux (unused if d inactive).
 hipError_t launch_unsplit_update(bool mhd, const UnsplitArgs &a, const double *const F[3], hipStream_t st);
+// Flux along dir and its difference added in one pass (src != dst): the
+// running sum starts from base when `first`, is kept in dst between
+// directions, and gets the source passes when `last` (first and last active
+// direction of the stage).  Same result as the two launchers above.
+hipError_t launch_unsplit_fused(int dir, bool mhd, const UnsplitArgs &a, int first, int last, hipStream_t st);
 
 }  // namespace pk
 
@@ -40,4 +45,5 @@ hipError_t launch_unsplit_update(bool mhd, const UnsplitArgs &a, const double *c
 namespace pk_fast {
 hipError_t launch_unsplit_flux(int dir, bool mhd, const pk::UnsplitArgs &a, double *F, hipStream_t st);
 hipError_t launch_unsplit_update(bool mhd, const pk::UnsplitArgs &a, const double *const F[3], hipStream_t st);
+hipError_t launch_unsplit_fused(int dir, bool mhd, const pk::UnsplitArgs &a, int first, int last, hipStream_t st);
 }  // namespace pk_fast
