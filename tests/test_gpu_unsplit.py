@@ -67,6 +67,29 @@ def test_3d_steps_bit_exact(oracle, mag, lim, limb):
     pk.finalize()
 
 
+def test_euler_in_place_bit_exact(oracle):
+    """integration_order = 1: the stage reads and writes one buffer, so the
+    fluxes of every direction are stored before the update (the two-pass
+    form); bit for bit with the oracle."""
+    n = (12, 10, 8)
+    cg = setups.smooth_random(n, seed=35, magnetized=True)
+    cg.psi[...] = np.random.default_rng(7).normal(0, 0.01, cg.psi.shape)
+    setups.fill_ghosts_periodic(cg)
+    cfg = pk.Config(magnetized=True, limiter="vanleer", integration_order=1, cfl=0.3, cfl_glm=0.3, smalld=1e-10,
+                    smallei=1e-10)
+    print(f"unsplit Euler {n}", flush=True)
+    start(cfg)
+    p = oracle.params_from_config(cfg)
+    ob = oracle.OBlock.from_cg(cg)
+    cg.upload()
+    t_g, n_g = pk.run_unsplit(cg, 1.0, nmax=3)
+    t_o, n_o = oracle.run_unsplit(p, ob, 1.0, nmax=3)
+    cg.download()
+    assert (t_g, n_g) == (t_o, n_o) and n_g == 3
+    assert_same(cg, ob, "euler")
+    pk.finalize()
+
+
 def test_stages_bit_exact(oracle):
     """The two stages one at a time: uh after the first, u after the second."""
     cg = setups.smooth_random((10, 12, 6), seed=32, magnetized=True)
