@@ -27,6 +27,7 @@
 #include "pk_kernels.hpp"
 #include "pk_unsplit.hpp"
 #include "pk_datafields.hpp"
+#include "pk_xfer.hpp"
 
 namespace {
 
@@ -103,6 +104,7 @@ double *g_pinned = nullptr;  // small pinned buffer for scalar read-backs
 double *g_stage = nullptr;   // AoS staging for host transfers, grown on demand
 size_t g_stage_bytes = 0;
 double *g_part = nullptr;    // partial results of the standalone timestep reduction
+pk_xfer::Ring g_ring;        // pinned chunks for host <-> device copies of the caller's arrays
 
 // Tile shapes chosen by pk_autotune, per block shape, direction and physics.
 struct TuneKey {
@@ -848,7 +850,7 @@ int upload_array(Block *b, const double *h, const pk::PlaneMap &m, double *dst, 
     const double *src = h;
     if (!from_device) {
         RCCHK(ensure_stage(sizeof(double) * ncell * m.nc));
-        HIPCHK(hipMemcpyAsync(g_stage, h, sizeof(double) * ncell * m.nc, hipMemcpyHostToDevice, g_stream));
+        HIPCHK(g_ring.h2d(g_stage, h, sizeof(double) * ncell * m.nc, g_stream));
         src = g_stage;
     }
     HIPCHK(pk::launch_aos_to_soa(src, m, dst, b->sv, b->ng, b->sy, b->sz, g_stream));
@@ -864,7 +866,7 @@ int download_array(Block *b, double *h, const pk::PlaneMap &m, const double *src
     } else {
         RCCHK(ensure_stage(sizeof(double) * ncell * m.nc));
         HIPCHK(pk::launch_soa_to_aos(src, b->sv, m, g_stage, b->ng, b->sy, b->sz, g_stream));
-        HIPCHK(hipMemcpyAsync(h, g_stage, sizeof(double) * ncell * m.nc, hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(g_ring.d2h(h, g_stage, sizeof(double) * ncell * m.nc, g_stream));
     }
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
@@ -2153,6 +2155,7 @@ int pk_finalize(void)
     g_stage_bytes = 0;
     if (g_part) (void)hipFree(g_part);
     g_part = nullptr;
+    g_ring.release();
     g_init = false;
     return PK_OK;
 }
