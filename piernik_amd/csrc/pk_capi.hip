@@ -2412,7 +2412,7 @@ int datafield(int64_t cg_id, const char *name, void *out, bool fp32)
     if (a.kind == pk::DF_MAGB) reads = 3;
     const double algo = 8.0 * cells * reads + (double)bytes;
     RCCHK(launch_timed(3, algo, [&]() { return pk::launch_datafield(a, g_stream); }));
-    HIPCHK(hipMemcpyAsync(out, g_stage, bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(g_ring.d2h(out, g_stage, bytes, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
 }
