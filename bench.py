@@ -344,7 +344,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": load_traffic(f"sweep_{args.workload}_traffic.json") if n in (256, 64) else None,
-            "kernel": f"k_sweep (fused RK2, {'HLLD+GLM' if nv == 9 else 'HLLC'}), x/y/z",
+            "kernel": (f"fused RK2 sweep ({'HLLD+GLM' if nv == 9 else 'HLLC'}), x/y/z: "
+                       + ", ".join(("k_march" if (tiles or {}).get(d, (0, 0))[1] == -1 else "k_sweep") + " " + "xyz"[d]
+                                   for d in range(3))),
             "algo_bytes_per_launch": algo / launches if launches else None,
             "avg_launch_ms": ms / launches if launches else None,
             "per_dir_gbps": per_dir,
@@ -555,7 +557,7 @@ def run_unsplit(args, cpu):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "whole step (k_unsplit_flux_tile x3 + k_unsplit_update per stage)",
+            "kernel": "whole step (k_unsplit_fused_tile x3 per stage)",
             "algo_bytes_per_step": algo / args.steps,
         },
         "cpu_baseline": cpu,
