@@ -118,7 +118,10 @@ private:
     bool quit_ = false;
 };
 
-// Pinned chunks and one event per chunk (the last DMA that used it).
+// Pinned chunks and one event per chunk (the last DMA that used it).  Every
+// transfer of one Ring goes on the same stream, so a DMA into a slot is
+// ordered after the earlier DMA out of it; the host waits on the slot's
+// event before it writes or reads the slot itself.
 struct Ring {
     static constexpr int kSlots = 4;
     static constexpr size_t kChunk = 32u << 20;
@@ -163,7 +166,8 @@ struct Ring {
         for (size_t off = 0; off < n; off += kChunk, k++) {
             const size_t len = std::min(kChunk, n - off);
             const int s = k % kSlots;
-            if (k >= kSlots && (e = hipEventSynchronize(ev[s])) != hipSuccess) return e;
+            // the last DMA out of this slot (of this call or an earlier one) is done
+            if ((e = hipEventSynchronize(ev[s])) != hipSuccess) return e;
             pool.copy(slot[s], static_cast<const char *>(src) + off, len);
             if ((e = hipMemcpyAsync(static_cast<char *>(dst) + off, slot[s], len, hipMemcpyHostToDevice, st)) !=
                 hipSuccess)
