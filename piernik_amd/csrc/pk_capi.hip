@@ -1832,7 +1832,7 @@ int unsplit_stage(Block *b, int istep, double dt, double ch)
         HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_update(g_mhd, a, F, g_stream)
                                : pk::launch_unsplit_update(g_mhd, a, F, g_stream));
     }
-    RCCHK(fill_buffer(b, dst));
+    if (src == dst) RCCHK(fill_buffer(b, dst));  // the fused passes wrote the periodic images already
     b->uh = euler ? -1 : (first ? dst : b->uh);
     b->dt_pending = false;
     return PK_OK;

@@ -138,9 +138,11 @@ __global__ __launch_bounds__(256) void k_unsplit_flux_tile(const UnsplitArgs a, 
 // flux from its neighbour through LDS, to the cell's running sum.  The sum
 // starts from base at the first active direction and waits in dst between
 // directions, each cell read and written by its own lane only; after the
-// last active direction the source passes run and dst holds the stage's
-// result.  The operands and their order are those of k_unsplit_update, so
-// the result is the same bit for bit, without the flux arrays.
+// last active direction the source passes run, dst holds the stage's
+// result and the pass also writes each cell's periodic ghost images (the
+// stage's ghost refresh).  The operands and their order are those of
+// k_unsplit_update, so the result is the same bit for bit, without the flux
+// arrays.
 template <int DIR, class P, int LIM>
 __global__ __launch_bounds__(256) void k_unsplit_fused_tile(const UnsplitArgs a, int TP, int TS, int chunks,
                                                             int first, int last)
@@ -259,6 +261,28 @@ __global__ __launch_bounds__(256) void k_unsplit_fused_tile(const UnsplitArgs a,
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + ci] = T[v];
+    if (!last) return;
+    // periodic images of the finished cell (the stage's ghost refresh): every
+    // ghost has exactly one interior pre-image, since n >= nb
+    int off[3][3], cnt[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int ic = xyz[d] - a.g[d];
+        cnt[d] = 1;
+        off[d][0] = 0;
+        if (a.g[d] > 0) {
+            if (ic < a.g[d]) off[d][cnt[d]++] = a.n[d];
+            if (ic >= a.n[d] - a.g[d]) off[d][cnt[d]++] = -a.n[d];
+        }
+    }
+    for (int i0 = 0; i0 < cnt[0]; i0++)
+        for (int i1 = 0; i1 < cnt[1]; i1++)
+            for (int i2 = 0; i2 < cnt[2]; i2++) {
+                if ((i0 | i1 | i2) == 0) continue;
+                const long long j = ci + off[0][i0] + (long long)a.sy * off[1][i1] + (long long)a.sz * off[2][i2];
+#pragma unroll
+                for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + j] = T[v];
+            }
 }
 
 template <class P>

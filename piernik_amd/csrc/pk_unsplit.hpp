@@ -34,8 +34,9 @@ hipError_t launch_unsplit_flux(int dir, bool mhd, const UnsplitArgs &a, double *
 hipError_t launch_unsplit_update(bool mhd, const UnsplitArgs &a, const double *const F[3], hipStream_t st);
 // Flux along dir and its difference added in one pass (src != dst): the
 // running sum starts from base when `first`, is kept in dst between
-// directions, and gets the source passes when `last` (first and last active
-// direction of the stage).  Same result as the two launchers above.
+// directions, and gets the source passes and the periodic ghost images when
+// `last` (first and last active direction of the stage).  Same result as the
+// two launchers above followed by the ghost refresh.
 hipError_t launch_unsplit_fused(int dir, bool mhd, const UnsplitArgs &a, int first, int last, hipStream_t st);
 
 }  // namespace pk
