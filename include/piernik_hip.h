@@ -79,7 +79,8 @@ typedef struct pk_config {
                                   cells = -1: one lane per whole pencil (march kernel) */
     int32_t tile_yz[2];        /* sweep tile shape (pencils, cells) along y and z; 0 = default */
     int32_t fast_math;         /* 0: every operation rounds as the reference's FP64 code (bit-exact);
-                                  1: FMA contraction and fast division/sqrt in the sweep (~1e-15 per op) */
+                                  1: FMA contraction and fast division/sqrt in the split sweeps and
+                                     the unsplit passes (~1e-15 per op) */
     int32_t reserved;
 } pk_config;
 
