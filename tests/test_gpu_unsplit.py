@@ -164,3 +164,36 @@ def test_relaxed_build_within_1e10_of_oracle(oracle, mag):
     assert cg.interior(cg.u)[0].min() > 0.0
     print(f"worst rel-L1 {worst:.3e}")
     pk.finalize()
+
+
+def test_relaxed_build_mhd256_within_1e10_of_exact():
+    """At the benchmark's size: the 256^3 MHD turbulence block, 5 unsplit
+    steps at CFL 0.3, relaxed build against the bit-exact one (rel-L1 <= 1e-10
+    per conserved variable, positive density)."""
+    cg0, cfg = setups.mhd_turbulence(n=256, seed=1)
+    cfg.cfl = 0.3
+    out = {}
+    for fast in (False, True):
+        print(f"unsplit 256^3 MHD, 5 steps, fast_math={fast}", flush=True)
+        cfg.fast_math = fast
+        start(cfg)
+        cg = pk.GridContainer(cg0.n, magnetized=True)
+        cg.u[...] = cg0.u
+        cg.b[...] = cg0.b
+        cg.psi[...] = cg0.psi
+        cg.upload()
+        t, n = pk.run_unsplit(cg, 1e30, nmax=5)
+        cg.download()
+        out[fast] = (cg, t)
+        pk.finalize()
+    (ex, t_e), (fa, t_f) = out[False], out[True]
+    assert abs(t_f - t_e) <= 1e-12 * t_e
+    worst = 0.0
+    for name in ("u", "b"):
+        a, o = ex.interior(getattr(fa, name)), ex.interior(getattr(ex, name))
+        for v in range(a.shape[0]):
+            err = rel_l1(a[v], o[v])
+            worst = max(worst, err)
+            assert err <= 1e-10, (name, v, err)
+    assert ex.interior(fa.u)[0].min() > 0.0
+    print(f"worst rel-L1 {worst:.3e}")
