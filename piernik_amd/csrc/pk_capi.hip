@@ -806,6 +806,15 @@ void free_block(Block *b)
     delete b;
 }
 
+// The pinned ring, set up on first use; if pinned memory cannot be had the
+// transfers fall back to plain copies from the pageable arrays.
+bool ring_ready()
+{
+    if (g_ring.ensure() == hipSuccess) return true;
+    (void)hipGetLastError();  // clear the failed allocation's error
+    return false;
+}
+
 int ensure_stage(size_t bytes)
 {
     if (bytes <= g_stage_bytes) return PK_OK;
@@ -850,7 +859,10 @@ int upload_array(Block *b, const double *h, const pk::PlaneMap &m, double *dst, 
     const double *src = h;
     if (!from_device) {
         RCCHK(ensure_stage(sizeof(double) * ncell * m.nc));
-        HIPCHK(g_ring.h2d(g_stage, h, sizeof(double) * ncell * m.nc, g_stream));
+        if (ring_ready())
+            HIPCHK(g_ring.h2d(g_stage, h, sizeof(double) * ncell * m.nc, g_stream));
+        else
+            HIPCHK(hipMemcpyAsync(g_stage, h, sizeof(double) * ncell * m.nc, hipMemcpyHostToDevice, g_stream));
         src = g_stage;
     }
     HIPCHK(pk::launch_aos_to_soa(src, m, dst, b->sv, b->ng, b->sy, b->sz, g_stream));
@@ -866,7 +878,10 @@ int download_array(Block *b, double *h, const pk::PlaneMap &m, const double *src
     } else {
         RCCHK(ensure_stage(sizeof(double) * ncell * m.nc));
         HIPCHK(pk::launch_soa_to_aos(src, b->sv, m, g_stage, b->ng, b->sy, b->sz, g_stream));
-        HIPCHK(g_ring.d2h(h, g_stage, sizeof(double) * ncell * m.nc, g_stream));
+        if (ring_ready())
+            HIPCHK(g_ring.d2h(h, g_stage, sizeof(double) * ncell * m.nc, g_stream));
+        else
+            HIPCHK(hipMemcpyAsync(h, g_stage, sizeof(double) * ncell * m.nc, hipMemcpyDeviceToHost, g_stream));
     }
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
@@ -2412,7 +2427,10 @@ int datafield(int64_t cg_id, const char *name, void *out, bool fp32)
     if (a.kind == pk::DF_MAGB) reads = 3;
     const double algo = 8.0 * cells * reads + (double)bytes;
     RCCHK(launch_timed(3, algo, [&]() { return pk::launch_datafield(a, g_stream); }));
-    HIPCHK(g_ring.d2h(out, g_stage, bytes, g_stream));
+    if (ring_ready())
+        HIPCHK(g_ring.d2h(out, g_stage, bytes, g_stream));
+    else
+        HIPCHK(hipMemcpyAsync(out, g_stage, bytes, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
 }
