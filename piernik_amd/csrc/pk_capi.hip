@@ -1767,8 +1767,11 @@ int fill_buffer(Block *b, int i)
 // then update_boundaries of the stage's arrays, unsplit_sweeps.F90:126-190):
 // the first stage reads u and writes uh (the other buffer), the last reads
 // uh and adds its fluxes to u in place.
-int unsplit_stage(Block *b, int istep, double dt, double ch)
+// psi_fac: with the fused passes, psi of the stage's result is multiplied by
+// it in the last pass (*fac_applied set); the in-place Euler stage leaves it.
+int unsplit_stage(Block *b, int istep, double dt, double ch, double psi_fac = 1.0, bool *fac_applied = nullptr)
 {
+    if (fac_applied) *fac_applied = false;
     RCCHK(no_fc(b, "the unsplit solver"));
     if (g_nfl != 1) return fail(PK_EINVAL, "the unsplit solver covers one fluid");
     if (block_linked(b)) return fail(PK_EINVAL, "the unsplit solver covers blocks periodic on themselves");
@@ -1813,6 +1816,7 @@ int unsplit_stage(Block *b, int istep, double dt, double ch)
     a.dt = dt;
     a.rk = rk_coef(istep);
     a.last = last ? 1 : 0;
+    a.psi_fac = 1.0;
     a.smalld = g_cfg.smalld;
     a.smallei = g_cfg.smallei;
     a.use_smalld = g_cfg.use_smalld;
@@ -1830,8 +1834,10 @@ int unsplit_stage(Block *b, int istep, double dt, double ch)
         for (int d = 0; d < 3; d++) {
             if (!g_cfg.has_dir[d]) continue;
             const int fi = d == d_first, la = d == d_last;
-            HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_fused(d, g_mhd, a, fi, la, g_stream)
-                                   : pk::launch_unsplit_fused(d, g_mhd, a, fi, la, g_stream));
+            pk::UnsplitArgs ad = a;
+            if (la) ad.psi_fac = psi_fac;
+            HIPCHK(g_cfg.fast_math ? pk_fast::launch_unsplit_fused(d, g_mhd, ad, fi, la, g_stream)
+                                   : pk::launch_unsplit_fused(d, g_mhd, ad, fi, la, g_stream));
         }
     } else {
         // in place (the Euler stage): every flux before any update
@@ -1848,6 +1854,7 @@ int unsplit_stage(Block *b, int istep, double dt, double ch)
                                : pk::launch_unsplit_update(g_mhd, a, F, g_stream));
     }
     if (src == dst) RCCHK(fill_buffer(b, dst));  // the fused passes wrote the periodic images already
+    else if (fac_applied) *fac_applied = true;
     b->uh = euler ? -1 : (first ? dst : b->uh);
     b->dt_pending = false;
     return PK_OK;
@@ -1879,9 +1886,10 @@ int fluid_update_unsplit(Block *b, double dt)
         RCCHK(unsplit_stage(b, PK_EULER, dt, ch));
     } else {
         RCCHK(unsplit_stage(b, PK_RK2_1, dt, ch));
-        RCCHK(unsplit_stage(b, PK_RK2_2, dt, ch));
     }
-    if (g_mhd) RCCHK(damp_psi(b, fac));
+    bool damped = false;  // the last fused pass applies glmdamping's second half itself
+    if (g_cfg.integration_order != 1) RCCHK(unsplit_stage(b, PK_RK2_2, dt, ch, g_mhd ? fac : 1.0, &damped));
+    if (g_mhd && !damped) RCCHK(damp_psi(b, fac));
     return PK_OK;
 }
 

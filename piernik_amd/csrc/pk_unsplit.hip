@@ -260,6 +260,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
         if (flags) atomicOr(a.flags, flags);
         if (dmass != 0.0) atomicAdd(a.dmass, dmass);
+        if constexpr (P::MAG) T[8] = T[8] * a.psi_fac;  // glmdamping after the stages (1 at the first stage)
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + ci] = T[v];

@@ -18,6 +18,7 @@ struct UnsplitArgs {
     double gam, gam_1;
     double ch;           // GLM cleaning speed
     double dt, rk;       // dt and rk_coef of the stage
+    double psi_fac;      // fused passes: psi of the finished state times this (glmdamping after the stages)
     int last;            // last stage: care_for_positives runs
     double smalld, smallei;
     int use_smalld, use_smallei;
