@@ -46,12 +46,14 @@ typedef struct or_params {
     double smalld, smallei;
     double cfl, cfl_glm, glm_alpha;
     int32_t has_dir[3];
+    int32_t disallow_negatives; /* global.F90:237: ei_negative is raised only with it (sources.F90:425) */
 } or_params;
 
 typedef struct or_report {
     int32_t dn_negative;        /* sources.F90:350-351 */
     int32_t ei_negative;        /* sources.F90:425 */
-    double magic_mass[2];       /* sources.F90:356-358, per fluid */
+    double magic_mass[2];       /* local_magic_mass, sources.F90:356-358, per fluid: per pencil
+                                   (mm - sum(rho_before)*dvol) + sum(rho_after)*dvol */
 } or_report;
 
 typedef struct or_block {

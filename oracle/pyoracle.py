@@ -32,7 +32,7 @@ class Params(ctypes.Structure):
         ("use_smalld", ctypes.c_int32), ("use_smallei", ctypes.c_int32),
         ("smalld", ctypes.c_double), ("smallei", ctypes.c_double),
         ("cfl", ctypes.c_double), ("cfl_glm", ctypes.c_double), ("glm_alpha", ctypes.c_double),
-        ("has_dir", ctypes.c_int32 * 3),
+        ("has_dir", ctypes.c_int32 * 3), ("disallow_negatives", ctypes.c_int32),
     ]
 
 
@@ -116,8 +116,10 @@ def _dp(a):
 
 def params(gamma=5.0 / 3.0, magnetized=False, limiter="vanleer", limiter_b=None, integration_order=2,
            smalld=1e-10, smallei=1e-10, use_smalld=True, use_smallei=True, cfl=0.7, cfl_glm=None,
-           glm_alpha=0.1, has_dir=(True, True, True), fluids=None):
-    """fluids: (gamma, magnetized) per fluid; default the one fluid (gamma, magnetized)."""
+           glm_alpha=0.1, has_dir=(True, True, True), fluids=None, disallow_negatives=True):
+    """fluids: (gamma, magnetized) per fluid; default the one fluid (gamma, magnetized).
+    disallow_negatives: global.F90:237 (default .true.); the floors raise
+    ei_negative only with it (sources.F90:425)."""
     if fluids is None:
         fluids = [(gamma, magnetized)]
     p = Params()
@@ -137,15 +139,18 @@ def params(gamma=5.0 / 3.0, magnetized=False, limiter="vanleer", limiter_b=None,
     p.glm_alpha = glm_alpha
     for d in range(3):
         p.has_dir[d] = int(has_dir[d])
+    p.disallow_negatives = int(disallow_negatives)
     return p
 
 
-def params_from_config(cfg):
-    """Oracle parameters equal to a piernik_amd.Config."""
+def params_from_config(cfg, disallow_negatives=True):
+    """Oracle parameters equal to a piernik_amd.Config (disallow_negatives is
+    set on the GPU side by piernik_amd.set_cfl_control)."""
     return params(gamma=cfg.gamma, magnetized=cfg.magnetized, limiter=cfg.limiter, limiter_b=cfg.limiter_b,
                   integration_order=cfg.integration_order, smalld=cfg.smalld, smallei=cfg.smallei,
                   use_smalld=cfg.use_smalld, use_smallei=cfg.use_smallei, cfl=cfg.cfl, cfl_glm=cfg.cfl_glm,
-                  glm_alpha=cfg.glm_alpha, has_dir=cfg.has_dir, fluids=cfg.fluids)
+                  glm_alpha=cfg.glm_alpha, has_dir=cfg.has_dir, fluids=cfg.fluids,
+                  disallow_negatives=disallow_negatives)
 
 
 class OBlock:
@@ -405,3 +410,129 @@ def run_unsplit(p, ob, tend, nmax=1 << 30):
     ns = ctypes.c_int32()
     t = lib().or_run_unsplit(ctypes.byref(p), ctypes.byref(blk), tend, nmax, ctypes.byref(ns), ctypes.byref(ob.rep))
     return t, ns.value
+
+
+# ---- the main loop with step retries (cflcontrol) -------------------------
+
+def cfl_max_sanitized(cfl, cfl_max):
+    """cfl_max as init_global sanitizes it (global.F90:304)."""
+    return min(max(cfl_max, min(cfl * 1.1, cfl + 0.05, (1.0 + cfl) / 2.0)), 1.0)
+
+
+def main_loop(nsteps, time_step, fluid_update, save, restore, negatives, cfl, mode="warn", cfl_max=0.9,
+              dt_shrink=0.5, max_attempts=10, disallow_negatives=True):
+    """The reference's main loop (piernik.F90:125-170) with its step controls,
+    over callbacks, so that the same control flow drives the oracle's blocks
+    and synthetic steps:
+
+      time_step(s, dt_old) -> (dt, c_max)  time_step of the current state
+      fluid_update(dt)                     one step (two dt-substeps)
+      save() -> snapshot, restore(snap)    repeat_fluidstep's backup and restore
+                                           (floor bookkeeping included)
+      negatives() -> (dn, ei)              the flags the floors raised since the
+                                           last call; reading clears them
+                                           (check_cfl_violation, timestep.F90:291-300)
+
+    mode 'redo' / 'flexible': after a step check_cfl_violation recomputes
+    c_max; cfl_warn (timestep.F90:317-361) asks for a redo when
+    stepcfl = c_new/c_old*cfl*dt_cur_shrink > cfl_max (sanitized) or, with
+    disallow_negatives, when the step raised a flag; the step is redone from
+    the backup with dt = dt_full*dt_cur_shrink, dt_cur_shrink = dt_shrink**attempt
+    ('redo') or cfl_max/stepcfl, times dt_shrink from the second attempt on
+    ('flexible') (repeat_fluidstep, timestep_retry.F90:86-146).  mode 'auto':
+    cfl_auto (timestep.F90:368-405) scales each dt by cfl_c.  'warn': no retries.
+    Returns (t, redone, attempts), attempts a list of (step, dt) of every
+    update in order."""
+    cmax = cfl_max_sanitized(cfl, cfl_max)
+    redo = mode in ("redo", "flexible")
+    t, dt_old, dt_full, c_old, shrink = 0.0, -1.0, 0.0, 0.0, 1.0
+    s, attempt, redone, repeat = 0, 0, 0, False
+    c_main_old, stepcfl, cfl_c = 0.0, 0.0, 1.0  # timestep_pub.F90:42
+    snap = None
+    attempts = []
+    while s < nsteps:
+        if not repeat:
+            dt, c_old = time_step(s, dt_old)
+            if mode == "auto":
+                stepcfl_old, stepcfl = stepcfl, cfl
+                if c_main_old > 0.0:
+                    stepcfl = c_old / c_main_old * cfl
+                else:
+                    stepcfl_old = cfl
+                if stepcfl > 0.0 and dt_old > 0.0:
+                    cfl_c = min(1.0, 0.5 * (cfl_c + min(1.0, stepcfl_old / stepcfl * dt / dt_old)))
+                    dt = dt * cfl_c
+                else:
+                    cfl_c = 1.0
+            c_main_old = c_old
+            dt_full = dt
+            if redo:
+                snap = save()
+        else:
+            restore(snap)
+            dt = dt_full * shrink
+        negatives()  # this step's flags only
+        fluid_update(dt)
+        attempts.append((s, dt))
+        dt_old = dt
+        if redo:
+            _, c_new = time_step(s + 1, dt_old)
+            dn, ei = negatives()
+            stepcfl = cfl * shrink
+            if c_old > 0.0:
+                stepcfl = c_new / c_old * cfl * shrink
+            repeat = stepcfl > cmax or (disallow_negatives and bool(dn or ei))
+            if repeat:
+                if mode == "flexible":
+                    shrink = cmax / stepcfl
+                    if attempt >= 1:
+                        shrink = shrink * dt_shrink
+                else:
+                    shrink = dt_shrink ** (attempt + 1)
+                attempt += 1
+                redone += 1
+                if attempt > max_attempts:
+                    raise RuntimeError(f"step {s} redone more than max_redostep_attempts = {max_attempts} times")
+                continue
+            shrink, attempt = 1.0, 0
+        t = t + dt
+        t = t + dt
+        s += 1
+    return t, redone, attempts
+
+
+def run_redo(p, ob, nsteps, mode, cfl_max=0.9, dt_shrink=0.5, max_attempts=10):
+    """main_loop over one oracle block (periodic on itself).  A redone step
+    starts from the backup's state and floor bookkeeping (downgrade_magic_mass);
+    the block's report accumulates the flags of the steps that were kept.
+    Returns (t, redone)."""
+    fill_ghosts(p, ob)
+    arrays = [a for a in (ob.u, ob.b, ob.psi) if a is not None]
+    last = [(0, 0)]
+
+    def save():
+        return ([a.copy() for a in arrays], list(ob.rep.magic_mass),
+                (ob.rep.dn_negative, ob.rep.ei_negative))
+
+    def restore(snap):
+        for a, sv in zip(arrays, snap[0]):
+            a[...] = sv
+        ob.rep.magic_mass[0], ob.rep.magic_mass[1] = snap[1]
+        ob.rep.dn_negative, ob.rep.ei_negative = snap[2]
+
+    def step(dt):
+        keep = (ob.rep.dn_negative, ob.rep.ei_negative)
+        ob.rep.dn_negative = ob.rep.ei_negative = 0
+        fluid_update(p, ob, dt)
+        last[0] = (ob.rep.dn_negative, ob.rep.ei_negative)
+        ob.rep.dn_negative |= keep[0]
+        ob.rep.ei_negative |= keep[1]
+
+    def negatives():
+        out, last[0] = last[0], (0, 0)
+        return out
+
+    t, redone, _ = main_loop(nsteps, lambda s, dt_old: time_step(p, ob, s, dt_old), step, save, restore,
+                             negatives, p.cfl, mode, cfl_max, dt_shrink, max_attempts,
+                             bool(p.disallow_negatives))
+    return t, redone

@@ -459,15 +459,18 @@ void or_floors(const or_params *p, int n, double *u1, const double *b1, int nbp,
             if (dn[i] < 0.0) rep->dn_negative = 1;
     }
     if (p->use_smalld) {
-        /* the mass added by the floor over the pencil interior nbp..n-nbp-1
-         * (the reference's local_magic_mass bookkeeping, :356-358) */
+        /* local_magic_mass (sources.F90:356-358): per fluid, the pencil-interior
+         * mass nbp..n-nbp-1 (nb+1:n-nb) before the floor is taken off and the
+         * mass after it added back, each sum left to right times dvol */
         for (int f = 0; f < p->nfluids; f++) {
             double *dn = u1 + (size_t)(p->fl[f].beg + IDN) * n;
-            for (int i = 0; i < n; i++) {
-                double nd = fmax2(dn[i], p->smalld);
-                if (nd != dn[i] && i >= nbp && i < n - nbp) rep->magic_mass[f] += (nd - dn[i]) * dvol;
-                dn[i] = nd;
-            }
+            double s = 0.0;
+            for (int i = nbp; i < n - nbp; i++) s += dn[i];
+            rep->magic_mass[f] = rep->magic_mass[f] - s * dvol;
+            for (int i = 0; i < n; i++) dn[i] = fmax2(dn[i], p->smalld);
+            s = 0.0;
+            for (int i = nbp; i < n - nbp; i++) s += dn[i];
+            rep->magic_mass[f] = rep->magic_mass[f] + s * dvol;
         }
     }
     /* limit_minimal_intener, sources.F90:394-433 */
@@ -486,7 +489,7 @@ void or_floors(const or_params *p, int n, double *u1, const double *b1, int nbp,
             } else {
                 ei = en[i] - kin;
             }
-            if (ei < 0.0) rep->ei_negative = 1;
+            if (p->disallow_negatives && ei < 0.0) rep->ei_negative = 1; /* sources.F90:425 */
             if (p->use_smallei) ei = fmax2(ei, p->smallei);
             en[i] = ei + kin;
             if (fl->magnetized) en[i] = en[i] + mag;

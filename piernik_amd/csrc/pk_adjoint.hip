@@ -218,7 +218,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
         }
         unsigned fl = 0u;
         double dm = 0.0;
-        floors<P>(Us, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+        floors<P>(Us, a.smalld, a.smallei, a.use_smalld, a.use_smallei, false, fl, dm);
     }
     park(K1, R1);
     park(K2, Us_pre);
@@ -314,7 +314,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &a, int bid, double *
             for (int v = 0; v < NV; v++) Usr[v] = Us_pre[v];
             unsigned fl = 0u;
             double dm = 0.0;
-            floors<P>(Usr, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+            floors<P>(Usr, a.smalld, a.smallei, a.use_smalld, a.use_smallei, false, fl, dm);
             prim_vjp<P>(a, Usr, qb, lsp);
         }
         floors_vjp<P>(a, Us_pre, lsp);

@@ -27,6 +27,7 @@
 #include "pk_kernels.hpp"
 #include "pk_unsplit.hpp"
 #include "pk_datafields.hpp"
+#include "pk_mass.hpp"
 #include "pk_xfer.hpp"
 
 namespace {
@@ -44,7 +45,7 @@ struct Block {
     double *h_u = nullptr, *h_uh = nullptr, *h_b = nullptr, *h_bh = nullptr, *h_psi = nullptr,
            *h_psih = nullptr;
     unsigned *d_flags = nullptr;
-    double *d_dmass = nullptr;  // per fluid
+    unsigned long long *d_mass = nullptr;  // per fluid: the floor's mass counters (pk_mass.hpp)
     double *d_dtout = nullptr;
     unsigned long long *d_dtslots = nullptr;  // timestep reduction written by the last sweep of a step
     bool dt_pending = false;                  // d_dtout holds timestep_fluid of the current state
@@ -67,7 +68,7 @@ struct Block {
     // the state at the start of a step that may be redone (cflcontrol 'redo')
     double *backup = nullptr;
     unsigned saved_flags = 0u;
-    double saved_dmass[2] = {0.0, 0.0};
+    unsigned long long saved_mass[2 * pk::kMassDigits] = {};
     // fine/coarse flux points (pk_register_fcflx): host arrays per direction,
     // kind (0 finebnd, 1 coarsebnd) and side, and their device staging
     struct FcReg {
@@ -91,6 +92,8 @@ const int kDtSlots = 1024;
 
 pk_config g_cfg;
 bool g_init = false;
+int g_ei_flag = 1;        // disallow_negatives (global.F90:237): the floors record negative internal energies
+constexpr size_t kMassBytes = 2 * pk::kMassDigits * sizeof(unsigned long long);  // both fluids' counters
 bool g_mhd = false;       // some fluid is magnetized: B and psi planes exist
 int g_nfl = 1;
 int g_fplane[2] = {0, 5};  // first plane of each fluid's group
@@ -204,8 +207,9 @@ void fill_common(const Block *b, int f, pk::SweepArgs &a)
     a.smallei = g_cfg.smallei;
     a.use_smalld = g_cfg.use_smalld;
     a.use_smallei = g_cfg.use_smallei;
+    a.ei_flag = g_ei_flag;
     a.flags = b->d_flags;
-    a.dmass = b->d_dmass + f;
+    a.dmass = b->d_mass + pk::kMassDigits * f;
     a.psi_fac = 1.0;
     a.ch = 0.0;
     a.images = 0;
@@ -755,9 +759,9 @@ int alloc_block(Block *b)
         HIPCHK(hipMemsetAsync(b->buf[i], 0, bytes, g_stream));
     }
     HIPCHK(hipMalloc((void **)&b->d_flags, sizeof(unsigned)));
-    HIPCHK(hipMalloc((void **)&b->d_dmass, 2 * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->d_mass, kMassBytes));
     HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
-    HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
+    HIPCHK(hipMemsetAsync(b->d_mass, 0, kMassBytes, g_stream));
     HIPCHK(hipMalloc((void **)&b->d_dtout, sizeof(double) * 8));
     HIPCHK(hipMalloc((void **)&b->d_dtslots, sizeof(unsigned long long) * 4 * kDtSlots));
     return PK_OK;
@@ -789,7 +793,7 @@ void free_block(Block *b)
     for (int i = 0; i < 2; i++)
         if (b->buf[i]) (void)hipFree(b->buf[i]);
     if (b->d_flags) (void)hipFree(b->d_flags);
-    if (b->d_dmass) (void)hipFree(b->d_dmass);
+    if (b->d_mass) (void)hipFree(b->d_mass);
     if (b->d_dtout) (void)hipFree(b->d_dtout);
     if (b->d_dtslots) (void)hipFree(b->d_dtslots);
     if (b->d_flux) (void)hipFree(b->d_flux);
@@ -1821,8 +1825,9 @@ int unsplit_stage(Block *b, int istep, double dt, double ch, double psi_fac = 1.
     a.smallei = g_cfg.smallei;
     a.use_smalld = g_cfg.use_smalld;
     a.use_smallei = g_cfg.use_smallei;
+    a.ei_flag = g_ei_flag;
     a.flags = b->d_flags;
-    a.dmass = b->d_dmass;
+    a.dmass = b->d_mass;
     if (src != dst) {
         // one pass per direction: flux and flux difference, no flux arrays
         int d_first = -1, d_last = -1;
@@ -1933,7 +1938,7 @@ int step_save(const std::vector<Block *> &bl)
         if (!b->backup) HIPCHK(hipMalloc((void **)&b->backup, state_bytes(b)));
         HIPCHK(hipMemcpyAsync(b->backup, b->buf[b->cur], state_bytes(b), hipMemcpyDeviceToDevice, g_stream));
         HIPCHK(hipMemcpyAsync(&b->saved_flags, b->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, g_stream));
-        HIPCHK(hipMemcpyAsync(b->saved_dmass, b->d_dmass, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipMemcpyAsync(b->saved_mass, b->d_mass, kMassBytes, hipMemcpyDeviceToHost, g_stream));
     }
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
@@ -1943,7 +1948,7 @@ int step_restore(const std::vector<Block *> &bl)
 {
     for (Block *b : bl) {
         HIPCHK(hipMemcpyAsync(b->buf[b->cur], b->backup, state_bytes(b), hipMemcpyDeviceToDevice, g_stream));
-        HIPCHK(hipMemcpyAsync(b->d_dmass, b->saved_dmass, 2 * sizeof(double), hipMemcpyHostToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(b->d_mass, b->saved_mass, kMassBytes, hipMemcpyHostToDevice, g_stream));
         b->dt_pending = false;
         b->uh = -1;
     }
@@ -2154,6 +2159,7 @@ int pk_init(const pk_config *cfg)
     g_nplanes = p;
     g_mhd = g_magf >= 0;
     g_cflc = CflControl();
+    g_ei_flag = g_cflc.disallow_negatives;
     g_tc = TimeControl();
     g_init = true;
     return PK_OK;
@@ -2545,6 +2551,10 @@ int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t m
     g_cflc.dt_shrink = dt_shrink;
     g_cflc.max_attempts = max_attempts;
     g_cflc.disallow_negatives = disallow_negatives ? 1 : 0;
+    if (g_ei_flag != g_cflc.disallow_negatives) {
+        g_ei_flag = g_cflc.disallow_negatives;
+        list_invalidate();  // the list's argument tables carry the flag
+    }
     return PK_OK;
 }
 
@@ -2622,9 +2632,9 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset)
     Block *b;
     RCCHK(get_block(cg_id, &b));
     unsigned fl = 0;
-    double dm[2] = {0.0, 0.0};
+    unsigned long long acc[2][pk::kMassDigits] = {};
     HIPCHK(hipMemcpyAsync(&fl, b->d_flags, sizeof fl, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(dm, b->d_dmass, sizeof dm, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(acc, b->d_mass, kMassBytes, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     if (rep) {
         double dvol = 1.0;
@@ -2632,11 +2642,12 @@ int pk_get_report(int64_t cg_id, pk_stage_report *rep, int32_t reset)
             if (g_cfg.has_dir[d]) dvol *= b->dl[d];
         rep->dn_negative = (fl & 1u) ? 1 : 0;
         rep->ei_negative = (fl & 2u) ? 1 : 0;
-        for (int f = 0; f < 2; f++) rep->magic_mass[f] = f < g_nfl ? dm[f] * dvol : 0.0;
+        // dvol times the exact sum of the floor's additions, rounded once
+        for (int f = 0; f < 2; f++) rep->magic_mass[f] = f < g_nfl ? pk::mass_value(acc[f]) * dvol : 0.0;
     }
     if (reset) {
         HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
-        HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
+        HIPCHK(hipMemsetAsync(b->d_mass, 0, kMassBytes, g_stream));
         HIPCHK(hipStreamSynchronize(g_stream));
     }
     return PK_OK;
@@ -2732,7 +2743,7 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
     // the candidates wrote the other buffer only; the state is unchanged.
     // The floor bookkeeping they accumulated is discarded.
     HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
-    HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
+    HIPCHK(hipMemsetAsync(b->d_mass, 0, kMassBytes, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return PK_OK;
 }
@@ -2877,7 +2888,7 @@ int pk_autotune_all(double dt, double chspeed)
     // the candidates wrote the other buffers only; their floor bookkeeping is discarded
     for (Block *b : g_list.blocks) {
         HIPCHK(hipMemsetAsync(b->d_flags, 0, sizeof(unsigned), g_stream));
-        HIPCHK(hipMemsetAsync(b->d_dmass, 0, 2 * sizeof(double), g_stream));
+        HIPCHK(hipMemsetAsync(b->d_mass, 0, kMassBytes, g_stream));
     }
     HIPCHK(hipStreamSynchronize(g_stream));
     const bool fresh = g_list.fresh;

@@ -15,6 +15,7 @@
 // (solve_cg_riemann.F90:170-228), pencil ends included.
 #include <hip/hip_runtime.h>
 #include "pk_math.hpp"
+#include "pk_mass.hpp"
 #include "pk_kernels.hpp"
 
 namespace PK_KNS {
@@ -288,18 +289,21 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int bid, unsigned
 #pragma unroll
     for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx];
 
+    // Floor bookkeeping.  Flags: stage 1 over every cell whose stage-1 value
+    // the tile completes within two cells of the interior (lanes 2..TS-3;
+    // the reference's stage computes these cells with the same stencil), the
+    // last stage over the cells the tile writes.  Mass: the interior cells,
+    // each stage's events added exactly (pk_mass.hpp).
     unsigned flags = 0u;
-    double dmass = 0.0;
     if constexpr (FUSED) {
         double Ws[NV];
         tile_stage<P, LIM>(a, W, W, a.c1, true, false, false, LA, LB, t, tm, tp, nt, Ws);
         double dm = 0.0;
         unsigned fl = 0u;
-        floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
-        if (outl) {
-            flags |= fl;
-            dmass += dm;
-        }
+        floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
+        const int g = a.g[ds];
+        if (pvalid && s >= 2 && s <= a.TS - 3 && cs >= g - 2 && cs < g + a.n[ds] + 2) flags |= fl;
+        pk::mass_add(a.dmass, outl ? dm : 0.0);
         tile_stage<P, LIM>(a, Ws, W, a.c2, true, false, false, LB, LA, t, tm, tp, nt, Wn);
     } else {
         const bool lo = cs == 0, hi = cs == ng - 1;
@@ -316,12 +320,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int bid, unsigned
     {
         double dm = 0.0;
         unsigned fl = 0u;
-        floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+        floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
         const bool inner = cs >= a.g[ds] && cs < a.g[ds] + a.n[ds];
-        if (outl) {
-            flags |= fl;
-            if (inner) dmass += dm;
-        }
+        if (outl) flags |= fl;
+        pk::mass_add(a.dmass, (outl && inner) ? dm : 0.0);
     }
     if constexpr (FUSED) {
         if (a.dtslots) dt_contribution<DIR, P>(a, Wn, outl && svalid, slot_id);
@@ -333,9 +335,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int bid, unsigned
         for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + idx] = Wn[k];
         if (FUSED && a.images) write_images<DIR, P>(a, xyz, idx, Wn);
     }
-    // floor bookkeeping: rare, one atomic per lane that saw something
+    // negative values are rare: one atomic per lane that saw one
     if (flags) atomicOr(a.flags, flags);
-    if (dmass != 0.0) atomicAdd(a.dmass, dmass);
 }
 
 // ---------------------------------------------------------------- march ----
@@ -429,7 +430,6 @@ __global__ __launch_bounds__(64) void k_march(const SweepArgs a)
     for (int v = 0; v < NV; v++) qa[v] = qb[v] = fpp[v] = F1m[v] = sa[v] = sb[v] = fps[v] = F2m[v] = 1.0;
 
     unsigned flags = 0u;
-    double dmass = 0.0;
     const double huge = 1.7976931348623157e308;
     double mn[3] = {huge, huge, huge}, cmax = 0.0;
     double Wnext[NV];
@@ -449,14 +449,14 @@ __global__ __launch_bounds__(64) void k_march(const SweepArgs a)
         double Ws[NV];
         cell_update<P>(W2, a.c1, F1m, F1, Ws);
         {
+            // the stage-1 value of cell t-2 is complete from t = g on (cells
+            // from g-2: the two cells next to the interior, as the tiles)
             unsigned fl = 0u;
             double dm = 0.0;
-            floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
+            floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
             const int c = t - 2;
-            if (c >= g && c < g + n) {
-                flags |= fl;
-                dmass += dm;
-            }
+            if (c >= g - 2) flags |= fl;
+            pk::mass_add(a.dmass, (pvalid && c >= g && c < g + n) ? dm : 0.0);
         }
         double qs[NV];
         prim<P>(Ws, a.gam_1, qs);
@@ -467,12 +467,14 @@ __global__ __launch_bounds__(64) void k_march(const SweepArgs a)
         double Wn[NV];
         cell_update<P>(W4, a.c2, F2m, F2, Wn);
         const int c = t - 4;
-        if (c >= g) {
+        {
             unsigned fl = 0u;
             double dm = 0.0;
-            floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
-            flags |= fl;
-            dmass += dm;
+            floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
+            if (c >= g) flags |= fl;
+            pk::mass_add(a.dmass, (pvalid && c >= g) ? dm : 0.0);
+        }
+        if (c >= g) {
             if (a.dtslots && pvalid) dt_cell<DIR, P>(a, Wn, mn, cmax);
             if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
             if (pvalid) {
@@ -495,7 +497,6 @@ __global__ __launch_bounds__(64) void k_march(const SweepArgs a)
     }
     if (a.dtslots) dt_reduce(a, mn, cmax, blockIdx.x);
     if (pvalid && flags) atomicOr(a.flags, flags);
-    if (pvalid && dmass != 0.0) atomicAdd(a.dmass, dmass);
 }
 
 // The hydro sweep fits four waves per SIMD in 128 VGPRs (2 spilled; 5 % faster
@@ -561,7 +562,6 @@ __global__ __launch_bounds__(512) void k_sweep_list(const ListArgs L)
         a.psi_fac = L.apply_fac ? sfac : 1.0;
         a.dtslots = L.dtslots ? L.dtslots + (L.per_block ? 4ll * L.nslots * blk : 0ll) : nullptr;
         a.nslots = L.nslots;
-        a.images = 1;
         sweep_tile<DIR, P, true, LIM>(a, (int)(item - first), item, lds, lds + P::NV * blockDim.x);
         item++;
     }

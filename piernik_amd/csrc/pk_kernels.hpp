@@ -30,8 +30,9 @@ struct SweepArgs {
     double psi_fac;      // multiplier applied to psi on output (fused GLM damping), else 1
     double smalld, smallei;
     int use_smalld, use_smallei;
+    int ei_flag;         // record negative internal energies (disallow_negatives, sources.F90:425)
     unsigned *flags;     // device: bit 0 negative density, bit 1 negative internal energy
-    double *dmass;       // device: floor-added mass / cell volume
+    unsigned long long *dmass;  // device: floor-added mass / cell volume, exact counters (pk_mass.hpp)
     // fused mode only:
     int images;          // also write every output cell's periodic ghost images (one periodic block)
     unsigned long long *dtslots;  // if set: per slot, min of dl_d/c_d (d = x, y, z) and max c of the

@@ -385,13 +385,14 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, do
 
 // Floors after the update (care_for_positives): density floor, then
 // internal-energy floor with the unconditional energy re-sum.  The B used for
-// the magnetic energy is the updated one, in sweep order.  Bit 0 of *flags:
-// a negative density was seen; bit 1: a negative internal energy was seen.
-// *dmass receives the mass the density floor added.
+// the magnetic energy is the updated one, in sweep order.  Bit 0 of flags: a
+// negative density was seen (sources.F90:349-350); bit 1: a negative internal
+// energy was seen, recorded only when ei_flag is set (disallow_negatives,
+// sources.F90:425).  dmass receives the mass the density floor added to the
+// cell (max(rho, smalld) - rho, which is >= 0).
 template <class P>
-PK_HD void floors(double (&u)[P::NV], double smalld, double smallei,
-                                       bool use_smalld, bool use_smallei, unsigned &flags,
-                                       double &dmass)
+PK_HD void floors(double (&u)[P::NV], double smalld, double smallei, bool use_smalld, bool use_smallei,
+                  bool ei_flag, unsigned &flags, double &dmass)
 {
     if (u[0] < 0.0) flags |= 1u;
     if (use_smalld) {
@@ -407,7 +408,7 @@ PK_HD void floors(double (&u)[P::NV], double smalld, double smallei,
     } else {
         ei = u[4] - kin;
     }
-    if (ei < 0.0) flags |= 2u;
+    if (ei_flag && ei < 0.0) flags |= 2u;
     if (use_smallei) ei = fmx(ei, smallei);
     double e = ei + kin;
     if constexpr (P::MAG) e = e + mag;

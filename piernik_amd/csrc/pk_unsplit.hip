@@ -28,6 +28,7 @@
 // contraction and fast division into namespace pk_fast.
 #include <hip/hip_runtime.h>
 #include "pk_math.hpp"
+#include "pk_mass.hpp"
 #include "pk_unsplit.hpp"
 
 namespace PK_KNS {
@@ -226,44 +227,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int v = 0; v < NV; v++) A[v * nt + t] = f[v];
     __syncthreads();
     const bool upd = pvalid && sl >= 2 && sl < 2 + out && cs < a.g[DIR] + a.n[DIR];
-    if (!upd) return;
-    const long long ci = idx;  // an interior cell: csc == cs
-    const double *S = first ? a.base : a.dst;
+    const long long ci = idx;  // an interior cell when upd: csc == cs
     double T[NV];
+    unsigned flags = 0u;
+    double dmass = 0.0;
+    if (upd) {
+        const double *S = first ? a.base : a.dst;
 #pragma unroll
-    for (int v = 0; v < NV; v++) T[v] = S[(long long)v * a.sv + ci];
-    const double coef = (a.dt / a.dl[DIR]) * a.rk;
+        for (int v = 0; v < NV; v++) T[v] = S[(long long)v * a.sv + ci];
+        const double coef = (a.dt / a.dl[DIR]) * a.rk;
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-        const int v = uplane(DIR, k);
-        T[v] = T[v] + coef * (A[k * nt + tm] - f[k]);
-    }
-    if (last) {
-        unsigned flags = 0u;
-        double dmass = 0.0;
-        for (int d = 0; d < 3; d++) {
-            if (!a.has_dir[d]) continue;
-            double Ws[NV];
-#pragma unroll
-            for (int kk = 0; kk < NV; kk++) Ws[kk] = T[uplane(d, kk)];
-#pragma unroll
-            for (int v = 0; v < 5; v++) Ws[v] = Ws[v] + 0.0;
-            if (a.last) {
-                unsigned fl = 0u;
-                double dm = 0.0;
-                floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
-                flags |= fl;
-                dmass += dm;
-            }
-#pragma unroll
-            for (int kk = 0; kk < NV; kk++) T[uplane(d, kk)] = Ws[kk];
+        for (int k = 0; k < NV; k++) {
+            const int v = uplane(DIR, k);
+            T[v] = T[v] + coef * (A[k * nt + tm] - f[k]);
         }
-        if (flags) atomicOr(a.flags, flags);
-        if (dmass != 0.0) atomicAdd(a.dmass, dmass);
-        if constexpr (P::MAG) T[8] = T[8] * a.psi_fac;  // glmdamping after the stages (1 at the first stage)
-    }
+        if (last) {
+            for (int d = 0; d < 3; d++) {
+                if (!a.has_dir[d]) continue;
+                double Ws[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + ci] = T[v];
+                for (int kk = 0; kk < NV; kk++) Ws[kk] = T[uplane(d, kk)];
+#pragma unroll
+                for (int v = 0; v < 5; v++) Ws[v] = Ws[v] + 0.0;
+                if (a.last) {
+                    unsigned fl = 0u;
+                    double dm = 0.0;
+                    floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
+                    flags |= fl;
+                    dmass += dm;  // one cell's floor events, in the direction order
+                }
+#pragma unroll
+                for (int kk = 0; kk < NV; kk++) T[uplane(d, kk)] = Ws[kk];
+            }
+            if constexpr (P::MAG) T[8] = T[8] * a.psi_fac;  // glmdamping after the stages (1 at the first stage)
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + ci] = T[v];
+    }
+    // last and a.last are the same for every lane: the whole wave takes part
+    if (last && a.last) pk::mass_add(a.dmass, dmass);
+    if (!upd) return;
+    if (flags) atomicOr(a.flags, flags);
     if (!last) return;
     // periodic images of the finished cell (the stage's ghost refresh): every
     // ghost has exactly one interior pre-image, since n >= nb
@@ -297,46 +301,47 @@ __global__ __launch_bounds__(256) void k_unsplit_update(const UnsplitArgs a, con
     const long long nx = a.n[0], nxy = nx * a.n[1];
     const long long total = nxy * a.n[2];
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= total) return;
-    const int i = (int)(e % nx), j = (int)((e / nx) % a.n[1]), k = (int)(e / nxy);
-    const long long idx =
-        (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
-    double T[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) T[v] = a.base[(long long)v * a.sv + idx];
-    const double *Fd[3] = {F0, F1, F2};
-    const long long st[3] = {1, a.sy, a.sz};
-    for (int d = 0; d < 3; d++) {
-        if (!a.has_dir[d]) continue;
-        const double coef = (a.dt / a.dl[d]) * a.rk;
-        const double *F = Fd[d];
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-            T[v] = T[v] + coef * (F[(long long)v * a.sv + idx] - F[(long long)v * a.sv + idx + st[d]]);
-    }
     unsigned flags = 0u;
     double dmass = 0.0;
-    for (int d = 0; d < 3; d++) {
-        if (!a.has_dir[d]) continue;
-        double W[NV];
+    if (e < total) {
+        const int i = (int)(e % nx), j = (int)((e / nx) % a.n[1]), k = (int)(e / nxy);
+        const long long idx =
+            (long long)(i + a.g[0]) + (long long)a.sy * (j + a.g[1]) + (long long)a.sz * (k + a.g[2]);
+        double T[NV];
 #pragma unroll
-        for (int kk = 0; kk < NV; kk++) W[kk] = T[uplane(d, kk)];
+        for (int v = 0; v < NV; v++) T[v] = a.base[(long long)v * a.sv + idx];
+        const double *Fd[3] = {F0, F1, F2};
+        const long long st[3] = {1, a.sy, a.sz};
+        for (int d = 0; d < 3; d++) {
+            if (!a.has_dir[d]) continue;
+            const double coef = (a.dt / a.dl[d]) * a.rk;
+            const double *F = Fd[d];
 #pragma unroll
-        for (int v = 0; v < 5; v++) W[v] = W[v] + 0.0;
-        if (a.last) {
-            unsigned fl = 0u;
-            double dm = 0.0;
-            floors<P>(W, a.smalld, a.smallei, a.use_smalld, a.use_smallei, fl, dm);
-            flags |= fl;
-            dmass += dm;
+            for (int v = 0; v < NV; v++)
+                T[v] = T[v] + coef * (F[(long long)v * a.sv + idx] - F[(long long)v * a.sv + idx + st[d]]);
+        }
+        for (int d = 0; d < 3; d++) {
+            if (!a.has_dir[d]) continue;
+            double W[NV];
+#pragma unroll
+            for (int kk = 0; kk < NV; kk++) W[kk] = T[uplane(d, kk)];
+#pragma unroll
+            for (int v = 0; v < 5; v++) W[v] = W[v] + 0.0;
+            if (a.last) {
+                unsigned fl = 0u;
+                double dm = 0.0;
+                floors<P>(W, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
+                flags |= fl;
+                dmass += dm;
+            }
+#pragma unroll
+            for (int kk = 0; kk < NV; kk++) T[uplane(d, kk)] = W[kk];
         }
 #pragma unroll
-        for (int kk = 0; kk < NV; kk++) T[uplane(d, kk)] = W[kk];
+        for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + idx] = T[v];
     }
-#pragma unroll
-    for (int v = 0; v < NV; v++) a.dst[(long long)v * a.sv + idx] = T[v];
+    if (a.last) pk::mass_add(a.dmass, dmass);
     if (flags) atomicOr(a.flags, flags);
-    if (dmass != 0.0) atomicAdd(a.dmass, dmass);
 }
 
 // Tile shapes of the flux kernel, pencils x cells: along x one wave per

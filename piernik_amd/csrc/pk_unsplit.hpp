@@ -22,8 +22,9 @@ struct UnsplitArgs {
     int last;            // last stage: care_for_positives runs
     double smalld, smallei;
     int use_smalld, use_smallei;
+    int ei_flag;         // record negative internal energies (disallow_negatives)
     unsigned *flags;     // device: bit 0 negative density, bit 1 negative internal energy
-    double *dmass;       // device: floor-added mass / cell volume
+    unsigned long long *dmass;  // device: floor-added mass / cell volume, exact counters (pk_mass.hpp)
 };
 
 // Interface fluxes along dir of the transverse-interior pencils into F (tiled)

@@ -1,10 +1,26 @@
 // Host build of the sweep's per-interface arithmetic (piernik_amd/csrc/pk_math.hpp)
 // for bitwise comparison with the oracle on the CPU.  Test helper only.
 #include "../../piernik_amd/csrc/pk_adjoint.hpp"
+#include "../../piernik_amd/csrc/pk_mass.hpp"
 
 extern "C" {
 
 double hm_spacing(double x) { return pk::spacing(x); }
+
+// The floor's mass counters (pk_mass.hpp) over n values, each added digit by
+// digit as a lane adds it on the device; returns the rounded total.
+double hm_mass_sum(int n, const double *x)
+{
+    unsigned long long acc[pk::kMassDigits] = {};
+    for (int k = 0; k < n; k++) {
+        uint32_t d[3];
+        const int i = pk::mass_place(x[k], d[0], d[1], d[2]);
+        if (i < 0) continue;
+        for (int j = 0; j < 3; j++)
+            if (i + j < pk::kMassDigits) acc[i + j] += d[j];
+    }
+    return pk::mass_value(acc);
+}
 
 double hm_limit(int lim, double dl, double dr) { return pk::limit(lim, dl, dr); }
 
@@ -55,20 +71,20 @@ void hm_hlld(int m, const double *ql, const double *qr, double gam, double ch, d
 }
 
 // u: (m, nv) column-major; applies the floors cell by cell
-void hm_floors(int m, int mhd, double smalld, double smallei, int use_smalld, int use_smallei, double *u,
-               unsigned *flags)
+void hm_floors(int m, int mhd, double smalld, double smallei, int use_smalld, int use_smallei, int ei_flag,
+               double *u, unsigned *flags)
 {
     for (int i = 0; i < m; i++) {
         double dm = 0.0;
         if (mhd) {
             double a[9];
             for (int v = 0; v < 9; v++) a[v] = u[v * m + i];
-            pk::floors<pk::MHD>(a, smalld, smallei, use_smalld, use_smallei, *flags, dm);
+            pk::floors<pk::MHD>(a, smalld, smallei, use_smalld, use_smallei, ei_flag, *flags, dm);
             for (int v = 0; v < 9; v++) u[v * m + i] = a[v];
         } else {
             double a[5];
             for (int v = 0; v < 5; v++) a[v] = u[v * m + i];
-            pk::floors<pk::HD>(a, smalld, smallei, use_smalld, use_smallei, *flags, dm);
+            pk::floors<pk::HD>(a, smalld, smallei, use_smalld, use_smallei, ei_flag, *flags, dm);
             for (int v = 0; v < 5; v++) u[v * m + i] = a[v];
         }
     }
@@ -103,7 +119,7 @@ void hm_floors_hd(double smalld, double smallei, int use_smalld, int use_smallei
     for (int v = 0; v < 5; v++) a[v] = u[v];
     unsigned fl = 0;
     double dm = 0.0;
-    pk::floors<pk::HD>(a, smalld, smallei, use_smalld, use_smallei, fl, dm);
+    pk::floors<pk::HD>(a, smalld, smallei, use_smalld, use_smallei, true, fl, dm);
     for (int v = 0; v < 5; v++) u[v] = a[v];
 }
 
@@ -152,7 +168,7 @@ void hm_floors_mhd(double smalld, double smallei, int use_smalld, int use_smalle
     for (int v = 0; v < 9; v++) a[v] = u[v];
     unsigned fl = 0;
     double dm = 0.0;
-    pk::floors<pk::MHD>(a, smalld, smallei, use_smalld, use_smallei, fl, dm);
+    pk::floors<pk::MHD>(a, smalld, smallei, use_smalld, use_smallei, true, fl, dm);
     for (int v = 0; v < 9; v++) u[v] = a[v];
 }
 

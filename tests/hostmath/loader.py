@@ -6,7 +6,8 @@ import subprocess
 
 HM = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HM, "..", "..", "piernik_amd", "csrc")
-SRC = [os.path.join(HM, "hostmath.cpp"), os.path.join(CSRC, "pk_math.hpp"), os.path.join(CSRC, "pk_adjoint.hpp")]
+SRC = [os.path.join(HM, "hostmath.cpp"), os.path.join(CSRC, "pk_math.hpp"), os.path.join(CSRC, "pk_adjoint.hpp"),
+       os.path.join(CSRC, "pk_mass.hpp")]
 LIB = os.path.join(HM, "libhostmath.so")
 P = ctypes.POINTER(ctypes.c_double)
 
@@ -25,12 +26,14 @@ def load():
     d, i = ctypes.c_double, ctypes.c_int
     L.hm_spacing.argtypes = [d]
     L.hm_spacing.restype = d
+    L.hm_mass_sum.argtypes = [i, P]
+    L.hm_mass_sum.restype = d
     L.hm_limit.argtypes = [i, d, d]
     L.hm_limit.restype = d
     L.hm_prim.argtypes = [i, i, d, P, P]
     L.hm_hllc.argtypes = [i, P, P, d, P]
     L.hm_hlld.argtypes = [i, P, P, d, d, P]
-    L.hm_floors.argtypes = [i, i, d, d, i, i, P, ctypes.POINTER(ctypes.c_uint)]
+    L.hm_floors.argtypes = [i, i, d, d, i, i, i, P, ctypes.POINTER(ctypes.c_uint)]
     L.hm_prim_vjp_hd.argtypes = [d, P, P, P]
     L.hm_limit_vjp.argtypes = [i, d, d, d, P]
     L.hm_hllc_vjp.argtypes = [P, P, d, P, P, P]

@@ -1,5 +1,6 @@
 """Step retries (cflcontrol = 'redo' / 'flexible', pk_set_cfl_control) on the
-GPU against the reference's control flow driving the oracle.
+GPU against the reference's control flow driving the oracle
+(oracle.pyoracle.main_loop, pinned in tests/test_oracle_retry.py).
 
 The reference's main loop (piernik.F90:125-170) saves the state before each
 step (repeat_fluidstep, timestep_retry.F90:86-273), and after the step
@@ -19,78 +20,13 @@ from piernik_amd import setups
 pytestmark = pytest.mark.gpu
 
 
-def oracle_run_redo(po, p, ob, nsteps, mode, cfl_max, dt_shrink=0.5):
-    """The reference's main loop with repetitive steps (or cfl_auto), over the oracle."""
-    c = p.cfl
-    cmax = min(max(cfl_max, min(c * 1.1, c + 0.05, (1.0 + c) / 2.0)), 1.0)  # global.F90:304
-    t, dt_old, dt_full, c_old, shrink = 0.0, -1.0, 0.0, 0.0, 1.0
-    s, attempt, redone, repeat = 0, 0, 0, False
-    c_main_old, stepcfl, cfl_c = 0.0, 0.0, 1.0  # timestep_pub.F90:42
-    po.fill_ghosts(p, ob)
-    saved = None
-    while s < nsteps:
-        if not repeat:
-            dt, c_old = po.time_step(p, ob, s, dt_old)
-            if mode == "auto":  # cfl_auto, timestep.F90:368-405
-                stepcfl_old, stepcfl = stepcfl, c
-                if c_main_old > 0.0:
-                    stepcfl = c_old / c_main_old * c
-                else:
-                    stepcfl_old = c
-                if stepcfl > 0.0 and dt_old > 0.0:
-                    cfl_c = min(1.0, 0.5 * (cfl_c + min(1.0, stepcfl_old / stepcfl * dt / dt_old)))
-                    dt = dt * cfl_c
-                else:
-                    cfl_c = 1.0
-            c_main_old = c_old
-            dt_full = dt
-            saved = [a.copy() for a in (ob.u, ob.b, ob.psi) if a is not None]
-            mm = list(ob.rep.magic_mass)
-            neg0 = (ob.rep.dn_negative, ob.rep.ei_negative)
-        else:
-            for a, sv in zip([a for a in (ob.u, ob.b, ob.psi) if a is not None], saved):
-                a[...] = sv
-            ob.rep.magic_mass[0], ob.rep.magic_mass[1] = mm
-            dt = dt_full * shrink
-        ob.rep.dn_negative = ob.rep.ei_negative = 0
-        po.fluid_update(p, ob, dt)
-        dt_old = dt
-        if mode == "auto":
-            t = t + dt
-            t = t + dt
-            s += 1
-            continue
-        _, c_new = po.time_step(p, ob, s + 1, dt_old)
-        neg = ob.rep.dn_negative or ob.rep.ei_negative
-        ob.rep.dn_negative, ob.rep.ei_negative = neg0[0] | ob.rep.dn_negative, neg0[1] | ob.rep.ei_negative
-        stepcfl = p.cfl * shrink
-        if c_old > 0.0:
-            stepcfl = c_new / c_old * p.cfl * shrink
-        repeat = stepcfl > cmax or bool(neg)
-        if repeat:
-            if mode == "flexible":
-                shrink = cmax / stepcfl
-                if attempt >= 1:
-                    shrink = shrink * dt_shrink
-            else:
-                shrink = dt_shrink ** (attempt + 1)
-            attempt += 1
-            redone += 1
-            continue
-        shrink, attempt = 1.0, 0
-        t = t + dt
-        t = t + dt
-        s += 1
-    return t, redone
-
-
 @pytest.mark.parametrize("mode,driver", [("redo", "block"), ("flexible", "block"), ("redo", "list"),
                                          ("auto", "block"), ("auto", "list")])
 def test_redo_matches_reference_control_flow(oracle, mode, driver):
     cg, cfg = setups.sedov_hd(n=24)
     p = oracle.params_from_config(cfg)
     ob = oracle.OBlock.from_cg(cg)
-    t_o, redone_o = oracle_run_redo(oracle, p, ob, 6, mode, 0.8)
+    t_o, redone_o = oracle.run_redo(p, ob, 6, mode, 0.8)
     assert redone_o >= 1 or mode == "auto"
     print(f"cflcontrol {mode} via {driver}: oracle redid {redone_o} step(s)", flush=True)
     pk.load_library()
@@ -158,4 +94,40 @@ def test_time_control_matches_reference(oracle, driver):
     with pytest.raises(pk.PiernikError, match="below dt_min"):
         pk.set_time_control(dt_min=1.0)
         pk.run(cg, 1)
+    pk.finalize()
+
+
+@pytest.mark.parametrize("disallow", [False, True])
+def test_negatives_and_redo(oracle, disallow):
+    """Floors that raise ei_negative in every attempt (cells with zero total
+    energy): without disallow_negatives the steps are kept and the state
+    matches the oracle bit for bit; with it every attempt is redone until
+    max_redostep_attempts is exceeded, on the GPU as in the reference's
+    control flow (which dies there)."""
+    from floors_setup import floor_block
+
+    cg, cfg = floor_block(magnetized=True)
+    p = oracle.params_from_config(cfg, disallow_negatives=disallow)
+    ob = oracle.OBlock.from_cg(cg)
+    pk.load_library()
+    pk.finalize()
+    pk.init(cfg)
+    pk.set_cfl_control("redo", cfl_max=0.95, max_redostep_attempts=2, disallow_negatives=disallow)
+    cg.upload()
+    print(f"negatives with cflcontrol redo, disallow_negatives={disallow}", flush=True)
+    if disallow:
+        with pytest.raises(RuntimeError, match="max_redostep_attempts"):
+            oracle.run_redo(p, ob, 2, "redo", 0.95, max_attempts=2)
+        with pytest.raises(pk.PiernikError, match="max_redostep_attempts"):
+            pk.run(cg, 2)
+    else:
+        t_o, redone_o = oracle.run_redo(p, ob, 2, "redo", 0.95, max_attempts=2)
+        t_g, _ = pk.run(cg, 2)
+        assert t_g == t_o and pk.redone_steps() == redone_o
+        cg.download()
+        for name in ("u", "b", "psi"):
+            assert np.array_equal(getattr(cg, name), getattr(ob, name)), name
+        rep = pk.get_report(cg)
+        assert rep["dn_negative"] == bool(ob.rep.dn_negative)
+        assert rep["ei_negative"] == bool(ob.rep.ei_negative) == False  # noqa: E712
     pk.finalize()

@@ -132,10 +132,12 @@ def test_hlld(hm, oracle, gam, ch):
     assert bad.size == 0, (bad[:5], f[tuple(bad[0])], ref[tuple(bad[0])])
 
 
+@pytest.mark.parametrize("disallow", [False, True])
 @pytest.mark.parametrize("mhd", [False, True])
-def test_floors(hm, oracle, mhd):
+def test_floors(hm, oracle, mhd, disallow):
     """Floors inside the full pencil stage: one pencil with cells that cross
-    smalld and smallei, against the oracle's care_for_positives."""
+    smalld and smallei, against the oracle's care_for_positives; the
+    negative-internal-energy flag only with disallow_negatives."""
     rng = np.random.default_rng(5)
     m = 64
     nv = 9 if mhd else 5
@@ -147,13 +149,37 @@ def test_floors(hm, oracle, mhd):
         u[:, 5:9] = rng.normal(0, 1, (m, 4))
     a = u.copy(order="F")
     fl = ctypes.c_uint(0)
-    hm.hm_floors(m, int(mhd), 0.05, 0.01, 1, 1, dp(a), ctypes.byref(fl))
-    p = oracle.params(magnetized=mhd, smalld=0.05, smallei=0.01)
+    hm.hm_floors(m, int(mhd), 0.05, 0.01, 1, 1, int(disallow), dp(a), ctypes.byref(fl))
+    p = oracle.params(magnetized=mhd, smalld=0.05, smallei=0.01, disallow_negatives=disallow)
     u1 = np.asfortranarray(u[:, :5]).copy(order="F")
     b1 = np.asfortranarray(u[:, 5:8]) if mhd else None
     rep = oracle.Report()
     oracle.lib().or_floors(ctypes.byref(p), m, dp(u1), None if b1 is None else dp(b1), 0, 1.0,
                            ctypes.byref(rep))
     assert np.array_equal(a[:, :5], u1)
-    assert bool(fl.value & 1) == bool(rep.dn_negative)
-    assert bool(fl.value & 2) == bool(rep.ei_negative)
+    assert rep.dn_negative == 1 and bool(fl.value & 1)
+    assert bool(fl.value & 2) == bool(rep.ei_negative) == disallow
+
+
+def test_floor_mass_counters_exact(hm):
+    """The device's floor-mass counters (pk_mass.hpp): the total of values
+    spread over 70 binary orders of magnitude, added digit by digit in any
+    order, is the exact sum correctly rounded to FP64 (the same bits in
+    every order); zeros and values below 2^-160 add nothing."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(21)
+    for trial in range(20):
+        n = int(rng.integers(1, 400))
+        x = np.abs(rng.standard_normal(n)) * np.exp2(rng.integers(-60, 10, n).astype(np.float64))
+        x[rng.random(n) < 0.1] = 0.0
+        exact = float(sum(Fraction(float(v)) for v in x))
+        a = np.ascontiguousarray(x)
+        got = hm.hm_mass_sum(n, dp(a))
+        assert got == exact, (trial, got, exact)
+        b = np.ascontiguousarray(rng.permutation(x))
+        assert hm.hm_mass_sum(n, dp(b)) == got
+    tiny = np.array([2.0 ** -170, 0.0, 2.0 ** -200])
+    assert hm.hm_mass_sum(3, dp(tiny)) == 0.0
+    one = np.array([1.0, 2.0 ** -160, 2.0 ** 80])
+    assert hm.hm_mass_sum(3, dp(one)) == float(Fraction(1) + Fraction(2) ** -160 + Fraction(2) ** 80)
