@@ -76,7 +76,9 @@ typedef struct pk_config {
     double glm_alpha;          /* GLM damping (hdc.F90:325) */
     int32_t has_dir[3];        /* dom%has_dir: active directions */
     int32_t tile_x[2];         /* sweep tile shape (pencils, cells) along x; 0 = autotuned default;
-                                  cells = -1: one lane per whole pencil (march kernel) */
+                                  cells = -1: one lane per whole pencil (march kernel);
+                                  cells = -m <= -2: the stream kernel, pencils per lane group
+                                  (1 along x; 1, 2 or 4 along y and z), m iterations per workgroup */
     int32_t tile_yz[2];        /* sweep tile shape (pencils, cells) along y and z; 0 = default */
     int32_t fast_math;         /* 0: every operation rounds as the reference's FP64 code (bit-exact);
                                   1: FMA contraction and fast division/sqrt in the split sweeps and
@@ -331,9 +333,11 @@ int pk_run_unsplit(int64_t cg_id, double tend, int32_t nmax, double *t_out, int3
  * tiles_out (may be NULL) receives (pencils, cells) for x, y, z; 0 for an
  * inactive direction; cells = -1 is the march kernel (one lane per whole
  * pencil, 64 pencils per workgroup), a candidate for y and z sweeps of
- * blocks with at least 65536 pencils.  A shape given in pk_config
- * (tile_x, tile_yz; (64, -1) selects the march kernel) overrides the tuned
- * one. */
+ * blocks with at least 65536 pencils; cells = -m <= -2 is the stream kernel
+ * (a one-wave workgroup walks the pencils of m * 64 stream positions, each
+ * cell's two stages computed once).  A shape given in pk_config (tile_x,
+ * tile_yz; (64, -1) selects the march kernel, (tp, -m) the stream kernel)
+ * overrides the tuned one. */
 int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6]);
 
 /* Tile shapes of the persistent list sweep: per direction and fluid kind the

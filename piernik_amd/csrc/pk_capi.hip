@@ -250,6 +250,24 @@ void tile_geometry(const Block *b, int d, bool fused, int tp, int ts, int &TP, i
 // Tile length that selects the march kernel: one lane per whole pencil.
 const int kMarch = -1;
 
+// Tile lengths of -2 and below select the stream kernel: a one-wave
+// workgroup walks m = -TS iterations of the stream of TP-pencil groups
+// (64/TP positions each).
+bool is_stream(int ts) { return ts <= -2; }
+
+void stream_geometry(const Block *b, int d, int tp, int m, int &TP, int &out, int &chunks, int &ptiles)
+{
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    if (d == 0 || (tp != 2 && tp != 4)) tp = 1;
+    if (m < 2) m = 2;
+    TP = tp;
+    out = m * (64 / tp);  // positions per workgroup
+    const long long npen = (long long)b->n[d1] * b->n[d2];
+    const long long npos = (npen + tp - 1) / tp * (long long)(b->n[d] + 8);
+    chunks = 1;
+    ptiles = (int)((npos + out - 1) / out);
+}
+
 // Precedence: the shape given in pk_config, then the one pk_autotune
 // measured for this block shape, then 3 x 72 (4-wave workgroups, two per CU).
 // A tile length of kMarch selects the march kernel for the fused sweep.
@@ -257,13 +275,18 @@ void choose_tile(const Block *b, int d, bool fused, bool mag, int &TP, int &TS, 
 {
     const int *ov = (d == 0) ? g_cfg.tile_x : g_cfg.tile_yz;
     int tp = ov[0], ts = ov[1];
-    if (ts == kMarch && !fused) ts = 0;
-    if (ts <= 0 && ts != kMarch && fused) {
+    if ((ts == kMarch || is_stream(ts)) && !fused) ts = 0;
+    if (ts == 0 && fused) {
         auto it = g_tuned.find(tune_key(b, d, mag));
         if (it != g_tuned.end()) {
             tp = it->second.first;
             ts = it->second.second;
         }
+    }
+    if (is_stream(ts) && fused) {
+        TS = ts;
+        stream_geometry(b, d, tp, -ts, TP, out, chunks, ptiles);
+        return;
     }
     if (ts == kMarch && fused) {
         // whole pencils, one lane each, 64-lane workgroups (k_march)
@@ -2710,6 +2733,26 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
                     if (ms < best) {
                         best = ms;
                         best_t = {TP, TS};
+                    }
+                }
+            }
+            // the stream kernel: one pencil per lane group along x, 1, 2 or 4
+            // along y and z, workgroups of 8 to 64 iterations
+            for (int tp : {1, 2, 4}) {
+                if (d == 0 && tp != 1) continue;
+                for (int m : {8, 16, 32, 64}) {
+                    a.TS = -m;
+                    stream_geometry(b, d, tp, m, a.TP, a.out, a.chunks, a.ptiles);
+                    HIPCHK(sweep_kernel(d, g_fmag[f], true, a));  // warm-up
+                    HIPCHK(hipEventRecord(e0, g_stream));
+                    for (int rep = 0; rep < 3; rep++) HIPCHK(sweep_kernel(d, g_fmag[f], true, a));
+                    HIPCHK(hipEventRecord(e1, g_stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        best_t = {tp, -m};
                     }
                 }
             }

@@ -499,6 +499,8 @@ __global__ __launch_bounds__(64) void k_march(const SweepArgs a)
     if (pvalid && flags) atomicOr(a.flags, flags);
 }
 
+#include "pk_stream.hpp"
+
 // The hydro sweep fits four waves per SIMD in 128 VGPRs (2 spilled; 5 % faster
 // at 128^3 than three waves); the MHD sweep keeps its two (227-256 VGPRs).
 template <int DIR, class P, bool FUSED, int LIM>
@@ -914,10 +916,55 @@ static hipError_t launch_march_t(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
+// The stream kernel: a.TP pencils per lane group, a.out positions per
+// workgroup (a multiple of 64 / a.TP), a.ptiles workgroups.
+template <int DIR, class P, int LIM, int TP>
+static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
+{
+    constexpr int d1 = (DIR == 0) ? 1 : 0, d2 = (DIR == 2) ? 1 : 2;
+    const long long npen = (long long)a.n[d1] * a.n[d2];
+    const long long npos = (npen + TP - 1) / TP * (long long)(a.n[DIR] + 8);
+    const int span = a.out;
+    if (a.g[DIR] < 4 || span <= 0 || span % (64 / TP) != 0 || npos + span + 64 > 0x7fffffffll)
+        return hipErrorInvalidValue;
+    const long long nblk = (npos + span - 1) / span;
+    if (nblk != a.ptiles) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stream<DIR, P, LIM, TP>), dim3((unsigned)nblk), dim3(64), 0, st, a, span);
+    return hipGetLastError();
+}
+
+template <int DIR, class P, int LIM>
+static hipError_t launch_stream_t(const SweepArgs &a, hipStream_t st)
+{
+    if constexpr (DIR == 0) {
+        // x pencils are rows: one per lane group
+        return a.TP == 1 ? launch_stream_tp<DIR, P, LIM, 1>(a, st) : hipErrorInvalidValue;
+    } else {
+        switch (a.TP) {
+        case 1: return launch_stream_tp<DIR, P, LIM, 1>(a, st);
+        case 2: return launch_stream_tp<DIR, P, LIM, 2>(a, st);
+        case 4: return launch_stream_tp<DIR, P, LIM, 4>(a, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+}
+
 template <int DIR, class P, bool FUSED>
 static hipError_t launch_lim(const SweepArgs &a, hipStream_t st)
 {
     if constexpr (FUSED) {
+        if (a.TS <= -2) {  // the stream kernel (k_stream)
+            if (a.lim == a.limb) {
+                switch (a.lim) {
+                case LIM_VANLEER: return launch_stream_t<DIR, P, LIM_VANLEER>(a, st);
+                case LIM_MINMOD: return launch_stream_t<DIR, P, LIM_MINMOD>(a, st);
+                case LIM_MONCEN: return launch_stream_t<DIR, P, LIM_MONCEN>(a, st);
+                case LIM_SUPERBEE: return launch_stream_t<DIR, P, LIM_SUPERBEE>(a, st);
+                default: break;
+                }
+            }
+            return launch_stream_t<DIR, P, -1>(a, st);
+        }
         if (a.TS < 0) {  // whole pencils, one lane each (k_march)
             if (a.lim == a.limb) {
                 switch (a.lim) {

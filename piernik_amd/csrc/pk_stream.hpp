@@ -1,0 +1,327 @@
+// pk_stream.hpp -- the fused RK2 sweep as a stream (included by
+// pk_kernels.hip inside its namespace; uses plane_of, tile helpers, floors,
+// the timestep fold and the periodic images defined there).
+//
+// One wave per workgroup walks a stream of positions along the sweep.  TP
+// pencils (x-consecutive for y and z sweeps, one row for x) advance together;
+// each pencil contributes L = n + 8 positions (its interior and four cells
+// on each side), and the pencils follow each other: the stream is
+//     position s  ->  pencil group s / L, cell (s mod L) - 4 + g.
+// Per iteration the wave takes C = 64 / TP new positions per pencil (lane
+// (p, i): pencil p, position base + i) and finishes everything the new
+// values complete, lagging behind the loads:
+//     q(s)                  primitives of the loaded cell
+//     faces of cell s-1     from q(s-2), q(s-1), q(s)
+//     F1 at (s-2)+1/2       stage-1 flux
+//     U*(s-2)               stage-1 state (floors), its primitives qs
+//     faces* of cell s-3    from qs(s-4), qs(s-3), qs(s-2)
+//     F2 at (s-4)+1/2       stage-2 flux
+//     U1(s-4)               the new state (floors, timestep fold, stored)
+// A lane reads its lagged neighbours from LDS: positions of this iteration
+// from the shared buffer, those of the previous iteration from small carry
+// areas that the last lanes fill before moving on.  Every cell's two RK2
+// stages are computed once, with the operands and operations of the tile
+// kernel in the same order (bit-identical results); only the 8 halo
+// positions per pencil and the start-up positions of each workgroup's range
+// (8 per range) are extra work.  A wave is a whole workgroup, so the
+// barriers between phases only order LDS accesses within the wave.
+//
+// Workgroup w owns positions [w*span, (w+1)*span) (span a multiple of C)
+// and starts 8 positions early so that its first owned outputs have their
+// stencil; it writes only outputs at owned positions.
+
+// LDS carry areas: slot offsets (in positions) of each lagged quantity in
+// the carry block, and how many positions it keeps.
+struct StreamCarry {
+    static constexpr int Q = 0, NQ = 2;    // primitives, lags 1 and 2
+    static constexpr int FP = 2, NFP = 1;  // right face state, lag 1
+    static constexpr int F1 = 3, NF1 = 1;  // stage-1 flux, lag 1
+    static constexpr int QS = 4, NQS = 2;  // stage-1 primitives, lags 1 and 2
+    static constexpr int GP = 6, NGP = 1;  // stage-2 right face state, lag 1
+    static constexpr int F2 = 7, NF2 = 1;  // stage-2 flux, lag 1
+    static constexpr int N = 8;            // positions kept in the carry block
+    static constexpr int ULAG = 4;         // U0 is read 2 and 4 positions back
+};
+
+// LDS bytes of one stream workgroup.
+__host__ __device__ constexpr int stream_lds_doubles(int nv, int tp)
+{
+    return nv * 64 /* shared buffer */ + nv * 64 /* carry block */ + nv * tp * (64 / tp + StreamCarry::ULAG);
+}
+
+template <int DIR, class P, int LIM, int TP>
+__global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span)
+{
+    constexpr int NV = P::NV;
+    constexpr int C = 64 / TP;
+    constexpr int UL = StreamCarry::ULAG;
+    constexpr int SU = TP * (C + UL);  // slots per variable of the U0 buffer
+    static_assert(C >= UL && TP * StreamCarry::N <= 64, "stream shape");
+    constexpr int ds = DIR;
+    constexpr int d1 = (DIR == 0) ? 1 : 0;
+    constexpr int d2 = (DIR == 2) ? 1 : 2;
+    __shared__ double A[NV * 64];  // the quantity of the current phase, slot TP*i + p
+    __shared__ double K[NV * 64];  // carry block: per variable, StreamCarry::N positions x TP
+    __shared__ double U[NV * SU];  // U0 with UL positions of the previous iteration in front
+
+    const int lane = threadIdx.x;
+    const int p = lane % TP, i = lane / TP;
+    const int L = a.n[ds] + 8;
+    const int n1 = a.n[d1];
+    const int npen = n1 * a.n[d2];
+    const int ngroups = (npen + TP - 1) / TP;
+    const int npos = ngroups * L;
+    const int s0 = blockIdx.x * span;
+    const int s1 = min(s0 + span, npos);
+    const long long st = (DIR == 0) ? 1ll : ((DIR == 1) ? (long long)a.sy : (long long)a.sz);
+
+    // LDS slots read at each lag (loop-invariant): this iteration's positions
+    // from A, earlier ones from the quantity's carry positions in K
+    auto rd = [&](int lag, int koff, int klag) -> const double * {
+        return (i >= lag) ? &A[TP * (i - lag) + p] : &K[TP * (koff + klag + i - lag) + p];
+    };
+    const double *q1 = rd(1, StreamCarry::Q, StreamCarry::NQ), *q2 = rd(2, StreamCarry::Q, StreamCarry::NQ);
+    const double *fp1 = rd(1, StreamCarry::FP, StreamCarry::NFP);
+    const double *f11 = rd(1, StreamCarry::F1, StreamCarry::NF1);
+    const double *qs1 = rd(1, StreamCarry::QS, StreamCarry::NQS), *qs2 = rd(2, StreamCarry::QS, StreamCarry::NQS);
+    const double *gp1 = rd(1, StreamCarry::GP, StreamCarry::NGP);
+    const double *f21 = rd(1, StreamCarry::F2, StreamCarry::NF2);
+    double *const aw = &A[TP * i + p];
+    // carry slot this lane fills for a quantity keeping klag positions (null: none)
+    auto cw = [&](int koff, int klag) -> double * {
+        return (i >= C - klag) ? &K[TP * (koff + i - (C - klag)) + p] : nullptr;
+    };
+    double *const cq = cw(StreamCarry::Q, StreamCarry::NQ), *const cfp = cw(StreamCarry::FP, StreamCarry::NFP);
+    double *const cf1 = cw(StreamCarry::F1, StreamCarry::NF1), *const cqs = cw(StreamCarry::QS, StreamCarry::NQS);
+    double *const cgp = cw(StreamCarry::GP, StreamCarry::NGP), *const cf2 = cw(StreamCarry::F2, StreamCarry::NF2);
+    double *const uw = &U[TP * (i + UL) + p];
+    const double *const u2 = &U[TP * (i + UL - 2) + p], *const u4 = &U[TP * (i + UL - 4) + p];
+    double *const uc = (i >= C - UL) ? &U[TP * (i + UL - C) + p] : nullptr;
+
+    // position of this lane in the stream: cell j of pencil group grp, its
+    // pencil's transverse coordinates, advanced by C per iteration
+    // (a position before the stream, in the first workgroup's start-up, is
+    // in group -1, whose pencils do not exist)
+    int sp = s0 - 8 + i;
+    int grp, j, c1, c2;
+    if (sp < 0) {
+        grp = -1;
+        j = sp + L;
+        c1 = p - TP;
+        c2 = 0;
+    } else {
+        grp = sp / L;
+        j = sp - grp * L;
+        const int pf = grp * TP + p;
+        c1 = pf % n1;
+        c2 = pf / n1;
+    }
+    auto advance = [&]() {
+        sp += C;
+        j += C;
+        while (j >= L) {
+            j -= L;
+            grp++;
+            c1 += TP;
+            while (c1 >= n1) {
+                c1 -= n1;
+                c2++;
+            }
+        }
+    };
+    // global index (plane 0) of cell j of this lane's pencil; lanes outside
+    // the stream or past the last pencil read a valid cell of the block
+    auto cell_index = [&](int jj, bool &pv) -> long long {
+        pv = grp >= 0 && (grp * TP + p) < npen;
+        const int cc1 = pv ? c1 : 0, cc2 = pv ? c2 : 0;
+        int xyz[3];
+        xyz[ds] = a.g[ds] - 4 + jj;
+        xyz[d1] = a.g[d1] + cc1;
+        xyz[d2] = a.g[d2] + cc2;
+        return (long long)xyz[0] + (long long)a.sy * xyz[1] + (long long)a.sz * xyz[2];
+    };
+
+    unsigned flags = 0u;
+    const double huge = 1.7976931348623157e308;
+    double mn[3] = {huge, huge, huge}, cmax = 0.0;
+    const int g = a.g[ds];
+
+    double W[NV];
+    {
+        bool pv;
+        const long long idx = cell_index(j, pv);
+#pragma unroll
+        for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx];
+    }
+    // outputs lag 4 positions behind the loads: the last owned output,
+    // position s1-1, is finished when position s1+3 is loaded
+    const int iters = (s1 + 4 - (s0 - 8) + C - 1) / C;
+    for (int it = 0; it < iters; it++) {
+        const int jc = j, cc1 = c1, cc2 = c2;  // this iteration's cell (position sp)
+        bool pv;
+        const long long idx = cell_index(jc, pv);
+        advance();
+
+        // U0 and primitives of position sp
+        double q[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) uw[v * SU] = W[v];
+        prim<P>(W, a.gam_1, q);
+#pragma unroll
+        for (int v = 0; v < NV; v++) aw[v * 64] = q[v];
+        __syncthreads();
+
+        // faces of cell sp-1 from q(sp-2), q(sp-1), q(sp)
+        double fp[NV], fm[NV];
+        {
+            double qa[NV], qb[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                qa[v] = q2[v * 64];
+                qb[v] = q1[v * 64];
+            }
+            cell_faces<P, LIM>(a, qa, qb, q, fp, fm);
+        }
+        if (cq) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) cq[v * 64] = q[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) aw[v * 64] = fp[v];
+        __syncthreads();
+
+        // stage-1 flux at (sp-2)+1/2: right face of sp-2, left face of sp-1
+        double F1[NV];
+        {
+            double fl[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) fl[v] = fp1[v * 64];
+            riemann<P>(a, fl, fm, F1);
+        }
+        if (cfp) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) cfp[v * 64] = fp[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) aw[v * 64] = F1[v];
+        __syncthreads();
+
+        // stage-1 state of cell sp-2 and its primitives
+        double qs[NV];
+        {
+            double Fm[NV], W0[NV], Ws[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                Fm[v] = f11[v * 64];
+                W0[v] = u2[v * SU];
+            }
+            cell_update<P>(W0, a.c1, Fm, F1, Ws);
+            unsigned fl = 0u;
+            double dm = 0.0;
+            floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
+            // cells g-2 .. g+n+1 (j-2 in [2, L-2)) raise flags, the interior counts mass
+            const int jj = jc - 2;
+            const bool own = pv && sp - 2 >= s0 && sp - 2 < s1 && jc >= 2;  // same pencil as sp
+            if (own && jj >= 2 && jj < L - 2) flags |= fl;
+            pk::mass_add(a.dmass, (own && jj >= 4 && jj < L - 4) ? dm : 0.0);
+            prim<P>(Ws, a.gam_1, qs);
+        }
+        if (cf1) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) cf1[v * 64] = F1[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) aw[v * 64] = qs[v];
+        __syncthreads();
+
+        // stage-2 faces of cell sp-3 from qs(sp-4), qs(sp-3), qs(sp-2)
+        double gp[NV], gm[NV];
+        {
+            double qa[NV], qb[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                qa[v] = qs2[v * 64];
+                qb[v] = qs1[v * 64];
+            }
+            cell_faces<P, LIM>(a, qa, qb, qs, gp, gm);
+        }
+        if (cqs) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) cqs[v * 64] = qs[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) aw[v * 64] = gp[v];
+        __syncthreads();
+
+        // stage-2 flux at (sp-4)+1/2
+        double F2[NV];
+        {
+            double fl[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) fl[v] = gp1[v * 64];
+            riemann<P>(a, fl, gm, F2);
+        }
+        if (cgp) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) cgp[v * 64] = gp[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) aw[v * 64] = F2[v];
+        __syncthreads();
+
+        // the new state of cell sp-4
+        {
+            double Fm[NV], W0[NV], Wn[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                Fm[v] = f21[v * 64];
+                W0[v] = u4[v * SU];
+            }
+            cell_update<P>(W0, a.c2, Fm, F2, Wn);
+            unsigned fl = 0u;
+            double dm = 0.0;
+            floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
+            const int jo = jc - 4;  // output cell: interior when jo in [4, L-4)
+            const bool out = pv && sp - 4 >= s0 && sp - 4 < s1 && jo >= 4 && jo < L - 4;
+            if (out) flags |= fl;
+            pk::mass_add(a.dmass, out ? dm : 0.0);
+            if (out) {
+                if (a.dtslots) dt_cell<DIR, P>(a, Wn, mn, cmax);
+                if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
+                const long long oidx = idx - 4 * st;
+#pragma unroll
+                for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + oidx] = Wn[k];
+                if (a.images) {
+                    int xyz[3];
+                    xyz[ds] = g - 4 + jo;
+                    xyz[d1] = a.g[d1] + cc1;
+                    xyz[d2] = a.g[d2] + cc2;
+                    write_images<DIR, P>(a, xyz, oidx, Wn);
+                }
+            }
+        }
+        if (cf2) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) cf2[v * 64] = F2[v];
+        }
+        // U0 of the last UL positions goes in front of the next iteration's
+        if (uc) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) uc[v * SU] = uw[v * SU];
+        }
+        __syncthreads();
+        {
+            bool pv2;
+            const long long idx2 = cell_index(j, pv2);
+#pragma unroll
+            for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx2];
+        }
+    }
+    if (a.dtslots) dt_reduce(a, mn, cmax, blockIdx.x);
+    if (flags) atomicOr(a.flags, flags);
+}

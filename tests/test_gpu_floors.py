@@ -145,11 +145,11 @@ def test_floors_list_all_modes(oracle):
 
 
 def test_floor_mass_independent_of_tiles(oracle):
-    """The fused sweep with different tile shapes and with the march kernel
-    (one lane per pencil along y and z) gives the same state and the same
-    magic-mass bits."""
+    """The fused sweep with different tile shapes, the march kernel (one lane
+    per pencil along y and z) and the stream kernel gives the same state and
+    the same magic-mass bits."""
     out = []
-    for tx, tyz in [((1, 64), (4, 64)), ((3, 40), (8, 24)), ((2, 72), (64, -1))]:
+    for tx, tyz in [((1, 64), (4, 64)), ((3, 40), (8, 24)), ((2, 72), (64, -1)), ((1, -4), (4, -3)), ((1, -8), (2, -5))]:
         cg, cfg = fs.floor_block(shape=(24, 20, 16), magnetized=True)
         cfg.tile_x, cfg.tile_yz = tx, tyz
         print(f"floors tiles {tx} {tyz}", flush=True)

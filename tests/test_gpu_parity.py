@@ -201,3 +201,53 @@ def test_march_kernel_bit_exact(oracle, case, tile_x):
     assert t_g == t_o
     assert_same(cg, ob, f"march {case}")
     pk.finalize()
+
+
+# the stream kernel: (pencils per lane group, -iterations per workgroup)
+STREAM_CASES = [("mhd", (1, -8), (4, -8)), ("mhd", (1, -3), (2, -5)), ("mhd", (1, -16), (1, -16)),
+                ("sedov", (1, -8), (4, -16)), ("sedov", (1, -2), (2, -2)), ("mhd2d", (1, -4), (4, -4)),
+                ("mhd_vanleer_moncen", (1, -8), (4, -8)), ("mhd_odd", (1, -5), (4, -3))]
+
+
+@pytest.mark.parametrize("case,tile_x,tile_yz", STREAM_CASES)
+def test_stream_kernel_bit_exact(oracle, case, tile_x, tile_yz):
+    """The stream kernel (one wave walking concatenated pencils, each cell's
+    stages computed once) gives the oracle's bits over whole stepping loops,
+    periodic images and the fused timestep included; workgroup ranges that
+    end inside pencils and short pencils (shorter than a wave) included."""
+    if case == "mhd":
+        cg, cfg = setups.mhd_turbulence(n=24, seed=17)
+        steps = 3
+    elif case == "sedov":
+        cg, cfg = setups.sedov_hd(n=24)
+        steps = 6
+    elif case == "mhd_vanleer_moncen":
+        cg, cfg = setups.mhd_turbulence(n=20, seed=18)
+        cfg.limiter, cfg.limiter_b = "vanleer", "moncen"
+        steps = 3
+    elif case == "mhd_odd":
+        cfg = pk.Config(magnetized=True, limiter="minmod", smalld=1e-3, smallei=1e-4)
+        cg = None
+        steps = 3
+    else:
+        cfg = pk.Config(magnetized=True, limiter="vanleer", has_dir=(True, True, False), smalld=1e-3, smallei=1e-4)
+        cg = None
+        steps = 3
+    cfg.tile_x = tile_x
+    cfg.tile_yz = tile_yz
+    print(f"stream {case}, tile_x {tile_x}, tile_yz {tile_yz}", flush=True)
+    setup_lib(cfg)
+    p = oracle.params_from_config(cfg)
+    if cg is None and case == "mhd_odd":
+        cg, ob = make_pair(oracle, (7, 13, 5), True, seed=23, cfg=cfg)
+    elif cg is None:
+        cg, ob = make_pair(oracle, (16, 12, 1), True, seed=19, cfg=cfg, has_dir=(True, True, False))
+    else:
+        ob = oracle.OBlock.from_cg(cg)
+        cg.upload()
+    t_g, _ = pk.run(cg, steps)
+    t_o = oracle.run(p, ob, steps)
+    cg.download()
+    assert t_g == t_o
+    assert_same(cg, ob, f"stream {case}")
+    pk.finalize()
