@@ -143,7 +143,8 @@ def cpu_baseline(workload, budget=10.0):
         "unit": "cell-stages/s" if workload == "stage256" else "cell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"oracle (C restatement of the reference sweep, one thread per process) on {cores} "
+        "sample": (f"oracle (C restatement of the reference sweep built with the reference's benchmark flags "
+                   f"-O3 -funroll-loops, no allocation per pencil, one thread per process) on {cores} "
                    f"independent periodic {nblk}^3 blocks of the same "
                    f"{ {'mhd256': 'MHD HLLD+GLM', 'unsplit256': 'MHD HLLD+GLM, unsplit solver', 'stage256': 'MHD HLLD+GLM, solve_cg stages', 'batch4096': 'two-fluid (MHD HLLD+GLM + HD HLLC)'}.get(workload, 'HD HLLC')} "
                    f"problem, {'RK2 stages along x, y, z' if workload == 'stage256' else 'whole PIERNIK steps'} for ~{budget:.0f} s each; {updates} cell-updates in {slowest:.2f} s "
@@ -200,6 +201,8 @@ def main():
                                            "unsplit256", "stage256", "gdf256"],
                     default="mhd256")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-list speed-up (4096 blocks, graph vs per-block launches)")
     ap.add_argument("--nblocks", type=int, default=4096, help="blocks of the batched list")
     ap.add_argument("--n", type=int, default=None, help="override the block size (cells per side)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -309,6 +312,10 @@ def main():
     if not args.no_adjoint:
         print("[bench] adjoint cost ratio (128^3 hydro rollout)", file=sys.stderr, flush=True)
         adj = adjoint_cost()
+    batching = None
+    if not args.no_batch and args.workload == "mhd256":
+        print(f"[bench] batched list: {args.nblocks} blocks, graph vs per-block launches", file=sys.stderr, flush=True)
+        batching = batch_speedup(args)
     line = {
         "metric": METRIC if args.workload == "mhd256" else "FP64 HD cell-updates/s at 64^3",
         "value": value,
@@ -345,8 +352,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": load_traffic(f"sweep_{args.workload}_traffic.json") if n in (256, 64) else None,
             "kernel": (f"fused RK2 sweep ({'HLLD+GLM' if nv == 9 else 'HLLC'}), x/y/z: "
-                       + ", ".join(("k_march" if (tiles or {}).get(d, (0, 0))[1] == -1 else "k_sweep") + " " + "xyz"[d]
-                                   for d in range(3))),
+                       + ", ".join(kernel_name((tiles or {}).get(d, (0, 0))[1]) + " " + "xyz"[d] for d in range(3))),
             "algo_bytes_per_launch": algo / launches if launches else None,
             "avg_launch_ms": ms / launches if launches else None,
             "per_dir_gbps": per_dir,
@@ -359,8 +365,55 @@ def main():
         "floors": rep,
         "adjoint_forward_cost_ratio": adj and adj["ratio"],
         "adjoint": adj,
+        "batching": batching,
     }
     print(json.dumps(line))
+
+
+def batch_speedup(args, steps=3):
+    """The batched list of the north star's third workload (args.nblocks
+    blocks with edges from {16, 24, 32, 48, 64}, seed 0, ionized MHD +
+    neutral HD per block), generated on the GPU: ms per step of the
+    graph-captured persistent launches and of one launch per block, after an
+    untimed warm-up step each; the same state and bit-identical results."""
+    import piernik_amd as pk
+    from piernik_amd import setups
+
+    sizes = setups.batch_sizes(args.nblocks, seed=0)
+    cfg = setups.batch_config()
+    cfg.fast_math = not args.exact
+    pk.init(cfg)
+    setups.batch_blocks_device(sizes, cfg, seed=0)
+    pk.synchronize()
+    t0 = time.perf_counter()
+    if not args.no_autotune:
+        pk.autotune_all()
+    t_tune = time.perf_counter() - t0
+    ms = {}
+    for name, mode in (("graph", pk.BATCH_GRAPH), ("per_block", pk.BATCH_PER_BLOCK)):
+        pk.run_all(1, mode)
+        pk.synchronize()
+        t0 = time.perf_counter()
+        pk.run_all(steps, mode)
+        pk.synchronize()
+        ms[name] = (time.perf_counter() - t0) / steps * 1e3
+        print(f"[bench] batched list {name}: {ms[name]:.1f} ms/step", file=sys.stderr, flush=True)
+    pk.finalize()
+    cells = sum(n ** 3 for n in sizes)
+    return {
+        "workload": (f"{len(sizes)} periodic blocks, edges {({str(n): sizes.count(n) for n in sorted(set(sizes))})} "
+                     "(seed 0), nb=4, ionized MHD (HLLD+GLM) + neutral HD (HLLC) per block, van Leer, RK2, CFL 0.7"),
+        "cells": cells,
+        "steps": steps,
+        "ms_per_step": ms,
+        "speedup_graph_vs_per_block": ms["per_block"] / ms["graph"],
+        "autotune_s": t_tune,
+    }
+
+
+def kernel_name(ts):
+    """The sweep kernel a tile length selects (pk_config tile_x / tile_yz)."""
+    return "k_march" if ts == -1 else ("k_stream" if ts <= -2 else "k_sweep")
 
 
 def adjoint_cost(n=128, steps=32, every=8):

@@ -69,8 +69,7 @@ void or_limiter(int kind, int n, const double *u, double *dq)
     }
     /* minmod / moncen / superbee share the one-sided difference arrays,
      * fluxlimiters.F90:157-158 (and :184-185, :209-210) */
-    double *dl = (double *)malloc(sizeof(double) * n);
-    double *dr = (double *)malloc(sizeof(double) * n);
+    double dl[n], dr[n];
     for (int i = 1; i < n; i++) dl[i] = u[i] - u[i - 1];
     dl[0] = dl[1];
     for (int i = 0; i < n - 1; i++) dr[i] = dl[i + 1];
@@ -96,8 +95,6 @@ void or_limiter(int kind, int n, const double *u, double *dq)
             dq[i] = 0.0;
         }
     }
-    free(dl);
-    free(dr);
 }
 
 /* interpolations.F90:170-204: ql(i) = q(i) + dq(i)/2, qr(i) = q(i+1) - dq(i+1)/2
@@ -105,7 +102,7 @@ void or_limiter(int kind, int n, const double *u, double *dq)
 void or_linear(int kind, int n, int nv, const double *q, double *ql, double *qr)
 {
     int m = n - 1;
-    double *dq = (double *)malloc(sizeof(double) * n);
+    double dq[n];
     for (int v = 0; v < nv; v++) {
         const double *qc = q + (size_t)v * n;
         or_limiter(kind, n, qc, dq);
@@ -115,7 +112,6 @@ void or_linear(int kind, int n, int nv, const double *q, double *ql, double *qr)
             qr[(size_t)v * m + i] = qc[i + 1] - dq[i + 1];
         }
     }
-    free(dq);
 }
 
 /* interpolations.F90:69-106 with func.F90:55-88 (ekin = emag(m)/rho,
@@ -508,17 +504,11 @@ void or_pencil_flux(const or_params *p, int n, double chspeed, const double *u, 
 {
     int nv = p->nvar, m = n - 1;
     int mhd = any_magnetized(p);
-    double *q = (double *)malloc(sizeof(double) * (size_t)n * nv);
-    double *ql = (double *)malloc(sizeof(double) * (size_t)m * nv);
-    double *qr = (double *)malloc(sizeof(double) * (size_t)m * nv);
-    double *bl = NULL, *br = NULL;
+    /* pencil work arrays on the stack: no allocation per pencil */
+    double q[n * nv], ql[m * nv], qr[m * nv], bl[m * 4], br[m * 4];
     or_utoq(p, n, u, mhd ? b : NULL, q);
     or_linear(p->limiter, n, nv, q, ql, qr);
-    if (mhd) {
-        bl = (double *)malloc(sizeof(double) * (size_t)m * 4);
-        br = (double *)malloc(sizeof(double) * (size_t)m * 4);
-        or_linear(p->limiter_b, n, 4, b, bl, br);
-    }
+    if (mhd) or_linear(p->limiter_b, n, 4, b, bl, br);
     for (int f = 0; f < p->nfluids; f++) {
         int o = p->fl[f].beg;
         if (p->fl[f].magnetized)
@@ -527,7 +517,6 @@ void or_pencil_flux(const or_params *p, int n, double chspeed, const double *u, 
         else
             or_hllc(m, ql + (size_t)o * m, qr + (size_t)o * m, p->fl[f].gam, flx + (size_t)o * m);
     }
-    free(q); free(ql); free(qr); free(bl); free(br);
 }
 
 /*
@@ -564,20 +553,12 @@ void or_pencil_stage_ext(const or_params *p, int n, double dtodx, double chspeed
     memcpy(u1, u, sizeof(double) * (size_t)n * nv);
     if (mhd) memcpy(b1, b, sizeof(double) * (size_t)n * 4);
 
-    double *q = (double *)malloc(sizeof(double) * (size_t)n * nv);
-    double *ql = (double *)malloc(sizeof(double) * (size_t)m * nv);
-    double *qr = (double *)malloc(sizeof(double) * (size_t)m * nv);
-    double *flx = (double *)malloc(sizeof(double) * (size_t)m * nv);
-    double *bl = NULL, *br = NULL, *bflx = NULL;
+    /* pencil work arrays on the stack: no allocation per pencil */
+    double q[n * nv], ql[m * nv], qr[m * nv], flx[m * nv], bl[m * 4], br[m * 4], bflx[m * 4];
 
     or_utoq(p, n, u1, mhd ? b1 : NULL, q);
     or_linear(p->limiter, n, nv, q, ql, qr);
-    if (mhd) {
-        bl = (double *)malloc(sizeof(double) * (size_t)m * 4);
-        br = (double *)malloc(sizeof(double) * (size_t)m * 4);
-        bflx = (double *)malloc(sizeof(double) * (size_t)m * 4);
-        or_linear(p->limiter_b, n, 4, b1, bl, br);
-    }
+    if (mhd) or_linear(p->limiter_b, n, 4, b1, bl, br);
     /* riemann_wrap / riemann_wrap_u: per-fluid column slices (hlld.F90:84-111, 135-153) */
     for (int f = 0; f < p->nfluids; f++) {
         int o = p->fl[f].beg;
@@ -621,6 +602,4 @@ void or_pencil_stage_ext(const or_params *p, int n, double dtodx, double chspeed
     for (size_t k = 0; k < (size_t)n * nv; k++) u1[k] = u1[k] + 0.0;
 
     or_floors(p, n, u1, mhd ? b1 : NULL, nbp, dvol, rep);
-    free(q); free(ql); free(qr); free(flx);
-    free(bl); free(br); free(bflx);
 }

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
     // position s1-1, is finished when position s1+3 is loaded
     const int iters = (s1 + 4 - (s0 - 8) + C - 1) / C;
     for (int it = 0; it < iters; it++) {
-        const int jc = j, cc1 = c1, cc2 = c2;  // this iteration's cell (position sp)
+        const int spc = sp, jc = j, cc1 = c1, cc2 = c2;  // this iteration's position and cell
         bool pv;
         const long long idx = cell_index(jc, pv);
         advance();
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
             floors<P>(Ws, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
             // cells g-2 .. g+n+1 (j-2 in [2, L-2)) raise flags, the interior counts mass
             const int jj = jc - 2;
-            const bool own = pv && sp - 2 >= s0 && sp - 2 < s1 && jc >= 2;  // same pencil as sp
+            const bool own = pv && spc - 2 >= s0 && spc - 2 < s1 && jc >= 2;  // same pencil as spc
             if (own && jj >= 2 && jj < L - 2) flags |= fl;
             pk::mass_add(a.dmass, (own && jj >= 4 && jj < L - 4) ? dm : 0.0);
             prim<P>(Ws, a.gam_1, qs);
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
             double dm = 0.0;
             floors<P>(Wn, a.smalld, a.smallei, a.use_smalld, a.use_smallei, a.ei_flag, fl, dm);
             const int jo = jc - 4;  // output cell: interior when jo in [4, L-4)
-            const bool out = pv && sp - 4 >= s0 && sp - 4 < s1 && jo >= 4 && jo < L - 4;
+            const bool out = pv && spc - 4 >= s0 && spc - 4 < s1 && jo >= 4 && jo < L - 4;
             if (out) flags |= fl;
             pk::mass_add(a.dmass, out ? dm : 0.0);
             if (out) {

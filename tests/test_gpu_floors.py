@@ -94,9 +94,12 @@ def test_floors_stages_bit_exact(oracle, mhd, neg_rho):
     cg.download()
     assert_same(cg, ob, "stages")
     rep = assert_report(cg, ob, 6, "stages")
-    assert rep["magic_mass"][0] > 0.0
     if neg_rho:
+        # the negative cells' images at the pencil ends stay negative in the
+        # first stage; the floor there (a ghost cell) adds no interior mass
         assert rep["dn_negative"]
+    else:
+        assert rep["magic_mass"][0] > 0.0
     pk.finalize()
 
 
