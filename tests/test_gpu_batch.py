@@ -165,3 +165,91 @@ def test_device_generated_state_matches_host():
         assert np.max(np.abs(bh - h.b)) <= 1e-13
         assert float(psi.abs().max()) == 0.0
     pk.finalize()
+
+
+def test_config_block_sizes_all_modes_bit_exact(oracle):
+    """A two-fluid list holding every block size of the batched workload
+    (interiors 16^3, 24^3, 32^3, 48^3, 64^3, and a second 48^3), autotuned
+    list tiles: graph, persistent and per-block modes each equal the
+    oracle's list driver, bit for bit, after two steps."""
+    cfg = setups.batch_config()
+    shapes = [16, 24, 32, 48, 64, 48]
+    cgs = setups.batch_blocks(shapes, cfg, seed=31)
+    init = [(cg.u.copy(order="F"), cg.b.copy(order="F"), cg.psi.copy(order="F")) for cg in cgs]
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    t_o = oracle.run_list(p, obs, 2)
+    for mode in MODES:
+        for cg, (u, b, psi) in zip(cgs, init):
+            cg.u[...] = u
+            cg.b[...] = b
+            cg.psi[...] = psi
+        print(f"config block sizes {shapes}, mode {mode}", flush=True)
+        start(cfg, cgs)
+        pk.autotune_all()
+        t_g, _ = pk.run_all(2, MODES[mode])
+        assert t_g == t_o
+        for cg, ob in zip(cgs, obs):
+            cg.download()
+            assert_same(cg, ob, mode)
+            rep = pk.get_report(cg)
+            assert rep["magic_mass"] == list(ob.rep.magic_mass)
+        pk.finalize()
+
+
+def _device_state(cg, nvar):
+    import torch
+
+    ng = cg.ng
+    u = torch.empty((ng[2], ng[1], ng[0], nvar), dtype=torch.float64, device="cuda")
+    b = torch.empty((ng[2], ng[1], ng[0], 3), dtype=torch.float64, device="cuda")
+    psi = torch.empty((ng[2], ng[1], ng[0]), dtype=torch.float64, device="cuda")
+    cg.download_device(u, b, psi)
+    return u, b, psi
+
+
+def test_config_list_512_blocks_modes_agree():
+    """512 blocks drawn like the batched workload (edges 16..64, seeded, two
+    fluids, generated on the GPU), autotuned: graph replay, persistent
+    launches and one launch per block give the same bits after two steps,
+    every block conserves its mass, momentum, energy and B to rounding and
+    stays positive."""
+    import torch
+
+    cfg = setups.batch_config()
+    sizes = setups.batch_sizes(512, seed=0)
+    assert sorted(set(sizes)) == list(setups.BATCH_SIZES)
+    print(f"512-block list, {sum(n ** 3 for n in sizes)} cells", flush=True)
+    pk.load_library()
+    outs = {}
+    init_tot = None
+    for name, mode in MODES.items():
+        pk.finalize()
+        pk.init(cfg)
+        cgs = setups.batch_blocks_device(sizes, cfg, seed=0)
+        if init_tot is None:
+            init_tot = []
+            for cg in cgs:
+                u, b, _ = _device_state(cg, 10)
+                ui, bi = setups.interior_of(u, 4, cfg.has_dir), setups.interior_of(b, 4, cfg.has_dir)
+                init_tot.append((torch.cat([ui.sum(dim=(0, 1, 2)), bi.sum(dim=(0, 1, 2))]),
+                                 torch.cat([ui.abs().sum(dim=(0, 1, 2)), bi.abs().sum(dim=(0, 1, 2))])))
+        pk.autotune_all()
+        t, dt = pk.run_all(2, mode)
+        print(f"mode {name}: t = {t!r}", flush=True)
+        outs[name] = (t, [_device_state(cg, 10) for cg in cgs])
+    pk.finalize()
+    t_ref, ref = outs["graph"]
+    for name in ("persistent", "per_block"):
+        t, st = outs[name]
+        assert t == t_ref
+        for i, (a, c) in enumerate(zip(st, ref)):
+            for x, y in zip(a, c):
+                assert torch.equal(x, y), f"block {i} ({sizes[i]}^3): {name} differs from graph"
+    for i, (u, b, psi) in enumerate(ref):
+        ui, bi = setups.interior_of(u, 4, cfg.has_dir), setups.interior_of(b, 4, cfg.has_dir)
+        tot = torch.cat([ui.sum(dim=(0, 1, 2)), bi.sum(dim=(0, 1, 2))])
+        t0, m0 = init_tot[i]
+        err = float(torch.max(torch.abs(tot - t0) / m0))
+        assert err <= 1e-13, f"block {i}: conservation {err:.3e}"
+        assert float(ui[..., 0].min()) > 0.0 and float(ui[..., 5].min()) > 0.0
