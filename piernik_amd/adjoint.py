@@ -13,6 +13,11 @@ depends on the state at the start of the step (through the minimising
 cell), except where the growth cap dt <= 2 dt_old binds, where it depends on
 the previous dt.
 
+The last segment's sweep inputs are recorded during the forward pass itself
+(record_last): its backward needs no recomputation, and with
+checkpoint_every >= nsteps the whole rollout is recorded and nothing is
+recomputed.
+
 A state is a tuple of tensors in the reference layout: u(5, nxg, nyg, nzg)
 in Fortran order, i.e. torch shape (nzg, nyg, nxg, 5), and for MHD also
 b (nzg, nyg, nxg, 3) and psi (nzg, nyg, nxg); float64, on the GPU, ghost
@@ -31,7 +36,7 @@ from .setups import _DeviceBlock
 class RolloutPlan:
     """One registered device block and the slots a checkpointed rollout uses."""
 
-    def __init__(self, cfg, n, nsteps, checkpoint_every=8, nb=4, cg_id=900001):
+    def __init__(self, cfg, n, nsteps, checkpoint_every=8, nb=4, cg_id=900001, record_last=True):
         if cfg.nfluids != 1:
             raise PiernikError("the adjoint covers one fluid per block")
         self.cfg = cfg
@@ -45,6 +50,8 @@ class RolloutPlan:
         _call("pk_adjoint_setup", self.cg.cg_id, self.nck + self.nsw * self.k)
         self.dts = []
         self.capped = []
+        self.record_last = bool(record_last)
+        self.last_recorded = False  # the slots hold the last segment's sweep inputs
 
     @property
     def shape(self):
@@ -68,6 +75,8 @@ class RolloutPlan:
         cg.upload_device(*[t.contiguous() for t in state])
         update_boundaries(cg)
         self.dts, self.capped = [], []
+        self.last_recorded = False
+        last0 = (self.nck - 1) * self.k  # first step of the last segment
         dt_old = -1.0
         for s in range(self.nsteps):
             if record and s % self.k == 0:
@@ -76,10 +85,15 @@ class RolloutPlan:
             cap = False
             if s >= 1 and dt_old > 0.0 and dt_old * 2.0 < dt:  # timestep.F90:200-208
                 dt, cap = dt_old * 2.0, True
-            fluid_update(cg, dt)
+            if record and self.record_last and s >= last0:
+                # the same step, keeping the state before every sweep
+                _call("pk_fluid_update_record", cg.cg_id, self.nck + (s - last0) * self.nsw, dt)
+            else:
+                fluid_update(cg, dt)
             self.dts.append(dt)
             self.capped.append(cap)
             dt_old = dt
+        self.last_recorded = record and self.record_last
         out = self._empty(state[0].device)
         cg.download_device(*out)
         return out[0] if single else out
@@ -94,9 +108,10 @@ class RolloutPlan:
         carry = 0.0
         for seg in reversed(range(self.nck)):
             s0, s1 = seg * self.k, min(seg * self.k + self.k, self.nsteps)
-            _call("pk_state_load", cg.cg_id, seg)
-            for s in range(s0, s1):
-                _call("pk_fluid_update_record", cg.cg_id, self.nck + (s - s0) * self.nsw, self.dts[s])
+            if not (seg == self.nck - 1 and self.last_recorded):
+                _call("pk_state_load", cg.cg_id, seg)
+                for s in range(s0, s1):
+                    _call("pk_fluid_update_record", cg.cg_id, self.nck + (s - s0) * self.nsw, self.dts[s])
             for s in reversed(range(s0, s1)):
                 slot = self.nck + (s - s0) * self.nsw
                 dtb = ctypes.c_double(0.0)
@@ -107,6 +122,7 @@ class RolloutPlan:
                 else:
                     _call("pk_timestep_adjoint", cg.cg_id, slot, total)
                     carry = 0.0
+        self.last_recorded = False  # the recomputed segments have overwritten the slots
         out = self._empty(g[0].device)
         ptrs = [ctypes.c_void_p(_device_ptr(t)) for t in out] + [None] * (3 - len(out))
         _call("pk_adjoint_get", cg.cg_id, *ptrs)

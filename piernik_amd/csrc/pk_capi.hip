@@ -258,7 +258,7 @@ bool is_stream(int ts) { return ts <= -2; }
 void stream_geometry(const Block *b, int d, int tp, int m, int &TP, int &out, int &chunks, int &ptiles)
 {
     const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
-    if (d == 0 || (tp != 2 && tp != 4)) tp = 1;
+    if (d == 0 || (tp != 2 && tp != 4 && tp != 8)) tp = 1;
     if (m < 2) m = 2;
     TP = tp;
     out = m * (64 / tp);  // positions per workgroup
@@ -1232,6 +1232,7 @@ constexpr unsigned kListGrab = 4;
 
 int list_default_ts(const Block *b, int d, int nt)
 {
+    if (nt == pk::kStreamListNt) return -16;  // the stream: 16 iterations per item
     const int span = b->n[d];
     const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
     const long long npen = (long long)b->n[d1] * b->n[d2];
@@ -1277,8 +1278,15 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
         a.dst = fluid_state(b, 1 - b->cur, f);
         a.images = g_list.linked ? 0 : 1;
         const int ts = ts_override ? (*ts_override)[i] : list_ts(b, d, f, nt);
-        tile_geometry(b, d, true, nt / ts, ts, a.TP, a.TS, a.out, a.chunks, a.ptiles);
-        if (a.TP * a.TS > nt) return fail(PK_EINVAL, "tile %d x %d exceeds %d lanes", a.TP, a.TS, nt);
+        if (nt == pk::kStreamListNt) {
+            // the stream form: items are ranges of -ts iterations of the block's stream
+            if (!is_stream(ts)) return fail(PK_EINVAL, "stream list item of %d iterations", -ts);
+            a.TS = ts;
+            stream_geometry(b, d, pk::stream_list_tp(d), -ts, a.TP, a.out, a.chunks, a.ptiles);
+        } else {
+            tile_geometry(b, d, true, nt / ts, ts, a.TP, a.TS, a.out, a.chunks, a.ptiles);
+            if (a.TP * a.TS > nt) return fail(PK_EINVAL, "tile %d x %d exceeds %d lanes", a.TP, a.TS, nt);
+        }
         t.h_first[i] = items;
         const unsigned long long n = (unsigned long long)a.chunks * (unsigned long long)a.ptiles;
         // headroom for the fetches past the end (at most grab per resident workgroup)
@@ -2736,9 +2744,9 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
                     }
                 }
             }
-            // the stream kernel: one pencil per lane group along x, 1, 2 or 4
-            // along y and z, workgroups of 8 to 64 iterations
-            for (int tp : {1, 2, 4}) {
+            // the stream kernel: one pencil per lane group along x, 1, 2, 4 or
+            // 8 along y and z, workgroups of 8 to 64 iterations
+            for (int tp : {1, 2, 4, 8}) {
                 if (d == 0 && tp != 1) continue;
                 for (int m : {8, 16, 32, 64}) {
                     a.TS = -m;
@@ -2857,7 +2865,7 @@ int pk_autotune_all(double dt, double chspeed)
     }
     LaunchTab tmp;
     int rc = PK_OK;
-    const int nt_list[] = {128, 256, 512};
+    const int nt_list[] = {pk::kStreamListNt, 128, 256, 512};
     for (int d = 0; d < 3 && !rc; d++) {
         if (!g_cfg.has_dir[d]) continue;
         for (int f = 0; f < g_nfl && !rc; f++) {
@@ -2871,10 +2879,14 @@ int pk_autotune_all(double dt, double chspeed)
                 for (auto &cl : classes) {
                     Block *b0 = cl.second[0];
                     std::vector<int> cand;
-                    for (int k = 1; k <= 8; k++) {
-                        const int out = (b0->n[d] + k - 1) / k;
-                        const int ts = out + 8;
-                        if (ts <= nt && std::find(cand.begin(), cand.end(), ts) == cand.end()) cand.push_back(ts);
+                    if (nt == pk::kStreamListNt) {
+                        cand = {-4, -8, -16, -32};  // iterations of the stream per item
+                    } else {
+                        for (int k = 1; k <= 8; k++) {
+                            const int out = (b0->n[d] + k - 1) / k;
+                            const int ts = out + 8;
+                            if (ts <= nt && std::find(cand.begin(), cand.end(), ts) == cand.end()) cand.push_back(ts);
+                        }
                     }
                     float best = 1e30f;
                     int best_ts = cand.empty() ? 9 : cand[0];
@@ -2919,9 +2931,11 @@ int pk_autotune_all(double dt, double chspeed)
                 }
             }
             g_list.nt_pref[d][g_fmag[f]] = best_nt;
-            for (auto &cl : classes)
-                g_tuned[tune_key(cl.second[0], d, g_fmag[f], best_nt)] = {best_nt / best_ts_at[cl.first],
-                                                                          best_ts_at[cl.first]};
+            for (auto &cl : classes) {
+                const int ts = best_ts_at[cl.first];
+                g_tuned[tune_key(cl.second[0], d, g_fmag[f], best_nt)] = {
+                    is_stream(ts) ? pk::stream_list_tp(d) : best_nt / ts, ts};
+            }
         }
     }
     free_tab(tmp);

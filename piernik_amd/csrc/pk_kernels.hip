@@ -499,6 +499,41 @@ __global__ __launch_bounds__(64) void k_march(const SweepArgs a)
     if (pvalid && flags) atomicOr(a.flags, flags);
 }
 
+// The kernel arguments of list item `item` (its block's table entry with
+// the step's scalars and buffer parity applied); returns the block's first
+// item.  The tables are read-only while a kernel runs: they are read through
+// the constant address space, so the uniform loads go to scalar registers.
+template <int DIR>
+__device__ __forceinline__ unsigned load_list_args(const ListArgs &L, unsigned item, SweepArgs &a)
+{
+    typedef const __attribute__((address_space(4))) unsigned cu32;
+    const int blk = __builtin_amdgcn_readfirstlane((int)((cu32 *)L.items)[item]);
+    {
+        static_assert(sizeof(SweepArgs) % 4 == 0, "SweepArgs is copied in dwords");
+        cu32 *w = (cu32 *)(L.args + blk);
+        unsigned *aw = reinterpret_cast<unsigned *>(&a);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(SweepArgs) / 4); i++) aw[i] = w[i];
+    }
+    const unsigned first = ((cu32 *)L.first)[blk];
+    if (L.parity) {
+        const double *s0 = a.src;
+        a.src = a.dst;
+        a.dst = const_cast<double *>(s0);
+    }
+    a.base = a.src;
+    typedef const __attribute__((address_space(4))) double cf64;
+    const cf64 *stp = (const cf64 *)(L.step + (L.per_block ? blk : 0));
+    const double sdt = stp[0], sch = stp[1], sfac = stp[2];
+    a.c1 = (0.5 * sdt) / a.dl[DIR];  // rk_coef * dt / dl, constants.F90:237-238
+    a.c2 = (1.0 * sdt) / a.dl[DIR];
+    a.ch = sch;
+    a.psi_fac = L.apply_fac ? sfac : 1.0;
+    a.dtslots = L.dtslots ? L.dtslots + (L.per_block ? 4ll * L.nslots * blk : 0ll) : nullptr;
+    a.nslots = L.nslots;
+    return first;
+}
+
 #include "pk_stream.hpp"
 
 // The hydro sweep fits four waves per SIMD in 128 VGPRs (2 spilled; 5 % faster
@@ -536,34 +571,8 @@ __global__ __launch_bounds__(512) void k_sweep_list(const ListArgs L)
             if (item >= L.nitems) break;
             end = min(item + grab, L.nitems);
         }
-        // the tables are read-only while the kernel runs: read them through the
-        // constant address space so the uniform loads go to scalar registers
-        typedef const __attribute__((address_space(4))) unsigned cu32;
-        const int blk = __builtin_amdgcn_readfirstlane((int)((cu32 *)L.items)[item]);
         SweepArgs a;
-        {
-            static_assert(sizeof(SweepArgs) % 4 == 0, "SweepArgs is copied in dwords");
-            cu32 *w = (cu32 *)(L.args + blk);
-            unsigned *aw = reinterpret_cast<unsigned *>(&a);
-#pragma unroll
-            for (int i = 0; i < (int)(sizeof(SweepArgs) / 4); i++) aw[i] = w[i];
-        }
-        const unsigned first = ((cu32 *)L.first)[blk];
-        if (L.parity) {
-            const double *s0 = a.src;
-            a.src = a.dst;
-            a.dst = const_cast<double *>(s0);
-        }
-        a.base = a.src;
-        typedef const __attribute__((address_space(4))) double cf64;
-        const cf64 *stp = (const cf64 *)(L.step + (L.per_block ? blk : 0));
-        const double sdt = stp[0], sch = stp[1], sfac = stp[2];
-        a.c1 = (0.5 * sdt) / a.dl[DIR];  // rk_coef * dt / dl, constants.F90:237-238
-        a.c2 = (1.0 * sdt) / a.dl[DIR];
-        a.ch = sch;
-        a.psi_fac = L.apply_fac ? sfac : 1.0;
-        a.dtslots = L.dtslots ? L.dtslots + (L.per_block ? 4ll * L.nslots * blk : 0ll) : nullptr;
-        a.nslots = L.nslots;
+        const unsigned first = load_list_args<DIR>(L, item, a);
         sweep_tile<DIR, P, true, LIM>(a, (int)(item - first), item, lds, lds + P::NV * blockDim.x);
         item++;
     }
@@ -944,6 +953,7 @@ static hipError_t launch_stream_t(const SweepArgs &a, hipStream_t st)
         case 1: return launch_stream_tp<DIR, P, LIM, 1>(a, st);
         case 2: return launch_stream_tp<DIR, P, LIM, 2>(a, st);
         case 4: return launch_stream_tp<DIR, P, LIM, 4>(a, st);
+        case 8: return launch_stream_tp<DIR, P, LIM, 8>(a, st);
         default: return hipErrorInvalidValue;
         }
     }
@@ -1004,9 +1014,30 @@ hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipSt
 
 // Persistent list sweep: launch (resident == nullptr) or report how many
 // workgroups of nt lanes the device holds at once.
+// Persistent stream over a list (nt == 64: k_stream_list, one pencil per
+// lane group along x, four along y and z).
+template <int DIR, class P, int LIM>
+static hipError_t stream_list_t(const ListArgs *L, int grid, hipStream_t st, int *resident)
+{
+    constexpr int TP = pk::stream_list_tp(DIR);
+    const void *fn = (const void *)k_stream_list<DIR, P, LIM, TP>;
+    if (resident) {
+        int per = 0, dev = 0, ncu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64, 0);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        *resident = per * ncu;
+        return e;
+    }
+    if (grid <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stream_list<DIR, P, LIM, TP>), dim3((unsigned)grid), dim3(64), 0, st, *L);
+    return hipGetLastError();
+}
+
 template <int DIR, class P, int LIM>
 static hipError_t list_t(const ListArgs *L, int nt, int grid, hipStream_t st, int *resident)
 {
+    if (nt == pk::kStreamListNt) return stream_list_t<DIR, P, LIM>(L, grid, st, resident);
     const void *fn = (const void *)k_sweep_list<DIR, P, LIM>;
     const size_t lds = sizeof(double) * 2 * P::NV * nt;
     const size_t lds_max = 160 * 1024 - 256;  // static LDS (the queue slot) comes out of the same 160 KiB

@@ -128,6 +128,13 @@ struct FaceJob {
 };
 
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st);
+// Workgroup size that selects the stream form of the persistent list sweep
+// (k_stream_list: one-wave workgroups, each list item a range of a block's
+// stream of a.out positions, a.TP pencils per lane group: 1 along x, 4 along
+// y and z).
+constexpr int kStreamListNt = 64;
+// Pencils per lane group of the list stream along direction d.
+constexpr int stream_list_tp(int d) { return d == 0 ? 1 : 4; }
 // Persistent batched fused sweep with workgroups of nt lanes; grid = 0 picks
 // the number of workgroups the device holds at once.
 hipError_t launch_sweep_list(int dir, bool mhd, int lim, int limb, const ListArgs &L, int nt, int grid,

@@ -43,26 +43,36 @@ struct StreamCarry {
     static constexpr int ULAG = 4;         // U0 is read 2 and 4 positions back
 };
 
-// LDS bytes of one stream workgroup.
-__host__ __device__ constexpr int stream_lds_doubles(int nv, int tp)
+// Orders the LDS accesses of the lanes of the one wave that is the whole
+// workgroup.  The LDS unit performs a wave's accesses in the order the wave
+// issues them, so a write is seen by any later read of the same wave; only
+// the compiler must keep them in program order, which a fence at wavefront
+// scope does without waiting for their completion.
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+// Slots per variable of the U0 buffer of a stream workgroup.
+template <int TP>
+__host__ __device__ constexpr int stream_su()
 {
-    return nv * 64 /* shared buffer */ + nv * 64 /* carry block */ + nv * tp * (64 / tp + StreamCarry::ULAG);
+    return TP * (64 / TP + StreamCarry::ULAG);
 }
 
+// The positions [s0, s0 + span) of a block's stream, walked by the one wave
+// of the calling workgroup.  A (NV*64), K (NV*64) and U (NV*stream_su<TP>())
+// are the workgroup's LDS; the wave's earlier contents there are never read
+// for an owned output.  slot_id picks the timestep slot.
 template <int DIR, class P, int LIM, int TP>
-__global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span)
+__device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, const int span, unsigned slot_id,
+                                             double *A, double *K, double *U)
 {
     constexpr int NV = P::NV;
     constexpr int C = 64 / TP;
     constexpr int UL = StreamCarry::ULAG;
-    constexpr int SU = TP * (C + UL);  // slots per variable of the U0 buffer
+    constexpr int SU = stream_su<TP>();
     static_assert(C >= UL && TP * StreamCarry::N <= 64, "stream shape");
     constexpr int ds = DIR;
     constexpr int d1 = (DIR == 0) ? 1 : 0;
     constexpr int d2 = (DIR == 2) ? 1 : 2;
-    __shared__ double A[NV * 64];  // the quantity of the current phase, slot TP*i + p
-    __shared__ double K[NV * 64];  // carry block: per variable, StreamCarry::N positions x TP
-    __shared__ double U[NV * SU];  // U0 with UL positions of the previous iteration in front
 
     const int lane = threadIdx.x;
     const int p = lane % TP, i = lane / TP;
@@ -71,7 +81,6 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
     const int npen = n1 * a.n[d2];
     const int ngroups = (npen + TP - 1) / TP;
     const int npos = ngroups * L;
-    const int s0 = blockIdx.x * span;
     const int s1 = min(s0 + span, npos);
     const long long st = (DIR == 0) ? 1ll : ((DIR == 1) ? (long long)a.sy : (long long)a.sz);
 
@@ -169,7 +178,7 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
         prim<P>(W, a.gam_1, q);
 #pragma unroll
         for (int v = 0; v < NV; v++) aw[v * 64] = q[v];
-        __syncthreads();
+        wave_lds_order();
 
         // faces of cell sp-1 from q(sp-2), q(sp-1), q(sp)
         double fp[NV], fm[NV];
@@ -186,10 +195,10 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
 #pragma unroll
             for (int v = 0; v < NV; v++) cq[v * 64] = q[v];
         }
-        __syncthreads();
+        wave_lds_order();
 #pragma unroll
         for (int v = 0; v < NV; v++) aw[v * 64] = fp[v];
-        __syncthreads();
+        wave_lds_order();
 
         // stage-1 flux at (sp-2)+1/2: right face of sp-2, left face of sp-1
         double F1[NV];
@@ -203,10 +212,10 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
 #pragma unroll
             for (int v = 0; v < NV; v++) cfp[v * 64] = fp[v];
         }
-        __syncthreads();
+        wave_lds_order();
 #pragma unroll
         for (int v = 0; v < NV; v++) aw[v * 64] = F1[v];
-        __syncthreads();
+        wave_lds_order();
 
         // stage-1 state of cell sp-2 and its primitives
         double qs[NV];
@@ -232,10 +241,10 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
 #pragma unroll
             for (int v = 0; v < NV; v++) cf1[v * 64] = F1[v];
         }
-        __syncthreads();
+        wave_lds_order();
 #pragma unroll
         for (int v = 0; v < NV; v++) aw[v * 64] = qs[v];
-        __syncthreads();
+        wave_lds_order();
 
         // stage-2 faces of cell sp-3 from qs(sp-4), qs(sp-3), qs(sp-2)
         double gp[NV], gm[NV];
@@ -252,10 +261,10 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
 #pragma unroll
             for (int v = 0; v < NV; v++) cqs[v * 64] = qs[v];
         }
-        __syncthreads();
+        wave_lds_order();
 #pragma unroll
         for (int v = 0; v < NV; v++) aw[v * 64] = gp[v];
-        __syncthreads();
+        wave_lds_order();
 
         // stage-2 flux at (sp-4)+1/2
         double F2[NV];
@@ -269,10 +278,10 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
 #pragma unroll
             for (int v = 0; v < NV; v++) cgp[v * 64] = gp[v];
         }
-        __syncthreads();
+        wave_lds_order();
 #pragma unroll
         for (int v = 0; v < NV; v++) aw[v * 64] = F2[v];
-        __syncthreads();
+        wave_lds_order();
 
         // the new state of cell sp-4
         {
@@ -314,7 +323,7 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
 #pragma unroll
             for (int v = 0; v < NV; v++) uc[v * SU] = uw[v * SU];
         }
-        __syncthreads();
+        wave_lds_order();
         {
             bool pv2;
             const long long idx2 = cell_index(j, pv2);
@@ -322,6 +331,37 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
             for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx2];
         }
     }
-    if (a.dtslots) dt_reduce(a, mn, cmax, blockIdx.x);
+    if (a.dtslots) dt_reduce(a, mn, cmax, slot_id);
     if (flags) atomicOr(a.flags, flags);
+}
+
+template <int DIR, class P, int LIM, int TP>
+__global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span)
+{
+    __shared__ double A[P::NV * 64];              // the quantity of the current phase, slot TP*i + p
+    __shared__ double K[P::NV * 64];              // carry block: per variable, StreamCarry::N positions x TP
+    __shared__ double U[P::NV * stream_su<TP>()]; // U0 with ULAG positions of the previous iteration in front
+    stream_range<DIR, P, LIM, TP>(a, (int)blockIdx.x * span, span, blockIdx.x, A, K, U);
+}
+
+// The stream over a block list: one-wave workgroups take items from a
+// device queue until it is empty; item `item` of block items[item] is the
+// range of a.out positions starting at (item - first[block]) * a.out of
+// that block's stream.  Each wave leaves after its first fetch past the
+// end, so the grid drains after at most nitems + gridDim.x fetches.
+template <int DIR, class P, int LIM, int TP>
+__global__ __launch_bounds__(64) void k_stream_list(const ListArgs L)
+{
+    __shared__ double A[P::NV * 64];
+    __shared__ double K[P::NV * 64];
+    __shared__ double U[P::NV * stream_su<TP>()];
+    for (;;) {
+        unsigned got = 0u;
+        if (threadIdx.x == 0) got = atomicAdd(L.queue, 1u);
+        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl((int)got, 0, 64));
+        if (item >= L.nitems) break;
+        SweepArgs a;
+        const unsigned first = load_list_args<DIR>(L, item, a);
+        stream_range<DIR, P, LIM, TP>(a, (int)(item - first) * a.out, a.out, item, A, K, U);
+    }
 }

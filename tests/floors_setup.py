@@ -19,9 +19,10 @@ EPS = np.finfo(np.float64).eps
 
 
 def floor_block(shape=(20, 12, 10), magnetized=False, seed=0, neg_rho=False, fluids=None):
-    """Block and Config; neg_rho: also four isolated cells at rest with
-    density -1e-4 (hydro only: their periodic images at the pencil ends stay
-    negative through the first stage, which is what raises dn_negative)."""
+    """Block and Config (at least 19 x 11 x 9 cells, for the cells set
+    below); neg_rho: also four isolated cells at rest with density -1e-4
+    (hydro only: their periodic images at the pencil ends stay negative
+    through the first stage, which is what raises dn_negative)."""
     cg = setups.smooth_random(shape, seed=seed, magnetized=magnetized)
     ui = cg.interior(cg.u)
     for (i, j, k) in ZERO_E_CELLS:

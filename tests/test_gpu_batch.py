@@ -25,6 +25,7 @@ def start(cfg, cgs):
     pk.finalize()
     pk.init(cfg)
     for cg in cgs:
+        cg.registered = False  # pk_finalize dropped every registration
         cg.upload()
 
 
@@ -232,8 +233,9 @@ def test_config_list_512_blocks_modes_agree():
             for cg in cgs:
                 u, b, _ = _device_state(cg, 10)
                 ui, bi = setups.interior_of(u, 4, cfg.has_dir), setups.interior_of(b, 4, cfg.has_dir)
-                init_tot.append((torch.cat([ui.sum(dim=(0, 1, 2)), bi.sum(dim=(0, 1, 2))]),
-                                 torch.cat([ui.abs().sum(dim=(0, 1, 2)), bi.abs().sum(dim=(0, 1, 2))])))
+                # interior_of gives (component, x, y, z): one total per component
+                init_tot.append((torch.cat([ui.sum(dim=(1, 2, 3)), bi.sum(dim=(1, 2, 3))]),
+                                 torch.cat([ui.abs().sum(dim=(1, 2, 3)), bi.abs().sum(dim=(1, 2, 3))])))
         pk.autotune_all()
         t, dt = pk.run_all(2, mode)
         print(f"mode {name}: t = {t!r}", flush=True)
@@ -248,8 +250,8 @@ def test_config_list_512_blocks_modes_agree():
                 assert torch.equal(x, y), f"block {i} ({sizes[i]}^3): {name} differs from graph"
     for i, (u, b, psi) in enumerate(ref):
         ui, bi = setups.interior_of(u, 4, cfg.has_dir), setups.interior_of(b, 4, cfg.has_dir)
-        tot = torch.cat([ui.sum(dim=(0, 1, 2)), bi.sum(dim=(0, 1, 2))])
+        tot = torch.cat([ui.sum(dim=(1, 2, 3)), bi.sum(dim=(1, 2, 3))])
         t0, m0 = init_tot[i]
         err = float(torch.max(torch.abs(tot - t0) / m0))
         assert err <= 1e-13, f"block {i}: conservation {err:.3e}"
-        assert float(ui[..., 0].min()) > 0.0 and float(ui[..., 5].min()) > 0.0
+        assert float(ui[0].min()) > 0.0 and float(ui[5].min()) > 0.0

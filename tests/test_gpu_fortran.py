@@ -91,7 +91,7 @@ def test_fortran_drop_in_floors(oracle, tmp_path):
     """The same with both floors firing (smalld = 0.9, smallei = 1.5, three
     cells of zero total energy): state bit for bit, flags equal, magic mass
     within the rounding of the reference's per-pencil form."""
-    cg, cfg = floor_block(shape=(12, 10, 8), magnetized=True, seed=2)
+    cg, cfg = floor_block(shape=(20, 12, 10), magnetized=True, seed=2)
     dt, ch = 1.0e-3, 1.1
     print("fortran driver, floors", flush=True)
     rcs, flags, mm, u, b, psi = run_driver(tmp_path, cg, cfg, dt, ch)

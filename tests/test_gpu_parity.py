@@ -206,7 +206,8 @@ def test_march_kernel_bit_exact(oracle, case, tile_x):
 # the stream kernel: (pencils per lane group, -iterations per workgroup)
 STREAM_CASES = [("mhd", (1, -8), (4, -8)), ("mhd", (1, -3), (2, -5)), ("mhd", (1, -16), (1, -16)),
                 ("sedov", (1, -8), (4, -16)), ("sedov", (1, -2), (2, -2)), ("mhd2d", (1, -4), (4, -4)),
-                ("mhd_vanleer_moncen", (1, -8), (4, -8)), ("mhd_odd", (1, -5), (4, -3))]
+                ("mhd_vanleer_moncen", (1, -8), (4, -8)), ("mhd_odd", (1, -5), (4, -3)),
+                ("mhd", (1, -4), (8, -8)), ("sedov", (1, -3), (8, -3)), ("mhd_odd", (1, -2), (8, -5))]
 
 
 @pytest.mark.parametrize("case,tile_x,tile_yz", STREAM_CASES)
