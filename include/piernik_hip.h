@@ -183,6 +183,31 @@ int pk_register_fcflx(int64_t cg_id, int32_t ddim, int32_t side, int32_t kind, i
  * into the last sweep of a substep; 1 otherwise). */
 int pk_sweep(int64_t cg_id, int32_t ddim, double dt, double chspeed, double psi_fac);
 
+/* pk_sweep over caller-owned device memory, allocating nothing.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  src and dst are distinct, non-overlapping device
+ * arrays holding every plane of one block in the library's device layout:
+ * plane p of ghosted cell (x, y, z) at [p*sv + x + sy*y + sz*z], the planes
+ * of each fluid in flind order (rho, m_x, m_y, m_z, E, then B_x, B_y, B_z,
+ * psi after the magnetized fluid's five), n interior cells and nb >= 4 ghost
+ * layers in each active direction (n = 1, no ghosts, in an inactive one).
+ * dst receives pk_sweep's interior result (both RK2 stages along ddim, psi
+ * times psi_fac); its ghost cells are not written.  report (device, may be
+ * NULL) receives 5 + nfluids doubles: min over the cells written of dl_d / c
+ * for d = x, y, z and max c (timestep_fluid of the new state, cfl not
+ * applied; as pk_timestep_fluid combines them), the floor flags (bit 0
+ * negative density, bit 1 negative internal energy) and per fluid the mass
+ * the density floor added divided by the cell volume.  The work is queued on
+ * `stream` (a hipStream_t; NULL: the library's stream) and not waited for.
+ * Scratch: one static device array of 33 KiB per GPU (timestep slots, floor
+ * counters, flags), held until the process ends; its size does not depend on
+ * the block, so no block size is refused for it.  Every call on a GPU uses
+ * that GPU's one array: calls that overlap on one GPU race. */
+int pk_sweep_noalloc(const double *src, double *dst, int64_t sv, int32_t sy, int32_t sz, const int32_t n[3],
+                     int32_t nb, const double dl[3], int32_t ddim, double dt, double chspeed, double psi_fac,
+                     double *report, void *stream);
+
 /* Ghost refresh of all variables of one block, all directions, corners
  * included (update_boundaries, sweeps.F90:66-107): periodic on itself, or
  * for a block with neighbours (pk_set_neighbour) the exchange over the

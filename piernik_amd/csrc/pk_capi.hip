@@ -305,10 +305,50 @@ void choose_tile(const Block *b, int d, bool fused, bool mag, int &TP, int &TS, 
     tile_geometry(b, d, fused, tp, ts, TP, TS, out, chunks, ptiles);
 }
 
-hipError_t sweep_kernel(int d, bool mag, bool fused, const pk::SweepArgs &a)
+hipError_t sweep_kernel(int d, bool mag, bool fused, const pk::SweepArgs &a, hipStream_t st = nullptr)
 {
-    return g_cfg.fast_math ? pk_fast::launch_sweep(d, mag, fused, a, g_stream)
-                           : pk::launch_sweep(d, mag, fused, a, g_stream);
+    if (!st) st = g_stream;
+    return g_cfg.fast_math ? pk_fast::launch_sweep(d, mag, fused, a, st) : pk::launch_sweep(d, mag, fused, a, st);
+}
+
+// ---- scratch of pk_sweep_noalloc -------------------------------------------
+// The one entry point that allocates nothing keeps its scratch in a static
+// device array: the timestep slots, both fluids' floor-mass counters and the
+// floor flags word (the low 32 bits of its last element).  Each GPU holds its
+// own copy, at the address hipGetSymbolAddress gives with that GPU current;
+// host code uses only that address (the array's name is not a device
+// address in host code).
+constexpr int kNaSlots = 1024;                            // timestep slots, 4 words each
+constexpr int kNaMass = 4 * kNaSlots;                     // first mass counter (2 fluids x kMassDigits)
+constexpr int kNaFlags = kNaMass + 2 * pk::kMassDigits;   // the flags word
+constexpr int kNaWords = kNaFlags + 1;
+static __device__ __align__(256) unsigned long long scratch[kNaWords];
+constexpr int kNaMaxDevices = 64;
+unsigned long long *g_scratch_dev[kNaMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per GPU.
+int scratch_ptr(unsigned long long **out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) return fail(PK_EINVAL, "device %d beyond the %d supported", dev, kNaMaxDevices);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(PK_EHIP, "hipGetSymbolAddress gave no address for the scratch array");
+        g_scratch_dev[dev] = (unsigned long long *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return PK_OK;
+}
+
+// report[4] = the floor flags, report[5 + f] = fluid f's floor-added mass
+// divided by the cell volume (the exact counters rounded once).
+__global__ void k_noalloc_report(const unsigned long long *s, int nfl, double *report)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    report[4] = (double)(unsigned)s[kNaFlags];
+    for (int f = 0; f < nfl; f++) report[5 + f] = pk::mass_value(s + kNaMass + pk::kMassDigits * f);
 }
 
 hipError_t list_kernel(int d, bool mag, const pk::ListArgs &L, int nt, int grid)
@@ -2177,6 +2217,8 @@ int pk_init(const pk_config *cfg)
     if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     if (!g_pinned) HIPCHK(hipHostMalloc((void **)&g_pinned, 16 * sizeof(double), hipHostMallocDefault));
     if (!g_part) HIPCHK(hipMalloc((void **)&g_part, sizeof(double) * 4 * 2048));
+    unsigned long long *sp = nullptr;
+    RCCHK(scratch_ptr(&sp));  // pk_sweep_noalloc's scratch on this GPU
     g_cfg = *cfg;
     g_nfl = cfg->nfluids;
     g_magf = -1;
@@ -2356,6 +2398,71 @@ int pk_sweep(int64_t cg_id, int32_t ddim, double dt, double chspeed, double psi_
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
     list_touch();
     return do_sweep(b, ddim, dt, chspeed, psi_fac);
+}
+
+int pk_sweep_noalloc(const double *src, double *dst, int64_t sv, int32_t sy, int32_t sz, const int32_t n[3],
+                     int32_t nb, const double dl[3], int32_t ddim, double dt, double chspeed, double psi_fac,
+                     double *report, void *stream)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (!src || !dst || !n || !dl) return fail(PK_EINVAL, "null argument");
+    if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
+    if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "the fused sweep is RK2 only");
+    if (nb < 4) return fail(PK_EINVAL, "fused RK2 sweep needs nb >= 4 (nb = %d)", nb);
+    // the geometry as a block of the library's own (no buffers): the tile
+    // choice and the kernel arguments come from it
+    Block b;
+    b.nb = nb;
+    for (int d = 0; d < 3; d++) {
+        if (!g_cfg.has_dir[d] && n[d] != 1) return fail(PK_EINVAL, "inactive direction %d must have n = 1", d);
+        if (g_cfg.has_dir[d] && n[d] < nb) return fail(PK_EINVAL, "n[%d] = %d smaller than nb = %d", d, n[d], nb);
+        if (!(dl[d] > 0.0)) return fail(PK_EINVAL, "dl[%d] must be positive", d);
+        b.n[d] = n[d];
+        b.g[d] = g_cfg.has_dir[d] ? nb : 0;
+        b.ng[d] = n[d] + 2 * b.g[d];
+        b.dl[d] = dl[d];
+    }
+    const long long plane = (long long)sz * b.ng[2];
+    if (sy < b.ng[0] || (long long)sz < (long long)sy * b.ng[1] || sv < plane)
+        return fail(PK_EINVAL, "strides (sy %d, sz %d, sv %lld) do not hold the ghosted box %d x %d x %d", sy, sz,
+                    (long long)sv, b.ng[0], b.ng[1], b.ng[2]);
+    const long long span = sv * g_nplanes;
+    if ((dst < src ? src - dst : dst - src) < span)
+        return fail(PK_EINVAL, "src and dst overlap (each holds %d planes of %lld values)", g_nplanes, (long long)sv);
+    b.sv = sv;
+    b.sy = sy;
+    b.sz = sz;
+    unsigned long long *s = nullptr;
+    RCCHK(scratch_ptr(&s));
+    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    // the counters and flags start at zero (a fill kernel: safe under graph capture)
+    HIPCHK(pk::launch_zero_u32((unsigned *)(s + kNaMass), 2 * (kNaWords - kNaMass), st));
+    if (report) HIPCHK(pk::launch_dtslots_init(s, kNaSlots, st));
+    for (int f = 0; f < g_nfl; f++) {
+        pk::SweepArgs a{};
+        fill_common(&b, f, a);
+        a.flags = (unsigned *)(s + kNaFlags);
+        a.dmass = s + kNaMass + pk::kMassDigits * f;
+        a.nslots = kNaSlots;
+        a.dtslots = report ? s : nullptr;
+        a.ch = chspeed;
+        a.c1 = (0.5 * dt) / b.dl[ddim];
+        a.c2 = (1.0 * dt) / b.dl[ddim];
+        a.psi_fac = psi_fac;
+        a.src = src + (long long)g_fplane[f] * sv;
+        a.base = a.src;
+        a.dst = dst + (long long)g_fplane[f] * sv;
+        a.images = 0;
+        choose_tile(&b, ddim, true, g_fmag[f], a.TP, a.TS, a.out, a.chunks, a.ptiles);
+        HIPCHK(sweep_kernel(ddim, g_fmag[f], true, a, st));
+    }
+    if (report) {
+        HIPCHK(pk::launch_dtslots_final(s, kNaSlots, report, st));
+        hipLaunchKernelGGL(k_noalloc_report, dim3(1), dim3(64), 0, st, s, g_nfl, report);
+        HIPCHK(hipGetLastError());
+    }
+    return PK_OK;
 }
 
 int pk_fill_ghosts(int64_t cg_id)
