@@ -153,12 +153,13 @@ def cpu_baseline(workload, budget=10.0):
 
 
 class TimedRegion:
-    """Marks the timed steps for rocprofv3 --selected-regions, which
-    profiles only what runs between roctxProfilerResume(0) and
-    roctxProfilerPause(0) (rocprofiler-sdk's roctx), so a kernel-trace or
-    counter profile of a bench run covers exactly the timed launches: no
-    warm-up, no autotuning, no other workload.  Without a profiler attached
-    the calls do nothing; without the library the region is not marked."""
+    """Marks the timed steps for rocprofv3 (rocprofiler-sdk's roctx): a range
+    named TIMED_RANGE, which --marker-trace records with its timestamps, so
+    that trace_summary() keeps exactly the timed launches of a kernel trace
+    (no warm-up, no autotuning, no other workload); and
+    roctxProfilerResume(0) / roctxProfilerPause(0) around it for
+    --selected-regions.  Without a profiler attached the calls do nothing;
+    without the library the region is not marked."""
 
     def __init__(self):
         import ctypes
@@ -172,13 +173,52 @@ class TimedRegion:
 
     def __enter__(self):
         if self.lib is not None:
+            self.lib.roctxRangePushA(TIMED_RANGE.encode())
             self.lib.roctxProfilerResume(0)
         return self
 
     def __exit__(self, *exc):
         if self.lib is not None:
             self.lib.roctxProfilerPause(0)
+            self.lib.roctxRangePop()
         return False
+
+
+# the roctx range around the timed steps (rocprofv3 --marker-trace records it)
+TIMED_RANGE = "bench timed steps"
+
+
+def trace_summary(d):
+    """Per-kernel statistics of the dispatches inside the timed range, from
+    a rocprofv3 run with --kernel-trace --marker-trace -f csv (output under
+    directory d): calls, total ms, average, min and max us, share."""
+    import csv
+    import glob
+
+    def rows(pattern):
+        out = []
+        for path in glob.glob(os.path.join(d, "**", pattern), recursive=True):
+            with open(path, newline="") as f:
+                out.extend(csv.DictReader(f))
+        return out
+
+    marks = [r for r in rows("*marker_api_trace.csv") if TIMED_RANGE in " ".join(str(v) for v in r.values())]
+    if not marks:
+        raise SystemExit(f"no '{TIMED_RANGE}' range in the marker trace under {d}")
+    t0 = min(int(r["Start_Timestamp"]) for r in marks)
+    t1 = max(int(r["End_Timestamp"]) for r in marks)
+    stats = {}
+    for r in rows("*kernel_trace.csv"):
+        a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        if a >= t0 and b <= t1:
+            stats.setdefault(r["Kernel_Name"], []).append((b - a) * 1e-3)
+    total = sum(sum(v) for v in stats.values())
+    lines = [f"# timed range: {(t1 - t0) * 1e-6:.3f} ms, kernels inside it: {total * 1e-3:.3f} ms",
+             "# columns: calls, total ms, average us, min us, max us, share %, kernel"]
+    for name, v in sorted(stats.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{len(v):6d} {sum(v) * 1e-3:10.3f} {sum(v) / len(v):12.2f} {min(v):12.2f} {max(v):12.2f} "
+                     f"{100.0 * sum(v) / total:7.2f}  {name}")
+    return "\n".join(lines)
 
 
 def load_traffic(name):
@@ -227,7 +267,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["mhd256", "sedov64", "batch4096", "adjoint128", "ensemble256", "copy",
-                                           "unsplit256", "stage256", "gdf256"],
+                                           "unsplit256", "stage256", "gdf256", "variants"],
                     default="mhd256")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--no-batch", action="store_true",
@@ -245,7 +285,13 @@ def main():
                     help="stage256: time only the plain stages (no flux points, no PCIe round trips)")
     ap.add_argument("--exact", action="store_true",
                     help="time the bit-exact build of the sweep instead of the FMA-contracted one")
+    ap.add_argument("--trace-summary", metavar="DIR", default=None,
+                    help="print per-kernel statistics of the timed steps from a rocprofv3 "
+                         "--kernel-trace --marker-trace -f csv output directory, then exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: there is no multi-GPU path to run "
@@ -275,6 +321,9 @@ def main():
         return
     if args.workload == "gdf256":
         print(json.dumps(run_gdf(args, cpu)))
+        return
+    if args.workload == "variants":
+        print(json.dumps(run_variants(args)))
         return
     if args.workload == "mhd256" and args.blocks:
         print(json.dumps(run_blocks(args)))
@@ -398,6 +447,62 @@ def main():
         "batching": batching,
     }
     print(json.dumps(line))
+
+
+# Sweep kernel shapes compared by --workload variants: (pencils, cells) as in
+# pk_config tile_x / tile_yz; cells > 0 the tile kernel (TP x TS lanes),
+# -1 the march kernel, -m <= -2 the stream kernel (m iterations per workgroup).
+VARIANTS_X = [(1, 64), (1, 72), (1, 136), (1, -8), (1, -16), (1, -32), (1, -64)]
+VARIANTS_YZ = [(4, 64), (8, 32), (2, 136), (64, -1), (2, -16), (4, -16), (4, -32), (8, -8), (8, -16), (8, -32)]
+
+
+def run_variants(args, reps=10):
+    """Each sweep kernel shape of VARIANTS_X / VARIANTS_YZ on the 256^3 MHD
+    block (relaxed build unless --exact): `reps` fused sweeps along each
+    direction after one untimed, HIP events around every launch; GB/s of the
+    algorithmic bytes (144 B per cell and sweep) and the same as a fraction
+    of 8 TB/s."""
+    import piernik_amd as pk
+    from piernik_amd import setups
+
+    n = args.n or 256
+    cg, cfg = setups.mhd_turbulence(n=n, seed=1)
+    cfg.fast_math = not args.exact
+    dt = 0.3 * min(cg.dl)
+    out = {}
+    for d in range(3):
+        rows = []
+        for shape in (VARIANTS_X if d == 0 else VARIANTS_YZ):
+            if d == 0:
+                cfg.tile_x, cfg.tile_yz = shape, (0, 0)
+            else:
+                cfg.tile_x, cfg.tile_yz = (0, 0), shape
+            pk.init(cfg)
+            cg.registered = False
+            cg.upload()
+            ch = pk.update_chspeed(cg, dt, cfg)
+            pk.sweep(cg, d, dt, ch)
+            pk.synchronize()
+            pk.profile_begin()
+            for _ in range(reps):
+                pk.sweep(cg, d, dt, ch)
+            prof = pk.profile_end()
+            ms = prof["ms"][d] / prof["launches"][d]
+            gbps = prof["algo_bytes"][d] / prof["launches"][d] / (ms * 1e-3) / 1e9
+            rows.append({"shape": list(shape), "kernel": kernel_name(shape[1]), "ms": ms, "gbps": gbps,
+                         "frac": gbps / HBM_PEAK_GBPS})
+            print(f"[bench] {'xyz'[d]} {shape}: {ms:.3f} ms, {gbps:.0f} GB/s", file=sys.stderr, flush=True)
+            pk.finalize()
+        out["xyz"[d]] = rows
+    best = {k: max(v, key=lambda r: r["gbps"]) for k, v in out.items()}
+    return {
+        "metric": "fused sweep GB/s per kernel shape, 256^3 MHD", "value": max(r["gbps"] for r in best.values()),
+        "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D ideal MHD, HLLD+GLM, minmod, single periodic {n}^3 block, one fused sweep per "
+                               f"launch, {'bit-exact' if args.exact else 'relaxed'} build"},
+        "variants": out, "best": best,
+    }
 
 
 def batch_speedup(args, steps=3):
@@ -967,7 +1072,7 @@ def run_batch(args, cpu):
     cfg.fast_math = not args.exact
     pk.init(cfg)
     t0 = time.perf_counter()
-    setups.batch_blocks_device(sizes, cfg, seed=0)
+    cgs = setups.batch_blocks_device(sizes, cfg, seed=0)
     pk.synchronize()
     t_setup = time.perf_counter() - t0
     cells = sum(n ** 3 for n in sizes)
@@ -977,7 +1082,14 @@ def run_batch(args, cpu):
     if not args.no_autotune:
         pk.autotune_all()
     t_tune = time.perf_counter() - t0
-    print(f"[bench] autotune {t_tune:.1f} s", file=sys.stderr, flush=True)
+    # the list shapes chosen: (workgroup lanes, pencils, cells) per direction,
+    # fluid and block edge; lanes 64 with cells -m is the stream form
+    first_of = {}
+    for i, nn in enumerate(sizes):
+        first_of.setdefault(nn, i)
+    list_shapes = {"xyz"[d]: {fl: {str(nn): list(pk.list_tiles(d, f, cgs[i])) for nn, i in sorted(first_of.items())}
+                              for f, fl in enumerate(("ionized", "neutral"))} for d in range(3)}
+    print(f"[bench] autotune {t_tune:.1f} s, list shapes {list_shapes}", file=sys.stderr, flush=True)
     modes = [("graph", pk.BATCH_GRAPH), ("persistent", pk.BATCH_PERSISTENT), ("per_block", pk.BATCH_PER_BLOCK)]
     for name, mode in modes:
         pk.run_all(max(1, args.warmup), mode)
@@ -1025,6 +1137,7 @@ def run_batch(args, cpu):
                            "FP64, every operation rounded as the reference's code (bit-exact)"),
             "setup_s": t_setup,
             "autotune_s": t_tune,
+            "list_shapes_lanes_pencils_cells": list_shapes,
         },
         "batching": {
             "ms_per_step": ms,
