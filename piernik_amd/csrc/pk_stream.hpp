@@ -341,7 +341,13 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
     __shared__ double A[P::NV * 64];              // the quantity of the current phase, slot TP*i + p
     __shared__ double K[P::NV * 64];              // carry block: per variable, StreamCarry::N positions x TP
     __shared__ double U[P::NV * stream_su<TP>()]; // U0 with ULAG positions of the previous iteration in front
-    stream_range<DIR, P, LIM, TP>(a, (int)blockIdx.x * span, span, blockIdx.x, A, K, U);
+    // XCD-aware order: workgroup i runs on XCD i % 8, so each XCD gets a
+    // contiguous range of the stream; neighbouring pencil groups of a y or z
+    // sweep, which share cache lines, then share an L2
+    int bid = blockIdx.x;
+    const int nfull = (int)(gridDim.x / 8u) * 8;
+    if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
+    stream_range<DIR, P, LIM, TP>(a, bid * span, span, blockIdx.x, A, K, U);
 }
 
 // The stream over a block list: one-wave workgroups take items from a
