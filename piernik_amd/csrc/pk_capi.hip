@@ -268,6 +268,16 @@ void stream_geometry(const Block *b, int d, int tp, int m, int &TP, int &out, in
     ptiles = (int)((npos + out - 1) / out);
 }
 
+// The march kernel: whole pencils, one lane each, 64-lane workgroups (k_march).
+void march_geometry(const Block *b, int d, int &TP, int &out, int &chunks, int &ptiles)
+{
+    const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
+    TP = 64;
+    ptiles = (b->n[d1] * b->n[d2] + 63) / 64;
+    chunks = 1;
+    out = b->n[d];
+}
+
 // Precedence: the shape given in pk_config, then the one pk_autotune
 // measured for this block shape, then 3 x 72 (4-wave workgroups, two per CU).
 // A tile length of kMarch selects the march kernel for the fused sweep.
@@ -289,13 +299,8 @@ void choose_tile(const Block *b, int d, bool fused, bool mag, int &TP, int &TS, 
         return;
     }
     if (ts == kMarch && fused) {
-        // whole pencils, one lane each, 64-lane workgroups (k_march)
-        const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
-        TP = 64;
         TS = kMarch;
-        out = b->n[d];
-        chunks = 1;
-        ptiles = (b->n[d1] * b->n[d2] + 63) / 64;
+        march_geometry(b, d, TP, out, chunks, ptiles);
         return;
     }
     if (ts <= 0) {
@@ -2872,11 +2877,8 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
                 }
             }
             if (try_march) {
-                a.TP = 64;
                 a.TS = kMarch;
-                a.out = b->n[d];
-                a.chunks = 1;
-                a.ptiles = (b->n[d1] * b->n[d2] + 63) / 64;
+                march_geometry(b, d, a.TP, a.out, a.chunks, a.ptiles);
                 HIPCHK(sweep_kernel(d, g_fmag[f], true, a));  // warm-up
                 HIPCHK(hipEventRecord(e0, g_stream));
                 for (int rep = 0; rep < 3; rep++) HIPCHK(sweep_kernel(d, g_fmag[f], true, a));

@@ -138,25 +138,25 @@ __device__ __forceinline__ void dt_cell(const SweepArgs &a, const double (&Wn)[P
     double v[3] = {0.0, 0.0, 0.0};
     if (d > 0.0) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) v[q] = fabs(m[q] / d);
+        for (int q = 0; q < 3; q++) v[q] = fabs(qdiv(m[q], d));
     }
-    const double ek = (0.5 * ((mx * mx + my * my) + mz * mz)) / d;
+    const double ek = qdiv(0.5 * ((mx * mx + my * my) + mz * mz), d);
     double cs;
     if constexpr (P::MAG) {
         const double bx = Wn[plane_of<DIR>(5)], by = Wn[plane_of<DIR>(6)], bz = Wn[plane_of<DIR>(7)];
         const double pmag = 0.5 * ((bx * bx + by * by) + bz * bz);
         const double ps = (e - ek) * a.gam_1 + (2.0 - a.gam) * pmag;
         const double pr = ps - pmag;
-        cs = sqrt(fabs((2.0 * pmag + a.gam * pr) / d));
+        cs = qsqrt(fabs(qdiv(2.0 * pmag + a.gam * pr, d)));
     } else {
         const double pr = (e - ek) * a.gam_1;
-        cs = sqrt(fabs((a.gam * pr) / d));
+        cs = qsqrt(fabs(qdiv(a.gam * pr, d)));
     }
 #pragma unroll
     for (int q = 0; q < 3; q++) {
         const double c = fmx(v[q] + cs, small);
         cmax = fmx(cmax, c);
-        if (a.has_dir[q]) mn[q] = fmn(mn[q], a.dl[q] / c);
+        if (a.has_dir[q]) mn[q] = fmn(mn[q], qdiv(a.dl[q], c));
     }
 }
 
@@ -728,11 +728,11 @@ __global__ __launch_bounds__(256) void k_dt_partial(const DtArgs a, double *__re
         const double d = u[0], mx = u[a.sv], my = u[2 * a.sv], mz = u[3 * a.sv], e = u[4 * a.sv];
         double v0 = 0.0, v1 = 0.0, v2 = 0.0;
         if (d > 0.0) {
-            v0 = fabs(mx / d);
-            v1 = fabs(my / d);
-            v2 = fabs(mz / d);
+            v0 = fabs(qdiv(mx, d));
+            v1 = fabs(qdiv(my, d));
+            v2 = fabs(qdiv(mz, d));
         }
-        const double ek = (0.5 * ((mx * mx + my * my) + mz * mz)) / d;
+        const double ek = qdiv(0.5 * ((mx * mx + my * my) + mz * mz), d);
         double cs;
         if (MAG) {
             const double *b = a.u + (long long)a.bplane * a.sv + idx;
@@ -740,17 +740,17 @@ __global__ __launch_bounds__(256) void k_dt_partial(const DtArgs a, double *__re
             const double pmag = 0.5 * ((bx * bx + by * by) + bz * bz);
             const double ps = (e - ek) * a.gam_1 + (2.0 - a.gam) * pmag;
             const double pr = ps - pmag;
-            cs = sqrt(fabs((2.0 * pmag + a.gam * pr) / d));
+            cs = qsqrt(fabs(qdiv(2.0 * pmag + a.gam * pr, d)));
         } else {
             const double pr = (e - ek) * a.gam_1;
-            cs = sqrt(fabs((a.gam * pr) / d));
+            cs = qsqrt(fabs(qdiv(a.gam * pr, d)));
         }
         const double small = 1.17549435082228750797e-38;
         const double c0 = fmx(v0 + cs, small), c1 = fmx(v1 + cs, small), c2 = fmx(v2 + cs, small);
         cmax = fmx(cmax, fmx(fmx(c0, c1), c2));
-        if (a.has_dir[0]) m0 = fmn(m0, a.dl[0] / c0);
-        if (a.has_dir[1]) m1 = fmn(m1, a.dl[1] / c1);
-        if (a.has_dir[2]) m2 = fmn(m2, a.dl[2] / c2);
+        if (a.has_dir[0]) m0 = fmn(m0, qdiv(a.dl[0], c0));
+        if (a.has_dir[1]) m1 = fmn(m1, qdiv(a.dl[1], c1));
+        if (a.has_dir[2]) m2 = fmn(m2, qdiv(a.dl[2], c2));
     }
     red[0][t] = m0;
     red[1][t] = m1;

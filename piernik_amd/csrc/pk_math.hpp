@@ -17,6 +17,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifndef PK_RELAXED
+#define PK_RELAXED 0
+#endif
+
 // Callable from device code and, for host-side checks of the same
 // arithmetic, from host code.
 #define PK_HD __host__ __device__ __forceinline__
@@ -36,6 +40,41 @@ enum Limiter : int { LIM_VANLEER = 0, LIM_MINMOD = 1, LIM_MONCEN = 2, LIM_SUPERB
 PK_HD double fmn(double a, double b) { return fmin(a, b); }
 PK_HD double fmx(double a, double b) { return fmax(a, b); }
 
+// a / b and sqrt(x).  The relaxed build (PK_RELAXED: the FMA-contracted
+// kernels) forms them from the hardware reciprocal and reciprocal square root
+// with one Newton-Raphson step each and a final residual correction, which
+// leaves them within an ulp or two of the correctly rounded values for the
+// finite, nonzero (division) and positive normal (sqrt) operands of the
+// sweep, in fewer instructions than the compiler's fast expansions (no
+// range scaling, one refinement less); sqrt of 0 is 0.  The exact build
+// uses the correctly rounded operations, as the reference does.
+PK_HD double qdiv(double a, double b)
+{
+#if PK_RELAXED && defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+#else
+    return a / b;
+#endif
+}
+
+PK_HD double qsqrt(double x)
+{
+#if PK_RELAXED && defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, x), h, g);
+    return x > 0.0 ? g : 0.0;
+#else
+    return sqrt(x);
+#endif
+}
+
 // Fortran SPACING(x) for real(8): 2**max(exponent(x)-53, -1022); TINY for 0.
 // For a biased exponent field E >= 53 that is the power of two with exponent
 // field E-52, otherwise 2**-1022.
@@ -53,7 +92,7 @@ PK_HD double limit(int lim, double dl, double dr)
 {
     if (lim == LIM_VANLEER) {
         const double dcen = dl * dr;
-        return (dcen > 0.0) ? (2.0 * dcen) / (dl + dr) : 0.0;
+        return (dcen > 0.0) ? qdiv(2.0 * dcen, dl + dr) : 0.0;
     }
     const double s = copysign(1.0, dl) + copysign(1.0, dr);
     const double a = fabs(dl), b = fabs(dr);
@@ -79,10 +118,10 @@ PK_HD void prim(const double (&u)[P::NV], double gam_1, double (&q)[P::NV])
 {
     const double d = u[0];
     q[0] = d;
-    q[1] = u[1] / d;
-    q[2] = u[2] / d;
-    q[3] = u[3] / d;
-    const double ek = (0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3])) / d;
+    q[1] = qdiv(u[1], d);
+    q[2] = qdiv(u[2], d);
+    q[3] = qdiv(u[3], d);
+    const double ek = qdiv(0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3]), d);
     double p = gam_1 * (u[4] - ek);
     if constexpr (P::MAG) {
         const double b2 = (u[5] * u[5] + u[6] * u[6]) + u[7] * u[7];
@@ -100,9 +139,9 @@ PK_HD void prim(const double (&u)[P::NV], double gam_1, double (&q)[P::NV])
 PK_HD void flux_hllc(const double *l, const double *r, double gam, double *f)
 {
     const double prl = l[4], prr = r[4];
-    const double enl = (l[4] / (gam - 1.0)) + (0.5 * l[0]) * ((l[1] * l[1] + l[2] * l[2]) + l[3] * l[3]);
-    const double enr = (r[4] / (gam - 1.0)) + (0.5 * r[0]) * ((r[1] * r[1] + r[2] * r[2]) + r[3] * r[3]);
-    const double c = sqrt(fmx((gam * l[4]) / l[0], (gam * r[4]) / r[0]));
+    const double enl = qdiv(l[4], gam - 1.0) + (0.5 * l[0]) * ((l[1] * l[1] + l[2] * l[2]) + l[3] * l[3]);
+    const double enr = qdiv(r[4], gam - 1.0) + (0.5 * r[0]) * ((r[1] * r[1] + r[2] * r[2]) + r[3] * r[3]);
+    const double c = qsqrt(fmx(qdiv(gam * l[4], l[0]), qdiv(gam * r[4], r[0])));
     const double sl = fmn(l[1], r[1]) - c;
     const double sr = fmx(l[1], r[1]) + c;
 
@@ -131,7 +170,7 @@ PK_HD void flux_hllc(const double *l, const double *r, double gam, double *f)
     const double auxr = (sr - r[1]) * r[0], auxl = (sl - l[1]) * l[0];
     const double sm = (fabs(auxr - auxl) < spacing(auxr + auxl))
                           ? (sl + sr) / 2.0
-                          : ((auxr * r[1] - prr) - (auxl * l[1] - prl)) / (auxr - auxl);
+                          : qdiv((auxr * r[1] - prr) - (auxl * l[1] - prl), auxr - auxl);
     const double slvxl = sl - l[1], srvxr = sr - r[1];
     const double dn_l = l[0] * slvxl, dn_r = r[0] * srvxr;
     const double prt_star = 0.5 * ((prl + dn_l * (sm - l[1])) + (prr + dn_r * (sm - r[1])));
@@ -147,8 +186,8 @@ PK_HD void flux_hllc(const double *l, const double *r, double gam, double *f)
         const double* q = L ? l : r;
         const double* f0 = L ? fl : fr;
         const double ssm = s_ - sm;
-        const double d = dn / ssm;
-        const double us[5] = {d, d * sm, d * q[2], d * q[3], (((svx * en) - (pr * q[1])) + (prt_star * sm)) / ssm};
+        const double d = qdiv(dn, ssm);
+        const double us[5] = {d, d * sm, d * q[2], d * q[3], qdiv(((svx * en) - (pr * q[1])) + (prt_star * sm), ssm)};
         const double U[5] = {q[0], q[0] * q[1], q[0] * q[2], q[0] * q[3], en};
 #pragma unroll
         for (int v = 0; v < 5; v++) f[v] = f0[v] + s_ * (us[v] - U[v]);
@@ -179,7 +218,7 @@ struct HlldSide {
 
 PK_HD void hlld_side(const double *q, double b2, double pr, double gam, HlldSide &s)
 {
-    s.en = ((q[4] / (gam - 1.0)) + (0.5 * q[0]) * ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3])) + 0.5 * b2;
+    s.en = (qdiv(q[4], gam - 1.0) + (0.5 * q[0]) * ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3])) + 0.5 * b2;
     s.f[0] = q[0] * q[1];
     s.f[1] = (q[0] * (q[1] * q[1]) + pr) - q[5] * q[5];
     s.f[2] = (q[0] * q[2]) * q[1] - q[5] * q[6];
@@ -206,7 +245,7 @@ PK_HD void hlld_side(const double *q, double b2, double pr, double gam, HlldSide
 PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, double *f)
 {
     // Dedner et al. eq. 42 (hlld.F90:229-230)
-    f[8] = ((ch * ch) * 0.5) * ((r[5] + l[5]) - (r[8] - l[8]) / ch);
+    f[8] = ((ch * ch) * 0.5) * ((r[5] + l[5]) - qdiv(r[8] - l[8], ch));
     f[5] = 0.5 * ((r[8] + l[8]) - ch * (r[5] - l[5]));
 
     const double bxl = l[5], byl = l[6], bzl = l[7];
@@ -217,9 +256,9 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, do
     const double prr = r[4] + 0.5 * b2r;
     const double gpl = gam * l[4], gpr = gam * r[4];
     const double tl = gpl + b2l, tr = gpr + b2r;
-    const double cl = (tl + sqrt(tl * tl - (4.0 * gpl) * (bxl * bxl))) / l[0];
-    const double cr = (tr + sqrt(tr * tr - (4.0 * gpr) * (bxr * bxr))) / r[0];
-    const double c = sqrt(0.5 * fmx(cl, cr));
+    const double cl = qdiv(tl + qsqrt(tl * tl - (4.0 * gpl) * (bxl * bxl)), l[0]);
+    const double cr = qdiv(tr + qsqrt(tr * tr - (4.0 * gpr) * (bxr * bxr)), r[0]);
+    const double c = qsqrt(0.5 * fmx(cl, cr));
     const double sl = fmn(l[1], r[1]) - c;
     const double sr = fmx(l[1], r[1]) + c;
 
@@ -238,13 +277,13 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, do
     const double auxr = (sr - r[1]) * r[0], auxl = (sl - l[1]) * l[0];
     const double sm = (fabs(auxr - auxl) < spacing(auxr + auxl))
                           ? (sl + sr) / 2.0
-                          : ((auxr * r[1] - prr) - (auxl * l[1] - prl)) / (auxr - auxl);
+                          : qdiv((auxr * r[1] - prr) - (auxl * l[1] - prl), auxr - auxl);
     const double slsm = sl - sm, srsm = sr - sm;
     const double slvxl = sl - l[1], srvxr = sr - r[1];
     const double srmsl = sr - sl;
     const double dn_l = l[0] * slvxl, dn_r = r[0] * srvxr;
     const double b_lr = bxl * bxr;
-    const double b_lrgam = b_lr / gam;
+    const double b_lrgam = qdiv(b_lr, gam);
     const double prt_star = 0.5 * ((prl + dn_l * (sm - l[1])) + (prr + dn_r * (sm - r[1])));
     const double fbyl = byl * l[1] - bxl * l[2], fbzl = bzl * l[1] - bxl * l[3];
     const double fbyr = byr * r[1] - bxr * r[2], fbzr = bzr * r[1] - bxr * r[3];
@@ -258,41 +297,41 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, do
         const bool okr = (fabs(c2) > spacing(b_lr)) && b_lrgam <= r[4];
         double hy = 0.0, hz = 0.0;
         if (!(okl && okr)) {
-            hy = ((sr * byr - sl * byl) - (fbyr - fbyl)) / srmsl;
-            hz = ((sr * bzr - sl * bzl) - (fbzr - fbzl)) / srmsl;
+            hy = qdiv((sr * byr - sl * byl) - (fbyr - fbyl), srmsl);
+            hz = qdiv((sr * bzr - sl * bzl) - (fbzr - fbzl), srmsl);
         }
         if (okl) {
             const double k = dn_l * slvxl - b_lr;
-            bsly = (byl * k) / c1;
-            bslz = (bzl * k) / c1;
+            bsly = qdiv(byl * k, c1);
+            bslz = qdiv(bzl * k, c1);
         } else {
             bsly = hy;
             bslz = hz;
         }
         if (okr) {
             const double k = dn_r * srvxr - b_lr;
-            bsry = (byr * k) / c2;
-            bsrz = (bzr * k) / c2;
+            bsry = qdiv(byr * k, c2);
+            bsrz = qdiv(bzr * k, c2);
         } else {
             bsry = hy;
             bsrz = hz;
         }
     }
-    const double vsly = l[2] + (bxl / dn_l) * (byl - bsly);
-    const double vslz = l[3] + (bxl / dn_l) * (bzl - bslz);
-    const double vsry = r[2] + (bxr / dn_r) * (byr - bsry);
-    const double vsrz = r[3] + (bxr / dn_r) * (bzr - bsrz);
-    const double dsl = dn_l / slsm, dsr = dn_r / srsm;
+    const double vsly = l[2] + qdiv(bxl, dn_l) * (byl - bsly);
+    const double vslz = l[3] + qdiv(bxl, dn_l) * (bzl - bslz);
+    const double vsry = r[2] + qdiv(bxr, dn_r) * (byr - bsry);
+    const double vsrz = r[3] + qdiv(bxr, dn_r) * (bzr - bsrz);
+    const double dsl = qdiv(dn_l, slsm), dsr = qdiv(dn_r, srsm);
 
     // which region holds x/t = 0 (hlld.F90:410-599)
     const bool bxnz = fabs(bxl) > 0.0;
     double dn_lsqt = 0.0, dn_rsqt = 0.0, alfven_l = 0.0, alfven_r = 0.0;
     bool dbl, L;
     if (bxnz) {
-        dn_lsqt = sqrt(dsl);
-        dn_rsqt = sqrt(dsr);
-        alfven_l = sm - fabs(bxl) / dn_lsqt;
-        alfven_r = sm + fabs(bxr) / dn_rsqt;
+        dn_lsqt = qsqrt(dsl);
+        dn_rsqt = qsqrt(dsr);
+        alfven_l = sm - qdiv(fabs(bxl), dn_lsqt);
+        alfven_r = sm + qdiv(fabs(bxr), dn_rsqt);
         const bool ls = alfven_l > 0.0, rs = !ls && (alfven_r < 0.0);
         dbl = !ls && !rs;
         L = dbl ? (sm > 0.0) : ls;
@@ -307,11 +346,11 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, do
     if (dbl) {
         const double add = dn_lsqt + dn_rsqt;
         const double mul = dn_lsqt * dn_rsqt;
-        v2y = ((dn_lsqt * vsly + dn_rsqt * vsry) + b_sig * (bsry - bsly)) / add;
-        v2z = ((dn_lsqt * vslz + dn_rsqt * vsrz) + b_sig * (bsrz - bslz)) / add;
+        v2y = qdiv((dn_lsqt * vsly + dn_rsqt * vsry) + b_sig * (bsry - bsly), add);
+        v2z = qdiv((dn_lsqt * vslz + dn_rsqt * vsrz) + b_sig * (bsrz - bslz), add);
         const double b2x = 0.5 * (bxl + bxr);
-        b2y = ((dn_lsqt * bsry + dn_rsqt * bsly) + (b_sig * mul) * (vsry - vsly)) / add;
-        b2z = ((dn_lsqt * bsrz + dn_rsqt * bslz) + (b_sig * mul) * (vsrz - vslz)) / add;
+        b2y = qdiv((dn_lsqt * bsry + dn_rsqt * bsly) + (b_sig * mul) * (vsry - vsly), add);
+        b2z = qdiv((dn_lsqt * bsrz + dn_rsqt * bslz) + (b_sig * mul) * (vsrz - vslz), add);
         vb2 = (sm * b2x + v2y * b2y) + v2z * b2z;
     }
 
@@ -363,7 +402,7 @@ PK_HD void flux_hlld(const double *l, const double *r, double gam, double ch, do
     const double vb = (q[1] * q[5] + q[2] * q[6]) + q[3] * q[7];
     const double vbs = (sm * q[5] + vsy * bsy) + vsz * bsz;
     const double us[5] = {ds, ds * sm, ds * vsy, ds * vsz,
-                          ((((svx * o.en) - (pr * q[1])) + (prt_star * sm)) + q[5] * (vb - vbs)) / ssm};
+                          qdiv((((svx * o.en) - (pr * q[1])) + (prt_star * sm)) + q[5] * (vb - vbs), ssm)};
     if (dbl) {
         // Alfven intermediate flux, Eq. 65 (hlld.F90:496-520)
         const double al = L ? alfven_l : alfven_r;
@@ -400,7 +439,7 @@ PK_HD void floors(double (&u)[P::NV], double smalld, double smallei, bool use_sm
         dmass = nd - u[0];
         u[0] = nd;
     }
-    const double kin = (0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3])) / u[0];
+    const double kin = qdiv(0.5 * ((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3]), u[0]);
     double ei, mag = 0.0;
     if constexpr (P::MAG) {
         mag = 0.5 * ((u[5] * u[5] + u[6] * u[6]) + u[7] * u[7]);

@@ -57,19 +57,30 @@ __host__ __device__ constexpr int stream_su()
     return TP * (64 / TP + StreamCarry::ULAG);
 }
 
+// The phase buffer A and the carry block K share one LDS array: per
+// variable, 64 slots of A then StreamCarry::N positions x TP of K.
+template <int TP>
+__host__ __device__ constexpr int stream_stride()
+{
+    return 64 + StreamCarry::N * TP;
+}
+
 // The positions [s0, s0 + span) of a block's stream, walked by the one wave
-// of the calling workgroup.  A (NV*64), K (NV*64) and U (NV*stream_su<TP>())
-// are the workgroup's LDS; the wave's earlier contents there are never read
-// for an owned output.  slot_id picks the timestep slot.
+// of the calling workgroup.  AK (NV * stream_stride<TP>()) and U
+// (NV * stream_su<TP>()) are the workgroup's LDS; the wave's earlier
+// contents there are never read for an owned output.  slot_id picks the
+// timestep slot.
 template <int DIR, class P, int LIM, int TP>
 __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, const int span, unsigned slot_id,
-                                             double *A, double *K, double *U)
+                                             double *AK, double *U)
 {
     constexpr int NV = P::NV;
     constexpr int C = 64 / TP;
     constexpr int UL = StreamCarry::ULAG;
     constexpr int SU = stream_su<TP>();
-    static_assert(C >= UL && TP * StreamCarry::N <= 64, "stream shape");
+    constexpr int S = stream_stride<TP>();  // per-variable stride of A and K
+    static_assert(C >= UL, "stream shape");
+    double *const A = AK, *const K = AK + 64;
     constexpr int ds = DIR;
     constexpr int d1 = (DIR == 0) ? 1 : 0;
     constexpr int d2 = (DIR == 2) ? 1 : 2;
@@ -177,7 +188,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         for (int v = 0; v < NV; v++) uw[v * SU] = W[v];
         prim<P>(W, a.gam_1, q);
 #pragma unroll
-        for (int v = 0; v < NV; v++) aw[v * 64] = q[v];
+        for (int v = 0; v < NV; v++) aw[v * S] = q[v];
         wave_lds_order();
 
         // faces of cell sp-1 from q(sp-2), q(sp-1), q(sp)
@@ -186,18 +197,18 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             double qa[NV], qb[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                qa[v] = q2[v * 64];
-                qb[v] = q1[v * 64];
+                qa[v] = q2[v * S];
+                qb[v] = q1[v * S];
             }
             cell_faces<P, LIM>(a, qa, qb, q, fp, fm);
         }
         if (cq) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cq[v * 64] = q[v];
+            for (int v = 0; v < NV; v++) cq[v * S] = q[v];
         }
         wave_lds_order();
 #pragma unroll
-        for (int v = 0; v < NV; v++) aw[v * 64] = fp[v];
+        for (int v = 0; v < NV; v++) aw[v * S] = fp[v];
         wave_lds_order();
 
         // stage-1 flux at (sp-2)+1/2: right face of sp-2, left face of sp-1
@@ -205,16 +216,16 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         {
             double fl[NV];
 #pragma unroll
-            for (int v = 0; v < NV; v++) fl[v] = fp1[v * 64];
+            for (int v = 0; v < NV; v++) fl[v] = fp1[v * S];
             riemann<P>(a, fl, fm, F1);
         }
         if (cfp) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cfp[v * 64] = fp[v];
+            for (int v = 0; v < NV; v++) cfp[v * S] = fp[v];
         }
         wave_lds_order();
 #pragma unroll
-        for (int v = 0; v < NV; v++) aw[v * 64] = F1[v];
+        for (int v = 0; v < NV; v++) aw[v * S] = F1[v];
         wave_lds_order();
 
         // stage-1 state of cell sp-2 and its primitives
@@ -223,7 +234,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             double Fm[NV], W0[NV], Ws[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                Fm[v] = f11[v * 64];
+                Fm[v] = f11[v * S];
                 W0[v] = u2[v * SU];
             }
             cell_update<P>(W0, a.c1, Fm, F1, Ws);
@@ -239,11 +250,11 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         }
         if (cf1) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cf1[v * 64] = F1[v];
+            for (int v = 0; v < NV; v++) cf1[v * S] = F1[v];
         }
         wave_lds_order();
 #pragma unroll
-        for (int v = 0; v < NV; v++) aw[v * 64] = qs[v];
+        for (int v = 0; v < NV; v++) aw[v * S] = qs[v];
         wave_lds_order();
 
         // stage-2 faces of cell sp-3 from qs(sp-4), qs(sp-3), qs(sp-2)
@@ -252,18 +263,18 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             double qa[NV], qb[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                qa[v] = qs2[v * 64];
-                qb[v] = qs1[v * 64];
+                qa[v] = qs2[v * S];
+                qb[v] = qs1[v * S];
             }
             cell_faces<P, LIM>(a, qa, qb, qs, gp, gm);
         }
         if (cqs) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cqs[v * 64] = qs[v];
+            for (int v = 0; v < NV; v++) cqs[v * S] = qs[v];
         }
         wave_lds_order();
 #pragma unroll
-        for (int v = 0; v < NV; v++) aw[v * 64] = gp[v];
+        for (int v = 0; v < NV; v++) aw[v * S] = gp[v];
         wave_lds_order();
 
         // stage-2 flux at (sp-4)+1/2
@@ -271,16 +282,16 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         {
             double fl[NV];
 #pragma unroll
-            for (int v = 0; v < NV; v++) fl[v] = gp1[v * 64];
+            for (int v = 0; v < NV; v++) fl[v] = gp1[v * S];
             riemann<P>(a, fl, gm, F2);
         }
         if (cgp) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cgp[v * 64] = gp[v];
+            for (int v = 0; v < NV; v++) cgp[v * S] = gp[v];
         }
         wave_lds_order();
 #pragma unroll
-        for (int v = 0; v < NV; v++) aw[v * 64] = F2[v];
+        for (int v = 0; v < NV; v++) aw[v * S] = F2[v];
         wave_lds_order();
 
         // the new state of cell sp-4
@@ -288,7 +299,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             double Fm[NV], W0[NV], Wn[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                Fm[v] = f21[v * 64];
+                Fm[v] = f21[v * S];
                 W0[v] = u4[v * SU];
             }
             cell_update<P>(W0, a.c2, Fm, F2, Wn);
@@ -316,7 +327,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         }
         if (cf2) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cf2[v * 64] = F2[v];
+            for (int v = 0; v < NV; v++) cf2[v * S] = F2[v];
         }
         // U0 of the last UL positions goes in front of the next iteration's
         if (uc) {
@@ -335,19 +346,24 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
     if (flags) atomicOr(a.flags, flags);
 }
 
+// Both stream kernels are held to two waves per SIMD (256 registers): at
+// 253-256 VGPRs the allocator otherwise spills into AGPRs for some template
+// instances and halves their occupancy.
 template <int DIR, class P, int LIM, int TP>
-__global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stream(const SweepArgs a,
+                                                                                     const int span)
 {
-    __shared__ double A[P::NV * 64];              // the quantity of the current phase, slot TP*i + p
-    __shared__ double K[P::NV * 64];              // carry block: per variable, StreamCarry::N positions x TP
-    __shared__ double U[P::NV * stream_su<TP>()]; // U0 with ULAG positions of the previous iteration in front
+    // the quantity of the current phase (slot TP*i + p) and the carry block
+    __shared__ double AK[P::NV * stream_stride<TP>()];
+    // U0 with ULAG positions of the previous iteration in front
+    __shared__ double U[P::NV * stream_su<TP>()];
     // XCD-aware order: workgroup i runs on XCD i % 8, so each XCD gets a
     // contiguous range of the stream; neighbouring pencil groups of a y or z
     // sweep, which share cache lines, then share an L2
     int bid = blockIdx.x;
     const int nfull = (int)(gridDim.x / 8u) * 8;
     if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
-    stream_range<DIR, P, LIM, TP>(a, bid * span, span, blockIdx.x, A, K, U);
+    stream_range<DIR, P, LIM, TP>(a, bid * span, span, blockIdx.x, AK, U);
 }
 
 // The stream over a block list: one-wave workgroups take items from a
@@ -356,10 +372,9 @@ __global__ __launch_bounds__(64) void k_stream(const SweepArgs a, const int span
 // that block's stream.  Each wave leaves after its first fetch past the
 // end, so the grid drains after at most nitems + gridDim.x fetches.
 template <int DIR, class P, int LIM, int TP>
-__global__ __launch_bounds__(64) void k_stream_list(const ListArgs L)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stream_list(const ListArgs L)
 {
-    __shared__ double A[P::NV * 64];
-    __shared__ double K[P::NV * 64];
+    __shared__ double AK[P::NV * stream_stride<TP>()];
     __shared__ double U[P::NV * stream_su<TP>()];
     for (;;) {
         unsigned got = 0u;
@@ -368,6 +383,6 @@ __global__ __launch_bounds__(64) void k_stream_list(const ListArgs L)
         if (item >= L.nitems) break;
         SweepArgs a;
         const unsigned first = load_list_args<DIR>(L, item, a);
-        stream_range<DIR, P, LIM, TP>(a, (int)(item - first) * a.out, a.out, item, A, K, U);
+        stream_range<DIR, P, LIM, TP>(a, (int)(item - first) * a.out, a.out, item, AK, U);
     }
 }

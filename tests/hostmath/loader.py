@@ -15,8 +15,12 @@ P = ctypes.POINTER(ctypes.c_double)
 def build():
     if os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(s) for s in SRC):
         return LIB
+    # built under a name of this process and renamed into place, so that
+    # parallel test workers never load a half-written library
+    tmp = f"{LIB}.tmp{os.getpid()}"
     subprocess.run(["hipcc", "-x", "hip", "--offload-host-only", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
-                    "-std=c++17", "-o", LIB, SRC[0]], check=True, capture_output=True)
+                    "-std=c++17", "-o", tmp, SRC[0]], check=True, capture_output=True)
+    os.replace(tmp, LIB)
     return LIB
 
 

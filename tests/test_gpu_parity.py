@@ -165,12 +165,13 @@ def test_autotuned_tiles_keep_bit_exactness(oracle):
 MARCH = (64, -1)  # tile length -1: one lane per whole pencil (k_march)
 
 
-@pytest.mark.parametrize("case,tile_x", [("mhd", MARCH), ("mhd", (0, 0)), ("sedov", MARCH), ("mhd2d", MARCH),
-                                          ("mhd_vanleer_moncen", MARCH)])
-def test_march_kernel_bit_exact(oracle, case, tile_x):
+@pytest.mark.parametrize("case,tile_x,tile_yz", [
+    ("mhd", MARCH, MARCH), ("mhd", (0, 0), MARCH), ("sedov", MARCH, MARCH), ("mhd2d", MARCH, MARCH),
+    ("mhd_vanleer_moncen", MARCH, MARCH), ("mhd_long", (0, 0), MARCH), ("sedov_long", MARCH, MARCH)])
+def test_march_kernel_bit_exact(oracle, case, tile_x, tile_yz):
     """The march kernel (y and z, and x too when tile_x selects it) gives
     the oracle's bits over whole stepping loops, periodic images and the fused
-    timestep included."""
+    timestep included; also on blocks 64 cells long along y and z."""
     if case == "mhd":
         cg, cfg = setups.mhd_turbulence(n=24, seed=17)
         steps = 3
@@ -181,16 +182,23 @@ def test_march_kernel_bit_exact(oracle, case, tile_x):
         cg, cfg = setups.mhd_turbulence(n=20, seed=18)
         cfg.limiter, cfg.limiter_b = "vanleer", "moncen"
         steps = 3
+    elif case.endswith("_long"):
+        cfg = pk.Config(magnetized=case == "mhd_long", limiter="minmod" if case == "mhd_long" else "vanleer",
+                        smalld=1e-3, smallei=1e-4)
+        cg = None
+        steps = 3
     else:
         cfg = pk.Config(magnetized=True, limiter="vanleer", has_dir=(True, True, False), smalld=1e-3, smallei=1e-4)
         cg = None
         steps = 3
     cfg.tile_x = tile_x
-    cfg.tile_yz = MARCH
-    print(f"march {case}, tile_x {tile_x}", flush=True)
+    cfg.tile_yz = tile_yz
+    print(f"march {case}, tile_x {tile_x}, tile_yz {tile_yz}", flush=True)
     setup_lib(cfg)
     p = oracle.params_from_config(cfg)
-    if cg is None:
+    if cg is None and case.endswith("_long"):
+        cg, ob = make_pair(oracle, (8, 64, 64), cfg.magnetized, seed=29, cfg=cfg)
+    elif cg is None:
         cg, ob = make_pair(oracle, (16, 12, 1), True, seed=19, cfg=cfg, has_dir=(True, True, False))
     else:
         ob = oracle.OBlock.from_cg(cg)
