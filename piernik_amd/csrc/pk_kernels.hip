@@ -925,20 +925,49 @@ static hipError_t launch_march_t(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-// The stream kernel: a.TP pencils per lane group, a.out positions per
-// workgroup (a multiple of 64 / a.TP), a.ptiles workgroups.
+constexpr int kMaxDevices = 64;
+
+// Workgroups of `fn` (one wave each) resident on the current device at
+// once, 0 if unknown; cached per device.
+static int stream_slots(const void *fn, int *cache)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 0;
+    if (cache[dev] == 0) {
+        int per = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();  // launch without rounding; leave no error for the launch check
+            return 0;
+        }
+        cache[dev] = per * ncu;
+    }
+    return cache[dev];
+}
+
+// The stream kernel: a.TP pencils per lane group, ranges of a.out positions
+// (a multiple of 64 / a.TP) asked for, a.ptiles of them.  A launch of more
+// ranges than the device holds at once runs in rounds; it is rounded to a
+// whole number of rounds, the ranges made equal to within one iteration,
+// so that the last round does not leave the device partly idle.
 template <int DIR, class P, int LIM, int TP>
 static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
 {
     constexpr int d1 = (DIR == 0) ? 1 : 0, d2 = (DIR == 2) ? 1 : 2;
+    constexpr int C = 64 / TP;
     const long long npen = (long long)a.n[d1] * a.n[d2];
     const long long npos = (npen + TP - 1) / TP * (long long)(a.n[DIR] + 8);
     const int span = a.out;
-    if (a.g[DIR] < 4 || span <= 0 || span % (64 / TP) != 0 || npos + span + 64 > 0x7fffffffll)
-        return hipErrorInvalidValue;
-    const long long nblk = (npos + span - 1) / span;
+    if (a.g[DIR] < 4 || span <= 0 || span % C != 0 || npos + span + 64 > 0x7fffffffll) return hipErrorInvalidValue;
+    long long nblk = (npos + span - 1) / span;
     if (nblk != a.ptiles) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_stream<DIR, P, LIM, TP>), dim3((unsigned)nblk), dim3(64), 0, st, a, span);
+    const void *fn = (const void *)k_stream<DIR, P, LIM, TP>;
+    static int cache[kMaxDevices] = {};
+    const long long slots = stream_slots(fn, cache);
+    if (slots > 0 && nblk >= slots) nblk = (nblk + slots / 2) / slots * slots;
+    const long long nq = (npos + C - 1) / C;
+    if (nblk > nq) nblk = nq;
+    hipLaunchKernelGGL((k_stream<DIR, P, LIM, TP>), dim3((unsigned)nblk), dim3(64), 0, st, a, (int)nq);
     return hipGetLastError();
 }
 

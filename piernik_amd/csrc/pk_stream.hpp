@@ -26,9 +26,11 @@
 // (8 per range) are extra work.  A wave is a whole workgroup, so the
 // barriers between phases only order LDS accesses within the wave.
 //
-// Workgroup w owns positions [w*span, (w+1)*span) (span a multiple of C)
-// and starts 8 positions early so that its first owned outputs have their
-// stencil; it writes only outputs at owned positions.
+// The stream, cut into nq chunks of C positions, is split among the G
+// workgroups of a launch: workgroup w owns chunks [w*nq/G, (w+1)*nq/G), so
+// the ranges differ by at most one chunk.  A range starts 8 positions early
+// so that its first owned outputs have their stencil; it writes only
+// outputs at owned positions.
 
 // LDS carry areas: slot offsets (in positions) of each lagged quantity in
 // the carry block, and how many positions it keeps.
@@ -351,7 +353,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
 // instances and halves their occupancy.
 template <int DIR, class P, int LIM, int TP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stream(const SweepArgs a,
-                                                                                     const int span)
+                                                                                     const int nq)
 {
     // the quantity of the current phase (slot TP*i + p) and the carry block
     __shared__ double AK[P::NV * stream_stride<TP>()];
@@ -363,7 +365,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     int bid = blockIdx.x;
     const int nfull = (int)(gridDim.x / 8u) * 8;
     if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
-    stream_range<DIR, P, LIM, TP>(a, bid * span, span, blockIdx.x, AK, U);
+    constexpr int C = 64 / TP;
+    const int q0 = (int)((long long)bid * nq / gridDim.x), q1 = (int)((long long)(bid + 1) * nq / gridDim.x);
+    stream_range<DIR, P, LIM, TP>(a, q0 * C, (q1 - q0) * C, blockIdx.x, AK, U);
 }
 
 // The stream over a block list: one-wave workgroups take items from a

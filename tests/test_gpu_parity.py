@@ -215,7 +215,8 @@ def test_march_kernel_bit_exact(oracle, case, tile_x, tile_yz):
 STREAM_CASES = [("mhd", (1, -8), (4, -8)), ("mhd", (1, -3), (2, -5)), ("mhd", (1, -16), (1, -16)),
                 ("sedov", (1, -8), (4, -16)), ("sedov", (1, -2), (2, -2)), ("mhd2d", (1, -4), (4, -4)),
                 ("mhd_vanleer_moncen", (1, -8), (4, -8)), ("mhd_odd", (1, -5), (4, -3)),
-                ("mhd", (1, -4), (8, -8)), ("sedov", (1, -3), (8, -3)), ("mhd_odd", (1, -2), (8, -5))]
+                ("mhd", (1, -4), (8, -8)), ("sedov", (1, -3), (8, -3)), ("mhd_odd", (1, -2), (8, -5)),
+                ("mhd_rounds", (1, -2), (8, -2)), ("mhd_rounds", (1, -3), (4, -3))]
 
 
 @pytest.mark.parametrize("case,tile_x,tile_yz", STREAM_CASES)
@@ -223,7 +224,10 @@ def test_stream_kernel_bit_exact(oracle, case, tile_x, tile_yz):
     """The stream kernel (one wave walking concatenated pencils, each cell's
     stages computed once) gives the oracle's bits over whole stepping loops,
     periodic images and the fused timestep included; workgroup ranges that
-    end inside pencils and short pencils (shorter than a wave) included."""
+    end inside pencils and short pencils (shorter than a wave) included.
+    mhd_rounds: a 64 x 64 x 96 block with more ranges asked for than the
+    device holds at once, so the launch is rounded to whole rounds (up and
+    down) and the ranges differ by one iteration."""
     if case == "mhd":
         cg, cfg = setups.mhd_turbulence(n=24, seed=17)
         steps = 3
@@ -234,10 +238,10 @@ def test_stream_kernel_bit_exact(oracle, case, tile_x, tile_yz):
         cg, cfg = setups.mhd_turbulence(n=20, seed=18)
         cfg.limiter, cfg.limiter_b = "vanleer", "moncen"
         steps = 3
-    elif case == "mhd_odd":
+    elif case in ("mhd_odd", "mhd_rounds"):
         cfg = pk.Config(magnetized=True, limiter="minmod", smalld=1e-3, smallei=1e-4)
         cg = None
-        steps = 3
+        steps = 3 if case == "mhd_odd" else 2
     else:
         cfg = pk.Config(magnetized=True, limiter="vanleer", has_dir=(True, True, False), smalld=1e-3, smallei=1e-4)
         cg = None
@@ -249,6 +253,8 @@ def test_stream_kernel_bit_exact(oracle, case, tile_x, tile_yz):
     p = oracle.params_from_config(cfg)
     if cg is None and case == "mhd_odd":
         cg, ob = make_pair(oracle, (7, 13, 5), True, seed=23, cfg=cfg)
+    elif cg is None and case == "mhd_rounds":
+        cg, ob = make_pair(oracle, (64, 64, 96), True, seed=31, cfg=cfg)
     elif cg is None:
         cg, ob = make_pair(oracle, (16, 12, 1), True, seed=19, cfg=cfg, has_dir=(True, True, False))
     else:
