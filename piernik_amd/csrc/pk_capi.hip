@@ -293,11 +293,12 @@ void choose_tile(const Block *b, int d, bool fused, bool mag, int &TP, int &TS, 
             ts = it->second.second;
         }
     }
-    if (is_stream(ts) && fused) {
+    if (is_stream(ts) && fused && pk::stream_stores_fit(b->sv)) {
         TS = ts;
         stream_geometry(b, d, tp, -ts, TP, out, chunks, ptiles);
         return;
     }
+    if (is_stream(ts)) ts = 0;  // planes too large for the stream's store ranges: the tile kernel
     if (ts == kMarch && fused) {
         TS = kMarch;
         march_geometry(b, d, TP, out, chunks, ptiles);
@@ -1326,6 +1327,7 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
         if (nt == pk::kStreamListNt) {
             // the stream form: items are ranges of -ts iterations of the block's stream
             if (!is_stream(ts)) return fail(PK_EINVAL, "stream list item of %d iterations", -ts);
+            if (!pk::stream_stores_fit(b->sv)) return fail(PK_EINVAL, "block planes of %lld cells exceed the stream's buffer range", b->sv);
             a.TS = ts;
             stream_geometry(b, d, pk::stream_list_tp(d), -ts, a.TP, a.out, a.chunks, a.ptiles);
         } else {
@@ -2972,6 +2974,9 @@ int pk_autotune_all(double dt, double chspeed)
         for (Block *o : v) cells += (long long)o->n[0] * o->n[1] * o->n[2];
         if (cells < 8ll * 1024 * 1024) v.push_back(b);
     }
+    // the stream form stores through 4 GiB buffer ranges: every block's planes must fit
+    bool streams_fit = true;
+    for (Block *b : g_list.blocks) streams_fit = streams_fit && pk::stream_stores_fit(b->sv);
     LaunchTab tmp;
     int rc = PK_OK;
     const int nt_list[] = {pk::kStreamListNt, 128, 256, 512};
@@ -2982,6 +2987,7 @@ int pk_autotune_all(double dt, double chspeed)
             int best_nt = 256;
             std::map<std::vector<int>, int> best_ts_at;
             for (int nt : nt_list) {
+                if (nt == pk::kStreamListNt && !streams_fit) continue;
                 if (list_resident(d, g_fmag[f], nt) <= 0) continue;
                 double total = 0.0;
                 std::map<std::vector<int>, int> ts_at;

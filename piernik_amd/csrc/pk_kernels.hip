@@ -958,7 +958,8 @@ static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
     const long long npen = (long long)a.n[d1] * a.n[d2];
     const long long npos = (npen + TP - 1) / TP * (long long)(a.n[DIR] + 8);
     const int span = a.out;
-    if (a.g[DIR] < 4 || span <= 0 || span % C != 0 || npos + span + 64 > 0x7fffffffll) return hipErrorInvalidValue;
+    if (a.g[DIR] < 4 || span <= 0 || span % C != 0 || npos + span + 64 > 0x7fffffffll || !pk::stream_stores_fit(a.sv))
+        return hipErrorInvalidValue;
     long long nblk = (npos + span - 1) / span;
     if (nblk != a.ptiles) return hipErrorInvalidValue;
     const void *fn = (const void *)k_stream<DIR, P, LIM, TP>;

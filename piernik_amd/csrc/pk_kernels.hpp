@@ -135,6 +135,10 @@ hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipSt
 constexpr int kStreamListNt = 64;
 // Pencils per lane group of the list stream along direction d.
 constexpr int stream_list_tp(int d) { return d == 0 ? 1 : 4; }
+// The stream kernels store a fluid's planes of sv cells through raw buffer
+// resources over three planes each, whose byte ranges (24 sv) must stay
+// below 2^31.
+constexpr bool stream_stores_fit(long long sv) { return sv >= 0 && 24 * sv < (1ll << 31); }
 // Persistent batched fused sweep with workgroups of nt lanes; grid = 0 picks
 // the number of workgroups the device holds at once.
 hipError_t launch_sweep_list(int dir, bool mhd, int lim, int limb, const ListArgs &L, int nt, int grid,

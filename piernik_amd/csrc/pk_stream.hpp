@@ -52,6 +52,32 @@ struct StreamCarry {
 // scope does without waiting for their completion.
 __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
+// Stores of a fluid's new state through raw buffer resources, one per three
+// planes (pk::stream_stores_fit(sv): each range, 24 sv bytes, below 2^31).
+// Plane k is at scalar offset (k % 3) * 8 sv of resource k / 3; a lane whose
+// vector offset is the range's size (drop()) is out of range with any of
+// these scalar offsets, and the hardware drops its store.  Both offsets stay
+// below 2^31, so their sum does not wrap.
+template <int NV>
+struct PlaneStores {
+    __amdgpu_buffer_rsrc_t r[(NV + 2) / 3];
+    unsigned plane_bytes;
+    __device__ __forceinline__ PlaneStores(double *dst, long long sv) : plane_bytes((unsigned)sv * 8u)
+    {
+#pragma unroll
+        for (int g = 0; g < (NV + 2) / 3; g++)
+            // dword 3 of a gfx9 buffer resource: 32-bit data format, no swizzle
+            r[g] = __builtin_amdgcn_make_buffer_rsrc(dst + 3ll * g * sv, (short)0, (int)(3u * plane_bytes), 0x00020000);
+    }
+    __device__ __forceinline__ unsigned drop() const { return 3u * plane_bytes; }
+    __device__ __forceinline__ void store(int plane, unsigned voff, double v) const
+    {
+        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r[plane / 3], voff,
+                                              (int)((unsigned)(plane % 3) * plane_bytes), 0);
+    }
+};
+
 // Slots per variable of the U0 buffer of a stream workgroup.
 template <int TP>
 __host__ __device__ constexpr int stream_su()
@@ -175,6 +201,10 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
 #pragma unroll
         for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx];
     }
+    // Wait for these loads before the loop: the compiler's wait at the top of
+    // the loop then only has to cover the loads of the previous iteration,
+    // which the stores issued after them do not delay.
+    __builtin_amdgcn_s_waitcnt(0);
     // outputs lag 4 positions behind the loads: the last owned output,
     // position s1-1, is finished when position s1+3 is loaded
     const int iters = (s1 + 4 - (s0 - 8) + C - 1) / C;
@@ -296,6 +326,18 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         for (int v = 0; v < NV; v++) aw[v * S] = F2[v];
         wave_lds_order();
 
+        // U0 of the next iteration's positions, loaded before this
+        // iteration's stores.  Vector memory counts loads and stores in one
+        // in-order counter; as the stores below are issued on every path,
+        // the wait for these loads at the top of the next iteration leaves
+        // the stores in flight.
+        {
+            bool pv2;
+            const long long idx2 = cell_index(j, pv2);
+#pragma unroll
+            for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx2];
+        }
+
         // the new state of cell sp-4
         {
             double Fm[NV], W0[NV], Wn[NV];
@@ -312,12 +354,10 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             const bool out = pv && spc - 4 >= s0 && spc - 4 < s1 && jo >= 4 && jo < L - 4;
             if (out) flags |= fl;
             pk::mass_add(a.dmass, out ? dm : 0.0);
+            if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
+            const long long oidx = idx - 4 * st;
             if (out) {
                 if (a.dtslots) dt_cell<DIR, P>(a, Wn, mn, cmax);
-                if constexpr (P::MAG) Wn[8] = Wn[8] * a.psi_fac;
-                const long long oidx = idx - 4 * st;
-#pragma unroll
-                for (int k = 0; k < NV; k++) a.dst[(long long)plane_of<DIR>(k) * a.sv + oidx] = Wn[k];
                 if (a.images) {
                     int xyz[3];
                     xyz[ds] = g - 4 + jo;
@@ -326,6 +366,12 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
                     write_images<DIR, P>(a, xyz, oidx, Wn);
                 }
             }
+            // every lane issues the stores: a lane without an output stores
+            // past the end of the plane's buffer range, which the hardware drops
+            const PlaneStores<NV> ps(a.dst, a.sv);
+            const unsigned voff = out ? (unsigned)oidx * 8u : ps.drop();
+#pragma unroll
+            for (int k = 0; k < NV; k++) ps.store(plane_of<DIR>(k), voff, Wn[k]);
         }
         if (cf2) {
 #pragma unroll
@@ -337,12 +383,6 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             for (int v = 0; v < NV; v++) uc[v * SU] = uw[v * SU];
         }
         wave_lds_order();
-        {
-            bool pv2;
-            const long long idx2 = cell_index(j, pv2);
-#pragma unroll
-            for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx2];
-        }
     }
     if (a.dtslots) dt_reduce(a, mn, cmax, slot_id);
     if (flags) atomicOr(a.flags, flags);
