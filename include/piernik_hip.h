@@ -366,13 +366,23 @@ int pk_run_unsplit(int64_t cg_id, double tend, int32_t nmax, double *t_out, int3
 int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6]);
 
 /* Tile shapes of the persistent list sweep: per direction and fluid kind the
- * workgroup size (128, 256 or 512 lanes) and, per block shape, the tile
- * length, timed on the registered list (the state is read, never advanced;
- * the floor bookkeeping of every block is reset).  chspeed <= 0: the
- * list's own update_chspeed for this dt. */
+ * workgroup size (64 lanes: the stream form, one-wave workgroups walking
+ * ranges of a block's pencils; 128, 256 or 512 lanes: tiles) and, per block
+ * shape, the tile length (stream: iterations per item), timed on the
+ * registered list (the state is read, never advanced; the floor bookkeeping
+ * of every block is reset).  chspeed <= 0: the list's own update_chspeed
+ * for this dt. */
 int pk_autotune_all(double dt, double chspeed);
-/* The list sweep's workgroup lanes and tile (pencils, cells) for one block,
- * direction and fluid: out = {lanes, pencils, cells}. */
+/* Sets the list sweep's shape for one direction and the kind (magnetized or
+ * not) of fluid ifluid, for every block, until the next pk_autotune_all:
+ * lanes 64 (the stream form; ts = -m, m >= 2 iterations per item) or 128,
+ * 256, 512 (tiles of ts cells with the 8 halo cells, 9 <= ts <= lanes);
+ * ts = 0: the default for each block.  No counterpart in the reference (a
+ * launch-shape control, as tile_x and tile_yz of pk_config_t). */
+int pk_set_list_tiles(int32_t ddim, int32_t ifluid, int32_t lanes, int32_t ts);
+/* The list sweep's workgroup lanes and tile for one block, direction and
+ * fluid: out = {lanes, pencils, cells} (stream form: {64, pencils per lane
+ * group, -iterations per item}). */
 int pk_list_tiles(int32_t ddim, int32_t ifluid, int64_t cg_id, int32_t out[3]);
 
 /* ---- reverse mode (no counterpart in the reference) ------------------------ */

@@ -31,7 +31,7 @@ __all__ = [
     "solve_cg", "sweep", "update_boundaries", "glmdamping", "timestep_fluid", "time_step",
     "update_chspeed", "glm_factor", "fluid_update", "run", "get_report", "synchronize",
     "profile_begin", "profile_end", "autotune", "timestep_all", "fluid_update_all", "run_all",
-    "autotune_all", "list_tiles",
+    "autotune_all", "list_tiles", "set_list_tiles",
     "EULER", "RK2_1", "RK2_2", "LIMITERS", "BATCH_GRAPH", "BATCH_PERSISTENT", "BATCH_PER_BLOCK",
 ]
 
@@ -113,6 +113,7 @@ _PROTOS = {
     "pk_run_all": [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "pk_autotune_all": [ctypes.c_double, ctypes.c_double],
     "pk_list_tiles": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)],
+    "pk_set_list_tiles": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     "pk_adjoint_setup": [ctypes.c_int64, ctypes.c_int32],
     "pk_state_save": [ctypes.c_int64, ctypes.c_int32],
     "pk_state_load": [ctypes.c_int64, ctypes.c_int32],
@@ -597,8 +598,16 @@ def autotune_all(dt=None, chspeed=0.0):
     _call("pk_autotune_all", float(dt), float(chspeed))
 
 
+def set_list_tiles(ddim, ifluid, lanes, ts=0):
+    """Pin the list sweep's shape for direction ddim and fluid ifluid's kind
+    until the next autotune_all: lanes 64 (the stream form, ts = -iterations
+    per item) or 128, 256, 512 (tiles of ts cells); ts 0: per-block default."""
+    _call("pk_set_list_tiles", int(ddim), int(ifluid), int(lanes), int(ts))
+
+
 def list_tiles(ddim, ifluid, cg):
-    """(workgroup lanes, pencils, cells) of the list sweep for one block."""
+    """(workgroup lanes, pencils, cells) of the list sweep for one block
+    (stream form: (64, pencils per lane group, -iterations per item))."""
     out = (ctypes.c_int32 * 3)()
     _call("pk_list_tiles", int(ddim), int(ifluid), cg.cg_id, out)
     return tuple(out)

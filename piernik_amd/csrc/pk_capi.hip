@@ -1145,6 +1145,7 @@ struct ListState {
     std::vector<double> mdl;   // min(dl) per block
     LaunchTab tab[3][2];
     int nt_pref[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // workgroup lanes chosen by pk_autotune_all (0: default)
+    int ts_force[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // tile length (stream: -iterations) set by pk_set_list_tiles
     unsigned *d_queue = nullptr;
     pk::StepScalars *d_step = nullptr;
     pk::StepScalars *h_step = nullptr;  // pinned
@@ -1301,6 +1302,8 @@ int list_default_ts(const Block *b, int d, int nt)
 
 int list_ts(const Block *b, int d, int f, int nt)
 {
+    const int forced = g_list.ts_force[d][g_fmag[f]];
+    if (forced != 0 && nt == g_list.nt_pref[d][g_fmag[f]]) return forced;
     auto it = g_tuned.find(tune_key(b, d, g_fmag[f], nt));
     if (it != g_tuned.end()) return it->second.second;
     return list_default_ts(b, d, nt);
@@ -1540,8 +1543,9 @@ std::vector<SweepStep> step_sweeps()
 }
 
 // Enqueue the launches of one step on g_stream.  mode PK_BATCH_PER_BLOCK
-// launches the single-block sweep once per block with the same tile shapes;
-// the persistent modes launch k_sweep_list once per sweep and fluid.
+// launches the single-block sweep once per block with the same tile shapes
+// (for items of the stream form, the block's single-block shape); the
+// persistent modes launch the list sweep once per sweep and fluid.
 int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
 {
     const std::vector<SweepStep> sw = step_sweeps();
@@ -1582,6 +1586,11 @@ int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
             } else {
                 for (size_t i = 0; i < g_list.blocks.size(); i++) {
                     pk::SweepArgs a = t.h_args[i];
+                    if (is_stream(a.TS)) {
+                        // list items of the stream form are sized for the whole list
+                        // sharing the device; one block alone gets the single-block shape
+                        choose_tile(g_list.blocks[i], d, true, g_fmag[f], a.TP, a.TS, a.out, a.chunks, a.ptiles);
+                    }
                     if (parity) {
                         const double *s0 = a.src;
                         a.src = a.dst;
@@ -3046,6 +3055,7 @@ int pk_autotune_all(double dt, double chspeed)
                 }
             }
             g_list.nt_pref[d][g_fmag[f]] = best_nt;
+            g_list.ts_force[d][g_fmag[f]] = 0;
             for (auto &cl : classes) {
                 const int ts = best_ts_at[cl.first];
                 g_tuned[tune_key(cl.second[0], d, g_fmag[f], best_nt)] = {
@@ -3067,6 +3077,28 @@ int pk_autotune_all(double dt, double chspeed)
     list_invalidate();
     RCCHK(list_build());
     g_list.fresh = fresh;
+    return PK_OK;
+}
+
+int pk_set_list_tiles(int32_t ddim, int32_t ifluid, int32_t lanes, int32_t ts)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
+    if (ifluid < 0 || ifluid >= g_nfl) return fail(PK_EINVAL, "bad fluid %d", ifluid);
+    if (lanes == pk::kStreamListNt) {
+        if (ts != 0 && !is_stream(ts)) return fail(PK_EINVAL, "stream list items of %d iterations", -ts);
+    } else if (lanes == 128 || lanes == 256 || lanes == 512) {
+        if (ts != 0 && (ts < 9 || ts > lanes)) return fail(PK_EINVAL, "tile length %d for %d lanes", ts, lanes);
+    } else {
+        return fail(PK_EINVAL, "list workgroups of %d lanes (64: the stream; 128, 256, 512: tiles)", lanes);
+    }
+    if (lanes == pk::kStreamListNt)
+        for (Block *b : g_list.blocks)
+            if (!pk::stream_stores_fit(b->sv))
+                return fail(PK_EINVAL, "block planes of %lld cells exceed the stream's buffer range", b->sv);
+    g_list.nt_pref[ddim][g_fmag[ifluid]] = lanes;
+    g_list.ts_force[ddim][g_fmag[ifluid]] = ts;
+    list_invalidate();
     return PK_OK;
 }
 

@@ -1045,7 +1045,7 @@ hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipSt
 // Persistent list sweep: launch (resident == nullptr) or report how many
 // workgroups of nt lanes the device holds at once.
 // Persistent stream over a list (nt == 64: k_stream_list, one pencil per
-// lane group along x, four along y and z).
+// lane group along x, eight along y and z).
 template <int DIR, class P, int LIM>
 static hipError_t stream_list_t(const ListArgs *L, int grid, hipStream_t st, int *resident)
 {

@@ -130,11 +130,11 @@ struct FaceJob {
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st);
 // Workgroup size that selects the stream form of the persistent list sweep
 // (k_stream_list: one-wave workgroups, each list item a range of a block's
-// stream of a.out positions, a.TP pencils per lane group: 1 along x, 4 along
+// stream of a.out positions, a.TP pencils per lane group: 1 along x, 8 along
 // y and z).
 constexpr int kStreamListNt = 64;
 // Pencils per lane group of the list stream along direction d.
-constexpr int stream_list_tp(int d) { return d == 0 ? 1 : 4; }
+constexpr int stream_list_tp(int d) { return d == 0 ? 1 : 8; }
 // The stream kernels store a fluid's planes of sv cells through raw buffer
 // resources over three planes each, whose byte ranges (24 sv) must stay
 // below 2^31.

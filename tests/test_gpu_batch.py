@@ -130,7 +130,10 @@ def test_autotuned_list_bit_exact(oracle):
         for d in range(3):
             for f in range(2):
                 lanes, tp, ts = pk.list_tiles(d, f, cg)
-                assert lanes in (128, 256, 512) and 1 <= tp and tp * ts <= lanes
+                if lanes == 64:  # the stream form: pencils per lane group, -iterations per item
+                    assert tp == (1 if d == 0 else 8) and ts <= -2
+                else:
+                    assert lanes in (128, 256, 512) and 1 <= tp and tp * ts <= lanes
                 seen.add(lanes)
     t_g, _ = pk.run_all(2)
     t_o = oracle.run_list(p, obs, 2)
@@ -138,6 +141,54 @@ def test_autotuned_list_bit_exact(oracle):
     for cg, ob in zip(cgs, obs):
         cg.download()
         assert_same(cg, ob, "autotuned")
+    pk.finalize()
+
+
+@pytest.mark.parametrize("ts", [-2, -5, -16])
+def test_stream_list_bit_exact(oracle, ts):
+    """The stream form of the list sweep (one-wave workgroups over ranges
+    of each block's pencils, pinned with set_list_tiles for every direction
+    and fluid) equals the oracle's list driver bit for bit after two steps,
+    in graph, persistent and per-block modes; items that end inside pencils
+    and blocks shorter than one item included."""
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    cgs = setups.batch_blocks(SHAPES + [(32, 24, 20)], cfg, seed=33)
+    init = [(cg.u.copy(order="F"), cg.b.copy(order="F"), cg.psi.copy(order="F")) for cg in cgs]
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    t_o = oracle.run_list(p, obs, 2)
+    for mode in MODES:
+        for cg, (u, b, psi) in zip(cgs, init):
+            cg.u[...] = u
+            cg.b[...] = b
+            cg.psi[...] = psi
+        print(f"stream list, {-ts} iterations per item, mode {mode}", flush=True)
+        start(cfg, cgs)
+        for d in range(3):
+            for f in range(2):
+                pk.set_list_tiles(d, f, 64, ts)
+        assert pk.list_tiles(0, 0, cgs[0]) == (64, 1, ts)
+        assert pk.list_tiles(2, 1, cgs[-1]) == (64, 8, ts)
+        t_g, _ = pk.run_all(2, MODES[mode])
+        assert t_g == t_o
+        for cg, ob in zip(cgs, obs):
+            cg.download()
+            assert_same(cg, ob, f"stream list {mode}")
+        pk.finalize()
+
+
+def test_set_list_tiles_rejects_bad_shapes():
+    """set_list_tiles refuses workgroup sizes other than 64, 128, 256 and
+    512, stream items shorter than two iterations and tiles longer than the
+    workgroup."""
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    cgs = setups.batch_blocks(SHAPES[:2], cfg, seed=3)
+    start(cfg, cgs)
+    for lanes, ts in ((96, 0), (64, -1), (64, 20), (128, 200), (256, 8)):
+        with pytest.raises(pk.PiernikError):
+            pk.set_list_tiles(1, 0, lanes, ts)
+    pk.set_list_tiles(1, 0, 256, 0)
+    assert pk.list_tiles(1, 0, cgs[0])[0] == 256
     pk.finalize()
 
 
