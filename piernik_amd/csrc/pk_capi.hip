@@ -261,11 +261,14 @@ void stream_geometry(const Block *b, int d, int tp, int m, int &TP, int &out, in
     if (d == 0 || (tp != 2 && tp != 4 && tp != 8)) tp = 1;
     if (m < 2) m = 2;
     TP = tp;
-    out = m * (64 / tp);  // positions per workgroup
+    // ranges of whole pencil groups (L positions each), about m iterations of 64 / tp positions
+    const int L = b->n[d] + 8;
+    const long long k = std::max(1ll, ((long long)m * (64 / tp) + L / 2) / L);
+    out = (int)(k * L);  // positions per range
     const long long npen = (long long)b->n[d1] * b->n[d2];
-    const long long npos = (npen + tp - 1) / tp * (long long)(b->n[d] + 8);
+    const long long ngroups = (npen + tp - 1) / tp;
     chunks = 1;
-    ptiles = (int)((npos + out - 1) / out);
+    ptiles = (int)((ngroups + k - 1) / k);
 }
 
 // The march kernel: whole pencils, one lane each, 64-lane workgroups (k_march).

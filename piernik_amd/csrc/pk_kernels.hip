@@ -945,30 +945,34 @@ static int stream_slots(const void *fn, int *cache)
     return cache[dev];
 }
 
-// The stream kernel: a.TP pencils per lane group, ranges of a.out positions
-// (a multiple of 64 / a.TP) asked for, a.ptiles of them.  A launch of more
-// ranges than the device holds at once runs in rounds; it is rounded to a
-// whole number of rounds, the ranges made equal to within one iteration,
-// so that the last round does not leave the device partly idle.
+// The stream kernel: a.TP pencils per lane group, a.ptiles ranges of whole
+// pencil groups (a.out positions each, the last one shorter) asked for.  A
+// launch of more ranges than the device holds at once runs in rounds; it is
+// rounded to a whole number of rounds, the ranges made equal to within one
+// group, so that the last round does not leave the device partly idle.
 template <int DIR, class P, int LIM, int TP>
 static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
 {
     constexpr int d1 = (DIR == 0) ? 1 : 0, d2 = (DIR == 2) ? 1 : 2;
-    constexpr int C = 64 / TP;
     const long long npen = (long long)a.n[d1] * a.n[d2];
-    const long long npos = (npen + TP - 1) / TP * (long long)(a.n[DIR] + 8);
+    const long long L = a.n[DIR] + 8;
+    const long long ngroups = (npen + TP - 1) / TP;
+    const long long npos = ngroups * L;
     const int span = a.out;
-    if (a.g[DIR] < 4 || span <= 0 || span % C != 0 || npos + span + 64 > 0x7fffffffll || !pk::stream_stores_fit(a.sv))
+    if (a.g[DIR] < 4 || span <= 0 || span % L != 0 || npos + span + 64 > 0x7fffffffll || !pk::stream_stores_fit(a.sv))
         return hipErrorInvalidValue;
     long long nblk = (npos + span - 1) / span;
     if (nblk != a.ptiles) return hipErrorInvalidValue;
     const void *fn = (const void *)k_stream<DIR, P, LIM, TP>;
     static int cache[kMaxDevices] = {};
     const long long slots = stream_slots(fn, cache);
-    if (slots > 0 && nblk >= slots) nblk = (nblk + slots / 2) / slots * slots;
-    const long long nq = (npos + C - 1) / C;
-    if (nblk > nq) nblk = nq;
-    hipLaunchKernelGGL((k_stream<DIR, P, LIM, TP>), dim3((unsigned)nblk), dim3(64), 0, st, a, (int)nq);
+    if (slots > 0 && nblk >= slots) {
+        // nblk <= ngroups, so one round of slots ranges has a group each
+        long long r = (nblk + slots / 2) / slots;
+        while (r > 1 && r * slots > ngroups) r--;
+        nblk = r * slots;
+    }
+    hipLaunchKernelGGL((k_stream<DIR, P, LIM, TP>), dim3((unsigned)nblk), dim3(64), 0, st, a, (int)ngroups);
     return hipGetLastError();
 }
 

@@ -26,11 +26,12 @@
 // (8 per range) are extra work.  A wave is a whole workgroup, so the
 // barriers between phases only order LDS accesses within the wave.
 //
-// The stream, cut into nq chunks of C positions, is split among the G
-// workgroups of a launch: workgroup w owns chunks [w*nq/G, (w+1)*nq/G), so
-// the ranges differ by at most one chunk.  A range starts 8 positions early
-// so that its first owned outputs have their stencil; it writes only
-// outputs at owned positions.
+// The stream's pencil groups are split among the W workgroups of a launch:
+// workgroup w owns groups [w*G/W, (w+1)*G/W) of the G groups, so the ranges
+// differ by at most one group.  A range that starts or ends inside a group
+// (items of the list form) starts 8 positions early, so that its first
+// owned outputs have their stencil, and loads 4 positions past its end; a
+// range writes only outputs at owned positions.
 
 // LDS carry areas: slot offsets (in positions) of each lagged quantity in
 // the carry block, and how many positions it keeps.
@@ -121,6 +122,13 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
     const int ngroups = (npen + TP - 1) / TP;
     const int npos = ngroups * L;
     const int s1 = min(s0 + span, npos);
+    // A range that starts at a pencil group's first position needs no early
+    // start: the group's own 8 leading positions (its ghost cells) fill the
+    // pipeline before its first output.  One that ends at a group's end (or
+    // the stream's) needs no loads past it: the group's last output, 4
+    // positions before its end, is finished when its last position is loaded.
+    const int first = (s0 % L == 0) ? s0 : s0 - 8;
+    const int end = (s1 % L == 0) ? s1 : s1 + 4;
     const long long st = (DIR == 0) ? 1ll : ((DIR == 1) ? (long long)a.sy : (long long)a.sz);
 
     // LDS slots read at each lag (loop-invariant): this iteration's positions
@@ -150,7 +158,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
     // pencil's transverse coordinates, advanced by C per iteration
     // (a position before the stream, in the first workgroup's start-up, is
     // in group -1, whose pencils do not exist)
-    int sp = s0 - 8 + i;
+    int sp = first + i;
     int grp, j, c1, c2;
     if (sp < 0) {
         grp = -1;
@@ -206,8 +214,9 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
     // which the stores issued after them do not delay.
     __builtin_amdgcn_s_waitcnt(0);
     // outputs lag 4 positions behind the loads: the last owned output,
-    // position s1-1, is finished when position s1+3 is loaded
-    const int iters = (s1 + 4 - (s0 - 8) + C - 1) / C;
+    // position s1-1, is finished when position s1+3 (or the group's last
+    // position) is loaded
+    const int iters = (end - first + C - 1) / C;
     for (int it = 0; it < iters; it++) {
         const int spc = sp, jc = j, cc1 = c1, cc2 = c2;  // this iteration's position and cell
         bool pv;
@@ -393,7 +402,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
 // instances and halves their occupancy.
 template <int DIR, class P, int LIM, int TP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stream(const SweepArgs a,
-                                                                                     const int nq)
+                                                                                     const int ngroups)
 {
     // the quantity of the current phase (slot TP*i + p) and the carry block
     __shared__ double AK[P::NV * stream_stride<TP>()];
@@ -405,9 +414,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     int bid = blockIdx.x;
     const int nfull = (int)(gridDim.x / 8u) * 8;
     if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
-    constexpr int C = 64 / TP;
-    const int q0 = (int)((long long)bid * nq / gridDim.x), q1 = (int)((long long)(bid + 1) * nq / gridDim.x);
-    stream_range<DIR, P, LIM, TP>(a, q0 * C, (q1 - q0) * C, blockIdx.x, AK, U);
+    const int L = a.n[DIR] + 8;
+    const int g0 = (int)((long long)bid * ngroups / gridDim.x), g1 = (int)((long long)(bid + 1) * ngroups / gridDim.x);
+    stream_range<DIR, P, LIM, TP>(a, g0 * L, (g1 - g0) * L, blockIdx.x, AK, U);
 }
 
 // The stream over a block list: one-wave workgroups take items from a

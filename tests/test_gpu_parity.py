@@ -216,7 +216,7 @@ STREAM_CASES = [("mhd", (1, -8), (4, -8)), ("mhd", (1, -3), (2, -5)), ("mhd", (1
                 ("sedov", (1, -8), (4, -16)), ("sedov", (1, -2), (2, -2)), ("mhd2d", (1, -4), (4, -4)),
                 ("mhd_vanleer_moncen", (1, -8), (4, -8)), ("mhd_odd", (1, -5), (4, -3)),
                 ("mhd", (1, -4), (8, -8)), ("sedov", (1, -3), (8, -3)), ("mhd_odd", (1, -2), (8, -5)),
-                ("mhd_rounds", (1, -2), (8, -2)), ("mhd_rounds", (1, -3), (4, -3))]
+                ("mhd_rounds", (1, -2), (2, -2)), ("mhd_rounds", (1, -3), (8, -64)), ("sedov", (1, -64), (8, -64))]
 
 
 @pytest.mark.parametrize("case,tile_x,tile_yz", STREAM_CASES)
@@ -225,9 +225,10 @@ def test_stream_kernel_bit_exact(oracle, case, tile_x, tile_yz):
     stages computed once) gives the oracle's bits over whole stepping loops,
     periodic images and the fused timestep included; workgroup ranges that
     end inside pencils and short pencils (shorter than a wave) included.
-    mhd_rounds: a 64 x 64 x 96 block with more ranges asked for than the
-    device holds at once, so the launch is rounded to whole rounds (up and
-    down) and the ranges differ by one iteration."""
+    Ranges are whole pencil groups; mhd_rounds: a 64 x 64 x 96 block with
+    more ranges asked for than the device holds at once, so the launch is
+    rounded to whole rounds (up along x, down along y) and the ranges differ
+    by one group."""
     if case == "mhd":
         cg, cfg = setups.mhd_turbulence(n=24, seed=17)
         steps = 3
