@@ -380,6 +380,14 @@ int pk_autotune_all(double dt, double chspeed);
  * ts = 0: the default for each block.  No counterpart in the reference (a
  * launch-shape control, as tile_x and tile_yz of pk_config_t). */
 int pk_set_list_tiles(int32_t ddim, int32_t ifluid, int32_t lanes, int32_t ts);
+/* Sweeps the blocks whose interior shape is one of shapes[0..3*nshapes) by a
+ * second launch of the stream form (ts = -m iterations per item, 0: the
+ * default), the others as before, for one direction and fluid kind, until
+ * the next pk_autotune_all or pk_set_list_tiles (nshapes = 0: one launch
+ * again).  pk_autotune_all makes this split itself where the per-shape
+ * times favour it.  No counterpart in the reference (a launch-shape
+ * control). */
+int pk_set_list_split(int32_t ddim, int32_t ifluid, const int32_t *shapes, int32_t nshapes, int32_t ts);
 /* The list sweep's workgroup lanes and tile for one block, direction and
  * fluid: out = {lanes, pencils, cells} (stream form: {64, pencils per lane
  * group, -iterations per item}). */

@@ -31,7 +31,7 @@ __all__ = [
     "solve_cg", "sweep", "update_boundaries", "glmdamping", "timestep_fluid", "time_step",
     "update_chspeed", "glm_factor", "fluid_update", "run", "get_report", "synchronize",
     "profile_begin", "profile_end", "autotune", "timestep_all", "fluid_update_all", "run_all",
-    "autotune_all", "list_tiles", "set_list_tiles",
+    "autotune_all", "list_tiles", "set_list_tiles", "set_list_split",
     "EULER", "RK2_1", "RK2_2", "LIMITERS", "BATCH_GRAPH", "BATCH_PERSISTENT", "BATCH_PER_BLOCK",
 ]
 
@@ -114,6 +114,8 @@ _PROTOS = {
     "pk_autotune_all": [ctypes.c_double, ctypes.c_double],
     "pk_list_tiles": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)],
     "pk_set_list_tiles": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
+    "pk_set_list_split": [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                          ctypes.c_int32],
     "pk_adjoint_setup": [ctypes.c_int64, ctypes.c_int32],
     "pk_state_save": [ctypes.c_int64, ctypes.c_int32],
     "pk_state_load": [ctypes.c_int64, ctypes.c_int32],
@@ -603,6 +605,15 @@ def set_list_tiles(ddim, ifluid, lanes, ts=0):
     until the next autotune_all: lanes 64 (the stream form, ts = -iterations
     per item) or 128, 256, 512 (tiles of ts cells); ts 0: per-block default."""
     _call("pk_set_list_tiles", int(ddim), int(ifluid), int(lanes), int(ts))
+
+
+def set_list_split(ddim, ifluid, shapes, ts=0):
+    """Sweep the blocks of the given interior shapes ((nx, ny, nz) tuples) by
+    a second launch of the stream form (ts = -iterations per item), the rest
+    as before, until the next autotune_all or set_list_tiles."""
+    flat = [int(v) for sh in shapes for v in sh]
+    arr = (ctypes.c_int32 * max(1, len(flat)))(*flat)
+    _call("pk_set_list_split", int(ddim), int(ifluid), arr, len(flat) // 3, int(ts))
 
 
 def list_tiles(ddim, ifluid, cg):

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <set>
+#include <unordered_map>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -1121,17 +1123,19 @@ std::vector<int> step_dirs(std::vector<bool> *damp = nullptr)
 
 struct LaunchTab {
     int nt = 256;                       // workgroup lanes of the persistent launch
+    std::vector<Block *> blocks;        // the blocks of the launch, in item order
     std::vector<pk::SweepArgs> h_args;  // per block, src = buffer of the block's cur at build time
-    std::vector<unsigned> h_first;      // per block first item, nblocks + 1 entries
+    std::vector<unsigned> h_first;      // per block first item, nblocks + 1 entries, then each
+                                        // block's position in the list (nblocks entries)
     pk::SweepArgs *d_args = nullptr;
-    unsigned *d_first = nullptr;
+    unsigned *d_first = nullptr;        // h_first on the device
     unsigned *d_items = nullptr;
     size_t cap_blocks = 0, cap_items = 0;
     unsigned nitems = 0;
     int grid = 0;
 };
 
-const int kQueues = 16;
+const int kQueues = 32;
 
 // The face jobs of one exchange direction over the list.
 struct FaceTab {
@@ -1147,6 +1151,11 @@ struct ListState {
     std::vector<int> cur0;     // each block's cur when the tables were built
     std::vector<double> mdl;   // min(dl) per block
     LaunchTab tab[3][2];
+    // per direction and fluid kind: block shapes swept by a second launch of
+    // the stream form (pk_autotune_all), the rest by tab's workgroup size
+    std::set<std::vector<int>> split_cls[3][2];
+    int split_ts[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // their iterations per item set by pk_set_list_split
+    LaunchTab stab[3][2];
     int nt_pref[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // workgroup lanes chosen by pk_autotune_all (0: default)
     int ts_force[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // tile length (stream: -iterations) set by pk_set_list_tiles
     unsigned *d_queue = nullptr;
@@ -1262,6 +1271,8 @@ void list_free()
     ens_free();
     for (auto &row : g_list.tab)
         for (auto &t : row) free_tab(t);
+    for (auto &row : g_list.stab)
+        for (auto &t : row) free_tab(t);
     if (g_list.d_queue) (void)hipFree(g_list.d_queue);
     if (g_list.d_step) (void)hipFree(g_list.d_step);
     if (g_list.h_step) (void)hipHostFree(g_list.h_step);
@@ -1305,6 +1316,8 @@ int list_default_ts(const Block *b, int d, int nt)
 
 int list_ts(const Block *b, int d, int f, int nt)
 {
+    if (nt == pk::kStreamListNt && nt != g_list.nt_pref[d][g_fmag[f]] && g_list.split_ts[d][g_fmag[f]] != 0)
+        return g_list.split_ts[d][g_fmag[f]];
     const int forced = g_list.ts_force[d][g_fmag[f]];
     if (forced != 0 && nt == g_list.nt_pref[d][g_fmag[f]]) return forced;
     auto it = g_tuned.find(tune_key(b, d, g_fmag[f], nt));
@@ -1317,8 +1330,23 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
               const std::vector<int> *ts_override = nullptr)
 {
     t.nt = nt;
+    t.blocks = blks;
+    if (blks.empty()) {  // nothing to launch
+        t.h_args.clear();
+        t.h_first.assign(1, 0u);
+        t.nitems = 0;
+        return PK_OK;
+    }
     t.h_args.resize(blks.size());
-    t.h_first.resize(blks.size() + 1);
+    t.h_first.resize(2 * blks.size() + 1);
+    {
+        std::unordered_map<const Block *, unsigned> at;
+        for (size_t i = 0; i < g_list.blocks.size(); i++) at[g_list.blocks[i]] = (unsigned)i;
+        for (size_t i = 0; i < blks.size(); i++) {
+            auto it = at.find(blks[i]);
+            t.h_first[blks.size() + 1 + i] = it != at.end() ? it->second : (unsigned)i;
+        }
+    }
     unsigned items = 0;
     for (size_t i = 0; i < blks.size(); i++) {
         Block *b = blks[i];
@@ -1354,7 +1382,7 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
         t.d_args = nullptr;
         t.d_first = nullptr;
         HIPCHK(hipMalloc((void **)&t.d_args, sizeof(pk::SweepArgs) * (blks.size() + 1)));
-        HIPCHK(hipMalloc((void **)&t.d_first, sizeof(unsigned) * (blks.size() + 1)));
+        HIPCHK(hipMalloc((void **)&t.d_first, sizeof(unsigned) * (2 * blks.size() + 1)));
         t.cap_blocks = blks.size() + 1;
     }
     if (t.cap_items < items) {
@@ -1365,7 +1393,7 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
     }
     HIPCHK(hipMemcpyAsync(t.d_args, t.h_args.data(), sizeof(pk::SweepArgs) * blks.size(), hipMemcpyHostToDevice,
                           g_stream));
-    HIPCHK(hipMemcpyAsync(t.d_first, t.h_first.data(), sizeof(unsigned) * (blks.size() + 1), hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(t.d_first, t.h_first.data(), sizeof(unsigned) * (2 * blks.size() + 1), hipMemcpyHostToDevice,
                           g_stream));
     HIPCHK(pk::launch_fill_items(t.d_first, (int)blks.size(), t.d_items, g_stream));
     t.grid = list_resident(d, g_fmag[f], nt);
@@ -1511,7 +1539,12 @@ int list_build()
         if (!g_cfg.has_dir[d]) continue;
         for (int f = 0; f < g_nfl; f++) {
             const int nt = g_list.nt_pref[d][g_fmag[f]] > 0 ? g_list.nt_pref[d][g_fmag[f]] : 256;
-            RCCHK(build_tab(g_list.tab[d][f], g_list.blocks, d, f, nt));
+            const std::set<std::vector<int>> &sc = g_list.split_cls[d][g_fmag[f]];
+            std::vector<Block *> main, split;
+            for (Block *b : g_list.blocks)
+                (nt != pk::kStreamListNt && sc.count({b->n[0], b->n[1], b->n[2]}) ? split : main).push_back(b);
+            RCCHK(build_tab(g_list.tab[d][f], main, d, f, nt));
+            RCCHK(build_tab(g_list.stab[d][f], split, d, f, pk::kStreamListNt));
         }
     }
     if (!g_list.d_queue) HIPCHK(hipMalloc((void **)&g_list.d_queue, sizeof(unsigned) * kQueues));
@@ -1553,7 +1586,7 @@ int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
 {
     const std::vector<SweepStep> sw = step_sweeps();
     const bool persistent = mode != PK_BATCH_PER_BLOCK;
-    if (sw.size() * (size_t)g_nfl > (size_t)kQueues) return fail(PK_EINVAL, "too many launches per step");
+    if (sw.size() * (size_t)g_nfl * 2 > (size_t)kQueues) return fail(PK_EINVAL, "too many launches per step");
     if (persistent) HIPCHK(pk::launch_zero_u32(g_list.d_queue, kQueues, g_stream));
     unsigned long long *slots = ens ? g_ens.d_dtslots : g_list.d_dtslots;
     const int nslots = ens ? kEnsSlots : kDtSlots;
@@ -1564,50 +1597,55 @@ int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
         const int parity = (int)(k % 2);
         if (g_list.linked && !ens) RCCHK(exchange(d, parity));
         for (int f = 0; f < g_nfl; f++) {
-            LaunchTab &t = g_list.tab[d][f];
-            double bytes = 0.0;
-            for (Block *b : g_list.blocks) bytes += (double)b->n[0] * b->n[1] * b->n[2] * nv_of(f) * 16.0;
-            if (persistent) {
-                pk::ListArgs L{};
-                L.args = t.d_args;
-                L.first = t.d_first;
-                L.items = t.d_items;
-                L.nitems = t.nitems;
-                L.queue = g_list.d_queue + q++;
-                L.grab = kListGrab;
-                L.parity = parity;
-                L.apply_fac = sw[k].fac && g_fmag[f];
-                L.step = ens ? g_ens.d_step : g_list.d_step;
-                L.dtslots = sw[k].dt ? slots : nullptr;
-                L.nslots = nslots;
-                L.per_block = ens ? 1 : 0;
-                if (mode == PK_BATCH_GRAPH) {
-                    HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
+            for (LaunchTab *tp : {&g_list.tab[d][f], &g_list.stab[d][f]}) {
+                LaunchTab &t = *tp;
+                if (t.nitems == 0) continue;
+                double bytes = 0.0;
+                for (Block *b : t.blocks) bytes += (double)b->n[0] * b->n[1] * b->n[2] * nv_of(f) * 16.0;
+                if (persistent) {
+                    pk::ListArgs L{};
+                    L.args = t.d_args;
+                    L.first = t.d_first;
+                    L.items = t.d_items;
+                    L.nitems = t.nitems;
+                    L.queue = g_list.d_queue + q++;
+                    L.grab = kListGrab;
+                    L.parity = parity;
+                    L.apply_fac = sw[k].fac && g_fmag[f];
+                    L.step = ens ? g_ens.d_step : g_list.d_step;
+                    L.dtslots = sw[k].dt ? slots : nullptr;
+                    L.nslots = nslots;
+                    L.per_block = ens ? 1 : 0;
+                    L.bpos = t.d_first + t.blocks.size() + 1;
+                    if (mode == PK_BATCH_GRAPH) {
+                        HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
+                    } else {
+                        RCCHK(launch_timed(d, bytes, [&]() { return list_kernel(d, g_fmag[f], L, t.nt, t.grid); }));
+                    }
                 } else {
-                    RCCHK(launch_timed(d, bytes, [&]() { return list_kernel(d, g_fmag[f], L, t.nt, t.grid); }));
-                }
-            } else {
-                for (size_t i = 0; i < g_list.blocks.size(); i++) {
-                    pk::SweepArgs a = t.h_args[i];
-                    if (is_stream(a.TS)) {
-                        // list items of the stream form are sized for the whole list
-                        // sharing the device; one block alone gets the single-block shape
-                        choose_tile(g_list.blocks[i], d, true, g_fmag[f], a.TP, a.TS, a.out, a.chunks, a.ptiles);
+                    for (size_t i = 0; i < t.blocks.size(); i++) {
+                        pk::SweepArgs a = t.h_args[i];
+                        if (is_stream(a.TS)) {
+                            // list items of the stream form are sized for the whole list
+                            // sharing the device; one block alone gets the single-block shape
+                            choose_tile(t.blocks[i], d, true, g_fmag[f], a.TP, a.TS, a.out, a.chunks, a.ptiles);
+                        }
+                        if (parity) {
+                            const double *s0 = a.src;
+                            a.src = a.dst;
+                            a.dst = const_cast<double *>(s0);
+                        }
+                        const size_t bi = ens ? t.h_first[t.blocks.size() + 1 + i] : 0;
+                        const pk::StepScalars &bs = ens ? g_ens.h_step[bi] : sc;
+                        a.base = a.src;
+                        a.c1 = (0.5 * bs.dt) / a.dl[d];
+                        a.c2 = (1.0 * bs.dt) / a.dl[d];
+                        a.ch = bs.ch;
+                        a.psi_fac = (sw[k].fac && g_fmag[f]) ? bs.psi_fac : 1.0;
+                        a.dtslots = sw[k].dt ? slots + (ens ? 4ll * kEnsSlots * bi : 0ll) : nullptr;
+                        a.nslots = nslots;
+                        HIPCHK(sweep_kernel(d, g_fmag[f], true, a));
                     }
-                    if (parity) {
-                        const double *s0 = a.src;
-                        a.src = a.dst;
-                        a.dst = const_cast<double *>(s0);
-                    }
-                    const pk::StepScalars &bs = ens ? g_ens.h_step[i] : sc;
-                    a.base = a.src;
-                    a.c1 = (0.5 * bs.dt) / a.dl[d];
-                    a.c2 = (1.0 * bs.dt) / a.dl[d];
-                    a.ch = bs.ch;
-                    a.psi_fac = (sw[k].fac && g_fmag[f]) ? bs.psi_fac : 1.0;
-                    a.dtslots = sw[k].dt ? slots + (ens ? 4ll * kEnsSlots * i : 0ll) : nullptr;
-                    a.nslots = nslots;
-                    HIPCHK(sweep_kernel(d, g_fmag[f], true, a));
                 }
             }
         }
@@ -2986,7 +3024,7 @@ int pk_autotune_all(double dt, double chspeed)
         for (Block *o : v) cells += (long long)o->n[0] * o->n[1] * o->n[2];
         if (cells < 8ll * 1024 * 1024) v.push_back(b);
     }
-    // the stream form stores through 4 GiB buffer ranges: every block's planes must fit
+    // the stream form stores through buffer ranges of 24 sv bytes: every block's planes must fit
     bool streams_fit = true;
     for (Block *b : g_list.blocks) streams_fit = streams_fit && pk::stream_stores_fit(b->sv);
     LaunchTab tmp;
@@ -2995,14 +3033,11 @@ int pk_autotune_all(double dt, double chspeed)
     for (int d = 0; d < 3 && !rc; d++) {
         if (!g_cfg.has_dir[d]) continue;
         for (int f = 0; f < g_nfl && !rc; f++) {
-            double best_total = 1e300;
-            int best_nt = 256;
-            std::map<std::vector<int>, int> best_ts_at;
+            // per block shape and workgroup size: the best tile length and its time
+            std::map<std::vector<int>, std::map<int, std::pair<int, float>>> timed;
             for (int nt : nt_list) {
                 if (nt == pk::kStreamListNt && !streams_fit) continue;
                 if (list_resident(d, g_fmag[f], nt) <= 0) continue;
-                double total = 0.0;
-                std::map<std::vector<int>, int> ts_at;
                 for (auto &cl : classes) {
                     Block *b0 = cl.second[0];
                     std::vector<int> cand;
@@ -3047,22 +3082,59 @@ int pk_autotune_all(double dt, double chspeed)
                         }
                     }
                     if (rc) break;
-                    ts_at[cl.first] = best_ts;
-                    total += best;
+                    timed[cl.first][nt] = {best_ts, best};
                 }
                 if (rc) break;
-                if (total < best_total) {
-                    best_total = total;
-                    best_nt = nt;
-                    best_ts_at = ts_at;
+            }
+            if (rc) break;
+            // one workgroup size for the whole list, or the stream form for the
+            // shapes where it is faster and one tile size for the rest (two
+            // launches per sweep), whichever the per-shape times favour
+            double best_total = 1e300;
+            int best_nt = 256;
+            bool best_split = false;
+            for (int nt : nt_list) {
+                for (int split = 0; split < 2; split++) {
+                    if (split && nt == pk::kStreamListNt) continue;
+                    double total = 0.0;
+                    bool ok = true;
+                    for (auto &cl : classes) {
+                        auto &tm = timed[cl.first];
+                        const auto it = tm.find(nt);
+                        const auto is = tm.find(pk::kStreamListNt);
+                        if (it == tm.end() && !(split && is != tm.end())) {
+                            ok = false;
+                            break;
+                        }
+                        double t = it != tm.end() ? it->second.second : 1e30;
+                        if (split && is != tm.end()) t = std::min(t, (double)is->second.second);
+                        total += t;
+                    }
+                    if (ok && total < best_total) {
+                        best_total = total;
+                        best_nt = nt;
+                        best_split = split != 0;
+                    }
                 }
             }
             g_list.nt_pref[d][g_fmag[f]] = best_nt;
             g_list.ts_force[d][g_fmag[f]] = 0;
+            g_list.split_cls[d][g_fmag[f]].clear();
+            g_list.split_ts[d][g_fmag[f]] = 0;
             for (auto &cl : classes) {
-                const int ts = best_ts_at[cl.first];
-                g_tuned[tune_key(cl.second[0], d, g_fmag[f], best_nt)] = {
-                    is_stream(ts) ? pk::stream_list_tp(d) : best_nt / ts, ts};
+                auto &tm = timed[cl.first];
+                int nt = best_nt;
+                if (best_split) {
+                    const auto it = tm.find(best_nt), is = tm.find(pk::kStreamListNt);
+                    if (is != tm.end() && (it == tm.end() || is->second.second < it->second.second)) {
+                        nt = pk::kStreamListNt;
+                        g_list.split_cls[d][g_fmag[f]].insert(cl.first);
+                    }
+                }
+                const auto it = tm.find(nt);
+                if (it == tm.end()) continue;
+                const int ts = it->second.first;
+                g_tuned[tune_key(cl.second[0], d, g_fmag[f], nt)] = {is_stream(ts) ? pk::stream_list_tp(d) : nt / ts, ts};
             }
         }
     }
@@ -3101,6 +3173,28 @@ int pk_set_list_tiles(int32_t ddim, int32_t ifluid, int32_t lanes, int32_t ts)
                 return fail(PK_EINVAL, "block planes of %lld cells exceed the stream's buffer range", b->sv);
     g_list.nt_pref[ddim][g_fmag[ifluid]] = lanes;
     g_list.ts_force[ddim][g_fmag[ifluid]] = ts;
+    g_list.split_cls[ddim][g_fmag[ifluid]].clear();
+    g_list.split_ts[ddim][g_fmag[ifluid]] = 0;
+    list_invalidate();
+    return PK_OK;
+}
+
+int pk_set_list_split(int32_t ddim, int32_t ifluid, const int32_t *shapes, int32_t nshapes, int32_t ts)
+{
+    if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
+    if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
+    if (ifluid < 0 || ifluid >= g_nfl) return fail(PK_EINVAL, "bad fluid %d", ifluid);
+    if (nshapes < 0 || (nshapes > 0 && !shapes)) return fail(PK_EINVAL, "null shapes");
+    if (ts != 0 && !is_stream(ts)) return fail(PK_EINVAL, "stream list items of %d iterations", -ts);
+    if (g_list.nt_pref[ddim][g_fmag[ifluid]] == pk::kStreamListNt)
+        return fail(PK_EINVAL, "the list already sweeps every block in the stream form");
+    std::set<std::vector<int>> cls;
+    for (int i = 0; i < nshapes; i++) cls.insert({shapes[3 * i], shapes[3 * i + 1], shapes[3 * i + 2]});
+    for (Block *b : g_list.blocks)
+        if (cls.count({b->n[0], b->n[1], b->n[2]}) && !pk::stream_stores_fit(b->sv))
+            return fail(PK_EINVAL, "block planes of %lld cells exceed the stream's buffer range", b->sv);
+    g_list.split_cls[ddim][g_fmag[ifluid]] = cls;
+    g_list.split_ts[ddim][g_fmag[ifluid]] = ts;
     list_invalidate();
     return PK_OK;
 }
@@ -3111,13 +3205,14 @@ int pk_list_tiles(int32_t ddim, int32_t ifluid, int64_t cg_id, int32_t out[3])
     if (ddim < 0 || ddim > 2 || !g_cfg.has_dir[ddim]) return fail(PK_EINVAL, "inactive or bad direction %d", ddim);
     if (ifluid < 0 || ifluid >= g_nfl) return fail(PK_EINVAL, "bad fluid %d", ifluid);
     RCCHK(list_build());
-    for (size_t i = 0; i < g_list.blocks.size(); i++) {
-        if (g_list.blocks[i]->id != cg_id) continue;
-        const LaunchTab &t = g_list.tab[ddim][ifluid];
-        out[0] = t.nt;
-        out[1] = t.h_args[i].TP;
-        out[2] = t.h_args[i].TS;
-        return PK_OK;
+    for (const LaunchTab *t : {&g_list.tab[ddim][ifluid], &g_list.stab[ddim][ifluid]}) {
+        for (size_t i = 0; i < t->blocks.size(); i++) {
+            if (t->blocks[i]->id != cg_id) continue;
+            out[0] = t->nt;
+            out[1] = t->h_args[i].TP;
+            out[2] = t->h_args[i].TS;
+            return PK_OK;
+        }
     }
     return fail(PK_ENOBLOCK, "unknown cg_id %lld", (long long)cg_id);
 }
@@ -3432,21 +3527,25 @@ int pk_ensemble_record_step(const double *dts, int32_t slot0)
         RCCHK(ens_copy_slots(slot0 + (int)k, true, (int)(k % 2)));
         const int d = dirs[k];
         for (int f = 0; f < g_nfl; f++) {
-            LaunchTab &t = g_list.tab[d][f];
-            pk::ListArgs L{};
-            L.args = t.d_args;
-            L.first = t.d_first;
-            L.items = t.d_items;
-            L.nitems = t.nitems;
-            L.queue = g_list.d_queue + (int)(k * g_nfl + f);
-            L.grab = kListGrab;
-            L.parity = (int)(k % 2);
-            L.apply_fac = damp[k] && g_fmag[f];
-            L.step = g_ens.d_step;
-            L.dtslots = nullptr;
-            L.nslots = kEnsSlots;
-            L.per_block = 1;
-            HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
+            for (int part = 0; part < 2; part++) {
+                LaunchTab &t = part ? g_list.stab[d][f] : g_list.tab[d][f];
+                if (t.nitems == 0) continue;
+                pk::ListArgs L{};
+                L.args = t.d_args;
+                L.first = t.d_first;
+                L.items = t.d_items;
+                L.nitems = t.nitems;
+                L.queue = g_list.d_queue + (int)((k * g_nfl + f) * 2 + part);
+                L.grab = kListGrab;
+                L.parity = (int)(k % 2);
+                L.apply_fac = damp[k] && g_fmag[f];
+                L.step = g_ens.d_step;
+                L.dtslots = nullptr;
+                L.nslots = kEnsSlots;
+                L.per_block = 1;
+                L.bpos = t.d_first + t.blocks.size() + 1;
+                HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
+            }
         }
     }
     list_touch();

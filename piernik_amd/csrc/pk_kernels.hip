@@ -523,13 +523,14 @@ __device__ __forceinline__ unsigned load_list_args(const ListArgs &L, unsigned i
     }
     a.base = a.src;
     typedef const __attribute__((address_space(4))) double cf64;
-    const cf64 *stp = (const cf64 *)(L.step + (L.per_block ? blk : 0));
+    const int inst = L.per_block ? (L.bpos ? (int)((cu32 *)L.bpos)[blk] : blk) : 0;
+    const cf64 *stp = (const cf64 *)(L.step + inst);
     const double sdt = stp[0], sch = stp[1], sfac = stp[2];
     a.c1 = (0.5 * sdt) / a.dl[DIR];  // rk_coef * dt / dl, constants.F90:237-238
     a.c2 = (1.0 * sdt) / a.dl[DIR];
     a.ch = sch;
     a.psi_fac = L.apply_fac ? sfac : 1.0;
-    a.dtslots = L.dtslots ? L.dtslots + (L.per_block ? 4ll * L.nslots * blk : 0ll) : nullptr;
+    a.dtslots = L.dtslots ? L.dtslots + 4ll * L.nslots * inst : nullptr;
     a.nslots = L.nslots;
     return first;
 }

@@ -104,6 +104,7 @@ struct ListArgs {
     int nslots;                   // slots (per block when per_block)
     int per_block;                // independent instances: each block has its own dt, ch, psi factor
                                   // and its own nslots dt slots
+    const unsigned *bpos;         // per_block: each block's position among the instances (null: its index)
     unsigned grab;                // consecutive items taken per queue fetch (0 or 1: one)
 };
 

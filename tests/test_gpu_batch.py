@@ -177,6 +177,43 @@ def test_stream_list_bit_exact(oracle, ts):
         pk.finalize()
 
 
+def test_split_list_bit_exact(oracle):
+    """A list split between two launches per sweep (set_list_split: the
+    16^3 and 32x24x20 blocks by the stream form, the others by 256-lane
+    tiles) equals the oracle's list driver bit for bit after two steps, in
+    every mode."""
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    cgs = setups.batch_blocks(SHAPES + [(32, 24, 20)], cfg, seed=35)
+    init = [(cg.u.copy(order="F"), cg.b.copy(order="F"), cg.psi.copy(order="F")) for cg in cgs]
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    t_o = oracle.run_list(p, obs, 2)
+    split = [(16, 16, 16), (32, 24, 20)]
+    for mode in MODES:
+        for cg, (u, b, psi) in zip(cgs, init):
+            cg.u[...] = u
+            cg.b[...] = b
+            cg.psi[...] = psi
+        print(f"split list, mode {mode}", flush=True)
+        start(cfg, cgs)
+        for d in range(3):
+            for f in range(2):
+                pk.set_list_tiles(d, f, 256, 0)
+                pk.set_list_split(d, f, split, -4)
+        for cg in cgs:
+            lanes, tp, ts = pk.list_tiles(1, 0, cg)
+            if tuple(cg.n) in split:
+                assert (lanes, tp, ts) == (64, 8, -4)
+            else:
+                assert lanes == 256
+        t_g, _ = pk.run_all(2, MODES[mode])
+        assert t_g == t_o
+        for cg, ob in zip(cgs, obs):
+            cg.download()
+            assert_same(cg, ob, f"split list {mode}")
+        pk.finalize()
+
+
 def test_set_list_tiles_rejects_bad_shapes():
     """set_list_tiles refuses workgroup sizes other than 64, 128, 256 and
     512, stream items shorter than two iterations and tiles longer than the

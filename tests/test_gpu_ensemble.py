@@ -63,19 +63,26 @@ def sim_time(dts):
     return t
 
 
-@pytest.mark.parametrize("mode", [pk.BATCH_GRAPH, pk.BATCH_PER_BLOCK])
-def test_ensemble_instances_match_oracle(oracle, mode):
+@pytest.mark.parametrize("mode,split", [(pk.BATCH_GRAPH, False), (pk.BATCH_PER_BLOCK, False),
+                                        (pk.BATCH_GRAPH, True), (pk.BATCH_PER_BLOCK, True)])
+def test_ensemble_instances_match_oracle(oracle, mode, split):
     """Every instance, with its own dt, c_h and GLM damping factor, equals
-    the oracle's main loop on that instance alone, bit for bit."""
+    the oracle's main loop on that instance alone, bit for bit; split: the
+    24^2 instance is swept by a second launch of the stream form, so the
+    launches index the instances' dt through their list positions."""
     cases, cfg = ot_cases()
     hd = cfg.has_dir
     steps = 6
     p = oracle.params_from_config(cfg)
-    print(f"ensemble vs oracle, {len(cases)} instances, mode {mode}", flush=True)
+    print(f"ensemble vs oracle, {len(cases)} instances, mode {mode}, split {split}", flush=True)
     pk.load_library()
     pk.finalize()
     pk.init(cfg)
     ens = adjoint.EnsemblePlan(cfg, [(n, n, 1) for n, *_ in cases], steps, checkpoint_every=steps)
+    if split:
+        for d in range(2):
+            pk.set_list_tiles(d, 0, 256, 0)
+            pk.set_list_split(d, 0, [(24, 24, 1)], -4)
     fin = ens.run([(dev(u, hd), dev(b, hd), dev(psi, hd)) for _, u, b, psi in cases], record=False, mode=mode)
     dts = ens.dts
     ens.close()
