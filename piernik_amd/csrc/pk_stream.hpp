@@ -252,17 +252,20 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         for (int v = 0; v < NV; v++) aw[v * S] = fp[v];
         wave_lds_order();
 
-        // stage-1 flux at (sp-2)+1/2: right face of sp-2, left face of sp-1
+        // stage-1 flux at (sp-2)+1/2: right face of sp-2, left face of sp-1;
+        // the carry of fp is written once the lagged faces are read, so fp
+        // holds no registers through the solver
         double F1[NV];
         {
             double fl[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) fl[v] = fp1[v * S];
-            riemann<P>(a, fl, fm, F1);
-        }
-        if (cfp) {
+            wave_lds_order();
+            if (cfp) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cfp[v * S] = fp[v];
+                for (int v = 0; v < NV; v++) cfp[v * S] = fp[v];
+            }
+            riemann<P>(a, fl, fm, F1);
         }
         wave_lds_order();
 #pragma unroll
@@ -318,17 +321,18 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         for (int v = 0; v < NV; v++) aw[v * S] = gp[v];
         wave_lds_order();
 
-        // stage-2 flux at (sp-4)+1/2
+        // stage-2 flux at (sp-4)+1/2 (the carry of gp as that of fp)
         double F2[NV];
         {
             double fl[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) fl[v] = gp1[v * S];
-            riemann<P>(a, fl, gm, F2);
-        }
-        if (cgp) {
+            wave_lds_order();
+            if (cgp) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) cgp[v * S] = gp[v];
+                for (int v = 0; v < NV; v++) cgp[v * S] = gp[v];
+            }
+            riemann<P>(a, fl, gm, F2);
         }
         wave_lds_order();
 #pragma unroll
