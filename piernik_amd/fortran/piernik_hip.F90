@@ -22,7 +22,8 @@ module piernik_hip
         &    pk_timestep_fluid, pk_fluid_update, pk_run, pk_get_report, pk_synchronize, &
         &    pk_autotune, pk_profile_begin, pk_profile_end, pk_error_message, &
         &    pk_upload_device, pk_download_device, pk_timestep_all, pk_fluid_update_all, pk_run_all, &
-        &    pk_autotune_all, pk_list_tiles, pk_adjoint_setup, pk_state_save, pk_state_load, &
+        &    pk_autotune_all, pk_list_tiles, pk_set_list_tiles, pk_set_list_split, &
+        &    pk_adjoint_setup, pk_state_save, pk_state_load, &
         &    pk_adjoint_set, pk_adjoint_get, pk_fluid_update_record, pk_sweep_adjoint, &
         &    pk_fluid_update_adjoint, pk_timestep_adjoint, pk_ensemble_ids, pk_ensemble_step, pk_ensemble_run, &
         &    pk_ensemble_state_save, pk_ensemble_state_load, pk_ensemble_record_step, &
@@ -310,6 +311,17 @@ module piernik_hip
          integer(c_int64_t), value       :: cg_id
          integer(c_int32_t), intent(out) :: tiles(3)
       end function pk_list_tiles
+
+      integer(c_int32_t) function pk_set_list_tiles(ddim, ifluid, lanes, ts) bind(C, name="pk_set_list_tiles")
+         import :: c_int32_t
+         integer(c_int32_t), value :: ddim, ifluid, lanes, ts
+      end function pk_set_list_tiles
+
+      integer(c_int32_t) function pk_set_list_split(ddim, ifluid, shapes, nshapes, ts) bind(C, name="pk_set_list_split")
+         import :: c_int32_t
+         integer(c_int32_t), value      :: ddim, ifluid, nshapes, ts
+         integer(c_int32_t), intent(in) :: shapes(3, *)
+      end function pk_set_list_split
 
       ! reverse mode of the fused sweep (one fluid, periodic blocks)
       integer(c_int32_t) function pk_adjoint_setup(cg_id, nslots) bind(C, name="pk_adjoint_setup")
