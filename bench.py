@@ -1086,7 +1086,8 @@ def run_blocks(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "k_sweep_list (persistent, fused RK2, HLLD+GLM), x/y/z",
+            "kernel": ("persistent fused RK2 list sweep (HLLD+GLM), x/y/z: " +
+                       ", ".join(("k_stream_list" if tiles["xyz"[d]][0] == 64 else "k_sweep_list") for d in range(3))),
             "per_dir_ms": prof["ms"],
         },
         "cpu_baseline": None,
