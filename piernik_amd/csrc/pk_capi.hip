@@ -51,6 +51,7 @@ struct Block {
     double *d_dtout = nullptr;
     unsigned long long *d_dtslots = nullptr;  // timestep reduction written by the last sweep of a step
     bool dt_pending = false;                  // d_dtout holds timestep_fluid of the current state
+    bool ghosts_stale = false;                // stream sweeps read the periodic interior and left the ghosts old
     // reverse mode (pk_adjoint_setup): saved states and the adjoint state
     std::vector<double *> slots;
     double *lam[2] = {nullptr, nullptr};
@@ -655,7 +656,7 @@ int do_stage(Block *b, int ddim, int istep, double dt, double ch)
 // reduce the next step's timestep_fluid from the cells written.  With
 // src/dst (whole state buffers, one fluid) the sweep reads src and writes
 // dst instead of the block's ping-pong pair, whose state is left alone.
-int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool images = false, bool want_dt = false,
+int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, int images = 0, bool want_dt = false,
              double *src = nullptr, double *dst = nullptr)
 {
     RCCHK(no_fc(b, "a whole sweep runs both stages in one pass"));
@@ -679,10 +680,11 @@ int do_sweep(Block *b, int ddim, double dt, double ch, double psi_fac, bool imag
         a.src = src ? src + b->xo + (long long)g_fplane[f] * b->sv : fluid_state(b, b->cur, f);
         a.base = a.src;
         a.dst = dst ? dst + b->xo + (long long)g_fplane[f] * b->sv : fluid_state(b, 1 - b->cur, f);
-        a.images = images ? 1 : 0;
+        a.images = images;
         a.dtslots = want_dt ? b->d_dtslots : nullptr;
         RCCHK(launch(ddim, true, a, b, f));
     }
+    if (images == 2) b->ghosts_stale = true;
     if (src || dst) return PK_OK;
     b->cur = 1 - b->cur;
     b->uh = -1;
@@ -710,6 +712,7 @@ int do_fill(Block *b)
     a.nplanes = g_nplanes;
     for (int d = 0; d < 3; d++)
         if (b->g[d] > 0) HIPCHK(pk::launch_fill_ghosts(d, a, g_stream));
+    b->ghosts_stale = false;
     return PK_OK;
 }
 
@@ -790,10 +793,30 @@ void glm_scalars(const std::vector<double> &mdls, double dt, double *ch_out, dou
     *fac_out = std::exp(-f);
 }
 
-int do_fluid_update(Block *b, double dt)
+// The fused sweep along every active direction of every fluid of b runs the
+// stream kernel (its shape in pk_config, tuned or given): then its sweeps may
+// read ghost positions from their periodic pre-images and leave the ghosts
+// old (images == 2), to be refilled once by the caller.
+bool stream_all_dirs(const Block *b)
+{
+    for (int f = 0; f < g_nfl; f++)
+        for (int d = 0; d < 3; d++) {
+            if (!g_cfg.has_dir[d]) continue;
+            int TP, TS, out, chunks, ptiles;
+            choose_tile(b, d, true, g_fmag[f], TP, TS, out, chunks, ptiles);
+            if (!is_stream(TS)) return false;
+        }
+    return true;
+}
+
+// One step of fluid_update.  allow_wrap: the sweeps may leave the periodic
+// ghosts old when every one of them is a stream sweep (b->ghosts_stale; the
+// caller refills them with do_fill before they are read).
+int do_fluid_update(Block *b, double dt, bool allow_wrap = false)
 {
     double ch = 0.0, fac = 1.0;
     if (g_mhd) glm_scalars({min_dl(b)}, dt, &ch, &fac);
+    const int images = (allow_wrap && g_cfg.integration_order == 2 && stream_all_dirs(b)) ? 2 : 1;
     for (int half = 0; half < 2; half++) {
         int last = -1;
         for (int s = 0; s < 3; s++) {
@@ -806,7 +829,7 @@ int do_fluid_update(Block *b, double dt)
             if (g_cfg.integration_order == 2) {
                 // the sweep refreshes the periodic ghosts itself; the last one of
                 // the step also reduces the next step's timestep
-                RCCHK(do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0, true, half == 1 && d == last));
+                RCCHK(do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0, images, half == 1 && d == last));
             } else {
                 RCCHK(do_sweep(b, d, dt, ch, (d == last) ? fac : 1.0));
                 RCCHK(do_fill(b));
@@ -1172,6 +1195,14 @@ struct ListState {
     bool linked = false;
     bool ghosts_stale = false;  // linked list after a step: the ghosts other than the swept faces are old
     FaceTab ex[3][2];           // per direction and buffer parity
+    // blocks periodic on themselves: each ghost face from the block's own
+    // opposite interior (one launch per direction fills the whole list), and
+    // whether some block's stream sweeps read ghost positions from their
+    // pre-images and leave the ghosts old (images == 2 in its tables)
+    FaceTab self[3];
+    bool self_built = false;
+    bool wrap_any = false;
+    bool wrap_stale = false;
 };
 ListState g_list;
 
@@ -1273,6 +1304,8 @@ void list_free()
         for (auto &t : row) free_tab(t);
     for (auto &row : g_list.stab)
         for (auto &t : row) free_tab(t);
+    for (auto &t : g_list.self)
+        if (t.d_jobs) (void)hipFree(t.d_jobs);
     if (g_list.d_queue) (void)hipFree(g_list.d_queue);
     if (g_list.d_step) (void)hipFree(g_list.d_step);
     if (g_list.h_step) (void)hipHostFree(g_list.h_step);
@@ -1326,8 +1359,21 @@ int list_ts(const Block *b, int d, int f, int nt)
 }
 
 // Kernel arguments and item table of one persistent launch over `blks`.
+// Block b's fluid f is swept in the stream form along every active direction
+// (the list's workgroup size, or its split shapes).
+bool list_wrap_ok(const Block *b, int f)
+{
+    const int mag = g_fmag[f];
+    for (int d = 0; d < 3; d++) {
+        if (!g_cfg.has_dir[d]) continue;
+        const int nt = g_list.nt_pref[d][mag] > 0 ? g_list.nt_pref[d][mag] : 256;
+        if (nt != pk::kStreamListNt && !g_list.split_cls[d][mag].count({b->n[0], b->n[1], b->n[2]})) return false;
+    }
+    return true;
+}
+
 int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int nt,
-              const std::vector<int> *ts_override = nullptr)
+              const std::vector<int> *ts_override = nullptr, bool allow_wrap = false)
 {
     t.nt = nt;
     t.blocks = blks;
@@ -1356,7 +1402,7 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
         a.src = fluid_state(b, b->cur, f);
         a.base = a.src;
         a.dst = fluid_state(b, 1 - b->cur, f);
-        a.images = g_list.linked ? 0 : 1;
+        a.images = g_list.linked ? 0 : ((allow_wrap && list_wrap_ok(b, f)) ? 2 : 1);
         const int ts = ts_override ? (*ts_override)[i] : list_ts(b, d, f, nt);
         if (nt == pk::kStreamListNt) {
             // the stream form: items are ranges of -ts iterations of the block's stream
@@ -1481,6 +1527,54 @@ int build_faces()
     return PK_OK;
 }
 
+// Face jobs that fill every ghost layer of each block of an unlinked list
+// from the block's own opposite interior (periodic on itself), x then y then
+// z over the full ghosted extents, as k_fill_ghosts does block by block.
+// Lists with more faces than one launch takes keep the per-block fills.
+int build_self_faces()
+{
+    g_list.self_built = false;
+    if (g_list.blocks.size() * 2 > 65535) return PK_OK;
+    for (int d = 0; d < 3; d++) {
+        FaceTab &t = g_list.self[d];
+        t.h.clear();
+        t.maxvals = 0;
+        if (!g_cfg.has_dir[d]) continue;
+        for (Block *b : g_list.blocks) {
+            for (int side = 0; side < 2; side++) {
+                pk::FaceJob j{};
+                j.src = state(b, b->cur);
+                j.dst = state(b, b->cur);
+                j.ssv = j.dsv = b->sv;
+                j.ssy = j.dsy = b->sy;
+                j.ssz = j.dsz = b->sz;
+                for (int q = 0; q < 3; q++) {
+                    j.ext[q] = b->ng[q];
+                    j.sorg[q] = 0;
+                    j.dorg[q] = 0;
+                }
+                j.ext[d] = b->g[d];
+                j.dorg[d] = side == 0 ? 0 : b->g[d] + b->n[d];
+                j.sorg[d] = side == 0 ? b->n[d] : b->g[d];
+                j.nplanes = g_nplanes;
+                const long long vals = (long long)j.ext[0] * j.ext[1] * j.ext[2] * j.nplanes;
+                if (vals > t.maxvals) t.maxvals = vals;
+                t.h.push_back(j);
+            }
+        }
+        if (t.cap < t.h.size()) {
+            if (t.d_jobs) (void)hipFree(t.d_jobs);
+            t.d_jobs = nullptr;
+            HIPCHK(hipMalloc((void **)&t.d_jobs, sizeof(pk::FaceJob) * t.h.size()));
+            t.cap = t.h.size();
+        }
+        HIPCHK(hipMemcpyAsync(t.d_jobs, t.h.data(), sizeof(pk::FaceJob) * t.h.size(), hipMemcpyHostToDevice, g_stream));
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    g_list.self_built = true;
+    return PK_OK;
+}
+
 int exchange(int d, int parity)
 {
     const FaceTab &t = g_list.ex[d][parity];
@@ -1501,7 +1595,16 @@ int exchange_all()
 int list_fill()
 {
     if (g_list.linked) return exchange_all();
-    for (Block *b : g_list.blocks) RCCHK(do_fill(b));
+    if (g_list.self_built) {
+        for (int d = 0; d < 3; d++) {
+            const FaceTab &t = g_list.self[d];
+            if (!t.h.empty()) HIPCHK(pk::launch_exchange_faces(t.d_jobs, (int)t.h.size(), t.maxvals, g_stream));
+        }
+        for (Block *b : g_list.blocks) b->ghosts_stale = false;
+    } else {
+        for (Block *b : g_list.blocks) RCCHK(do_fill(b));
+    }
+    g_list.wrap_stale = false;
     return PK_OK;
 }
 
@@ -1520,6 +1623,7 @@ int list_build()
     if (g_blocks.empty()) return fail(PK_ESTATE, "no registered blocks");
     if (g_cfg.integration_order != 2) return fail(PK_EINVAL, "the batched list sweep is the fused RK2 sweep");
     g_list.blocks.clear();
+    g_list.wrap_any = false;
     g_list.cur0.clear();
     g_list.mdl.clear();
     for (auto &kv : g_blocks) {
@@ -1534,7 +1638,10 @@ int list_build()
     }
     g_list.linked = false;
     for (Block *b : g_list.blocks) g_list.linked = g_list.linked || block_linked(b);
-    if (g_list.linked) RCCHK(build_faces());
+    if (g_list.linked)
+        RCCHK(build_faces());
+    else
+        RCCHK(build_self_faces());
     for (int d = 0; d < 3; d++) {
         if (!g_cfg.has_dir[d]) continue;
         for (int f = 0; f < g_nfl; f++) {
@@ -1543,8 +1650,11 @@ int list_build()
             std::vector<Block *> main, split;
             for (Block *b : g_list.blocks)
                 (nt != pk::kStreamListNt && sc.count({b->n[0], b->n[1], b->n[2]}) ? split : main).push_back(b);
-            RCCHK(build_tab(g_list.tab[d][f], main, d, f, nt));
-            RCCHK(build_tab(g_list.stab[d][f], split, d, f, pk::kStreamListNt));
+            const bool wrap = !g_list.linked && g_list.self_built;
+            RCCHK(build_tab(g_list.tab[d][f], main, d, f, nt, nullptr, wrap));
+            RCCHK(build_tab(g_list.stab[d][f], split, d, f, pk::kStreamListNt, nullptr, wrap));
+            for (const LaunchTab *t : {&g_list.tab[d][f], &g_list.stab[d][f]})
+                for (const pk::SweepArgs &a : t->h_args) g_list.wrap_any = g_list.wrap_any || a.images == 2;
         }
     }
     if (!g_list.d_queue) HIPCHK(hipMalloc((void **)&g_list.d_queue, sizeof(unsigned) * kQueues));
@@ -1617,6 +1727,7 @@ int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
                     L.nslots = nslots;
                     L.per_block = ens ? 1 : 0;
                     L.bpos = t.d_first + t.blocks.size() + 1;
+                    L.no_wrap = ens ? 1 : 0;  // an ensemble keeps its ghosts current (the adjoint reads them)
                     if (mode == PK_BATCH_GRAPH) {
                         HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
                     } else {
@@ -1625,6 +1736,7 @@ int enqueue_step(int mode, const pk::StepScalars &sc, bool ens = false)
                 } else {
                     for (size_t i = 0; i < t.blocks.size(); i++) {
                         pk::SweepArgs a = t.h_args[i];
+                        if (a.images == 2) a.images = 1;  // one block at a time writes its images
                         if (is_stream(a.TS)) {
                             // list items of the stream form are sized for the whole list
                             // sharing the device; one block alone gets the single-block shape
@@ -1726,6 +1838,7 @@ int list_step(int mode, double dt)
     }
     g_list.fresh = true;
     g_list.ghosts_stale = g_list.linked;
+    if (g_list.wrap_any && mode != PK_BATCH_PER_BLOCK) g_list.wrap_stale = true;
     g_ens.fresh = false;
     return PK_OK;
 }
@@ -2730,8 +2843,10 @@ int pk_run(int64_t cg_id, int32_t nsteps, double *t_out, double *dt_out)
     StepLoop L;
     L.blocks = {b};
     L.timestep = [b](double *dt, double *c) { return do_timestep(b, dt, c); };
-    L.update = [b](double dt) { return do_fluid_update(b, dt); };
-    return run_loop(L, nsteps, t_out, dt_out, &g_last_redone);
+    L.update = [b](double dt) { return do_fluid_update(b, dt, true); };
+    RCCHK(run_loop(L, nsteps, t_out, dt_out, &g_last_redone));
+    if (b->ghosts_stale) RCCHK(do_fill(b));  // the ghosts as the sweeps' images would have left them
+    return PK_OK;
 }
 
 int pk_set_cfl_control(int32_t mode, double cfl_max, double dt_shrink, int32_t max_attempts,
@@ -2977,7 +3092,9 @@ int pk_fluid_update_all(int32_t mode, double dt)
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
     RCCHK(list_build());
     if (!g_list.fresh) RCCHK(list_fill());
-    return list_step(mode, dt);
+    RCCHK(list_step(mode, dt));
+    // the ghosts the stream sweeps left old, as their images would have left them
+    return g_list.wrap_stale ? list_fill() : PK_OK;
 }
 
 int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out)
@@ -2994,7 +3111,9 @@ int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out)
         if (!g_list.fresh) RCCHK(list_fill());
         return list_step(mode, dt);
     };
-    return run_loop(L, nsteps, t_out, dt_out, &g_last_redone);
+    RCCHK(run_loop(L, nsteps, t_out, dt_out, &g_last_redone));
+    // the ghosts the stream sweeps left old, as their images would have left them
+    return g_list.wrap_stale ? list_fill() : PK_OK;
 }
 
 int pk_autotune_all(double dt, double chspeed)
@@ -3544,6 +3663,7 @@ int pk_ensemble_record_step(const double *dts, int32_t slot0)
                 L.nslots = kEnsSlots;
                 L.per_block = 1;
                 L.bpos = t.d_first + t.blocks.size() + 1;
+                L.no_wrap = 1;
                 HIPCHK(list_kernel(d, g_fmag[f], L, t.nt, t.grid));
             }
         }

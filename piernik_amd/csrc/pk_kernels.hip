@@ -516,6 +516,7 @@ __device__ __forceinline__ unsigned load_list_args(const ListArgs &L, unsigned i
         for (int i = 0; i < (int)(sizeof(SweepArgs) / 4); i++) aw[i] = w[i];
     }
     const unsigned first = ((cu32 *)L.first)[blk];
+    if (L.no_wrap && a.images == 2) a.images = 1;
     if (L.parity) {
         const double *s0 = a.src;
         a.src = a.dst;
@@ -1037,6 +1038,8 @@ static hipError_t launch_lim(const SweepArgs &a, hipStream_t st)
 
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st)
 {
+    // reading ghost positions from their interior pre-images is the stream kernel's only
+    if (a.images == 2 && !(fused && a.TS <= -2)) return hipErrorInvalidValue;
 #define PK_CASE(D, PP, F) \
     if (dir == D && (mhd == PP::MAG) && fused == F) return launch_lim<D, PP, F>(a, st);
     PK_CASE(0, HD, true) PK_CASE(1, HD, true) PK_CASE(2, HD, true)

@@ -34,7 +34,9 @@ struct SweepArgs {
     unsigned *flags;     // device: bit 0 negative density, bit 1 negative internal energy
     unsigned long long *dmass;  // device: floor-added mass / cell volume, exact counters (pk_mass.hpp)
     // fused mode only:
-    int images;          // also write every output cell's periodic ghost images (one periodic block)
+    int images;          // 1: also write every output cell's periodic ghost images (one periodic block);
+                         // 2 (stream kernels): read each ghost position's interior pre-image instead
+                         // and write no images (the ghosts are left stale)
     unsigned long long *dtslots;  // if set: per slot, min of dl_d/c_d (d = x, y, z) and max c of the
                                   // cells written, as FP64 bit patterns (timestep_fluid of the next step)
     int nslots;
@@ -105,6 +107,7 @@ struct ListArgs {
     int per_block;                // independent instances: each block has its own dt, ch, psi factor
                                   // and its own nslots dt slots
     const unsigned *bpos;         // per_block: each block's position among the instances (null: its index)
+    int no_wrap;                  // blocks with images == 2 write their images instead (ghosts kept current)
     unsigned grab;                // consecutive items taken per queue fetch (0 or 1: one)
 };
 

@@ -187,9 +187,14 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
     };
     // global index (plane 0) of cell j of this lane's pencil; lanes outside
     // the stream or past the last pencil read a valid cell of the block
-    auto cell_index = [&](int jj, bool &pv) -> long long {
+    // (wrap: a ghost position reads the interior cell it images, for a block
+    // periodic on itself whose ghosts this launch leaves unwritten)
+    const bool wrap = a.images == 2;
+    const int n = a.n[ds];
+    auto cell_index = [&](int jj, bool &pv, bool wr) -> long long {
         pv = grp >= 0 && (grp * TP + p) < npen;
         const int cc1 = pv ? c1 : 0, cc2 = pv ? c2 : 0;
+        if (wr) jj = (jj < 4) ? jj + n : ((jj >= n + 4) ? jj - n : jj);
         int xyz[3];
         xyz[ds] = a.g[ds] - 4 + jj;
         xyz[d1] = a.g[d1] + cc1;
@@ -205,7 +210,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
     double W[NV];
     {
         bool pv;
-        const long long idx = cell_index(j, pv);
+        const long long idx = cell_index(j, pv, wrap);
 #pragma unroll
         for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx];
     }
@@ -220,7 +225,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
     for (int it = 0; it < iters; it++) {
         const int spc = sp, jc = j, cc1 = c1, cc2 = c2;  // this iteration's position and cell
         bool pv;
-        const long long idx = cell_index(jc, pv);
+        const long long idx = cell_index(jc, pv, false);
         advance();
 
         // U0 and primitives of position sp
@@ -346,7 +351,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
         // the stores in flight.
         {
             bool pv2;
-            const long long idx2 = cell_index(j, pv2);
+            const long long idx2 = cell_index(j, pv2, wrap);
 #pragma unroll
             for (int k = 0; k < NV; k++) W[k] = a.src[(long long)plane_of<DIR>(k) * a.sv + idx2];
         }
@@ -371,7 +376,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             const long long oidx = idx - 4 * st;
             if (out) {
                 if (a.dtslots) dt_cell<DIR, P>(a, Wn, mn, cmax);
-                if (a.images) {
+                if (a.images == 1) {
                     int xyz[3];
                     xyz[ds] = g - 4 + jo;
                     xyz[d1] = a.g[d1] + cc1;
