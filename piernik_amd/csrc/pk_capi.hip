@@ -1372,8 +1372,12 @@ bool list_wrap_ok(const Block *b, int f)
     return true;
 }
 
+// wrap: 0 every block writes its images; 1 the blocks whose every sweep is a
+// stream sweep read their ghost positions from the pre-images (list_wrap_ok);
+// 2 every block of a stream table does (timing only: the autotuner's
+// candidates, whose results are discarded)
 int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int nt,
-              const std::vector<int> *ts_override = nullptr, bool allow_wrap = false)
+              const std::vector<int> *ts_override = nullptr, int wrap = 0)
 {
     t.nt = nt;
     t.blocks = blks;
@@ -1402,7 +1406,8 @@ int build_tab(LaunchTab &t, const std::vector<Block *> &blks, int d, int f, int 
         a.src = fluid_state(b, b->cur, f);
         a.base = a.src;
         a.dst = fluid_state(b, 1 - b->cur, f);
-        a.images = g_list.linked ? 0 : ((allow_wrap && list_wrap_ok(b, f)) ? 2 : 1);
+        const bool wr = wrap == 2 ? nt == pk::kStreamListNt : (wrap == 1 && list_wrap_ok(b, f));
+        a.images = g_list.linked ? 0 : (wr ? 2 : 1);
         const int ts = ts_override ? (*ts_override)[i] : list_ts(b, d, f, nt);
         if (nt == pk::kStreamListNt) {
             // the stream form: items are ranges of -ts iterations of the block's stream
@@ -1650,7 +1655,7 @@ int list_build()
             std::vector<Block *> main, split;
             for (Block *b : g_list.blocks)
                 (nt != pk::kStreamListNt && sc.count({b->n[0], b->n[1], b->n[2]}) ? split : main).push_back(b);
-            const bool wrap = !g_list.linked && g_list.self_built;
+            const int wrap = (!g_list.linked && g_list.self_built) ? 1 : 0;
             RCCHK(build_tab(g_list.tab[d][f], main, d, f, nt, nullptr, wrap));
             RCCHK(build_tab(g_list.stab[d][f], split, d, f, pk::kStreamListNt, nullptr, wrap));
             for (const LaunchTab *t : {&g_list.tab[d][f], &g_list.stab[d][f]})
@@ -3173,7 +3178,9 @@ int pk_autotune_all(double dt, double chspeed)
                     int best_ts = cand.empty() ? 9 : cand[0];
                     for (int ts : cand) {
                         std::vector<int> tsv(cl.second.size(), ts);
-                        if ((rc = build_tab(tmp, cl.second, d, f, nt, &tsv))) break;
+                        // stream candidates timed as they run when every sweep of the block is one:
+                        // ghost positions from their pre-images, no images written
+                        if ((rc = build_tab(tmp, cl.second, d, f, nt, &tsv, g_list.self_built ? 2 : 0))) break;
                         pk::ListArgs L{};
                         L.args = tmp.d_args;
                         L.first = tmp.d_first;
