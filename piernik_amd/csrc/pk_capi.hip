@@ -1165,8 +1165,50 @@ struct FaceTab {
     std::vector<pk::FaceJob> h;
     pk::FaceJob *d_jobs = nullptr;
     size_t cap = 0;
-    long long maxvals = 0;
+    std::vector<int> wgjob;  // per workgroup of the launch: its job
+    int *d_wgjob = nullptr;
+    size_t wcap = 0;
 };
+
+// Cuts the jobs of t into workgroups of pk::kFaceChunk values and copies the
+// tables to the device.
+int finish_faces(FaceTab &t)
+{
+    t.wgjob.clear();
+    for (size_t i = 0; i < t.h.size(); i++) {
+        pk::FaceJob &j = t.h[i];
+        const long long vals = (long long)j.ext[0] * j.ext[1] * j.ext[2] * j.nplanes;
+        if (vals >= (1ll << 31)) return fail(PK_EINVAL, "ghost face of %lld values exceeds one exchange job", vals);
+        j.wg0 = (int)t.wgjob.size();
+        const long long nwg = (vals + pk::kFaceChunk - 1) / pk::kFaceChunk;
+        if ((long long)t.wgjob.size() + nwg >= (1ll << 31)) return fail(PK_EINVAL, "too many ghost faces for one launch");
+        t.wgjob.insert(t.wgjob.end(), (size_t)nwg, (int)i);
+    }
+    if (t.cap < t.h.size()) {
+        if (t.d_jobs) (void)hipFree(t.d_jobs);
+        t.d_jobs = nullptr;
+        HIPCHK(hipMalloc((void **)&t.d_jobs, sizeof(pk::FaceJob) * t.h.size()));
+        t.cap = t.h.size();
+    }
+    if (t.wcap < t.wgjob.size()) {
+        if (t.d_wgjob) (void)hipFree(t.d_wgjob);
+        t.d_wgjob = nullptr;
+        HIPCHK(hipMalloc((void **)&t.d_wgjob, sizeof(int) * t.wgjob.size()));
+        t.wcap = t.wgjob.size();
+    }
+    if (!t.h.empty()) {
+        HIPCHK(hipMemcpyAsync(t.d_jobs, t.h.data(), sizeof(pk::FaceJob) * t.h.size(), hipMemcpyHostToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(t.d_wgjob, t.wgjob.data(), sizeof(int) * t.wgjob.size(), hipMemcpyHostToDevice,
+                              g_stream));
+    }
+    return PK_OK;
+}
+
+int launch_faces(const FaceTab &t)
+{
+    HIPCHK(pk::launch_exchange_faces(t.d_jobs, t.d_wgjob, (int)t.wgjob.size(), g_stream));
+    return PK_OK;
+}
 
 struct ListState {
     bool built = false;
@@ -1304,16 +1346,20 @@ void list_free()
         for (auto &t : row) free_tab(t);
     for (auto &row : g_list.stab)
         for (auto &t : row) free_tab(t);
-    for (auto &t : g_list.self)
+    for (auto &t : g_list.self) {
         if (t.d_jobs) (void)hipFree(t.d_jobs);
+        if (t.d_wgjob) (void)hipFree(t.d_wgjob);
+    }
     if (g_list.d_queue) (void)hipFree(g_list.d_queue);
     if (g_list.d_step) (void)hipFree(g_list.d_step);
     if (g_list.h_step) (void)hipHostFree(g_list.h_step);
     if (g_list.d_dtslots) (void)hipFree(g_list.d_dtslots);
     if (g_list.d_dtout) (void)hipFree(g_list.d_dtout);
     for (auto &row : g_list.ex)
-        for (auto &t : row)
+        for (auto &t : row) {
             if (t.d_jobs) (void)hipFree(t.d_jobs);
+            if (t.d_wgjob) (void)hipFree(t.d_wgjob);
+        }
     g_list = ListState();
 }
 
@@ -1485,7 +1531,6 @@ int build_faces()
         for (int p = 0; p < 2; p++) {
             FaceTab &t = g_list.ex[d][p];
             t.h.clear();
-            t.maxvals = 0;
             for (Block *b : g_list.blocks) {
                 for (int side = 0; side < 2; side++) {
                     auto it = g_blocks.find(b->nbr[d][side]);
@@ -1512,20 +1557,10 @@ int build_faces()
                     j.dorg[d] = side == 0 ? 0 : b->g[d] + b->n[d];
                     j.sorg[d] = side == 0 ? o->n[d] : o->g[d];
                     j.nplanes = g_nplanes;
-                    const long long vals = (long long)j.ext[0] * j.ext[1] * j.ext[2] * j.nplanes;
-                    if (vals > t.maxvals) t.maxvals = vals;
                     t.h.push_back(j);
                 }
             }
-            if (t.h.size() > 65535) return fail(PK_EINVAL, "too many faces for one exchange launch");
-            if (t.cap < t.h.size()) {
-                if (t.d_jobs) (void)hipFree(t.d_jobs);
-                t.d_jobs = nullptr;
-                HIPCHK(hipMalloc((void **)&t.d_jobs, sizeof(pk::FaceJob) * t.h.size()));
-                t.cap = t.h.size();
-            }
-            HIPCHK(hipMemcpyAsync(t.d_jobs, t.h.data(), sizeof(pk::FaceJob) * t.h.size(), hipMemcpyHostToDevice,
-                                  g_stream));
+            RCCHK(finish_faces(t));
         }
     }
     HIPCHK(hipStreamSynchronize(g_stream));
@@ -1535,15 +1570,13 @@ int build_faces()
 // Face jobs that fill every ghost layer of each block of an unlinked list
 // from the block's own opposite interior (periodic on itself), x then y then
 // z over the full ghosted extents, as k_fill_ghosts does block by block.
-// Lists with more faces than one launch takes keep the per-block fills.
 int build_self_faces()
 {
     g_list.self_built = false;
-    if (g_list.blocks.size() * 2 > 65535) return PK_OK;
     for (int d = 0; d < 3; d++) {
         FaceTab &t = g_list.self[d];
         t.h.clear();
-        t.maxvals = 0;
+        t.wgjob.clear();
         if (!g_cfg.has_dir[d]) continue;
         for (Block *b : g_list.blocks) {
             for (int side = 0; side < 2; side++) {
@@ -1562,18 +1595,10 @@ int build_self_faces()
                 j.dorg[d] = side == 0 ? 0 : b->g[d] + b->n[d];
                 j.sorg[d] = side == 0 ? b->n[d] : b->g[d];
                 j.nplanes = g_nplanes;
-                const long long vals = (long long)j.ext[0] * j.ext[1] * j.ext[2] * j.nplanes;
-                if (vals > t.maxvals) t.maxvals = vals;
                 t.h.push_back(j);
             }
         }
-        if (t.cap < t.h.size()) {
-            if (t.d_jobs) (void)hipFree(t.d_jobs);
-            t.d_jobs = nullptr;
-            HIPCHK(hipMalloc((void **)&t.d_jobs, sizeof(pk::FaceJob) * t.h.size()));
-            t.cap = t.h.size();
-        }
-        HIPCHK(hipMemcpyAsync(t.d_jobs, t.h.data(), sizeof(pk::FaceJob) * t.h.size(), hipMemcpyHostToDevice, g_stream));
+        RCCHK(finish_faces(t));
     }
     HIPCHK(hipStreamSynchronize(g_stream));
     g_list.self_built = true;
@@ -1582,9 +1607,7 @@ int build_self_faces()
 
 int exchange(int d, int parity)
 {
-    const FaceTab &t = g_list.ex[d][parity];
-    HIPCHK(pk::launch_exchange_faces(t.d_jobs, (int)t.h.size(), t.maxvals, g_stream));
-    return PK_OK;
+    return launch_faces(g_list.ex[d][parity]);
 }
 
 // every ghost cell of every block of a linked list, x then y then z
@@ -1603,7 +1626,7 @@ int list_fill()
     if (g_list.self_built) {
         for (int d = 0; d < 3; d++) {
             const FaceTab &t = g_list.self[d];
-            if (!t.h.empty()) HIPCHK(pk::launch_exchange_faces(t.d_jobs, (int)t.h.size(), t.maxvals, g_stream));
+            if (!t.h.empty()) RCCHK(launch_faces(t));
         }
         for (Block *b : g_list.blocks) b->ghosts_stale = false;
     } else {

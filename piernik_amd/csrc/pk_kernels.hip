@@ -637,20 +637,21 @@ __global__ __launch_bounds__(256) void k_copy_blocks(const pk::CopyJob *__restri
 }
 
 // Ghost exchange between blocks (internal_boundaries_local,
-// cg_list_bnd.F90:269-322): one job per face, workgroups stride over its
-// values with x fastest, so rows of y and z faces are coalesced.
-__global__ __launch_bounds__(256) void k_exchange_faces(const pk::FaceJob *__restrict__ jobs)
+// cg_list_bnd.F90:269-322): one job per face, cut into chunks of kFaceChunk
+// values, one workgroup per chunk (wgjob names its job); values run x
+// fastest, so rows of y and z faces are coalesced.
+__global__ __launch_bounds__(256) void k_exchange_faces(const pk::FaceJob *__restrict__ jobs,
+                                                      const int *__restrict__ wgjob)
 {
-    const pk::FaceJob &J = jobs[blockIdx.y];
-    const long long e0 = J.ext[0], e01 = e0 * J.ext[1], ncell = e01 * J.ext[2];
-    const long long total = ncell * J.nplanes;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const long long v = e / ncell, c = e - v * ncell;
-        const int k = (int)(c / e01);
-        const long long r = c - (long long)k * e01;
-        const int j = (int)(r / e0);
-        const int i = (int)(r - (long long)j * e0);
+    const pk::FaceJob &J = jobs[wgjob[blockIdx.x]];
+    const int e0 = J.ext[0], e01 = e0 * J.ext[1], ncell = e01 * J.ext[2];
+    const int total = ncell * J.nplanes;
+    const int base = ((int)blockIdx.x - J.wg0) * pk::kFaceChunk;
+    const int end = min(base + pk::kFaceChunk, total);
+    for (int e = base + (int)threadIdx.x; e < end; e += 256) {
+        const int v = e / ncell, c = e - v * ncell;
+        const int k = c / e01, r = c - k * e01;
+        const int j = r / e0, i = r - j * e0;
         const long long s = v * J.ssv + (J.sorg[0] + i) + (long long)J.ssy * (J.sorg[1] + j) +
                             (long long)J.ssz * (J.sorg[2] + k);
         const long long d = v * J.dsv + (J.dorg[0] + i) + (long long)J.dsy * (J.dorg[1] + j) +
@@ -1222,13 +1223,10 @@ hipError_t launch_dtslots_final_blocks(const unsigned long long *slots, int nslo
     return hipGetLastError();
 }
 
-hipError_t launch_exchange_faces(const pk::FaceJob *jobs, int njobs, long long maxvals, hipStream_t st)
+hipError_t launch_exchange_faces(const pk::FaceJob *jobs, const int *wgjob, int nwg, hipStream_t st)
 {
-    if (njobs <= 0) return hipSuccess;
-    if (njobs > 65535) return hipErrorInvalidValue;
-    const long long wg = (maxvals + 255) / 256;
-    const unsigned gx = (unsigned)(wg < 1 ? 1 : (wg > 512 ? 512 : wg));
-    hipLaunchKernelGGL(k_exchange_faces, dim3(gx, (unsigned)njobs), dim3(256), 0, st, jobs);
+    if (nwg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_exchange_faces, dim3((unsigned)nwg), dim3(256), 0, st, jobs, wgjob);
     return hipGetLastError();
 }
 

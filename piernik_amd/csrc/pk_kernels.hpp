@@ -129,7 +129,10 @@ struct FaceJob {
     int ssy, ssz, dsy, dsz;
     int sorg[3], dorg[3], ext[3];
     int nplanes;
+    int wg0;             // the job's first workgroup in its launch (kFaceChunk values each)
 };
+// Values one workgroup of k_exchange_faces copies.
+constexpr int kFaceChunk = 4096;
 
 hipError_t launch_sweep(int dir, bool mhd, bool fused, const SweepArgs &a, hipStream_t st);
 // Workgroup size that selects the stream form of the persistent list sweep
@@ -154,7 +157,8 @@ hipError_t launch_dtslots_final_blocks(const unsigned long long *slots, int nslo
                                        hipStream_t st);
 hipError_t launch_copy_blocks(const CopyJob *jobs, int njobs, hipStream_t st);
 // the faces of one exchange; maxvals = the largest ext product times nplanes
-hipError_t launch_exchange_faces(const FaceJob *jobs, int njobs, long long maxvals, hipStream_t st);
+// jobs[wgjob[w]] is the job of workgroup w (nwg workgroups)
+hipError_t launch_exchange_faces(const FaceJob *jobs, const int *wgjob, int nwg, hipStream_t st);
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st);
 size_t sweep_lds_bytes(bool mhd, int TP, int TS);
 hipError_t launch_fill_ghosts(int dir, const GhostArgs &a, hipStream_t st);
