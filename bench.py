@@ -481,7 +481,7 @@ def main():
 # Sweep kernel shapes compared by --workload variants: (pencils, cells) as in
 # pk_config tile_x / tile_yz; cells > 0 the tile kernel (TP x TS lanes),
 # -1 the march kernel, -m <= -2 the stream kernel (m iterations per workgroup).
-VARIANTS_X = [(1, 64), (1, -8), (1, -16), (1, -32)]
+VARIANTS_X = [(1, 64), (1, -64), (2, -32), (4, -32), (8, -32)]
 VARIANTS_YZ = [(4, 64), (64, -1), (4, -16), (8, -16), (8, -32)]
 
 

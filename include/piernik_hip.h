@@ -78,7 +78,7 @@ typedef struct pk_config {
     int32_t tile_x[2];         /* sweep tile shape (pencils, cells) along x; 0 = autotuned default;
                                   cells = -1: one lane per whole pencil (march kernel);
                                   cells = -m <= -2: the stream kernel, pencils per lane group
-                                  (1 along x; 1, 2, 4 or 8 along y and z), m iterations per workgroup */
+                                  (1, 2, 4 or 8), about m iterations per workgroup */
     int32_t tile_yz[2];        /* sweep tile shape (pencils, cells) along y and z; 0 = default */
     int32_t fast_math;         /* 0: every operation rounds as the reference's FP64 code (bit-exact);
                                   1: FMA contraction and fast division/sqrt in the split sweeps and

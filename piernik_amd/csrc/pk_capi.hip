@@ -261,7 +261,7 @@ bool is_stream(int ts) { return ts <= -2; }
 void stream_geometry(const Block *b, int d, int tp, int m, int &TP, int &out, int &chunks, int &ptiles)
 {
     const int d1 = (d == 0) ? 1 : 0, d2 = (d == 2) ? 1 : 2;
-    if (d == 0 || (tp != 2 && tp != 4 && tp != 8)) tp = 1;
+    if (tp != 2 && tp != 4 && tp != 8) tp = 1;
     if (m < 2) m = 2;
     TP = tp;
     // ranges of whole pencil groups (L positions each), about m iterations of 64 / tp positions
@@ -3051,10 +3051,9 @@ int pk_autotune(int64_t cg_id, double dt, double chspeed, int32_t tiles_out[6])
                     }
                 }
             }
-            // the stream kernel: one pencil per lane group along x, 1, 2, 4 or
-            // 8 along y and z, workgroups of 8 to 64 iterations
+            // the stream kernel: 1, 2, 4 or 8 pencils per lane group,
+            // workgroups of about 8 to 64 iterations
             for (int tp : {1, 2, 4, 8}) {
-                if (d == 0 && tp != 1) continue;
                 for (int m : {8, 16, 32, 64}) {
                     a.TS = -m;
                     stream_geometry(b, d, tp, m, a.TP, a.out, a.chunks, a.ptiles);

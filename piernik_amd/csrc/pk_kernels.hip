@@ -982,17 +982,12 @@ static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
 template <int DIR, class P, int LIM>
 static hipError_t launch_stream_t(const SweepArgs &a, hipStream_t st)
 {
-    if constexpr (DIR == 0) {
-        // x pencils are rows: one per lane group
-        return a.TP == 1 ? launch_stream_tp<DIR, P, LIM, 1>(a, st) : hipErrorInvalidValue;
-    } else {
-        switch (a.TP) {
-        case 1: return launch_stream_tp<DIR, P, LIM, 1>(a, st);
-        case 2: return launch_stream_tp<DIR, P, LIM, 2>(a, st);
-        case 4: return launch_stream_tp<DIR, P, LIM, 4>(a, st);
-        case 8: return launch_stream_tp<DIR, P, LIM, 8>(a, st);
-        default: return hipErrorInvalidValue;
-        }
+    switch (a.TP) {
+    case 1: return launch_stream_tp<DIR, P, LIM, 1>(a, st);
+    case 2: return launch_stream_tp<DIR, P, LIM, 2>(a, st);
+    case 4: return launch_stream_tp<DIR, P, LIM, 4>(a, st);
+    case 8: return launch_stream_tp<DIR, P, LIM, 8>(a, st);
+    default: return hipErrorInvalidValue;
     }
 }
 

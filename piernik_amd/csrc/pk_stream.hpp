@@ -3,7 +3,8 @@
 // the timestep fold and the periodic images defined there).
 //
 // One wave per workgroup walks a stream of positions along the sweep.  TP
-// pencils (x-consecutive for y and z sweeps, one row for x) advance together;
+// pencils (consecutive along the first transverse direction: x for y and z
+// sweeps, y for x sweeps) advance together;
 // each pencil contributes L = n + 8 positions (its interior and four cells
 // on each side), and the pencils follow each other: the stream is
 //     position s  ->  pencil group s / L, cell (s mod L) - 4 + g.

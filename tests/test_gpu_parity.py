@@ -216,7 +216,9 @@ STREAM_CASES = [("mhd", (1, -8), (4, -8)), ("mhd", (1, -3), (2, -5)), ("mhd", (1
                 ("sedov", (1, -8), (4, -16)), ("sedov", (1, -2), (2, -2)), ("mhd2d", (1, -4), (4, -4)),
                 ("mhd_vanleer_moncen", (1, -8), (4, -8)), ("mhd_odd", (1, -5), (4, -3)),
                 ("mhd", (1, -4), (8, -8)), ("sedov", (1, -3), (8, -3)), ("mhd_odd", (1, -2), (8, -5)),
-                ("mhd_rounds", (1, -2), (2, -2)), ("mhd_rounds", (1, -3), (8, -64)), ("sedov", (1, -64), (8, -64))]
+                ("mhd_rounds", (1, -2), (2, -2)), ("mhd_rounds", (1, -3), (8, -64)), ("sedov", (1, -64), (8, -64)),
+                ("mhd", (4, -8), (8, -8)), ("mhd_odd", (2, -3), (4, -3)), ("sedov", (8, -4), (2, -4)),
+                ("mhd_rounds", (8, -2), (8, -3))]
 
 
 @pytest.mark.parametrize("case,tile_x,tile_yz", STREAM_CASES)
