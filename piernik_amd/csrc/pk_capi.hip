@@ -1666,6 +1666,7 @@ int list_build()
     }
     g_list.linked = false;
     for (Block *b : g_list.blocks) g_list.linked = g_list.linked || block_linked(b);
+    g_list.self_built = false;
     if (g_list.linked)
         RCCHK(build_faces());
     else
