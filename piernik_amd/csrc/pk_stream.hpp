@@ -4,9 +4,11 @@
 //
 // One wave per workgroup walks a stream of positions along the sweep.  TP
 // pencils (consecutive along the first transverse direction: x for y and z
-// sweeps, y for x sweeps) advance together;
-// each pencil contributes L = n + 8 positions (its interior and four cells
-// on each side), and the pencils follow each other: the stream is
+// sweeps, y for x sweeps) advance together; each pencil contributes
+// L = n + 8 positions (its interior and four cells on each side; for a
+// block periodic on itself whose ghosts are left stale, the four on each
+// side are read from the interior cells they image), and the pencils follow
+// each other: the stream is
 //     position s  ->  pencil group s / L, cell (s mod L) - 4 + g.
 // Per iteration the wave takes C = 64 / TP new positions per pencil (lane
 // (p, i): pencil p, position base + i) and finishes everything the new
@@ -23,9 +25,10 @@
 // areas that the last lanes fill before moving on.  Every cell's two RK2
 // stages are computed once, with the operands and operations of the tile
 // kernel in the same order (bit-identical results); only the 8 halo
-// positions per pencil and the start-up positions of each workgroup's range
-// (8 per range) are extra work.  A wave is a whole workgroup, so the
-// barriers between phases only order LDS accesses within the wave.
+// positions per pencil, and the start-up and trailing positions of a range
+// that starts or ends inside a pencil group, are extra work.  A wave is a
+// whole workgroup, so the barriers between phases only order LDS accesses
+// within the wave.
 //
 // The stream's pencil groups are split among the W workgroups of a launch:
 // workgroup w owns groups [w*G/W, (w+1)*G/W) of the G groups, so the ranges
