@@ -274,11 +274,16 @@ def fp64_view(prof, cells, name="sweep_mhd256_fp64_flops.json"):
     if sec <= 0:
         return None
     achieved = flops / sec / 1e12
+    with open(path) as f:
+        busy = json.load(f).get("valu_busy_counters")
     return {
         "achieved_tflops": achieved,
         "peak_tflops": FP64_PEAK_TFLOPS,
         "frac": achieved / FP64_PEAK_TFLOPS,
+        # at the 2.4 GHz peak engine clock; the clock under this load is lower
         "valu_issue_frac": insts * 4.0 / (sec * SIMDS * CLOCK_HZ),
+        # SQ_ACTIVE_INST_VALU over the GPU's own cycles (GRBM_GUI_ACTIVE), per direction
+        "valu_busy_counters": busy,
         "fp64_flops_per_cell": [per[d]["fp64_flops_per_cell"] for d in range(3)],
         "counts": f"profiles/{name}",
     }
