@@ -171,12 +171,16 @@ int nv_of(int f) { return g_fmag[f] ? 9 : 5; }
 
 void list_invalidate();
 void list_touch();
+int settle_ghosts();
 
+// Every entry point that takes a block reads or replaces its state, so the
+// ghosts a list's stream sweeps left old are refilled first.
 int get_block(int64_t id, Block **b)
 {
     if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
     auto it = g_blocks.find(id);
     if (it == g_blocks.end()) return fail(PK_ENOBLOCK, "unknown cg_id %lld", (long long)id);
+    RCCHK(settle_ghosts());
     *b = it->second;
     return PK_OK;
 }
@@ -1290,6 +1294,7 @@ void ens_free();
 
 void list_invalidate()
 {
+    (void)settle_ghosts();  // the tables that know which ghosts are old go away
     g_list.built = false;
     g_list.fresh = false;
     destroy_graph();
@@ -1636,6 +1641,14 @@ int list_fill()
     return PK_OK;
 }
 
+// The ghosts the list's stream sweeps left old (every block of the list, as
+// their images would have left them), before anything reads them.
+int settle_ghosts()
+{
+    if (!g_list.wrap_stale || !g_list.built) return PK_OK;
+    return list_fill();
+}
+
 int list_build()
 {
     if (g_list.built) {
@@ -1846,6 +1859,7 @@ int list_dt(double *dt_out, double *c_out)
 
 int list_step(int mode, double dt)
 {
+    if (mode == PK_BATCH_PER_BLOCK) RCCHK(settle_ghosts());  // one block at a time reads its ghosts
     pk::StepScalars sc{};
     sc.dt = dt;
     sc.ch = 0.0;
@@ -1883,6 +1897,7 @@ int check_mode(int mode)
 
 int ens_build()
 {
+    RCCHK(settle_ghosts());  // the ensemble's sweeps read the ghosts
     RCCHK(list_build());
     if (g_list.linked) return fail(PK_EINVAL, "ensemble instances must be periodic on themselves (no neighbours)");
     const size_t n = g_list.blocks.size();
@@ -2378,6 +2393,7 @@ int solo(const Block *b)
 // neighbours, so a download returns what update_boundaries would leave.
 int ghosts_for_download()
 {
+    RCCHK(settle_ghosts());
     if (!g_list.ghosts_stale) return PK_OK;
     RCCHK(list_build());
     return g_list.linked ? exchange_all() : PK_OK;
@@ -3120,9 +3136,7 @@ int pk_fluid_update_all(int32_t mode, double dt)
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
     RCCHK(list_build());
     if (!g_list.fresh) RCCHK(list_fill());
-    RCCHK(list_step(mode, dt));
-    // the ghosts the stream sweeps left old, as their images would have left them
-    return g_list.wrap_stale ? list_fill() : PK_OK;
+    return list_step(mode, dt);  // ghosts left old are refilled when something reads them
 }
 
 int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out)
@@ -3139,15 +3153,14 @@ int pk_run_all(int32_t mode, int32_t nsteps, double *t_out, double *dt_out)
         if (!g_list.fresh) RCCHK(list_fill());
         return list_step(mode, dt);
     };
-    RCCHK(run_loop(L, nsteps, t_out, dt_out, &g_last_redone));
-    // the ghosts the stream sweeps left old, as their images would have left them
-    return g_list.wrap_stale ? list_fill() : PK_OK;
+    return run_loop(L, nsteps, t_out, dt_out, &g_last_redone);  // ghosts left old are refilled when read
 }
 
 int pk_autotune_all(double dt, double chspeed)
 {
     if (!g_init) return fail(PK_ESTATE, "pk_init has not been called");
     if (!(dt > 0.0)) return fail(PK_EINVAL, "dt must be positive");
+    RCCHK(settle_ghosts());
     RCCHK(list_build());
     pk::StepScalars sc{};
     sc.dt = dt;

@@ -177,6 +177,35 @@ def test_stream_list_bit_exact(oracle, ts):
         pk.finalize()
 
 
+def test_stream_list_calls_mix_modes(oracle):
+    """Blocks whose every sweep is in the stream form read their ghost
+    positions from the interior and leave the ghosts old between calls; the
+    next call that reads ghosts refills them first.  Graph, per-block
+    (which reads ghosts), persistent and single-step calls in a row, then a
+    download, against the oracle's list driver, bit for bit."""
+    cfg = setups.batch_config(fluids=(ION, NEU))
+    cgs = setups.batch_blocks(SHAPES[:4], cfg, seed=14)
+    obs = [oracle.OBlock.from_cg(c) for c in cgs]
+    p = oracle.params_from_config(cfg)
+    start(cfg, cgs)
+    for d in range(3):
+        for f in range(2):
+            pk.set_list_tiles(d, f, 64, -4)
+    t_g = t_o = 0.0
+    for mode in (pk.BATCH_GRAPH, pk.BATCH_PER_BLOCK, pk.BATCH_PERSISTENT, pk.BATCH_GRAPH):
+        print(f"stream list, one step in mode {mode}", flush=True)
+        t_g += pk.run_all(1, mode)[0]
+        t_o += oracle.run_list(p, obs, 1)
+    assert t_g == t_o
+    dt = 0.5 * pk.timestep_all()[0]
+    pk.fluid_update_all(dt, pk.BATCH_GRAPH)
+    oracle.fluid_update_list(p, obs, dt)
+    for cg, ob in zip(cgs, obs):
+        cg.download()
+        assert_same(cg, ob, "mixed modes")
+    pk.finalize()
+
+
 def test_split_list_bit_exact(oracle):
     """A list split between two launches per sweep (set_list_split: the
     16^3 and 32x24x20 blocks by the stream form, the others by 256-lane
