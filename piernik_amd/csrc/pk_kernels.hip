@@ -953,7 +953,8 @@ static int stream_slots(const void *fn, int *cache)
 // launch of more ranges than the device holds at once runs in rounds; it is
 // rounded to a whole number of rounds, the ranges made equal to within one
 // group, so that the last round does not leave the device partly idle.
-template <int DIR, class P, int LIM, int TP>
+// IMG: the kernel that writes periodic ghost images (a.images == 1).
+template <int DIR, class P, int LIM, int TP, bool IMG>
 static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
 {
     constexpr int d1 = (DIR == 0) ? 1 : 0, d2 = (DIR == 2) ? 1 : 2;
@@ -966,7 +967,7 @@ static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
         return hipErrorInvalidValue;
     long long nblk = (npos + span - 1) / span;
     if (nblk != a.ptiles) return hipErrorInvalidValue;
-    const void *fn = (const void *)k_stream<DIR, P, LIM, TP>;
+    const void *fn = (const void *)k_stream<DIR, P, LIM, TP, IMG>;
     static int cache[kMaxDevices] = {};
     const long long slots = stream_slots(fn, cache);
     if (slots > 0 && nblk >= slots) {
@@ -975,18 +976,19 @@ static hipError_t launch_stream_tp(const SweepArgs &a, hipStream_t st)
         while (r > 1 && r * slots > ngroups) r--;
         nblk = r * slots;
     }
-    hipLaunchKernelGGL((k_stream<DIR, P, LIM, TP>), dim3((unsigned)nblk), dim3(64), 0, st, a, (int)ngroups);
+    hipLaunchKernelGGL((k_stream<DIR, P, LIM, TP, IMG>), dim3((unsigned)nblk), dim3(64), 0, st, a, (int)ngroups);
     return hipGetLastError();
 }
 
 template <int DIR, class P, int LIM>
 static hipError_t launch_stream_t(const SweepArgs &a, hipStream_t st)
 {
+    const bool img = a.images == 1;
     switch (a.TP) {
-    case 1: return launch_stream_tp<DIR, P, LIM, 1>(a, st);
-    case 2: return launch_stream_tp<DIR, P, LIM, 2>(a, st);
-    case 4: return launch_stream_tp<DIR, P, LIM, 4>(a, st);
-    case 8: return launch_stream_tp<DIR, P, LIM, 8>(a, st);
+    case 1: return img ? launch_stream_tp<DIR, P, LIM, 1, true>(a, st) : launch_stream_tp<DIR, P, LIM, 1, false>(a, st);
+    case 2: return img ? launch_stream_tp<DIR, P, LIM, 2, true>(a, st) : launch_stream_tp<DIR, P, LIM, 2, false>(a, st);
+    case 4: return img ? launch_stream_tp<DIR, P, LIM, 4, true>(a, st) : launch_stream_tp<DIR, P, LIM, 4, false>(a, st);
+    case 8: return img ? launch_stream_tp<DIR, P, LIM, 8, true>(a, st) : launch_stream_tp<DIR, P, LIM, 8, false>(a, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -102,8 +102,11 @@ __host__ __device__ constexpr int stream_stride()
 // of the calling workgroup.  AK (NV * stream_stride<TP>()) and U
 // (NV * stream_su<TP>()) are the workgroup's LDS; the wave's earlier
 // contents there are never read for an owned output.  slot_id picks the
-// timestep slot.
-template <int DIR, class P, int LIM, int TP>
+// timestep slot.  IMG: 1 the launch writes periodic ghost images
+// (a.images == 1), 0 it writes none (a.images 0 or 2), -1 as a.images says
+// (lists, whose blocks differ); a fixed IMG keeps the image code, and the
+// registers it holds, out of kernels that never run it.
+template <int DIR, class P, int LIM, int TP, int IMG>
 __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, const int span, unsigned slot_id,
                                              double *AK, double *U)
 {
@@ -380,7 +383,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
             const long long oidx = idx - 4 * st;
             if (out) {
                 if (a.dtslots) dt_cell<DIR, P>(a, Wn, mn, cmax);
-                if (a.images == 1) {
+                if ((IMG < 0) ? (a.images == 1) : (IMG == 1)) {
                     int xyz[3];
                     xyz[ds] = g - 4 + jo;
                     xyz[d1] = a.g[d1] + cc1;
@@ -413,7 +416,7 @@ __device__ __forceinline__ void stream_range(const SweepArgs &a, const int s0, c
 // Both stream kernels are held to two waves per SIMD (256 registers): at
 // 253-256 VGPRs the allocator otherwise spills into AGPRs for some template
 // instances and halves their occupancy.
-template <int DIR, class P, int LIM, int TP>
+template <int DIR, class P, int LIM, int TP, bool IMG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stream(const SweepArgs a,
                                                                                      const int ngroups)
 {
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     if (bid < nfull) bid = (bid % 8) * (nfull / 8) + bid / 8;
     const int L = a.n[DIR] + 8;
     const int g0 = (int)((long long)bid * ngroups / gridDim.x), g1 = (int)((long long)(bid + 1) * ngroups / gridDim.x);
-    stream_range<DIR, P, LIM, TP>(a, g0 * L, (g1 - g0) * L, blockIdx.x, AK, U);
+    stream_range<DIR, P, LIM, TP, IMG ? 1 : 0>(a, g0 * L, (g1 - g0) * L, blockIdx.x, AK, U);
 }
 
 // The stream over a block list: one-wave workgroups take items from a
@@ -449,6 +452,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         if (item >= L.nitems) break;
         SweepArgs a;
         const unsigned first = load_list_args<DIR>(L, item, a);
-        stream_range<DIR, P, LIM, TP>(a, (int)(item - first) * a.out, a.out, item, AK, U);
+        stream_range<DIR, P, LIM, TP, -1>(a, (int)(item - first) * a.out, a.out, item, AK, U);
     }
 }
