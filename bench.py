@@ -543,7 +543,7 @@ def run_variants(args, reps=10):
 
 
 def batch_speedup(args, steps=3):
-    """The batched list of the north star's third workload (args.nblocks
+    """The batched two-fluid block list (args.nblocks
     blocks with edges from {16, 24, 32, 48, 64}, seed 0, ionized MHD +
     neutral HD per block), generated on the GPU: ms per step of the
     graph-captured persistent launches and of one launch per block, after an
